@@ -1,0 +1,724 @@
+// context.hip - the kzg_* C-ABI: device buffers, uploads and the per-step
+// orchestration of the contact path on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "kz_device.h"
+#include "kz_math.h"
+#include "kzg.h"
+#include "schedule.h"
+
+namespace kz {
+void launch_narrowphase(hipStream_t s, const uint2* pairs, uint32_t n, const BodyBuf& b, const ShapeBuf& sb,
+                        const ManifoldBuf& m, uint32_t* work, uint32_t* work_count, float max_sep, int world_mode,
+                        uint32_t wave_blocks);
+void launch_integrate_velocities(hipStream_t s, const BodyBuf& b, uint32_t n, float dt, const float* g);
+void launch_build(hipStream_t s, const ContactBuf& c, const ManifoldBuf& m, const BodyBuf& b, const CacheBuf& cache,
+                  uint32_t nc, int warm);
+void launch_prepare(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32_t np, float dt, float rthr,
+                    float baum, float slop, float maxb);
+void launch_solve_groups(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const Schedule& ws,
+                         const Schedule& vel, const SolveParams& sp, uint32_t groups, uint32_t threads,
+                         float* resid);
+void launch_cache_store(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const CacheBuf& cache, uint32_t nc);
+void launch_cache_rehash(hipStream_t s, const CacheBuf& src, const CacheBuf& dst);
+void launch_finish(hipStream_t s, const BodyBuf& b, uint32_t n, float dt, int split);
+void launch_world_inertia(hipStream_t s, const BodyBuf& b, uint32_t n);
+}  // namespace kz
+
+using namespace kz;
+
+namespace {
+
+// Device array that grows geometrically and never copies on growth.
+template <class T> struct DevVec {
+    T* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= cap && p) return hipSuccess;
+        size_t nc = std::max<size_t>(n, cap + cap / 2);
+        nc = std::max<size_t>(nc, 64);
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, nc * sizeof(T));
+        if (e == hipSuccess) cap = nc;
+        return e;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Pinned host staging buffer.
+struct Pinned {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        size_t nc = std::max<size_t>(bytes, cap + cap / 2);
+        nc = std::max<size_t>(nc, 4096);
+        if (p) hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipHostMalloc(&p, nc, hipHostMallocDefault);
+        if (e == hipSuccess) cap = nc;
+        return e;
+    }
+    template <class T> T* as() { return static_cast<T*>(p); }
+    void release() {
+        if (p) hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+constexpr int kPointFloats = 44;
+
+}  // namespace
+
+struct kzg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    kz_world_desc world{};
+    uint32_t cu_count = 256;
+
+    // shapes
+    uint32_t nshapes = 0;
+    std::vector<uint32_t> h_shape_type;
+    DevVec<uint32_t> s_type;
+    DevVec<float4> s_params;
+    DevVec<uint4> s_hull;
+    DevVec<float4> s_hverts;
+    DevVec<uint32_t> s_htris;
+    DevVec<float4> s_hfn;
+
+    // bodies
+    uint32_t nbodies = 0;
+    std::vector<uint32_t> h_packed;
+    std::vector<uint8_t> h_dynamic, h_sensor, h_motion;
+    DevVec<float4> b_x, b_q, b_v, b_w, b_pv, b_pw, b_iw, b_il, b_mat, b_lim;
+    DevVec<uint32_t> b_meta, b_packed;
+
+    // pairs and manifolds
+    uint32_t npairs = 0;
+    std::vector<uint32_t> h_pairs;
+    DevVec<uint2> d_pairs;
+    DevVec<uint32_t> m_count;
+    DevVec<float4> m_pa, m_pb, m_n;
+    DevVec<uint32_t> work, work_count;
+
+    // constraints and points
+    uint32_t ncons = 0, npoints = 0;
+    std::vector<U4> h_info;
+    std::vector<uint32_t> h_cpair;
+    DevVec<uint4> c_info;
+    DevVec<uint32_t> c_pair;
+    DevVec<uint4> p_body;
+    DevVec<float4> p_pa, p_pb, p_n, p_t1, p_t2, p_ra, p_rb, p_la, p_lb, p_imp;
+    DevVec<float> p_nvel;
+
+    // schedule
+    ScheduleBuilder sched_builder;
+    ScheduleOutput sched;
+    DevVec<uint32_t> ws_idx, ws_off, vel_idx, vel_off;
+    DevVec<uint2> ws_grp, vel_grp;
+
+    // warm-start cache
+    DevVec<unsigned long long> cache_key;
+    DevVec<uint32_t> cache_count;
+    DevVec<float4> cache_p;
+    DevVec<float2> cache_t;
+    uint32_t cache_cap = 0;
+    std::unordered_set<uint64_t> cache_keys;
+
+    // residuals
+    DevVec<float> resid;
+    uint32_t resid_points = 0;
+
+    // staging and timing
+    Pinned stage, stage2;
+    hipEvent_t ev[8] = {};
+    kzg_timings timings{};
+    double build_host_ms = 0.0;
+    int phase = 0;  // 0 idle, 1 narrowphase done, 2 built, 3 solved
+
+    // standalone batch scratch
+    DevVec<float4> x_x, x_q;
+    DevVec<uint32_t> x_meta, x_packed;
+    DevVec<uint2> x_pairs;
+
+    BodyBuf bodies() {
+        return BodyBuf{b_x.p, b_q.p, b_v.p, b_w.p, b_pv.p, b_pw.p, b_iw.p, b_il.p, b_mat.p, b_lim.p, b_meta.p, b_packed.p};
+    }
+    ShapeBuf shapes() { return ShapeBuf{s_type.p, s_params.p, s_hull.p, s_hverts.p, s_htris.p, s_hfn.p}; }
+    ManifoldBuf manifolds() { return ManifoldBuf{m_count.p, m_pa.p, m_pb.p, m_n.p}; }
+    ContactBuf contacts() {
+        return ContactBuf{c_info.p, c_pair.p, p_body.p, p_pa.p, p_pb.p, p_n.p, p_t1.p, p_t2.p,
+                          p_ra.p, p_rb.p, p_la.p, p_lb.p, p_imp.p, p_nvel.p};
+    }
+    CacheBuf cache() { return CacheBuf{cache_key.p, cache_count.p, cache_p.p, cache_t.p, cache_cap}; }
+};
+
+namespace {
+
+int fail(kzg_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define KZG_HIP(ctx, call)                                                                            \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            return fail(ctx, KZG_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+template <class T> int upload(kzg_ctx* c, DevVec<T>& d, const T* src, size_t n) {
+    KZG_HIP(c, d.ensure(std::max<size_t>(n, 1)));
+    if (n) {
+        KZG_HIP(c, c->stage.ensure(n * sizeof(T)));
+        std::memcpy(c->stage.p, src, n * sizeof(T));
+        KZG_HIP(c, hipMemcpyAsync(d.p, c->stage.p, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+        KZG_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return KZG_OK;
+}
+
+template <class T> int download(kzg_ctx* c, T* dst, const DevVec<T>& d, size_t n) {
+    if (!n) return KZG_OK;
+    KZG_HIP(c, c->stage2.ensure(n * sizeof(T)));
+    KZG_HIP(c, hipMemcpyAsync(c->stage2.p, d.p, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    std::memcpy(dst, c->stage2.p, n * sizeof(T));
+    return KZG_OK;
+}
+
+float4 f4(float a, float b, float cc, float d) { return make_float4(a, b, cc, d); }
+
+int ensure_cache(kzg_ctx* c, size_t needed_keys) {
+    // Load factor at most 1/2 so that probing always ends quickly.
+    uint32_t want = c->cache_cap ? c->cache_cap : 1024u;
+    while (size_t(want) < 2 * needed_keys) want *= 2u;
+    if (want == c->cache_cap) return KZG_OK;
+    DevVec<unsigned long long> nk;
+    DevVec<uint32_t> ncnt;
+    DevVec<float4> np;
+    DevVec<float2> nt;
+    KZG_HIP(c, nk.ensure(want));
+    KZG_HIP(c, ncnt.ensure(want));
+    KZG_HIP(c, np.ensure(size_t(want) * kPairPoints));
+    KZG_HIP(c, nt.ensure(size_t(want) * kPairPoints));
+    KZG_HIP(c, hipMemsetAsync(nk.p, 0, sizeof(unsigned long long) * want, c->stream));
+    CacheBuf dst{nk.p, ncnt.p, np.p, nt.p, want};
+    if (c->cache_cap) launch_cache_rehash(c->stream, c->cache(), dst);
+    KZG_HIP(c, hipGetLastError());
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    c->cache_key.release();
+    c->cache_count.release();
+    c->cache_p.release();
+    c->cache_t.release();
+    c->cache_key = nk;
+    c->cache_count = ncnt;
+    c->cache_p = np;
+    c->cache_t = nt;
+    c->cache_cap = want;
+    return KZG_OK;
+}
+
+// Face normal of a hull triangle in the hull's frame, as GetShapeFace
+// computes it: NormalizedOr((b - a) x (c - a), +Y) (CollideShape.cpp:66-78).
+float4 hull_face_normal(const float* v, uint32_t a, uint32_t b, uint32_t cc) {
+    f3 A{v[3 * a], v[3 * a + 1], v[3 * a + 2]}, B{v[3 * b], v[3 * b + 1], v[3 * b + 2]},
+        C{v[3 * cc], v[3 * cc + 1], v[3 * cc + 2]};
+    f3 n = normalized_or(cross(sub(B, A), sub(C, A)), f3{0.0f, 1.0f, 0.0f});
+    return make_float4(n.x, n.y, n.z, 0.0f);
+}
+
+}  // namespace
+
+extern "C" {
+
+int kzg_create(kzg_ctx** out, const kz_world_desc* world, int device) {
+    if (!out || !world) return KZG_ERR_ARGUMENT;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0) return KZG_ERR_DEVICE;
+    kzg_ctx* c = new kzg_ctx();
+    c->device = device;
+    c->world = *world;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return KZG_ERR_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = uint32_t(prop.multiProcessorCount);
+    for (auto& e : c->ev) hipEventCreate(&e);
+    if (c->work_count.ensure(1) != hipSuccess || ensure_cache(c, 512) != KZG_OK) {
+        kzg_destroy(c);
+        return KZG_ERR_DEVICE;
+    }
+    *out = c;
+    return KZG_OK;
+}
+
+void kzg_destroy(kzg_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto* d : {&c->b_x, &c->b_q, &c->b_v, &c->b_w, &c->b_pv, &c->b_pw, &c->b_iw, &c->b_il, &c->b_mat, &c->b_lim,
+                    &c->s_params, &c->s_hverts, &c->s_hfn, &c->m_pa, &c->m_pb, &c->m_n, &c->p_pa, &c->p_pb, &c->p_n,
+                    &c->p_t1, &c->p_t2, &c->p_ra, &c->p_rb, &c->p_la, &c->p_lb, &c->p_imp, &c->cache_p, &c->x_x,
+                    &c->x_q})
+        d->release();
+    for (auto* d : {&c->b_meta, &c->b_packed, &c->s_type, &c->s_htris, &c->m_count, &c->work, &c->work_count,
+                    &c->c_pair, &c->ws_idx, &c->ws_off, &c->vel_idx, &c->vel_off, &c->cache_count, &c->x_meta,
+                    &c->x_packed})
+        d->release();
+    c->s_hull.release();
+    c->c_info.release();
+    c->p_body.release();
+    c->p_nvel.release();
+    c->ws_grp.release();
+    c->vel_grp.release();
+    c->d_pairs.release();
+    c->x_pairs.release();
+    c->cache_key.release();
+    c->cache_t.release();
+    c->resid.release();
+    c->stage.release();
+    c->stage2.release();
+    for (auto& e : c->ev)
+        if (e) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* kzg_last_error(const kzg_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int kzg_upload_shapes(kzg_ctx* c, const kz_shape_desc* shapes, uint32_t n, const kzg_hull_range* hulls,
+                      const float* hv, uint32_t nv, const uint32_t* ht, uint32_t nt) {
+    if (!c || (n && !shapes) || (n && !hulls)) return fail(c, KZG_ERR_ARGUMENT, "null shape arrays");
+    hipSetDevice(c->device);
+    std::vector<uint32_t> type(n);
+    std::vector<float4> prm(n);
+    std::vector<uint4> hr(n);
+    std::vector<float4> verts(nv);
+    std::vector<uint32_t> tris(nt);
+    std::vector<float4> fn(nt);
+    for (uint32_t i = 0; i < nv; ++i) verts[i] = f4(hv[3 * i], hv[3 * i + 1], hv[3 * i + 2], 0.0f);
+    for (uint32_t i = 0; i < n; ++i) {
+        const kz_shape_desc& s = shapes[i];
+        if (s.type != kSphere && s.type != kBox && s.type != kCapsule && s.type != kHull && s.type != kPlane)
+            return fail(c, KZG_ERR_UNSUPPORTED, "shape " + std::to_string(i) + ": type " + std::to_string(s.type) +
+                                                    " is outside the device path (sphere, box, capsule, hull, plane)");
+        type[i] = s.type;
+        prm[i] = f4(s.params[0], s.params[1], s.params[2], s.params[3]);
+        hr[i] = make_uint4(0, 0, 0, 0);
+        if (s.type == kHull) {
+            const kzg_hull_range& h = hulls[i];
+            if (h.vert_count < 4 || h.vert_count > uint32_t(kMaxHullVerts) || h.tri_count < 4 ||
+                h.tri_count > uint32_t(kMaxHullTris) || size_t(h.vert_offset) + h.vert_count > nv ||
+                size_t(h.tri_offset) + h.tri_count > nt)
+                return fail(c, KZG_ERR_ARGUMENT, "hull " + std::to_string(i) + ": bad vertex/triangle range");
+            hr[i] = make_uint4(h.vert_offset, h.vert_count, h.tri_offset, h.tri_count);
+            for (uint32_t t = 0; t < h.tri_count; ++t) {
+                const uint32_t* tri = ht + 3 * size_t(h.tri_offset + t);
+                if (tri[0] >= h.vert_count || tri[1] >= h.vert_count || tri[2] >= h.vert_count)
+                    return fail(c, KZG_ERR_ARGUMENT, "hull " + std::to_string(i) + ": triangle index out of range");
+                tris[h.tri_offset + t] = tri[0] | (tri[1] << 8) | (tri[2] << 16);
+                fn[h.tri_offset + t] = hull_face_normal(hv + 3 * size_t(h.vert_offset), tri[0], tri[1], tri[2]);
+            }
+        }
+    }
+    int rc;
+    if ((rc = upload(c, c->s_type, type.data(), n)) || (rc = upload(c, c->s_params, prm.data(), n)) ||
+        (rc = upload(c, c->s_hull, hr.data(), n)) || (rc = upload(c, c->s_hverts, verts.data(), nv)) ||
+        (rc = upload(c, c->s_htris, tris.data(), nt)) || (rc = upload(c, c->s_hfn, fn.data(), nt)))
+        return rc;
+    c->nshapes = n;
+    c->h_shape_type = type;
+    return KZG_OK;
+}
+
+int kzg_upload_bodies(kzg_ctx* c, const kzg_body* b, uint32_t n) {
+    if (!c || (n && !b)) return fail(c, KZG_ERR_ARGUMENT, "null body array");
+    if (n >= (1u << 24)) return fail(c, KZG_ERR_ARGUMENT, "too many bodies");
+    hipSetDevice(c->device);
+    std::vector<float4> x(n), q(n), v(n), w(n), z(n, f4(0, 0, 0, 0)), il(3 * size_t(n)), mat(n), lim(n);
+    std::vector<uint32_t> meta(n), packed(n);
+    c->h_packed.resize(n);
+    c->h_dynamic.resize(n);
+    c->h_sensor.resize(n);
+    c->h_motion.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const kzg_body& d = b[i];
+        if (d.shape >= c->nshapes) return fail(c, KZG_ERR_ARGUMENT, "body " + std::to_string(i) + ": bad shape index");
+        if (d.motion_type > 2) return fail(c, KZG_ERR_ARGUMENT, "body " + std::to_string(i) + ": bad motion type");
+        x[i] = f4(d.position[0], d.position[1], d.position[2], d.inverse_mass);
+        q[i] = f4(d.rotation[0], d.rotation[1], d.rotation[2], d.rotation[3]);
+        v[i] = f4(d.linear_velocity[0], d.linear_velocity[1], d.linear_velocity[2], 0.0f);
+        w[i] = f4(d.angular_velocity[0], d.angular_velocity[1], d.angular_velocity[2], 0.0f);
+        for (int col = 0; col < 3; ++col)
+            il[3 * size_t(i) + col] = f4(d.inverse_inertia_local[3 * col], d.inverse_inertia_local[3 * col + 1],
+                                         d.inverse_inertia_local[3 * col + 2], 0.0f);
+        mat[i] = f4(d.friction, d.restitution, d.linear_damping, d.angular_damping);
+        lim[i] = f4(d.gravity_factor, d.max_linear_velocity, d.max_angular_velocity, 0.0f);
+        meta[i] = d.shape | (d.motion_type << 24) | ((d.is_sensor ? 1u : 0u) << 26);
+        packed[i] = d.packed_id;
+        c->h_packed[i] = d.packed_id;
+        c->h_dynamic[i] = d.motion_type == 2;
+        c->h_sensor[i] = d.is_sensor != 0;
+        c->h_motion[i] = uint8_t(d.motion_type);
+    }
+    int rc;
+    if ((rc = upload(c, c->b_x, x.data(), n)) || (rc = upload(c, c->b_q, q.data(), n)) ||
+        (rc = upload(c, c->b_v, v.data(), n)) || (rc = upload(c, c->b_w, w.data(), n)) ||
+        (rc = upload(c, c->b_pv, z.data(), n)) || (rc = upload(c, c->b_pw, z.data(), n)) ||
+        (rc = upload(c, c->b_il, il.data(), 3 * size_t(n))) || (rc = upload(c, c->b_mat, mat.data(), n)) ||
+        (rc = upload(c, c->b_lim, lim.data(), n)) || (rc = upload(c, c->b_meta, meta.data(), n)) ||
+        (rc = upload(c, c->b_packed, packed.data(), n)))
+        return rc;
+    KZG_HIP(c, c->b_iw.ensure(3 * size_t(std::max<uint32_t>(n, 1))));
+    c->nbodies = n;
+    launch_world_inertia(c->stream, c->bodies(), n);
+    KZG_HIP(c, hipGetLastError());
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    c->phase = 0;
+    return KZG_OK;
+}
+
+int kzg_upload_states(kzg_ctx* c, const kz_body_state* s, uint32_t n) {
+    if (!c || !s || n != c->nbodies) return fail(c, KZG_ERR_ARGUMENT, "state count does not match the body table");
+    hipSetDevice(c->device);
+    std::vector<float4> x(n), q(n), v(n), w(n);
+    std::vector<float4> cur(n);
+    int rc = download(c, cur.data(), c->b_x, n);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < n; ++i) {
+        x[i] = f4(s[i].position[0], s[i].position[1], s[i].position[2], cur[i].w);
+        q[i] = f4(s[i].rotation[0], s[i].rotation[1], s[i].rotation[2], s[i].rotation[3]);
+        v[i] = f4(s[i].linear_velocity[0], s[i].linear_velocity[1], s[i].linear_velocity[2], 0.0f);
+        w[i] = f4(s[i].angular_velocity[0], s[i].angular_velocity[1], s[i].angular_velocity[2], 0.0f);
+    }
+    if ((rc = upload(c, c->b_x, x.data(), n)) || (rc = upload(c, c->b_q, q.data(), n)) ||
+        (rc = upload(c, c->b_v, v.data(), n)) || (rc = upload(c, c->b_w, w.data(), n)))
+        return rc;
+    launch_world_inertia(c->stream, c->bodies(), n);
+    KZG_HIP(c, hipGetLastError());
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    return KZG_OK;
+}
+
+int kzg_narrowphase(kzg_ctx* c, const uint32_t* pairs, uint32_t n) {
+    if (!c || (n && !pairs)) return fail(c, KZG_ERR_ARGUMENT, "null pair array");
+    hipSetDevice(c->device);
+    for (uint32_t i = 0; i < 2 * n; ++i)
+        if (pairs[i] >= c->nbodies) return fail(c, KZG_ERR_ARGUMENT, "pair " + std::to_string(i / 2) + ": bad body index");
+    c->h_pairs.assign(pairs, pairs + 2 * size_t(n));
+    c->npairs = n;
+    KZG_HIP(c, c->m_count.ensure(std::max<uint32_t>(n, 1)));
+    KZG_HIP(c, c->m_pa.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
+    KZG_HIP(c, c->m_pb.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
+    KZG_HIP(c, c->m_n.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
+    KZG_HIP(c, c->work.ensure(std::max<uint32_t>(n, 1)));
+    int rc = 0;
+    KZG_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    if ((rc = upload(c, c->d_pairs, reinterpret_cast<const uint2*>(pairs), n))) return rc;
+    launch_narrowphase(c->stream, c->d_pairs.p, n, c->bodies(), c->shapes(), c->manifolds(), c->work.p,
+                       c->work_count.p, 0.02f, 1, std::min<uint32_t>(n, c->cu_count * 16u));
+    KZG_HIP(c, hipGetLastError());
+    KZG_HIP(c, hipEventRecord(c->ev[1], c->stream));
+    c->phase = 1;
+    return KZG_OK;
+}
+
+int kzg_build_constraints(kzg_ctx* c, int warm) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (c->phase != 1) return fail(c, KZG_ERR_STATE, "kzg_build_constraints needs a preceding kzg_narrowphase");
+    hipSetDevice(c->device);
+    auto t0 = std::chrono::steady_clock::now();
+    const uint32_t n = c->npairs;
+    std::vector<uint32_t> counts(n);
+    int rc = download(c, counts.data(), c->m_count, n);
+    if (rc) return rc;
+    // Compaction in pair order (BuildContactConstraints: empty manifolds and
+    // pairs without a dynamic body are skipped; bodies ordered by packed id).
+    c->h_info.clear();
+    c->h_cpair.clear();
+    uint32_t np = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t k = counts[i];
+        if (k == 0) continue;
+        uint32_t a = c->h_pairs[2 * i], b = c->h_pairs[2 * i + 1];
+        if (c->h_packed[a] > c->h_packed[b]) std::swap(a, b);
+        const uint32_t sensor = (c->h_sensor[a] || c->h_sensor[b]) ? 1u : 0u;
+        c->h_info.push_back(U4{a, b, np, k | (sensor << 8)});
+        c->h_cpair.push_back(i);
+        c->cache_keys.insert(pair_key(c->h_packed[a], c->h_packed[b]));
+        np += k;
+    }
+    c->ncons = uint32_t(c->h_info.size());
+    c->npoints = np;
+    ScheduleInput si;
+    si.num_bodies = c->nbodies;
+    si.dynamic = c->h_dynamic.data();
+    si.num_constraints = c->ncons;
+    si.info = c->h_info.data();
+    si.use_simd = c->world.use_simd_solver != 0;
+    si.max_groups = c->cu_count * 4u;
+    c->sched_builder.build(si, c->sched);
+    const size_t nc1 = std::max<uint32_t>(c->ncons, 1), np1 = std::max<uint32_t>(np, 1);
+    KZG_HIP(c, c->p_body.ensure(np1));
+    for (auto* d : {&c->p_pa, &c->p_pb, &c->p_n, &c->p_t1, &c->p_t2, &c->p_ra, &c->p_rb, &c->p_la, &c->p_lb, &c->p_imp})
+        KZG_HIP(c, d->ensure(np1));
+    KZG_HIP(c, c->p_nvel.ensure(np1));
+    if ((rc = upload(c, c->c_info, reinterpret_cast<const uint4*>(c->h_info.data()), c->ncons)) ||
+        (rc = upload(c, c->c_pair, c->h_cpair.data(), c->ncons)) ||
+        (rc = upload(c, c->ws_idx, c->sched.ws.idx.data(), c->sched.ws.idx.size())) ||
+        (rc = upload(c, c->ws_off, c->sched.ws.off.data(), c->sched.ws.off.size())) ||
+        (rc = upload(c, c->ws_grp, reinterpret_cast<const uint2*>(c->sched.ws.grp.data()), c->sched.ws.grp.size())) ||
+        (rc = upload(c, c->vel_idx, c->sched.vel.idx.data(), c->sched.vel.idx.size())) ||
+        (rc = upload(c, c->vel_off, c->sched.vel.off.data(), c->sched.vel.off.size())) ||
+        (rc = upload(c, c->vel_grp, reinterpret_cast<const uint2*>(c->sched.vel.grp.data()), c->sched.vel.grp.size())))
+        return rc;
+    (void)nc1;
+    if ((rc = ensure_cache(c, c->cache_keys.size()))) return rc;
+    c->build_host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    KZG_HIP(c, hipEventRecord(c->ev[2], c->stream));
+    launch_build(c->stream, c->contacts(), c->manifolds(), c->bodies(), c->cache(), c->ncons, warm);
+    KZG_HIP(c, hipGetLastError());
+    KZG_HIP(c, hipEventRecord(c->ev[3], c->stream));
+    c->phase = 2;
+    return KZG_OK;
+}
+
+int kzg_integrate_velocities(kzg_ctx* c, float dt) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    hipSetDevice(c->device);
+    launch_integrate_velocities(c->stream, c->bodies(), c->nbodies, dt, c->world.gravity);
+    KZG_HIP(c, hipGetLastError());
+    KZG_HIP(c, hipEventRecord(c->ev[4], c->stream));
+    return KZG_OK;
+}
+
+int kzg_solve(kzg_ctx* c, float dt) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (c->phase != 2) return fail(c, KZG_ERR_STATE, "kzg_solve needs a preceding kzg_build_constraints");
+    hipSetDevice(c->device);
+    const kz_world_desc& w = c->world;
+    launch_prepare(c->stream, c->contacts(), c->bodies(), c->npoints, dt, w.restitution_threshold, w.baumgarte,
+                   w.penetration_slop, w.max_bias_velocity);
+    KZG_HIP(c, hipGetLastError());
+    KZG_HIP(c, c->resid.ensure(std::max<uint32_t>(w.velocity_iterations, 1)));
+    KZG_HIP(c, hipMemsetAsync(c->resid.p, 0, sizeof(float) * std::max<uint32_t>(w.velocity_iterations, 1), c->stream));
+    SolveParams sp{dt, w.warm_start_factor, w.velocity_iterations, w.position_iterations, w.use_simd_solver,
+                   w.use_velocity_bias};
+    Schedule ws{c->ws_idx.p, c->ws_off.p, c->ws_grp.p};
+    Schedule vel{c->vel_idx.p, c->vel_off.p, c->vel_grp.p};
+    const uint32_t width = std::max(c->sched.ws.max_width, c->sched.vel.max_width);
+    uint32_t threads = 64;
+    while (threads < width && threads < 1024) threads *= 2;
+    KZG_HIP(c, hipEventRecord(c->ev[5], c->stream));
+    if (c->ncons) launch_solve_groups(c->stream, c->contacts(), c->bodies(), ws, vel, sp, c->sched.groups, threads,
+                                      c->resid.p);
+    KZG_HIP(c, hipGetLastError());
+    KZG_HIP(c, hipEventRecord(c->ev[6], c->stream));
+    launch_cache_store(c->stream, c->contacts(), c->bodies(), c->cache(), c->ncons);
+    KZG_HIP(c, hipGetLastError());
+    c->resid_points = c->npoints;
+    c->phase = 3;
+    return KZG_OK;
+}
+
+int kzg_integrate_positions(kzg_ctx* c, float dt) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    hipSetDevice(c->device);
+    launch_finish(c->stream, c->bodies(), c->nbodies, dt, c->world.use_velocity_bias ? 0 : 1);
+    KZG_HIP(c, hipGetLastError());
+    KZG_HIP(c, hipEventRecord(c->ev[7], c->stream));
+    c->phase = 0;
+    return KZG_OK;
+}
+
+int kzg_step(kzg_ctx* c, const uint32_t* pairs, uint32_t n, float dt) {
+    int rc;
+    if ((rc = kzg_narrowphase(c, pairs, n))) return rc;
+    if ((rc = kzg_build_constraints(c, 0))) return rc;
+    if ((rc = kzg_integrate_velocities(c, dt))) return rc;
+    if ((rc = kzg_solve(c, dt))) return rc;
+    return kzg_integrate_positions(c, dt);
+}
+
+int kzg_query_sizes(kzg_ctx* c, uint32_t* np, uint32_t* nc, uint32_t* npts) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (np) *np = c->npairs;
+    if (nc) *nc = c->ncons;
+    if (npts) *npts = c->npoints;
+    return KZG_OK;
+}
+
+int kzg_download_states(kzg_ctx* c, kz_body_state* out, uint32_t n) {
+    if (!c || !out || n != c->nbodies) return fail(c, KZG_ERR_ARGUMENT, "state buffer does not match the body table");
+    hipSetDevice(c->device);
+    std::vector<float4> x(n), q(n), v(n), w(n);
+    int rc;
+    if ((rc = download(c, x.data(), c->b_x, n)) || (rc = download(c, q.data(), c->b_q, n)) ||
+        (rc = download(c, v.data(), c->b_v, n)) || (rc = download(c, w.data(), c->b_w, n)))
+        return rc;
+    for (uint32_t i = 0; i < n; ++i) {
+        kz_body_state& s = out[i];
+        s.position[0] = x[i].x; s.position[1] = x[i].y; s.position[2] = x[i].z;
+        s.rotation[0] = q[i].x; s.rotation[1] = q[i].y; s.rotation[2] = q[i].z; s.rotation[3] = q[i].w;
+        s.linear_velocity[0] = v[i].x; s.linear_velocity[1] = v[i].y; s.linear_velocity[2] = v[i].z;
+        s.angular_velocity[0] = w[i].x; s.angular_velocity[1] = w[i].y; s.angular_velocity[2] = w[i].z;
+    }
+    return KZG_OK;
+}
+
+int kzg_download_manifolds(kzg_ctx* c, uint32_t* counts, kz_contact_point* pts) {
+    if (!c || !counts || !pts) return fail(c, KZG_ERR_ARGUMENT, "null output");
+    hipSetDevice(c->device);
+    const uint32_t n = c->npairs;
+    std::vector<float4> pa(size_t(n) * kPairPoints), pb(pa.size()), nn(pa.size());
+    int rc;
+    if ((rc = download(c, counts, c->m_count, n)) || (rc = download(c, pa.data(), c->m_pa, pa.size())) ||
+        (rc = download(c, pb.data(), c->m_pb, pb.size())) || (rc = download(c, nn.data(), c->m_n, nn.size())))
+        return rc;
+    for (size_t i = 0; i < pa.size(); ++i) {
+        kz_contact_point& o = pts[i];
+        const bool live = (i % kPairPoints) < counts[i / kPairPoints];
+        o.point_on_a[0] = live ? pa[i].x : 0.0f; o.point_on_a[1] = live ? pa[i].y : 0.0f; o.point_on_a[2] = live ? pa[i].z : 0.0f;
+        o.point_on_b[0] = live ? pb[i].x : 0.0f; o.point_on_b[1] = live ? pb[i].y : 0.0f; o.point_on_b[2] = live ? pb[i].z : 0.0f;
+        o.normal[0] = live ? nn[i].x : 0.0f; o.normal[1] = live ? nn[i].y : 1.0f; o.normal[2] = live ? nn[i].z : 0.0f;
+        o.separation = live ? pa[i].w : 0.0f;
+        uint32_t fid;
+        std::memcpy(&fid, &pb[i].w, 4);
+        o.feature_id = live ? fid : 0u;
+    }
+    return KZG_OK;
+}
+
+int kzg_download_constraints(kzg_ctx* c, uint32_t* hdr, float* pts) {
+    if (!c || !hdr || !pts) return fail(c, KZG_ERR_ARGUMENT, "null output");
+    hipSetDevice(c->device);
+    const uint32_t np = c->npoints;
+    std::vector<uint4> body(np);
+    std::vector<float4> f[10];
+    DevVec<float4>* src[10] = {&c->p_pa, &c->p_pb, &c->p_n, &c->p_t1, &c->p_t2, &c->p_ra, &c->p_rb, &c->p_la, &c->p_lb, &c->p_imp};
+    std::vector<float> nvel(np);
+    int rc = download(c, body.data(), c->p_body, np);
+    if (rc) return rc;
+    for (int k = 0; k < 10; ++k) {
+        f[k].resize(np);
+        if ((rc = download(c, f[k].data(), *src[k], np))) return rc;
+    }
+    if ((rc = download(c, nvel.data(), c->p_nvel, np))) return rc;
+    for (uint32_t ci = 0; ci < c->ncons; ++ci) {
+        const U4& info = c->h_info[ci];
+        const uint32_t k = info.w & 0xFFu, first = info.z;
+        uint32_t flags = (info.w >> 8) & 1u;
+        if (k) flags = body[first].z;
+        hdr[4 * ci] = info.x; hdr[4 * ci + 1] = info.y; hdr[4 * ci + 2] = k; hdr[4 * ci + 3] = flags;
+        for (uint32_t j = 0; j < 8; ++j) {
+            float* o = pts + (8 * size_t(ci) + j) * kPointFloats;
+            std::memset(o, 0, sizeof(float) * kPointFloats);
+            if (j >= k) continue;
+            const uint32_t p = first + j;
+            auto v3 = [&](int at, const float4& v) { o[at] = v.x; o[at + 1] = v.y; o[at + 2] = v.z; };
+            const float4 &pa = f[0][p], &pb = f[1][p], &nn = f[2][p], &t1 = f[3][p], &t2 = f[4][p], &ra = f[5][p],
+                         &rb = f[6][p], &la = f[7][p], &lb = f[8][p], &imp = f[9][p];
+            v3(0, pa); v3(3, pb); v3(6, ra); v3(9, rb); v3(12, la); v3(15, lb); v3(18, nn); v3(21, t1); v3(24, t2);
+            o[27] = nn.w; o[28] = t1.w; o[29] = t2.w;
+            o[30] = imp.x; o[31] = imp.y; o[32] = imp.z;
+            o[33] = pa.w; o[34] = ra.w; o[35] = rb.w; o[36] = imp.w; o[37] = la.w; o[38] = lb.w; o[39] = nvel[p];
+            o[40] = pb.w;
+        }
+    }
+    return KZG_OK;
+}
+
+int kzg_get_residuals(kzg_ctx* c, double* out, uint32_t cap) {
+    if (!c || !out) return KZG_ERR_ARGUMENT;
+    hipSetDevice(c->device);
+    const uint32_t n = c->world.velocity_iterations;
+    std::vector<float> r(std::max<uint32_t>(n, 1));
+    int rc = download(c, r.data(), c->resid, n);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < n && i < cap; ++i)
+        out[i] = c->resid_points ? std::sqrt(double(r[i]) / double(c->resid_points)) : 0.0;
+    return int(std::min(n, cap));
+}
+
+int kzg_get_timings(kzg_ctx* c, kzg_timings* out) {
+    if (!c || !out) return KZG_ERR_ARGUMENT;
+    hipSetDevice(c->device);
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    kzg_timings t{};
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) t.narrowphase_ms = ms;
+    if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) t.build_ms = ms + float(c->build_host_ms);
+    if (hipEventElapsedTime(&ms, c->ev[3], c->ev[4]) == hipSuccess) t.integrate_velocities_ms = ms;
+    if (hipEventElapsedTime(&ms, c->ev[4], c->ev[6]) == hipSuccess) t.solve_ms = ms;
+    if (hipEventElapsedTime(&ms, c->ev[5], c->ev[6]) == hipSuccess) t.solve_kernel_ms = ms;
+    if (hipEventElapsedTime(&ms, c->ev[6], c->ev[7]) == hipSuccess) t.integrate_positions_ms = ms;
+    t.velocity_levels = c->sched.vel.max_depth;
+    t.bias_levels = c->sched.ws.max_depth;
+    t.groups = c->sched.groups;
+    t.islands = c->sched.islands;
+    *out = t;
+    return KZG_OK;
+}
+
+int kzg_collide_batch(kzg_ctx* c, const uint32_t* sa, const uint32_t* sb, const float* xa, const float* xb,
+                      uint32_t n, float max_sep, uint32_t* counts, kz_contact_point* pts, float* ms) {
+    if (!c || (n && (!sa || !sb || !xa || !xb || !counts || !pts))) return fail(c, KZG_ERR_ARGUMENT, "null argument");
+    hipSetDevice(c->device);
+    std::vector<float4> x(2 * size_t(n)), q(2 * size_t(n));
+    std::vector<uint32_t> meta(2 * size_t(n));
+    std::vector<uint2> pr(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (sa[i] >= c->nshapes || sb[i] >= c->nshapes) return fail(c, KZG_ERR_ARGUMENT, "bad shape index");
+        const float* a = xa + 7 * size_t(i);
+        const float* b = xb + 7 * size_t(i);
+        q[i] = f4(a[0], a[1], a[2], a[3]);
+        x[i] = f4(a[4], a[5], a[6], 0.0f);
+        q[n + i] = f4(b[0], b[1], b[2], b[3]);
+        x[n + i] = f4(b[4], b[5], b[6], 0.0f);
+        meta[i] = sa[i] | (2u << 24);
+        meta[n + i] = sb[i] | (2u << 24);
+        pr[i] = make_uint2(i, n + i);
+    }
+    int rc;
+    if ((rc = upload(c, c->x_x, x.data(), x.size())) || (rc = upload(c, c->x_q, q.data(), q.size())) ||
+        (rc = upload(c, c->x_meta, meta.data(), meta.size())) || (rc = upload(c, c->x_pairs, pr.data(), n)))
+        return rc;
+    KZG_HIP(c, c->m_count.ensure(std::max<uint32_t>(n, 1)));
+    KZG_HIP(c, c->m_pa.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
+    KZG_HIP(c, c->m_pb.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
+    KZG_HIP(c, c->m_n.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
+    KZG_HIP(c, c->work.ensure(std::max<uint32_t>(n, 1)));
+    BodyBuf bb{c->x_x.p, c->x_q.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->x_meta.p, nullptr};
+    KZG_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    launch_narrowphase(c->stream, c->x_pairs.p, n, bb, c->shapes(), c->manifolds(), c->work.p, c->work_count.p,
+                       max_sep, 0, std::min<uint32_t>(std::max<uint32_t>(n, 1), c->cu_count * 16u));
+    KZG_HIP(c, hipGetLastError());
+    KZG_HIP(c, hipEventRecord(c->ev[1], c->stream));
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    if (ms) KZG_HIP(c, hipEventElapsedTime(ms, c->ev[0], c->ev[1]));
+    c->npairs = n;
+    c->phase = 0;
+    return kzg_download_manifolds(c, counts, pts);
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+// kz_device.h - device buffer layout and kernel entry points of the contact path.
+//
+// All per-body, per-pair and per-point data live in HBM as structure-of-arrays
+// of float4/uint4 so that every gather of a kernel is one 16-byte load.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kz {
+
+// Shape types (the reference's ShapeType numbering, Shape.h:13-25).
+enum : uint32_t { kSphere = 0, kBox = 1, kCapsule = 2, kHull = 4, kPlane = 8 };
+
+// Body metadata word: shape index (bits 0-23), motion type (24-25), sensor (26).
+__host__ __device__ inline uint32_t meta_shape(uint32_t m) { return m & 0x00FFFFFFu; }
+__host__ __device__ inline uint32_t meta_motion(uint32_t m) { return (m >> 24) & 3u; }
+__host__ __device__ inline bool meta_sensor(uint32_t m) { return (m >> 26) & 1u; }
+
+// Maximum manifold points kept per pair.  Every in-scope narrowphase path
+// (plane, sphere, GJK/EPA, polyhedral clipping) reduces to at most four
+// (src/Collision/CollideShape.cpp:216-224, 406-411).
+constexpr int kPairPoints = 4;
+// Hull limits: ConvexHullShape::Create caps input at 256 points; triangle
+// vertex indices are packed as bytes.
+constexpr int kMaxHullVerts = 256;
+constexpr int kMaxHullTris = 512;
+
+struct ShapeBuf {
+    uint32_t* type;   // [S]
+    float4* params;   // [S] sphere r | box hx hy hz | capsule hh r | plane nx ny nz d
+    uint4* hull;      // [S] vert offset, vert count, tri offset, tri count
+    float4* hverts;   // [V] local vertex xyz
+    uint32_t* htris;  // [T] a | b << 8 | c << 16
+    float4* hfn;      // [T] local face normal NormalizedOr((b-a)x(c-a), +Y)
+};
+
+struct BodyBuf {
+    float4* x;        // position xyz, inverse mass
+    float4* q;        // rotation
+    float4* v;        // linear velocity
+    float4* w;        // angular velocity
+    float4* pv;       // pseudo linear velocity
+    float4* pw;       // pseudo angular velocity
+    float4* iw;       // inverse world inertia, 3 columns per body
+    float4* il;       // inverse local inertia, 3 columns per body
+    float4* mat;      // friction, restitution, linear damping, angular damping
+    float4* lim;      // gravity factor, max linear velocity, max angular velocity, 0
+    uint32_t* meta;   // shape | motion << 24 | sensor << 26
+    uint32_t* packed; // packed BodyID
+};
+
+// Per-pair manifold slots (kPairPoints points each).
+struct ManifoldBuf {
+    uint32_t* count;  // [pairs]
+    float4* pa;       // [pairs*4] point on A xyz, separation
+    float4* pb;       // [pairs*4] point on B xyz, feature id (bits)
+    float4* n;        // [pairs*4] normal xyz
+};
+
+// Contact constraints (compacted in pair order) and their points.
+struct ContactBuf {
+    uint4* info;      // [C] body A, body B, first point, num points | flags << 8
+    uint32_t* pair;   // [C] source pair index
+    uint4* body;      // [P] body A, body B, flags (1 sensor, 2 dynA, 4 dynB), constraint
+    float4* pa;       // [P] point on A, separation
+    float4* pb;       // [P] point on B, feature id bits
+    float4* n;        // [P] normal, normal mass
+    float4* t1;       // [P] tangent 1, tangent mass 1
+    float4* t2;       // [P] tangent 2, tangent mass 2
+    float4* ra;       // [P] rA, restitution bias
+    float4* rb;       // [P] rB, position bias
+    float4* la;       // [P] local anchor A, combined friction
+    float4* lb;       // [P] local anchor B, combined restitution
+    float4* imp;      // [P] normal, tangent 1, tangent 2 impulse, bias impulse
+    float* nvel;      // [P] normal velocity before solving
+};
+
+// Warm-start manifold cache: open addressing, key 0 = empty, never evicted
+// (the semantics of PhysicsWorld::FindCached / GetOrCreateCached,
+// src/World/PhysicsWorld.cpp:405-452).
+struct CacheBuf {
+    unsigned long long* key;  // [cap]
+    uint32_t* count;          // [cap]
+    float4* p;                // [cap*4] position xyz, normal impulse
+    float2* t;                // [cap*4] tangent impulses
+    uint32_t cap;             // power of two
+};
+
+// Level schedule of one solve: per group a range of levels, per level a
+// range of point indices.
+struct Schedule {
+    const uint32_t* idx;      // point indices, grouped by level
+    const uint32_t* lvl_off;  // [levels + 1]
+    const uint2* grp;         // [groups] first level, end level
+};
+
+struct SolveParams {
+    float dt;
+    float warm_start_factor;
+    uint32_t velocity_iterations;
+    uint32_t position_iterations;
+    uint32_t use_simd;
+    uint32_t use_velocity_bias;
+};
+
+__host__ __device__ inline uint64_t pair_key(uint32_t a, uint32_t b) {
+    return a < b ? ((uint64_t(a) << 32) | b) : ((uint64_t(b) << 32) | a);
+}
+__host__ __device__ inline uint32_t cache_hash(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return uint32_t(k);
+}
+
+}  // namespace kz

@@ -1,0 +1,1020 @@
+// narrowphase.hip - convex narrowphase over broadphase pairs, hand-written
+// for gfx950 (wave64).
+//
+// Two kernels:
+//   k_collide_simple  one thread per pair.  Orders the pair (lower packed id
+//                     first, src/World/PhysicsWorld.cpp:381-392), emits the
+//                     closed-form paths (plane-convex, sphere-sphere) and
+//                     appends every other pair to a work list.
+//   k_collide_wave    one wavefront per work-list pair: GJK, EPA and
+//                     polyhedral face clipping.  Hull vertices are held in
+//                     registers, four per lane (256 max), and a support query
+//                     is a wave arg-max; the EPA polytope lives in LDS and
+//                     every per-face step (closest face, visibility, horizon,
+//                     stable compaction, new faces) is a wave-parallel pass
+//                     with ballot/prefix-sum compaction that preserves the
+//                     reference's sequential order.
+// Each function cites the reference routine whose arithmetic it follows;
+// operation order is kept so that results are bit-identical.
+#include "kz_device.h"
+#include "kz_math.h"
+
+namespace kz {
+
+__device__ __forceinline__ f3 l3(const float4& v) { return f3{v.x, v.y, v.z}; }
+__device__ __forceinline__ f4q lq(const float4& v) { return f4q{v.x, v.y, v.z, v.w}; }
+
+struct ShapeView {
+    uint32_t type;
+    float4 prm;
+    uint4 hull;  // vert offset, count, tri offset, count
+};
+
+struct Contact {
+    f3 pa, pb, n;
+    float sep;
+    uint32_t feature;
+};
+
+struct PairOut {
+    uint32_t count;
+};
+
+constexpr int kMaxFaceVerts = 16;
+
+// Closed-form supports (include/Kizuri/Geometry/Shape.h:152-154,183-187,223-226).
+__device__ __forceinline__ f3 support_closed(const ShapeView& s, f3 d) {
+    if (s.type == kSphere) return mul(normalized_or(d, f3{0.0f, 1.0f, 0.0f}), s.prm.x);
+    if (s.type == kBox)
+        return f3{d.x >= 0.0f ? s.prm.x : -s.prm.x, d.y >= 0.0f ? s.prm.y : -s.prm.y, d.z >= 0.0f ? s.prm.z : -s.prm.z};
+    // capsule
+    float h = d.y >= 0.0f ? s.prm.x : -s.prm.x;
+    return add(f3{0.0f, h, 0.0f}, mul(normalized_or(d, f3{0.0f, 1.0f, 0.0f}), s.prm.y));
+}
+
+// ===========================================================================
+// Thread-per-pair kernel: ordering, plane and sphere-sphere paths.
+// ===========================================================================
+__device__ __forceinline__ void swap_contact(Contact& c) {
+    f3 t = c.pa;
+    c.pa = c.pb;
+    c.pb = t;
+    c.n = neg(c.n);
+}
+
+__device__ __forceinline__ void write_contact(const ManifoldBuf& m, uint32_t pair, int k, const Contact& c) {
+    m.pa[pair * kPairPoints + k] = make_float4(c.pa.x, c.pa.y, c.pa.z, c.sep);
+    m.pb[pair * kPairPoints + k] = make_float4(c.pb.x, c.pb.y, c.pb.z, __uint_as_float(c.feature));
+    m.n[pair * kPairPoints + k] = make_float4(c.n.x, c.n.y, c.n.z, 0.0f);
+}
+
+// CollidePlaneConvex (src/Collision/CollideShape.cpp:370-430): A of the
+// result is the convex shape, the normal points from the plane to it.
+__device__ int plane_convex(const ShapeView& P, f4q qp, f3 tp, const ShapeView& B, f4q qb, f3 tb,
+                            const float4* hverts, float max_sep, Contact* out) {
+    const f3 pn = rotate(qp, l3(P.prm));
+    const float pd = P.prm.w + dot(pn, tp);
+    if (B.type == kBox || B.type == kHull) {
+        Contact cand[8];
+        int k = 0;
+        const int nv = B.type == kBox ? 8 : (B.hull.y < uint32_t(kMaxFaceVerts) ? int(B.hull.y) : kMaxFaceVerts);
+        for (int i = 0; i < nv && k < 8; ++i) {
+            f3 v;
+            if (B.type == kBox)
+                v = xform(qb, tb, f3{(i & 1) ? B.prm.x : -B.prm.x, (i & 2) ? B.prm.y : -B.prm.y, (i & 4) ? B.prm.z : -B.prm.z});
+            else
+                v = xform(qb, tb, l3(hverts[B.hull.x + i]));
+            const float d = dot(pn, v) - pd;
+            if (d > max_sep) continue;
+            cand[k].n = pn;
+            cand[k].pa = v;
+            cand[k].pb = sub(v, mul(pn, d));
+            cand[k].sep = d;
+            cand[k].feature = 0;
+            ++k;
+        }
+        if (k > 4) {
+            for (int i = 0; i < 4; ++i) out[i] = cand[i * (k - 1) / 3];
+            return 4;
+        }
+        for (int i = 0; i < k; ++i) out[i] = cand[i];
+        return k;
+    }
+    const f3 ld = inv_rotate(qb, neg(pn));
+    const f3 deep = xform(qb, tb, support_closed(B, ld));
+    const float d = dot(pn, deep) - pd;
+    if (d > max_sep) return 0;
+    out[0].n = pn;
+    out[0].pa = deep;
+    out[0].pb = sub(deep, mul(pn, d));
+    out[0].sep = d;
+    out[0].feature = 0;
+    return 1;
+}
+
+// CollideSphereSphere (CollideShape.cpp:349-365).
+__device__ int sphere_sphere(float ra, f3 ta, float rb, f3 tb, float max_sep, Contact* out) {
+    const f3 d = sub(ta, tb);
+    const float dist = len(d);
+    const float sep = dist - (ra + rb);
+    if (sep > max_sep) return 0;
+    const f3 n = dist > kEps ? divs(d, dist) : f3{0.0f, 1.0f, 0.0f};
+    out[0].n = n;
+    out[0].pa = sub(ta, mul(n, ra));
+    out[0].pb = add(tb, mul(n, rb));
+    out[0].sep = sep;
+    out[0].feature = 0;
+    return 1;
+}
+
+__device__ __forceinline__ ShapeView load_shape(const ShapeBuf& s, uint32_t i) {
+    ShapeView v;
+    v.type = s.type[i];
+    v.prm = s.params[i];
+    v.hull = v.type == kHull ? s.hull[i] : make_uint4(0, 0, 0, 0);
+    return v;
+}
+
+// Pair inputs: body indices.  For the world path body = slot; for the
+// standalone batch every pair side is its own "body" carrying a transform.
+__global__ void k_collide_simple(const uint2* pairs, uint32_t n, BodyBuf b, ShapeBuf s, ManifoldBuf m,
+                                 uint32_t* work, uint32_t* work_count, float max_sep, int world_mode) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint2 pr = pairs[i];
+    uint32_t ia = pr.x, ib = pr.y;
+    m.count[i] = 0;
+    bool swapped = false;
+    if (world_mode) {
+        // PhysicsWorld::Collide: skip unless one body is dynamic; collide in
+        // packed-id order and swap the manifold back afterwards.
+        if (meta_motion(b.meta[ia]) != 2u && meta_motion(b.meta[ib]) != 2u) return;
+        swapped = b.packed[ia] > b.packed[ib];
+        if (swapped) { uint32_t t = ia; ia = ib; ib = t; }
+    }
+    const ShapeView A = load_shape(s, meta_shape(b.meta[ia]));
+    const ShapeView B = load_shape(s, meta_shape(b.meta[ib]));
+    const float4 xa = b.x[ia], xb = b.x[ib];
+    const f4q qa = lq(b.q[ia]), qb = lq(b.q[ib]);
+    Contact c[4];
+    int k = -1;
+    bool swap_result = swapped;
+    // NarrowPhase::Collide dispatch order (CollideShape.cpp:509-591).
+    if (A.type == kPlane) {
+        k = plane_convex(A, qa, l3(xa), B, qb, l3(xb), s.hverts, max_sep, c);
+        if (k > 0) swap_result = !swap_result;
+    } else if (B.type == kPlane) {
+        k = plane_convex(B, qb, l3(xb), A, qa, l3(xa), s.hverts, max_sep, c);
+    } else if (A.type == kSphere && B.type == kSphere) {
+        k = sphere_sphere(A.prm.x, l3(xa), B.prm.x, l3(xb), max_sep, c);
+    } else {
+        // Iterative class: hand the (ordered) pair to the wave kernel.
+        const uint32_t slot = atomicAdd(work_count, 1u);
+        work[slot] = i | (swapped ? 0x80000000u : 0u);
+        return;
+    }
+    for (int j = 0; j < k; ++j) {
+        if (swap_result) swap_contact(c[j]);
+        write_contact(m, i, j, c[j]);
+    }
+    m.count[i] = uint32_t(k);
+}
+
+// ===========================================================================
+// Wave-per-pair kernel.
+// ===========================================================================
+constexpr int kEpaMaxVerts = 72;    // 4 seed vertices + one per iteration (64)
+constexpr int kEpaMaxFaces = 512;   // EPA.cpp:206
+constexpr int kEpaMaxHorizon = 256; // EPA.cpp:253
+
+struct SP {
+    f3 a, b, p;
+};
+
+struct WaveLds {
+    // EPA polytope.
+    float vx[kEpaMaxVerts * 9];           // SP records: a, b, p
+    float fnx[kEpaMaxFaces], fny[kEpaMaxFaces], fnz[kEpaMaxFaces], fd[kEpaMaxFaces];
+    uint32_t fabc[kEpaMaxFaces];          // a | b << 8 | c << 16
+    uint16_t vis[kEpaMaxFaces];
+    uint16_t hor[kEpaMaxHorizon];         // edge u | v << 8
+    // Clipping scratch (written by lane 0 only).
+    float poly[2][2 * kMaxFaceVerts][3];
+    float cand[8][11];
+};
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Wave arg-max of (value, index): larger value wins, ties go to the lower
+// index; indices never set stay at INT_MAX.  Equivalent to the sequential
+// "strict > from -inf, first wins" scans of the reference.
+__device__ __forceinline__ void wave_argmax(float& v, int& i) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        float ov = __shfl_xor(v, off, 64);
+        int oi = __shfl_xor(i, off, 64);
+        if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+    }
+}
+// Wave arg-min with the same tie rule ("strict <, first wins").
+__device__ __forceinline__ void wave_argmin(float& v, int& i) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        float ov = __shfl_xor(v, off, 64);
+        int oi = __shfl_xor(i, off, 64);
+        if (ov < v || (ov == v && oi < i)) { v = ov; i = oi; }
+    }
+}
+
+// A shape with its transform, as seen by the wave; hull vertices are held
+// four per lane (vertex lane + 64 j in slot j).
+struct WShape {
+    ShapeView s;
+    f4q q;
+    f3 t;
+    f3 hv[4];
+    int nverts;
+};
+
+__device__ __forceinline__ void load_wshape(WShape& w, const ShapeBuf& sb, uint32_t shape, float4 x, float4 q) {
+    w.s = load_shape(sb, shape);
+    w.q = lq(q);
+    w.t = l3(x);
+    w.nverts = 0;
+    if (w.s.type == kHull) {
+        w.nverts = int(w.s.hull.y);
+        const int lane = lane_id();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int vi = lane + 64 * j;
+            w.hv[j] = vi < w.nverts ? l3(sb.hverts[w.s.hull.x + vi]) : f3{0.0f, 0.0f, 0.0f};
+        }
+    }
+}
+
+// ConvexHullShape::GetSupport (src/Geometry/Shapes.cpp:538-547) as a wave
+// arg-max; other shapes are closed forms evaluated by every lane.
+__device__ __forceinline__ f3 wsupport(const WShape& w, f3 d) {
+    if (w.s.type != kHull) return support_closed(w.s, d);
+    float best = -kInf;
+    int bi = 0x7FFFFFFF;
+    const int lane = lane_id();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int vi = lane + 64 * j;
+        if (vi < w.nverts) {
+            float v = dot(w.hv[j], d);
+            if (v > best) { best = v; bi = vi; }
+        }
+    }
+    wave_argmax(best, bi);
+    if (bi == 0x7FFFFFFF) bi = 0;
+    const int slot = bi >> 6, src = bi & 63;
+    f3 mine = slot == 0 ? w.hv[0] : (slot == 1 ? w.hv[1] : (slot == 2 ? w.hv[2] : w.hv[3]));
+    return f3{__shfl(mine.x, src, 64), __shfl(mine.y, src, 64), __shfl(mine.z, src, 64)};
+}
+
+// SupportMap::GetSupport (include/Kizuri/Collision/GJK.h:31-40).
+__device__ __forceinline__ SP msupport(const WShape& A, const WShape& B, f3 d) {
+    SP r;
+    const f3 la = inv_rotate(A.q, d);
+    const f3 lb = inv_rotate(B.q, neg(d));
+    r.a = xform(A.q, A.t, wsupport(A, la));
+    r.b = xform(B.q, B.t, wsupport(B, lb));
+    r.p = sub(r.a, r.b);
+    return r;
+}
+
+// --------------------------- GJK (src/Collision/GJK.cpp) -------------------
+__device__ f3 closest_on_triangle(f3 p, f3 a, f3 b, f3 c, float& u, float& v, float& w) {
+    const f3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
+    const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) { u = 1; v = 0; w = 0; return a; }
+    const f3 bp = sub(p, b);
+    const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) { u = 0; v = 1; w = 0; return b; }
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        const float t = d1 / (d1 - d3);
+        u = 1 - t; v = t; w = 0;
+        return add(a, mul(ab, t));
+    }
+    const f3 cp = sub(p, c);
+    const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) { u = 0; v = 0; w = 1; return c; }
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        const float t = d2 / (d2 - d6);
+        u = 1 - t; v = 0; w = t;
+        return add(a, mul(ac, t));
+    }
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        const float t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        u = 0; v = 1 - t; w = t;
+        return add(b, mul(sub(c, b), t));
+    }
+    const float den = 1.0f / (va + vb + vc);
+    v = vb * den;
+    w = vc * den;
+    u = 1.0f - v - w;
+    return add(add(a, mul(ab, v)), mul(ac, w));
+}
+
+__device__ bool reduce_simplex(SP* s, int& n, f3& closest) {
+    const f3 O{0.0f, 0.0f, 0.0f};
+    if (n == 1) {
+        closest = s[0].p;
+    } else if (n == 2) {
+        const f3 a = s[0].p, b = s[1].p, ab = sub(b, a);
+        const float t = -dot(a, ab);
+        if (t <= 0.0f) { closest = a; n = 1; }
+        else if (t >= len2(ab)) { s[0] = s[1]; closest = b; n = 1; }
+        else closest = add(a, mul(ab, t / len2(ab)));
+    } else if (n == 3) {
+        float u, v, w;
+        closest = closest_on_triangle(O, s[0].p, s[1].p, s[2].p, u, v, w);
+        SP keep[3] = {};
+        int k = 0;
+        if (u > 0.0f) keep[k++] = s[0];
+        if (v > 0.0f) keep[k++] = s[1];
+        if (w > 0.0f) keep[k++] = s[2];
+        if (k == 0) k = 1;
+        for (int i = 0; i < 3; ++i)
+            if (i < k) s[i] = keep[i];
+        n = k;
+    } else if (n == 4) {
+        auto outside = [&](f3 p0, f3 p1, f3 p2, f3 opp) {
+            f3 nn = cross(sub(p1, p0), sub(p2, p0));
+            if (dot(nn, sub(opp, p0)) > 0.0f) nn = neg(nn);
+            return dot(nn, sub(O, p0)) > 0.0f;
+        };
+        const bool oABC = outside(s[0].p, s[1].p, s[2].p, s[3].p);
+        const bool oACD = outside(s[0].p, s[2].p, s[3].p, s[1].p);
+        const bool oADB = outside(s[0].p, s[3].p, s[1].p, s[2].p);
+        const bool oBDC = outside(s[1].p, s[3].p, s[2].p, s[0].p);
+        if (!oABC && !oACD && !oADB && !oBDC) return true;
+        float best = kInf;
+        SP tri[3] = {};
+        int cnt = 0;
+        const int fi[4][3] = {{0, 1, 2}, {0, 2, 3}, {0, 3, 1}, {1, 3, 2}};
+        const bool fo[4] = {oABC, oACD, oADB, oBDC};
+        for (int f = 0; f < 4; ++f) {
+            if (!fo[f]) continue;
+            float u, v, w;
+            const f3 p = closest_on_triangle(O, s[fi[f][0]].p, s[fi[f][1]].p, s[fi[f][2]].p, u, v, w);
+            const float d2 = len2(p);
+            if (d2 < best) {
+                best = d2;
+                int k = 0;
+                if (u > 0.0f) tri[k++] = s[fi[f][0]];
+                if (v > 0.0f) tri[k++] = s[fi[f][1]];
+                if (w > 0.0f) tri[k++] = s[fi[f][2]];
+                cnt = k;
+                closest = p;
+            }
+        }
+        for (int i = 0; i < 3; ++i)
+            if (i < cnt) s[i] = tri[i];
+        n = cnt;
+    }
+    return false;
+}
+
+struct Gjk {
+    bool intersect;
+    float distance;
+    f3 pa, pb, normal;
+    int n;
+    SP s[4];
+};
+
+// GJK::GetClosestPoints (GJK.cpp:155-244).  Uniform control flow: every lane
+// runs the same scalar code, the support queries are wave-cooperative.
+__device__ void gjk(const WShape& A, const WShape& B, float tol, Gjk& r) {
+    r.intersect = false;
+    r.distance = 0.0f;
+    r.pa = r.pb = r.normal = f3{0.0f, 0.0f, 0.0f};
+    SP* s = r.s;
+    int n = 0;
+    s[n++] = msupport(A, B, f3{1.0f, 0.0f, 0.0f});
+    f3 closest = s[0].p;
+    const float tol2 = tol * tol;
+    for (int it = 0; it < 32; ++it) {
+        if (len2(closest) <= tol2) { r.intersect = true; r.distance = 0.0f; break; }
+        const f3 d = neg(closest);
+        const SP w = msupport(A, B, d);
+        const float progress = len2(closest) - dot(closest, w.p);
+        if (progress <= tol * len2(closest)) break;
+        bool dup = false;
+        for (int i = 0; i < 4; ++i)
+            if (i < n && len2(sub(s[i].p, w.p)) < tol2) dup = true;
+        if (dup) break;
+        s[n++] = w;
+        if (reduce_simplex(s, n, closest)) { r.intersect = true; r.distance = 0.0f; break; }
+    }
+    r.n = n;
+    if (!r.intersect) {
+        const float dist = len(closest);
+        r.distance = dist;
+        if (dist < kEps) r.intersect = true;
+        else r.normal = divs(neg(closest), dist);
+        if (n == 2) {
+            const f3 a = s[0].p, b = s[1].p, ab = sub(b, a);
+            const float t = fclamp(-dot(a, ab) / fmax_(len2(ab), kEps), 0.0f, 1.0f);
+            r.pa = add(s[0].a, mul(sub(s[1].a, s[0].a), t));
+            r.pb = add(s[0].b, mul(sub(s[1].b, s[0].b), t));
+        } else if (n == 3) {
+            float u, v, w;
+            closest_on_triangle(f3{0.0f, 0.0f, 0.0f}, s[0].p, s[1].p, s[2].p, u, v, w);
+            r.pa = add(add(mul(s[0].a, u), mul(s[1].a, v)), mul(s[2].a, w));
+            r.pb = add(add(mul(s[0].b, u), mul(s[1].b, v)), mul(s[2].b, w));
+        } else {
+            r.pa = s[0].a;
+            r.pb = s[0].b;
+        }
+    }
+}
+
+// --------------------------- EPA (src/Collision/EPA.cpp) -------------------
+__device__ __forceinline__ f3 perp(f3 v) {
+    if (fabs_(v.x) <= fabs_(v.y) && fabs_(v.x) <= fabs_(v.z)) return f3{0.0f, -v.z, v.y};
+    if (fabs_(v.y) <= fabs_(v.z)) return f3{-v.z, 0.0f, v.x};
+    return f3{-v.y, v.x, 0.0f};
+}
+__device__ bool origin_in_tetra(const SP* v) {
+    const f3 O{0.0f, 0.0f, 0.0f};
+    auto ok = [&](f3 a, f3 b, f3 c, f3 d) {
+        f3 n = cross(sub(b, a), sub(c, a));
+        if (dot(n, sub(d, a)) > 0.0f) n = neg(n);
+        return dot(n, sub(O, a)) < -1.0e-9f;
+    };
+    return ok(v[0].p, v[1].p, v[2].p, v[3].p) && ok(v[0].p, v[2].p, v[3].p, v[1].p) &&
+           ok(v[0].p, v[3].p, v[1].p, v[2].p) && ok(v[1].p, v[3].p, v[2].p, v[0].p);
+}
+
+// BuildInitialTetra (EPA.cpp:58-147).
+__device__ bool initial_tetra(const WShape& A, const WShape& B, const Gjk& g, SP* out) {
+    const float tiny = 1.0e-12f;
+    int n = 0;
+    for (int i = 0; i < 4; ++i) {
+        if (i >= g.n || n >= 4) continue;
+        bool dup = false;
+        for (int j = 0; j < 4; ++j)
+            if (j < n && len2(sub(out[j].p, g.s[i].p)) < tiny) dup = true;
+        if (!dup) out[n++] = g.s[i];
+    }
+    if (n == 0) out[n++] = msupport(A, B, f3{1.0f, 0.0f, 0.0f});
+    if (n == 4 && !origin_in_tetra(out)) n = 3;
+    if (n == 1) {
+        out[1] = msupport(A, B, neg(out[0].p));
+        if (len2(sub(out[1].p, out[0].p)) > tiny) ++n;
+    }
+    if (n == 2) {
+        const f3 ab = sub(out[1].p, out[0].p);
+        if (len2(ab) > tiny) {
+            const f3 d1 = normalized_or(perp(ab), f3{0.0f, 0.0f, 0.0f});
+            const f3 d2 = normalized_or(cross(ab, d1), f3{0.0f, 0.0f, 0.0f});
+            const SP k0 = out[0], k1 = out[1];
+            out[0] = msupport(A, B, d1);
+            out[1] = msupport(A, B, neg(d1));
+            out[2] = msupport(A, B, d2);
+            out[3] = msupport(A, B, neg(d2));
+            if (origin_in_tetra(out)) return true;
+            out[0] = k0;
+            out[1] = k1;
+        }
+        const SP sp = msupport(A, B, perp(ab));
+        if (len2(sub(sp.p, out[0].p)) > tiny && len2(sub(sp.p, out[1].p)) > tiny) out[n++] = sp;
+    }
+    if (n == 3) {
+        f3 tn = cross(sub(out[1].p, out[0].p), sub(out[2].p, out[0].p));
+        const float l = len(tn);
+        if (l > tiny) {
+            tn = divs(tn, l);
+            const float sd = dot(tn, sub(f3{0.0f, 0.0f, 0.0f}, out[0].p));
+            const SP a = msupport(A, B, sd > 0.0f ? neg(tn) : tn);
+            const SP b = msupport(A, B, sd > 0.0f ? tn : neg(tn));
+            out[3] = a;
+            if (!origin_in_tetra(out)) out[3] = b;
+            if (origin_in_tetra(out)) return true;
+        }
+        n = 3;
+    }
+    if (n == 4 && origin_in_tetra(out)) return true;
+    // Exhaustive fallback (EPA.cpp:116-146).
+    SP cand[16];
+    int nc = 0;
+    auto addc = [&](const SP& sp) {
+        for (int i = 0; i < 16; ++i)
+            if (i < nc && len2(sub(cand[i].p, sp.p)) < tiny) return;
+        if (nc < 16) cand[nc++] = sp;
+    };
+    for (int i = 0; i < 4; ++i)
+        if (i < g.n) addc(g.s[i]);
+    const float s = 0.5773502692f;
+    const f3 dirs[14] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1},
+                         {s, s, s}, {s, s, -s}, {s, -s, s}, {-s, s, s},
+                         {-s, -s, s}, {-s, s, -s}, {s, -s, -s}, {-s, -s, -s}};
+    for (int i = 0; i < 14; ++i) addc(msupport(A, B, dirs[i]));
+    for (int i = 0; i + 3 < nc; ++i)
+        for (int j = i + 1; j + 2 < nc; ++j)
+            for (int k = j + 1; k + 1 < nc; ++k)
+                for (int l = k + 1; l < nc; ++l) {
+                    SP t[4] = {cand[i], cand[j], cand[k], cand[l]};
+                    const float vol = dot(cross(sub(t[1].p, t[0].p), sub(t[2].p, t[0].p)), sub(t[3].p, t[0].p));
+                    if (fabs_(vol) < tiny) continue;
+                    if (origin_in_tetra(t)) {
+                        for (int m = 0; m < 4; ++m) out[m] = t[m];
+                        return true;
+                    }
+                }
+    return false;
+}
+
+struct Epa {
+    bool valid;
+    float depth;
+    f3 normal, pa, pb;
+};
+
+__device__ __forceinline__ SP lds_vert(const WaveLds& L, int i) {
+    const float* v = L.vx + 9 * i;
+    return SP{f3{v[0], v[1], v[2]}, f3{v[3], v[4], v[5]}, f3{v[6], v[7], v[8]}};
+}
+
+// BuildFace (EPA.cpp:16-31) for vertex indices (a, b, c); may swap b and c.
+__device__ __forceinline__ bool build_face(const WaveLds& L, uint32_t& a, uint32_t& b, uint32_t& c, f3& n, float& d) {
+    const f3 A = lds_vert(L, int(a)).p, B = lds_vert(L, int(b)).p, C = lds_vert(L, int(c)).p;
+    n = cross(sub(B, A), sub(C, A));
+    const float l = len(n);
+    if (l < 1.0e-12f) return false;
+    n = divs(n, l);
+    d = dot(n, A);
+    if (d < 0.0f) {
+        n = neg(n);
+        d = -d;
+        uint32_t t = b;
+        b = c;
+        c = t;
+    }
+    return true;
+}
+
+// FinishFace (EPA.cpp:149-172).
+__device__ void finish_face(const WaveLds& L, int f, Epa& r) {
+    const f3 n{L.fnx[f], L.fny[f], L.fnz[f]};
+    const float dist = L.fd[f];
+    const uint32_t abc = L.fabc[f];
+    const SP va = lds_vert(L, int(abc & 0xFF)), vb = lds_vert(L, int((abc >> 8) & 0xFF)), vc = lds_vert(L, int(abc >> 16));
+    const f3 p = mul(n, dist);
+    const f3 v0 = sub(vb.p, va.p), v1 = sub(vc.p, va.p), v2 = sub(p, va.p);
+    const float d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1);
+    const float d20 = dot(v2, v0), d21 = dot(v2, v1);
+    const float den = d00 * d11 - d01 * d01;
+    float u, v, w;
+    if (fabs_(den) < 1.0e-18f) { u = 1; v = 0; w = 0; }
+    else {
+        v = (d11 * d20 - d01 * d21) / den;
+        w = (d00 * d21 - d01 * d20) / den;
+        u = 1.0f - v - w;
+    }
+    r.valid = true;
+    r.depth = dist;
+    r.normal = neg(n);
+    r.pa = add(add(mul(va.a, u), mul(vb.a, v)), mul(vc.a, w));
+    r.pb = add(add(mul(va.b, u), mul(vb.b, v)), mul(vc.b, w));
+}
+
+// Wave-parallel closest alive face: strict <, first wins.
+__device__ __forceinline__ int closest_face(const WaveLds& L, int nf) {
+    float best = kInf;
+    int bi = 0x7FFFFFFF;
+    for (int f = lane_id(); f < nf; f += 64) {
+        const float d = L.fd[f];
+        if (d < best) { best = d; bi = f; }
+    }
+    wave_argmin(best, bi);
+    return bi;
+}
+
+__device__ __forceinline__ void lds_write_vert(WaveLds& L, int i, const SP& s) {
+    if (lane_id() == 0) {
+        float* v = L.vx + 9 * i;
+        v[0] = s.a.x; v[1] = s.a.y; v[2] = s.a.z;
+        v[3] = s.b.x; v[4] = s.b.y; v[5] = s.b.z;
+        v[6] = s.p.x; v[7] = s.p.y; v[8] = s.p.z;
+    }
+}
+
+__device__ __forceinline__ uint32_t prefix_count(unsigned long long mask) {
+    // number of set bits below this lane
+    const int lane = lane_id();
+    const unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+    return uint32_t(__popcll(below));
+}
+
+// EPA::Compute (EPA.cpp:176-296).  All faces in [0, nf) are alive at the start
+// of every iteration (dead faces are compacted away), so the alive tests of
+// the reference reduce to index bounds.
+__device__ void epa(const WShape& A, const WShape& B, const Gjk& g, float tol, WaveLds& L, Epa& r) {
+    r.valid = false;
+    r.depth = 0.0f;
+    r.normal = f3{0.0f, 1.0f, 0.0f};
+    r.pa = r.pb = f3{0.0f, 0.0f, 0.0f};
+    SP tet[4];
+    if (!initial_tetra(A, B, g, tet)) return;
+    for (int i = 0; i < 4; ++i) lds_write_vert(L, i, tet[i]);
+    __syncthreads();
+    int nv = 4, nf = 0;
+    const uint32_t kT[4][3] = {{0, 1, 2}, {0, 2, 3}, {0, 3, 1}, {1, 3, 2}};
+    for (int t = 0; t < 4; ++t) {
+        uint32_t a = kT[t][0], b = kT[t][1], c = kT[t][2];
+        f3 n;
+        float d;
+        if (build_face(L, a, b, c, n, d)) {
+            if (lane_id() == 0) {
+                L.fnx[nf] = n.x; L.fny[nf] = n.y; L.fnz[nf] = n.z; L.fd[nf] = d;
+                L.fabc[nf] = a | (b << 8) | (c << 16);
+            }
+            ++nf;
+        }
+    }
+    __syncthreads();
+    if (nf < 4) return;
+    const int lane = lane_id();
+    for (int it = 0; it < 64; ++it) {
+        const int bf = closest_face(L, nf);
+        if (bf == 0x7FFFFFFF) break;
+        const f3 fn{L.fnx[bf], L.fny[bf], L.fnz[bf]};
+        const float fdist = L.fd[bf];
+        const SP w = msupport(A, B, fn);
+        const float d = dot(w.p, fn);
+        if (d - fdist < tol || nv >= 256) {
+            finish_face(L, bf, r);
+            return;
+        }
+        const int newi = nv;
+        lds_write_vert(L, nv, w);
+        ++nv;
+        // Visible faces, listed in index order.
+        int nvis = 0;
+        for (int base = 0; base < nf; base += 64) {
+            const int f = base + lane;
+            bool vis = false;
+            if (f < nf) vis = dot(f3{L.fnx[f], L.fny[f], L.fnz[f]}, w.p) - L.fd[f] > tol;
+            const unsigned long long m = __ballot(vis);
+            if (vis) L.vis[nvis + int(prefix_count(m))] = uint16_t(f);
+            nvis += __popcll(m);
+        }
+        __syncthreads();
+        // Horizon: edges (ab, bc, ca) of each visible face, in list order,
+        // that no other visible face shares reversed (EPA.cpp:253-270).
+        int nh = 0;
+        for (int base = 0; base < nvis; base += 64) {
+            const int e = base + lane;
+            uint32_t eu[3] = {0, 0, 0}, ev[3] = {0, 0, 0};
+            int keep = 0;
+            if (e < nvis) {
+                const int fi = L.vis[e];
+                const uint32_t abc = L.fabc[fi];
+                const uint32_t fa = abc & 0xFF, fb = (abc >> 8) & 0xFF, fc = abc >> 16;
+                const uint32_t es[3][2] = {{fa, fb}, {fb, fc}, {fc, fa}};
+                for (int k = 0; k < 3; ++k) {
+                    bool shared = false;
+                    for (int vj = 0; vj < nvis && !shared; ++vj) {
+                        const int fj = L.vis[vj];
+                        if (fj == fi) continue;
+                        const uint32_t g = L.fabc[fj];
+                        const uint32_t ga = g & 0xFF, gb = (g >> 8) & 0xFF, gc = g >> 16;
+                        if ((ga == es[k][1] && gb == es[k][0]) || (gb == es[k][1] && gc == es[k][0]) ||
+                            (gc == es[k][1] && ga == es[k][0]))
+                            shared = true;
+                    }
+                    if (!shared) { eu[keep] = es[k][0]; ev[keep] = es[k][1]; ++keep; }
+                }
+            }
+            // Exclusive prefix sum of `keep` over the lanes, in order.
+            int incl = keep;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                int o = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += o;
+            }
+            const int pos = nh + incl - keep;
+            for (int k = 0; k < 3; ++k)
+                if (k < keep && pos + k < kEpaMaxHorizon) L.hor[pos + k] = uint16_t(eu[k] | (ev[k] << 8));
+            nh += __shfl(incl, 63, 64);
+        }
+        if (nh > kEpaMaxHorizon) nh = kEpaMaxHorizon;
+        __syncthreads();
+        // Stable compaction of the surviving faces, chunk by chunk in order
+        // (destinations never exceed sources, so in-place is safe).
+        int nalive = 0;
+        for (int base = 0; base < nf; base += 64) {
+            const int f = base + lane;
+            bool alive = false;
+            float x = 0, y = 0, z = 0, dd = 0;
+            uint32_t abc = 0;
+            if (f < nf) {
+                alive = !(dot(f3{L.fnx[f], L.fny[f], L.fnz[f]}, w.p) - L.fd[f] > tol);
+                x = L.fnx[f]; y = L.fny[f]; z = L.fnz[f]; dd = L.fd[f]; abc = L.fabc[f];
+            }
+            const unsigned long long m = __ballot(alive);
+            __syncthreads();
+            if (alive) {
+                const int dst = nalive + int(prefix_count(m));
+                L.fnx[dst] = x; L.fny[dst] = y; L.fnz[dst] = z; L.fd[dst] = dd; L.fabc[dst] = abc;
+            }
+            nalive += __popcll(m);
+            __syncthreads();
+        }
+        nf = nalive;
+        // New faces (v, u, new) per horizon edge, in order, until 512 faces.
+        for (int base = 0; base < nh; base += 64) {
+            const int e = base + lane;
+            bool ok = false;
+            f3 n{0.0f, 0.0f, 0.0f};
+            float dd = 0.0f;
+            uint32_t a = 0, b = 0, c = 0;
+            if (e < nh) {
+                const uint32_t h = L.hor[e];
+                a = h >> 8;       // edge v
+                b = h & 0xFF;     // edge u
+                c = uint32_t(newi);
+                ok = build_face(L, a, b, c, n, dd);
+            }
+            const unsigned long long m = __ballot(ok);
+            const int dst = nf + int(prefix_count(m));
+            if (ok && dst < kEpaMaxFaces) {
+                L.fnx[dst] = n.x; L.fny[dst] = n.y; L.fnz[dst] = n.z; L.fd[dst] = dd;
+                L.fabc[dst] = a | (b << 8) | (c << 16);
+            }
+            nf += __popcll(m);
+            if (nf > kEpaMaxFaces) nf = kEpaMaxFaces;
+        }
+        __syncthreads();
+        if (nf >= kEpaMaxFaces && it < 63) {
+            // (the cap only stops appending; iteration continues as in the reference)
+        }
+    }
+    const int bf = closest_face(L, nf);
+    if (bf != 0x7FFFFFFF) finish_face(L, bf, r);
+}
+
+// ---------------------- Polyhedral clipping (CollideShape.cpp) -------------
+// GetShapeFace (CollideShape.cpp:30-98); the hull triangle scan is a wave
+// arg-max over precomputed local face normals.
+__device__ bool shape_face(const WShape& w, const ShapeBuf& sb, f3 dir, f3* out, int& cnt, f3& nw) {
+    if (w.s.type == kBox) {
+        const f3 ld = inv_rotate(w.q, dir);
+        const int ax = max_index(vabs(ld));
+        const float sg = comp(ld, ax) >= 0.0f ? 1.0f : -1.0f;
+        const f3 he = l3(w.s.prm);
+        f3 n{0.0f, 0.0f, 0.0f};
+        set_comp(n, ax, sg);
+        const f3 c{he.x * n.x, he.y * n.y, he.z * n.z};
+        const int a0 = (ax + 1) % 3, a1 = (ax + 2) % 3;
+        const float s0 = comp(he, a0), s1 = comp(he, a1);
+        f3 k[4] = {c, c, c, c};
+        set_comp(k[0], a0, s0); set_comp(k[0], a1, s1);
+        set_comp(k[1], a0, -s0); set_comp(k[1], a1, s1);
+        set_comp(k[2], a0, -s0); set_comp(k[2], a1, -s1);
+        set_comp(k[3], a0, s0); set_comp(k[3], a1, -s1);
+        for (int i = 0; i < 4; ++i) out[i] = xform(w.q, w.t, k[i]);
+        cnt = 4;
+        nw = rotate(w.q, n);
+        return true;
+    }
+    if (w.s.type == kHull) {
+        const f3 ld = inv_rotate(w.q, dir);
+        const int nt = int(w.s.hull.w);
+        float best = -kInf;
+        int bi = 0x7FFFFFFF;
+        for (int f = lane_id(); f < nt; f += 64) {
+            const float d = dot(l3(sb.hfn[w.s.hull.z + f]), ld);
+            if (d > best) { best = d; bi = f; }
+        }
+        wave_argmax(best, bi);
+        if (bi == 0x7FFFFFFF) return false;
+        const uint32_t abc = sb.htris[w.s.hull.z + bi];
+        const uint32_t vi[3] = {abc & 0xFF, (abc >> 8) & 0xFF, abc >> 16};
+        for (int i = 0; i < 3; ++i) out[i] = xform(w.q, w.t, l3(sb.hverts[w.s.hull.x + vi[i]]));
+        cnt = 3;
+        nw = rotate(w.q, l3(sb.hfn[w.s.hull.z + bi]));
+        return true;
+    }
+    return false;
+}
+
+// Sutherland-Hodgman against one plane, keeping d <= 0 (CollideShape.cpp:111-126).
+// Run by lane 0 on the LDS polygons.
+__device__ int clip_plane(const float (*in)[3], int nin, f3 pn, float pd, float (*out)[3]) {
+    int n = 0;
+    if (nin == 0) return 0;
+    for (int i = 0; i < nin; ++i) {
+        const int j = (i + 1) % nin;
+        const f3 cur{in[i][0], in[i][1], in[i][2]}, nxt{in[j][0], in[j][1], in[j][2]};
+        const float dc = dot(pn, cur) - pd, dn = dot(pn, nxt) - pd;
+        if (dc <= 0.0f && n < 2 * kMaxFaceVerts) { out[n][0] = cur.x; out[n][1] = cur.y; out[n][2] = cur.z; ++n; }
+        if ((dc <= 0.0f) != (dn <= 0.0f)) {
+            const float t = dc / (dc - dn);
+            const f3 p = add(cur, mul(sub(nxt, cur), t));
+            if (n < 2 * kMaxFaceVerts) { out[n][0] = p.x; out[n][1] = p.y; out[n][2] = p.z; ++n; }
+        }
+    }
+    return n;
+}
+
+// CollidePolyhedral (CollideShape.cpp:131-228).  Returns the point count;
+// points are written to `res`.
+__device__ int collide_polyhedral(const WShape& A, const WShape& B, const ShapeBuf& sb, f3 normal, float max_sep,
+                                  WaveLds& L, Contact* res) {
+    f3 fa[4], fb[4], na, nb;
+    int ca = 0, cb = 0;
+    const bool ha = shape_face(A, sb, neg(normal), fa, ca, na);
+    const bool hb = shape_face(B, sb, normal, fb, cb, nb);
+    if (!ha || !hb) return 0;
+    const float align_a = dot(na, neg(normal)), align_b = dot(nb, normal);
+    const bool ref_a = align_a >= align_b;
+    const f3* rv = ref_a ? fa : fb;
+    const int rc = ref_a ? ca : cb;
+    const f3* iv = ref_a ? fb : fa;
+    const int ic = ref_a ? cb : ca;
+    const f3 rn = ref_a ? na : nb;
+    const float rd = dot(rn, rv[0]);
+    int count = 0;
+    if (lane_id() == 0) {
+        int cur = 0, np = 0;
+        for (int i = 0; i < ic; ++i) { L.poly[0][np][0] = iv[i].x; L.poly[0][np][1] = iv[i].y; L.poly[0][np][2] = iv[i].z; ++np; }
+        f3 cen{0.0f, 0.0f, 0.0f};
+        for (int i = 0; i < rc; ++i) cen = add(cen, rv[i]);
+        cen = divs(cen, float(rc));
+        for (int i = 0; i < rc; ++i) {
+            const f3 v0 = rv[i], v1 = rv[(i + 1) % rc];
+            f3 sn = cross(sub(v1, v0), rn);
+            const float l = len(sn);
+            if (l < 1.0e-9f) continue;
+            sn = divs(sn, l);
+            if (dot(sn, sub(v0, cen)) < 0.0f) sn = neg(sn);
+            np = clip_plane(L.poly[cur], np, sn, dot(sn, v0), L.poly[cur ^ 1]);
+            cur ^= 1;
+            if (np == 0) break;
+        }
+        int k = 0;
+        for (int i = 0; i < np && k < 8; ++i) {
+            const f3 p{L.poly[cur][i][0], L.poly[cur][i][1], L.poly[cur][i][2]};
+            const float sep = dot(rn, p) - rd;
+            if (sep > max_sep) continue;
+            const f3 proj = sub(p, mul(rn, sep));
+            const f3 pa = ref_a ? proj : p, pb = ref_a ? p : proj;
+            float* c = L.cand[k];
+            c[0] = pa.x; c[1] = pa.y; c[2] = pa.z; c[3] = pb.x; c[4] = pb.y; c[5] = pb.z;
+            c[6] = normal.x; c[7] = normal.y; c[8] = normal.z; c[9] = sep; c[10] = __uint_as_float(uint32_t(i));
+            ++k;
+        }
+        count = k;
+    }
+    count = __shfl(count, 0, 64);
+    __syncthreads();
+    const int outn = count > 4 ? 4 : count;
+    for (int j = 0; j < outn; ++j) {
+        const int src = count > 4 ? j * (count - 1) / 3 : j;
+        const float* c = L.cand[src];
+        res[j].pa = f3{c[0], c[1], c[2]};
+        res[j].pb = f3{c[3], c[4], c[5]};
+        res[j].n = f3{c[6], c[7], c[8]};
+        res[j].sep = c[9];
+        res[j].feature = __float_as_uint(c[10]);
+    }
+    __syncthreads();
+    return outn;
+}
+
+__device__ __forceinline__ bool polyhedral(const WShape& w) { return w.s.type == kBox || w.s.type == kHull; }
+
+// CollideConvexGeneric (CollideShape.cpp:233-299).
+__device__ int collide_generic(const WShape& A, const WShape& B, const ShapeBuf& sb, float max_sep, WaveLds& L,
+                               Contact* res) {
+    const float tol = 1.0e-4f;
+    Gjk g;
+    gjk(A, B, tol, g);
+    if (!g.intersect) {
+        if (g.distance > max_sep) return 0;
+        if (polyhedral(A) && polyhedral(B) && len2(g.normal) > 0.0f) {
+            const int k = collide_polyhedral(A, B, sb, g.normal, max_sep, L, res);
+            if (k > 0) return k;
+        }
+        res[0].pa = g.pa; res[0].pb = g.pb; res[0].n = g.normal; res[0].sep = g.distance; res[0].feature = 0;
+        return 1;
+    }
+    if (polyhedral(A) && polyhedral(B)) {
+        Epa e;
+        epa(A, B, g, tol, L, e);
+        if (e.valid) {
+            const int k = collide_polyhedral(A, B, sb, e.normal, max_sep, L, res);
+            if (k > 0) return k;
+            res[0].pa = e.pa; res[0].pb = e.pb; res[0].n = e.normal; res[0].sep = -e.depth; res[0].feature = 0;
+            return 1;
+        }
+    }
+    Epa e;
+    epa(A, B, g, tol, L, e);
+    if (e.valid) {
+        res[0].pa = e.pa; res[0].pb = e.pb; res[0].n = e.normal; res[0].sep = -e.depth; res[0].feature = 0;
+        return 1;
+    }
+    res[0].pa = g.pa; res[0].pb = g.pb;
+    res[0].n = len2(g.normal) > 0.0f ? g.normal : f3{0.0f, 1.0f, 0.0f};
+    res[0].sep = 0.0f;
+    res[0].feature = 0;
+    return 1;
+}
+
+// CollideSphereConvex (CollideShape.cpp:304-344): GJK against a 1e-4 proxy
+// sphere at the sphere centre.
+__device__ int collide_sphere_convex(const WShape& Sph, const WShape& B, float max_sep, WaveLds& L, Contact* res) {
+    const float tol = 1.0e-4f;
+    WShape P = Sph;
+    P.s.type = kSphere;
+    P.s.prm.x = 1.0e-4f;
+    Gjk g;
+    gjk(P, B, tol, g);
+    const float r = Sph.s.prm.x;
+    const f3 c = Sph.t;
+    if (g.intersect) {
+        Epa e;
+        epa(P, B, g, tol, L, e);
+        if (e.valid) {
+            res[0].n = neg(e.normal);
+            res[0].pb = e.pb;
+            res[0].pa = add(c, mul(res[0].n, r));
+            res[0].sep = -(e.depth + r);
+            res[0].feature = 0;
+            return 1;
+        }
+        return 0;
+    }
+    const float dist = g.distance;
+    const float sep = dist - r;
+    if (sep > max_sep) return 0;
+    const f3 n = dist > kEps ? divs(sub(c, g.pb), dist) : f3{0.0f, 1.0f, 0.0f};
+    res[0].n = n;
+    res[0].pa = sub(c, mul(n, r));
+    res[0].pb = g.pb;
+    res[0].sep = sep;
+    res[0].feature = 0;
+    return 1;
+}
+
+__global__ void __launch_bounds__(64) k_collide_wave(const uint2* pairs, const uint32_t* work,
+                                                     const uint32_t* work_count, BodyBuf b, ShapeBuf sb,
+                                                     ManifoldBuf m, float max_sep) {
+    __shared__ WaveLds L;
+    const uint32_t total = *work_count;
+    for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
+        const uint32_t wv = work[item];
+        const uint32_t pi = wv & 0x7FFFFFFFu;
+        const bool swapped = (wv >> 31) != 0u;
+        uint32_t ia = pairs[pi].x, ib = pairs[pi].y;
+        if (swapped) { uint32_t t = ia; ia = ib; ib = t; }
+        WShape A, B;
+        load_wshape(A, sb, meta_shape(b.meta[ia]), b.x[ia], b.q[ia]);
+        load_wshape(B, sb, meta_shape(b.meta[ib]), b.x[ib], b.q[ib]);
+        Contact res[4];
+        int k;
+        bool swap_result = swapped;
+        if (A.s.type == kSphere) {
+            k = collide_sphere_convex(A, B, max_sep, L, res);
+        } else if (B.s.type == kSphere) {
+            k = collide_sphere_convex(B, A, max_sep, L, res);
+            if (k > 0) swap_result = !swap_result;
+        } else {
+            k = collide_generic(A, B, sb, max_sep, L, res);
+        }
+        const int lane = lane_id();
+        for (int j = 0; j < 4; ++j) {
+            if (j < k && lane == j) {
+                Contact c = res[j];
+                if (swap_result) swap_contact(c);
+                write_contact(m, pi, j, c);
+            }
+        }
+        if (lane == 0) m.count[pi] = uint32_t(k);
+        __syncthreads();
+    }
+}
+
+static inline uint32_t nblocks(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
+
+void launch_narrowphase(hipStream_t s, const uint2* pairs, uint32_t n, const BodyBuf& b, const ShapeBuf& sb,
+                        const ManifoldBuf& m, uint32_t* work, uint32_t* work_count, float max_sep, int world_mode,
+                        uint32_t wave_blocks) {
+    if (!n) return;
+    hipMemsetAsync(work_count, 0, sizeof(uint32_t), s);
+    k_collide_simple<<<nblocks(n, 256), 256, 0, s>>>(pairs, n, b, sb, m, work, work_count, max_sep, world_mode);
+    uint32_t grid = wave_blocks ? wave_blocks : (n < 65536u ? n : 65536u);
+    k_collide_wave<<<grid, 64, 0, s>>>(pairs, work, work_count, b, sb, m, max_sep);
+}
+
+}  // namespace kz

@@ -1,0 +1,510 @@
+// solver.hip - constraint build, velocity integration, Prepare, warm start,
+// projected Gauss-Seidel velocity and split-impulse bias sweeps, cache store
+// and position integration, hand-written for gfx950.
+//
+// Bit-exactness argument.  The reference solves each island sequentially:
+// warm start over the island's contacts in constraint order, then
+// `velocityIterations` passes over the first-fit SIMD batches, then
+// `positionIterations` bias passes in constraint order
+// (src/World/PhysicsWorld.cpp:666-743).  The per-point arithmetic reads and
+// writes only the two bodies of that point; static and kinematic bodies are
+// never written (src/Solver/ContactSolver.cpp:32-43,
+// ContactSolverSIMD.cpp:241-256).  Hence any schedule that applies, for every
+// DYNAMIC body, the point updates touching it in the reference's order gives
+// bit-identical results.  The host turns each sequence into levels
+// (level = 1 + max over the point's dynamic bodies of their last level); the
+// points of one level share no dynamic body and run in parallel, levels run
+// in order.  Islands are independent, so the islands of one group share
+// level numbers and one persistent workgroup sweeps a group.
+#include "kz_device.h"
+#include "kz_math.h"
+
+namespace kz {
+
+__device__ __forceinline__ f3 ld3(const float4& v) { return f3{v.x, v.y, v.z}; }
+__device__ __forceinline__ f4q ldq(const float4& v) { return f4q{v.x, v.y, v.z, v.w}; }
+__device__ __forceinline__ float4 st3(f3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+__device__ __forceinline__ m3 ldm(const float4* c) { return m3{ld3(c[0]), ld3(c[1]), ld3(c[2])}; }
+
+// ---------------------------------------------------------------------------
+// IntegrateVelocities (PhysicsWorld.cpp:757-783), one thread per body.
+// ---------------------------------------------------------------------------
+__global__ void k_integrate_velocities(BodyBuf b, uint32_t n, float dt, float gx, float gy, float gz) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || meta_motion(b.meta[i]) != 2u) return;
+    const float4 x = b.x[i], lim = b.lim[i], mat = b.mat[i];
+    f3 v = ld3(b.v[i]), w = ld3(b.w[i]);
+    const m3 iw = ldm(b.iw + 3 * i);
+    const f3 zero{0.0f, 0.0f, 0.0f};  // accumulated force / torque: none on this path
+    f3 acc = add(mul(f3{gx, gy, gz}, lim.x), mul(zero, x.w));
+    v = add(v, mul(acc, dt));
+    w = add(w, mul(mmul(iw, zero), dt));
+    v = mul(v, 1.0f / (1.0f + mat.z * dt));
+    w = mul(w, 1.0f / (1.0f + mat.w * dt));
+    auto clamp_len = [](f3 a, float mx) {
+        float l = len(a);
+        return (l > mx && l >= kEps) ? mul(a, mx / l) : a;
+    };
+    v = clamp_len(v, lim.y);
+    w = clamp_len(w, lim.z);
+    if (!(isfinite(v.x) && isfinite(v.y) && isfinite(v.z)) || !(isfinite(w.x) && isfinite(w.y) && isfinite(w.z))) {
+        v = zero;
+        w = zero;
+    }
+    b.v[i] = st3(v, 0.0f);
+    b.w[i] = st3(w, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// BuildContactConstraints (PhysicsWorld.cpp:454-535), one thread per
+// constraint.  The host has already compacted the non-empty manifolds in pair
+// order and ordered each constraint's bodies by packed id (info).
+// ---------------------------------------------------------------------------
+__global__ void k_build(ContactBuf c, ManifoldBuf m, BodyBuf b, CacheBuf cache, uint32_t nc, int warm) {
+    uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= nc) return;
+    const uint4 info = c.info[ci];
+    const uint32_t pair = c.pair[ci];
+    const uint32_t first = info.z, np = info.w & 0xFFu, flags = info.w >> 8;
+    const float4 ma = b.mat[info.x], mb = b.mat[info.y];
+    const float friction = sqrtf(ma.x * mb.x);
+    const float restitution = fmax_(ma.y, mb.y);
+    int slot = -1;
+    if (warm == 0) {
+        const unsigned long long key = pair_key(b.packed[info.x], b.packed[info.y]);
+        uint32_t mask = cache.cap - 1u, s = cache_hash(key) & mask;
+        for (uint32_t probe = 0; probe < cache.cap; ++probe) {
+            unsigned long long k = cache.key[s];
+            if (k == key) { slot = int(s); break; }
+            if (k == 0ull) break;
+            s = (s + 1u) & mask;
+        }
+    }
+    uint32_t ncached = slot >= 0 ? cache.count[slot] : 0u;
+    for (uint32_t k = 0; k < np; ++k) {
+        const uint32_t p = first + k;
+        const float4 pa = m.pa[pair * kPairPoints + k];
+        const float4 pb = m.pb[pair * kPairPoints + k];
+        const float4 nn = m.n[pair * kPairPoints + k];
+        float4 imp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (slot >= 0) {
+            // Nearest cached point within 0.1 m, strict <, first wins.
+            float best = 0.10f * 0.10f;
+            int bi = -1;
+            const f3 src = ld3(pa);
+            for (uint32_t j = 0; j < ncached; ++j) {
+                const float4 cp = cache.p[slot * kPairPoints + j];
+                float d = len2(sub(ld3(cp), src));
+                if (d < best) { best = d; bi = int(j); }
+            }
+            if (bi >= 0) {
+                const float4 cp = cache.p[slot * kPairPoints + bi];
+                const float2 ct = cache.t[slot * kPairPoints + bi];
+                imp = make_float4(cp.w, ct.x, ct.y, 0.0f);
+            }
+        }
+        c.body[p] = make_uint4(info.x, info.y, flags, ci);
+        c.pa[p] = pa;
+        c.pb[p] = pb;
+        c.n[p] = make_float4(nn.x, nn.y, nn.z, 0.0f);
+        c.la[p] = make_float4(0.0f, 0.0f, 0.0f, friction);
+        c.lb[p] = make_float4(0.0f, 0.0f, 0.0f, restitution);
+        c.imp[p] = imp;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PrepareConstraint (ContactSolver.cpp:56-112), one thread per point.
+// ---------------------------------------------------------------------------
+struct PrepParams {
+    float dt, restitution_threshold, baumgarte, slop, max_bias;
+};
+__device__ __forceinline__ float eff_mass(float ima, float imb, const m3& ia, const m3& ib, f3 ra, f3 rb, f3 d) {
+    f3 xa = cross(ra, d), xb = cross(rb, d);
+    float k = ima + imb;
+    k = k + dot(xa, mmul(ia, xa));
+    k = k + dot(xb, mmul(ib, xb));
+    return k > kEps ? 1.0f / k : 0.0f;
+}
+__global__ void k_prepare(ContactBuf c, BodyBuf b, uint32_t np, PrepParams pp) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const uint4 bd = c.body[p];
+    const float4 xa = b.x[bd.x], xb = b.x[bd.y];
+    const f4q qa = ldq(b.q[bd.x]), qb = ldq(b.q[bd.y]);
+    const m3 ia = ldm(b.iw + 3 * bd.x), ib = ldm(b.iw + 3 * bd.y);
+    const f3 va = ld3(b.v[bd.x]), wa = ld3(b.w[bd.x]), vb = ld3(b.v[bd.y]), wb = ld3(b.w[bd.y]);
+    const float4 pa4 = c.pa[p], pb4 = c.pb[p];
+    const f3 pa = ld3(pa4), pb = ld3(pb4);
+    const float sep = pa4.w;
+    const float cf = c.la[p].w, cr = c.lb[p].w;
+    float4 imp = c.imp[p];
+    const f3 posa = ld3(xa), posb = ld3(xb);
+    const f3 ra = sub(pa, posa), rb = sub(pb, posb);
+    const f3 la = inv_rotate(qa, ra), lb = inv_rotate(qb, rb);
+    const f3 n = normalized_or(ld3(c.n[p]), f3{0.0f, 1.0f, 0.0f});
+    const f3 t1 = perpendicular(n);
+    const f3 t2 = cross(n, t1);
+    const float nm = eff_mass(xa.w, xb.w, ia, ib, ra, rb, n);
+    const float tm1 = eff_mass(xa.w, xb.w, ia, ib, ra, rb, t1);
+    const float tm2 = eff_mass(xa.w, xb.w, ia, ib, ra, rb, t2);
+    const f3 pva = add(va, cross(wa, sub(pa, posa)));
+    const f3 pvb = add(vb, cross(wb, sub(pb, posb)));
+    const float nvel = dot(sub(pva, pvb), n);
+    float rbias = 0.0f;
+    if (nvel < -pp.restitution_threshold) rbias = -cr * nvel;
+    float pbias = 0.0f;
+    if (sep < -pp.slop) {
+        pbias = pp.baumgarte / pp.dt * (-sep - pp.slop);
+        pbias = fmin_(pbias, pp.max_bias);
+    }
+    const float mf = cf * imp.x;
+    imp.y = fclamp(imp.y, -mf, mf);
+    imp.z = fclamp(imp.z, -mf, mf);
+    imp.x = fmax_(imp.x, 0.0f);
+    imp.w = 0.0f;
+    uint4 bdo = bd;
+    const uint32_t ma = meta_motion(b.meta[bd.x]), mb = meta_motion(b.meta[bd.y]);
+    bdo.z = (bd.z & 1u) | (ma == 2u ? 2u : 0u) | (mb == 2u ? 4u : 0u);
+    c.body[p] = bdo;
+    c.n[p] = st3(n, nm);
+    c.t1[p] = st3(t1, tm1);
+    c.t2[p] = st3(t2, tm2);
+    c.ra[p] = st3(ra, rbias);
+    c.rb[p] = st3(rb, pbias);
+    c.la[p] = st3(la, cf);
+    c.lb[p] = st3(lb, cr);
+    c.imp[p] = imp;
+    c.nvel[p] = nvel;
+}
+
+// ---------------------------------------------------------------------------
+// Per-point sweep bodies.
+// ---------------------------------------------------------------------------
+// WarmStart (ContactSolver.cpp:117-132): stored rA / rB.
+__device__ __forceinline__ void warm_point(const ContactBuf& c, const BodyBuf& b, uint32_t p, float wsf) {
+    const uint4 bd = c.body[p];
+    const float4 imp = c.imp[p];
+    const f3 n = ld3(c.n[p]), t1 = ld3(c.t1[p]), t2 = ld3(c.t2[p]);
+    f3 j = add(add(mul(n, imp.x), mul(t1, imp.y)), mul(t2, imp.z));
+    j = mul(j, wsf);
+    if (bd.z & 2u) {
+        const float ima = b.x[bd.x].w;
+        const m3 ia = ldm(b.iw + 3 * bd.x);
+        b.v[bd.x] = st3(add(ld3(b.v[bd.x]), mul(j, ima)), 0.0f);
+        b.w[bd.x] = st3(add(ld3(b.w[bd.x]), mmul(ia, cross(ld3(c.ra[p]), j))), 0.0f);
+    }
+    if (bd.z & 4u) {
+        const float imb = b.x[bd.y].w;
+        const m3 ib = ldm(b.iw + 3 * bd.y);
+        b.v[bd.y] = st3(sub(ld3(b.v[bd.y]), mul(j, imb)), 0.0f);
+        b.w[bd.y] = st3(sub(ld3(b.w[bd.y]), mmul(ib, cross(ld3(c.rb[p]), j))), 0.0f);
+    }
+}
+
+// One lane of SolveVelocityBatchSIMD (ContactSolverSIMD.cpp:93-256).
+// _mm_max_ps(a, b) = a > b ? a : b and _mm_min_ps(a, b) = a < b ? a : b.
+__device__ __forceinline__ float velocity_point(const ContactBuf& c, const BodyBuf& b, uint32_t p, bool vbias) {
+    const uint4 bd = c.body[p];
+    const float4 xa = b.x[bd.x], xb = b.x[bd.y];
+    const f3 ra = rotate(ldq(b.q[bd.x]), ld3(c.la[p]));
+    const f3 rb = rotate(ldq(b.q[bd.y]), ld3(c.lb[p]));
+    const m3 ia = ldm(b.iw + 3 * bd.x), ib = ldm(b.iw + 3 * bd.y);
+    f3 va = ld3(b.v[bd.x]), wa = ld3(b.w[bd.x]), vb = ld3(b.v[bd.y]), wb = ld3(b.w[bd.y]);
+    const float4 n4 = c.n[p], t14 = c.t1[p], t24 = c.t2[p];
+    float4 imp = c.imp[p];
+    const float cf = c.la[p].w;
+    const float nbias = c.ra[p].w + (vbias ? c.rb[p].w : 0.0f);
+    const float zero = 0.0f;
+    float dsq = 0.0f;
+    auto rel = [&]() {
+        f3 xa2 = cross(wa, ra), xb2 = cross(wb, rb);
+        return f3{(va.x + xa2.x) - (vb.x + xb2.x), (va.y + xa2.y) - (vb.y + xb2.y), (va.z + xa2.z) - (vb.z + xb2.z)};
+    };
+    auto apply = [&](f3 j) {
+        va = f3{va.x + j.x * xa.w, va.y + j.y * xa.w, va.z + j.z * xa.w};
+        wa = add(wa, mmul(ia, cross(ra, j)));
+        vb = f3{vb.x - j.x * xb.w, vb.y - j.y * xb.w, vb.z - j.z * xb.w};
+        wb = sub(wb, mmul(ib, cross(rb, j)));
+    };
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const f3 tv = t == 0 ? ld3(t14) : ld3(t24);
+        const float mass = t == 0 ? t14.w : t24.w;
+        const float acc = t == 0 ? imp.y : imp.z;
+        const float vt = dot(rel(), tv);
+        const float lam = (zero - mass) * vt;
+        const float mf = cf * imp.x;
+        float na = acc + lam;
+        const float lo = zero - mf;
+        na = na > lo ? na : lo;
+        na = na < mf ? na : mf;
+        const float dl = na - acc;
+        if (t == 0) imp.y = na; else imp.z = na;
+        dsq += dl * dl;
+        apply(mul(tv, dl));
+    }
+    {
+        const f3 nv = ld3(n4);
+        const float vn = dot(rel(), nv);
+        const float lam = (zero - n4.w) * (vn - nbias);
+        float na = imp.x + lam;
+        na = na > zero ? na : zero;
+        const float dl = na - imp.x;
+        imp.x = na;
+        dsq += dl * dl;
+        apply(mul(nv, dl));
+    }
+    if (bd.z & 2u) { b.v[bd.x] = st3(va, 0.0f); b.w[bd.x] = st3(wa, 0.0f); }
+    if (bd.z & 4u) { b.v[bd.y] = st3(vb, 0.0f); b.w[bd.y] = st3(wb, 0.0f); }
+    c.imp[p] = imp;
+    return dsq;
+}
+
+// SolveContactVelocity, the scalar path (ContactSolver.cpp:137-181).
+__device__ __forceinline__ float scalar_point(const ContactBuf& c, const BodyBuf& b, uint32_t p, bool vbias) {
+    const uint4 bd = c.body[p];
+    const float4 xa = b.x[bd.x], xb = b.x[bd.y];
+    const f3 ra = rotate(ldq(b.q[bd.x]), ld3(c.la[p]));
+    const f3 rb = rotate(ldq(b.q[bd.y]), ld3(c.lb[p]));
+    const m3 ia = ldm(b.iw + 3 * bd.x), ib = ldm(b.iw + 3 * bd.y);
+    f3 va = ld3(b.v[bd.x]), wa = ld3(b.w[bd.x]), vb = ld3(b.v[bd.y]), wb = ld3(b.w[bd.y]);
+    const float4 n4 = c.n[p], t14 = c.t1[p], t24 = c.t2[p];
+    float4 imp = c.imp[p];
+    const float cf = c.la[p].w;
+    const bool da = bd.z & 2u, db = bd.z & 4u;
+    float dsq = 0.0f;
+    auto apply = [&](f3 j) {
+        if (da) { va = add(va, mul(j, xa.w)); wa = add(wa, mmul(ia, cross(ra, j))); }
+        if (db) { vb = sub(vb, mul(j, xb.w)); wb = sub(wb, mmul(ib, cross(rb, j))); }
+    };
+    f3 vr = sub(add(va, cross(wa, ra)), add(vb, cross(wb, rb)));
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const f3 tv = t == 0 ? ld3(t14) : ld3(t24);
+        const float mass = t == 0 ? t14.w : t24.w;
+        float acc = t == 0 ? imp.y : imp.z;
+        float lam = -mass * dot(vr, tv);
+        const float mf = cf * imp.x;
+        const float ni = fclamp(acc + lam, -mf, mf);
+        lam = ni - acc;
+        if (t == 0) imp.y = ni; else imp.z = ni;
+        dsq += lam * lam;
+        apply(mul(tv, lam));
+    }
+    vr = sub(add(va, cross(wa, ra)), add(vb, cross(wb, rb)));
+    float bias = c.ra[p].w;
+    if (vbias) bias = bias + c.rb[p].w;
+    float lam = -n4.w * (dot(vr, ld3(n4)) - bias);
+    const float ni = fmax_(imp.x + lam, 0.0f);
+    lam = ni - imp.x;
+    imp.x = ni;
+    dsq += lam * lam;
+    apply(mul(ld3(n4), lam));
+    if (da) { b.v[bd.x] = st3(va, 0.0f); b.w[bd.x] = st3(wa, 0.0f); }
+    if (db) { b.v[bd.y] = st3(vb, 0.0f); b.w[bd.y] = st3(wb, 0.0f); }
+    c.imp[p] = imp;
+    return dsq;
+}
+
+// SolveBiasConstraint (ContactSolver.cpp:257-290).
+__device__ __forceinline__ void bias_point(const ContactBuf& c, const BodyBuf& b, uint32_t p) {
+    const uint4 bd = c.body[p];
+    const float4 xa = b.x[bd.x], xb = b.x[bd.y];
+    if (xa.w <= 0.0f && xb.w <= 0.0f) return;
+    const float pbias = c.rb[p].w;
+    if (pbias <= 0.0f) return;
+    const f3 ra = rotate(ldq(b.q[bd.x]), ld3(c.la[p]));
+    const f3 rb = rotate(ldq(b.q[bd.y]), ld3(c.lb[p]));
+    const f3 pva = ld3(b.pv[bd.x]), pwa = ld3(b.pw[bd.x]), pvb = ld3(b.pv[bd.y]), pwb = ld3(b.pw[bd.y]);
+    const f3 vr = sub(add(pva, cross(pwa, ra)), add(pvb, cross(pwb, rb)));
+    const float4 n4 = c.n[p];
+    float4 imp = c.imp[p];
+    float lam = -n4.w * (dot(vr, ld3(n4)) - pbias);
+    const float ni = fmax_(imp.w + lam, 0.0f);
+    lam = ni - imp.w;
+    imp.w = ni;
+    c.imp[p] = imp;
+    const f3 j = mul(ld3(n4), lam);
+    if (bd.z & 2u) {
+        const m3 ia = ldm(b.iw + 3 * bd.x);
+        b.pv[bd.x] = st3(add(pva, mul(j, xa.w)), 0.0f);
+        b.pw[bd.x] = st3(add(pwa, mmul(ia, cross(ra, j))), 0.0f);
+    }
+    if (bd.z & 4u) {
+        const m3 ib = ldm(b.iw + 3 * bd.y);
+        b.pv[bd.y] = st3(sub(pvb, mul(j, xb.w)), 0.0f);
+        b.pw[bd.y] = st3(sub(pwb, mmul(ib, cross(rb, j))), 0.0f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The persistent group sweep: one workgroup per group of islands runs the
+// warm start, every velocity iteration and every bias iteration level by
+// level.  The points of a level share no dynamic body, so the threads of
+// the workgroup process a level in parallel; __syncthreads() orders levels
+// (the waves of a workgroup share the CU's L1, so no cache maintenance is
+// needed between levels).  `resid` (optional) receives the sum of squared
+// impulse increments of each velocity sweep.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_solve_groups(ContactBuf c, BodyBuf b, Schedule ws, Schedule vel,
+                                                       SolveParams sp, float* resid) {
+    const uint2 wr = ws.grp[blockIdx.x];
+    const uint2 vr = vel.grp[blockIdx.x];
+    for (uint32_t l = wr.x; l < wr.y; ++l) {
+        for (uint32_t e = ws.lvl_off[l] + threadIdx.x; e < ws.lvl_off[l + 1]; e += blockDim.x)
+            warm_point(c, b, ws.idx[e], sp.warm_start_factor);
+        __syncthreads();
+    }
+    const bool vbias = sp.use_velocity_bias != 0;
+    const Schedule& vs = sp.use_simd ? vel : ws;
+    const uint2 vrange = sp.use_simd ? vr : wr;
+    for (uint32_t it = 0; it < sp.velocity_iterations; ++it) {
+        float dsq = 0.0f;
+        for (uint32_t l = vrange.x; l < vrange.y; ++l) {
+            for (uint32_t e = vs.lvl_off[l] + threadIdx.x; e < vs.lvl_off[l + 1]; e += blockDim.x) {
+                // Sensor points are absent from the SIMD schedule (the batch
+                // builder skips them) and present in the scalar one.
+                const uint32_t p = vs.idx[e];
+                dsq += sp.use_simd ? velocity_point(c, b, p, vbias) : scalar_point(c, b, p, vbias);
+            }
+            __syncthreads();
+        }
+        if (resid && dsq != 0.0f) atomicAdd(resid + it, dsq);
+    }
+    if (!vbias) {
+        for (uint32_t it = 0; it < sp.position_iterations; ++it) {
+            for (uint32_t l = wr.x; l < wr.y; ++l) {
+                for (uint32_t e = ws.lvl_off[l] + threadIdx.x; e < ws.lvl_off[l + 1]; e += blockDim.x)
+                    bias_point(c, b, ws.idx[e]);
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cache store (PhysicsWorld.cpp:568-581), one thread per constraint.  The
+// host guarantees free capacity (load factor <= 1/2), so probing ends.
+// ---------------------------------------------------------------------------
+__global__ void k_cache_store(ContactBuf c, BodyBuf b, CacheBuf cache, uint32_t nc) {
+    uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= nc) return;
+    const uint4 info = c.info[ci];
+    const unsigned long long key = pair_key(b.packed[info.x], b.packed[info.y]);
+    uint32_t mask = cache.cap - 1u, s = cache_hash(key) & mask;
+    for (uint32_t probe = 0; probe < cache.cap; ++probe) {
+        unsigned long long prev = atomicCAS(cache.key + s, 0ull, key);
+        if (prev == 0ull || prev == key) break;
+        s = (s + 1u) & mask;
+    }
+    const uint32_t np = info.w & 0xFFu;
+    cache.count[s] = np;
+    for (uint32_t k = 0; k < np; ++k) {
+        const uint32_t p = info.z + k;
+        const float4 imp = c.imp[p];
+        const float4 pa = c.pa[p];
+        cache.p[s * kPairPoints + k] = make_float4(pa.x, pa.y, pa.z, imp.x);
+        cache.t[s * kPairPoints + k] = make_float2(imp.y, imp.z);
+    }
+}
+
+__global__ void k_cache_rehash(CacheBuf src, CacheBuf dst) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= src.cap) return;
+    const unsigned long long key = src.key[i];
+    if (key == 0ull) return;
+    uint32_t mask = dst.cap - 1u, s = cache_hash(key) & mask;
+    for (uint32_t probe = 0; probe < dst.cap; ++probe) {
+        if (atomicCAS(dst.key + s, 0ull, key) == 0ull) break;
+        s = (s + 1u) & mask;
+    }
+    dst.count[s] = src.count[i];
+    for (int k = 0; k < kPairPoints; ++k) {
+        dst.p[s * kPairPoints + k] = src.p[i * kPairPoints + k];
+        dst.t[s * kPairPoints + k] = src.t[i * kPairPoints + k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// IntegratePositions (PhysicsWorld.cpp:785-802), pseudo-velocity integration
+// (:275-289) and UpdateBodyInertias (:854-861), one thread per body.
+// ---------------------------------------------------------------------------
+__global__ void k_finish(BodyBuf b, uint32_t n, float dt, int split_impulse) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t mo = meta_motion(b.meta[i]);
+    if (mo == 0u) return;
+    float4 x = b.x[i];
+    f3 t = ld3(x);
+    f4q q = ldq(b.q[i]);
+    const f3 v = ld3(b.v[i]), w = ld3(b.w[i]);
+    t = add(t, mul(v, dt));
+    if (len2(w) > 0.0f) q = integrate_rotation(q, w, dt);
+    if (split_impulse) {
+        const f3 pv = ld3(b.pv[i]), pw = ld3(b.pw[i]);
+        if (len2(pv) > 0.0f) t = add(t, mul(pv, dt));
+        if (len2(pw) > 0.0f) q = integrate_rotation(q, pw, dt);
+        b.pv[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        b.pw[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    b.x[i] = make_float4(t.x, t.y, t.z, x.w);
+    b.q[i] = make_float4(q.x, q.y, q.z, q.w);
+    if (mo == 2u) {
+        const m3 iw = world_inertia(q, ldm(b.il + 3 * i));
+        b.iw[3 * i + 0] = st3(iw.c0, 0.0f);
+        b.iw[3 * i + 1] = st3(iw.c1, 0.0f);
+        b.iw[3 * i + 2] = st3(iw.c2, 0.0f);
+    }
+}
+
+// World inertia from the local one, for uploads of new body state.
+__global__ void k_world_inertia(BodyBuf b, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (meta_motion(b.meta[i]) != 2u) {
+        for (int k = 0; k < 3; ++k) b.iw[3 * i + k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const m3 iw = world_inertia(ldq(b.q[i]), ldm(b.il + 3 * i));
+    b.iw[3 * i + 0] = st3(iw.c0, 0.0f);
+    b.iw[3 * i + 1] = st3(iw.c1, 0.0f);
+    b.iw[3 * i + 2] = st3(iw.c2, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+// ---------------------------------------------------------------------------
+static inline uint32_t blocks(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
+
+void launch_integrate_velocities(hipStream_t s, const BodyBuf& b, uint32_t n, float dt, const float* g) {
+    if (n) k_integrate_velocities<<<blocks(n, 256), 256, 0, s>>>(b, n, dt, g[0], g[1], g[2]);
+}
+void launch_build(hipStream_t s, const ContactBuf& c, const ManifoldBuf& m, const BodyBuf& b, const CacheBuf& cache,
+                  uint32_t nc, int warm) {
+    if (nc) k_build<<<blocks(nc, 256), 256, 0, s>>>(c, m, b, cache, nc, warm);
+}
+void launch_prepare(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32_t np, float dt, float rthr,
+                    float baum, float slop, float maxb) {
+    PrepParams pp{dt, rthr, baum, slop, maxb};
+    if (np) k_prepare<<<blocks(np, 256), 256, 0, s>>>(c, b, np, pp);
+}
+void launch_solve_groups(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const Schedule& ws,
+                         const Schedule& vel, const SolveParams& sp, uint32_t groups, uint32_t threads,
+                         float* resid) {
+    if (groups) k_solve_groups<<<groups, threads, 0, s>>>(c, b, ws, vel, sp, resid);
+}
+void launch_cache_store(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const CacheBuf& cache, uint32_t nc) {
+    if (nc) k_cache_store<<<blocks(nc, 256), 256, 0, s>>>(c, b, cache, nc);
+}
+void launch_cache_rehash(hipStream_t s, const CacheBuf& src, const CacheBuf& dst) {
+    if (src.cap) k_cache_rehash<<<blocks(src.cap, 256), 256, 0, s>>>(src, dst);
+}
+void launch_finish(hipStream_t s, const BodyBuf& b, uint32_t n, float dt, int split) {
+    if (n) k_finish<<<blocks(n, 256), 256, 0, s>>>(b, n, dt, split);
+}
+void launch_world_inertia(hipStream_t s, const BodyBuf& b, uint32_t n) {
+    if (n) k_world_inertia<<<blocks(n, 256), 256, 0, s>>>(b, n);
+}
+
+}  // namespace kz

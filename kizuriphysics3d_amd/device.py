@@ -1,0 +1,234 @@
+"""ctypes binding of the kzg_* C-ABI (include/kzg.h) in libkizuri_amd.so.
+
+There is no CPU fallback: loading fails loudly when the library is missing and
+``DeviceContext`` raises when no GPU is usable.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import scenes as S
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkizuri_amd.so")
+POINT_FLOATS = 44
+
+KZG_BODY_DTYPE = np.dtype([
+    ("packed_id", "<u4"), ("shape", "<u4"), ("motion_type", "<u4"), ("is_sensor", "<u4"),
+    ("position", "<f4", (3,)), ("rotation", "<f4", (4,)), ("linear_velocity", "<f4", (3,)),
+    ("angular_velocity", "<f4", (3,)), ("inverse_mass", "<f4"), ("inverse_inertia_local", "<f4", (9,)),
+    ("friction", "<f4"), ("restitution", "<f4"), ("linear_damping", "<f4"), ("angular_damping", "<f4"),
+    ("gravity_factor", "<f4"), ("max_linear_velocity", "<f4"), ("max_angular_velocity", "<f4"),
+])
+HULL_RANGE_DTYPE = np.dtype([("vert_offset", "<u4"), ("vert_count", "<u4"), ("tri_offset", "<u4"), ("tri_count", "<u4")])
+TIMINGS_DTYPE = np.dtype([
+    ("narrowphase_ms", "<f4"), ("build_ms", "<f4"), ("integrate_velocities_ms", "<f4"), ("solve_ms", "<f4"),
+    ("integrate_positions_ms", "<f4"), ("solve_kernel_ms", "<f4"), ("velocity_levels", "<u4"),
+    ("bias_levels", "<u4"), ("groups", "<u4"), ("islands", "<u4"),
+])
+assert KZG_BODY_DTYPE.itemsize == 136
+
+EXPORTS = [
+    "kzg_create", "kzg_destroy", "kzg_last_error", "kzg_upload_shapes", "kzg_upload_bodies", "kzg_upload_states",
+    "kzg_narrowphase", "kzg_build_constraints", "kzg_integrate_velocities", "kzg_solve", "kzg_integrate_positions",
+    "kzg_step", "kzg_query_sizes", "kzg_download_states", "kzg_download_manifolds", "kzg_download_constraints",
+    "kzg_get_residuals", "kzg_get_timings", "kzg_collide_batch",
+]
+
+_lib = None
+
+
+class KzgError(RuntimeError):
+    pass
+
+
+def library():
+    """Load libkizuri_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is missing: run __graft_entry__.build() or "
+                                    "python -m kizuriphysics3d_amd.build")
+        L = C.CDLL(LIB_PATH)
+        vp, u32, f32, i32 = C.c_void_p, C.c_uint32, C.c_float, C.c_int
+        sig = {
+            "kzg_create": ([vp, vp, i32], i32), "kzg_destroy": ([vp], None),
+            "kzg_last_error": ([vp], C.c_char_p),
+            "kzg_upload_shapes": ([vp, vp, u32, vp, vp, u32, vp, u32], i32),
+            "kzg_upload_bodies": ([vp, vp, u32], i32), "kzg_upload_states": ([vp, vp, u32], i32),
+            "kzg_narrowphase": ([vp, vp, u32], i32), "kzg_build_constraints": ([vp, i32], i32),
+            "kzg_integrate_velocities": ([vp, f32], i32), "kzg_solve": ([vp, f32], i32),
+            "kzg_integrate_positions": ([vp, f32], i32), "kzg_step": ([vp, vp, u32, f32], i32),
+            "kzg_query_sizes": ([vp, vp, vp, vp], i32), "kzg_download_states": ([vp, vp, u32], i32),
+            "kzg_download_manifolds": ([vp, vp, vp], i32), "kzg_download_constraints": ([vp, vp, vp], i32),
+            "kzg_get_residuals": ([vp, vp, u32], i32), "kzg_get_timings": ([vp, vp], i32),
+            "kzg_collide_batch": ([vp, vp, vp, vp, vp, u32, f32, vp, vp, vp], i32),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def pack_hulls(shapes: np.ndarray, hulls: dict):
+    """Built hulls {shape index: (verts (V,3), tris (T,3))} -> ranges + pools."""
+    ranges = np.zeros(len(shapes), HULL_RANGE_DTYPE)
+    vs, ts = [np.zeros((0, 3), np.float32)], [np.zeros((0, 3), np.uint32)]
+    nv = nt = 0
+    for i, (v, t) in sorted(hulls.items()):
+        ranges[i] = (nv, len(v), nt, len(t))
+        vs.append(np.asarray(v, np.float32))
+        ts.append(np.asarray(t, np.uint32))
+        nv += len(v)
+        nt += len(t)
+    verts = np.ascontiguousarray(np.concatenate(vs), np.float32) if nv else np.zeros((1, 3), np.float32)
+    tris = np.ascontiguousarray(np.concatenate(ts), np.uint32) if nt else np.zeros((1, 3), np.uint32)
+    return ranges, verts, tris, nv, nt
+
+
+def device_bodies(bodies: np.ndarray, mass: np.ndarray) -> np.ndarray:
+    """kz_body_desc table + mass data (inverse mass, inverse local inertia) -> kzg_body table."""
+    out = np.zeros(len(bodies), KZG_BODY_DTYPE)
+    out["packed_id"] = np.arange(len(bodies), dtype=np.uint32)
+    for f in ("shape", "motion_type", "is_sensor", "position", "rotation", "linear_velocity", "angular_velocity",
+              "friction", "restitution", "linear_damping", "angular_damping", "gravity_factor",
+              "max_linear_velocity", "max_angular_velocity"):
+        out[f] = bodies[f]
+    out["inverse_mass"] = mass[:, 0]
+    out["inverse_inertia_local"] = mass[:, 1:10]
+    return out
+
+
+class DeviceContext:
+    """A kzg_ctx: the device contact path for one world."""
+
+    def __init__(self, world: np.ndarray, device: int = 0):
+        L = library()
+        self._world = np.array(world, dtype=S.WORLD_DTYPE)
+        h = C.c_void_p()
+        rc = L.kzg_create(C.byref(h), _p(self._world), device)
+        if rc != 0:
+            raise KzgError(f"kzg_create failed ({rc}): no usable GPU")
+        self.h = h
+        self.num_bodies = 0
+        self.num_pairs = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            library().kzg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str):
+        if rc < 0:
+            msg = library().kzg_last_error(self.h)
+            raise KzgError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+        return rc
+
+    def upload_shapes(self, shapes: np.ndarray, hulls: dict):
+        shapes = np.ascontiguousarray(shapes)
+        ranges, verts, tris, nv, nt = pack_hulls(shapes, hulls)
+        self._check(library().kzg_upload_shapes(self.h, _p(shapes), len(shapes), _p(ranges), _p(verts), nv,
+                                                _p(tris), nt), "kzg_upload_shapes")
+
+    def upload_bodies(self, bodies: np.ndarray):
+        bodies = np.ascontiguousarray(bodies, dtype=KZG_BODY_DTYPE)
+        self._check(library().kzg_upload_bodies(self.h, _p(bodies), len(bodies)), "kzg_upload_bodies")
+        self.num_bodies = len(bodies)
+
+    def upload_states(self, st: np.ndarray):
+        st = np.ascontiguousarray(st, dtype=S.STATE_DTYPE)
+        self._check(library().kzg_upload_states(self.h, _p(st), len(st)), "kzg_upload_states")
+
+    def narrowphase(self, pairs: np.ndarray):
+        pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+        self.num_pairs = len(pairs)
+        self._check(library().kzg_narrowphase(self.h, _p(pairs), len(pairs)), "kzg_narrowphase")
+
+    def build_constraints(self, warm_start_mode: int = 0):
+        self._check(library().kzg_build_constraints(self.h, warm_start_mode), "kzg_build_constraints")
+
+    def integrate_velocities(self, dt=S.DT):
+        self._check(library().kzg_integrate_velocities(self.h, float(dt)), "kzg_integrate_velocities")
+
+    def solve(self, dt=S.DT):
+        self._check(library().kzg_solve(self.h, float(dt)), "kzg_solve")
+
+    def integrate_positions(self, dt=S.DT):
+        self._check(library().kzg_integrate_positions(self.h, float(dt)), "kzg_integrate_positions")
+
+    def step(self, pairs: np.ndarray, dt=S.DT, stop_after: int = 5, warm_start_mode: int = 0):
+        """StepFixed's contact path over `pairs`; stop_after as in the oracle."""
+        self.narrowphase(pairs)
+        if stop_after >= 2:
+            self.build_constraints(warm_start_mode)
+        if stop_after >= 3:
+            self.integrate_velocities(dt)
+        if stop_after >= 4:
+            self.solve(dt)
+        if stop_after >= 5:
+            self.integrate_positions(dt)
+
+    def sizes(self):
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._check(library().kzg_query_sizes(self.h, C.byref(a), C.byref(b), C.byref(c)), "kzg_query_sizes")
+        return a.value, b.value, c.value
+
+    def states(self) -> np.ndarray:
+        out = np.zeros(self.num_bodies, S.STATE_DTYPE)
+        self._check(library().kzg_download_states(self.h, _p(out), self.num_bodies), "kzg_download_states")
+        return out
+
+    def manifolds(self):
+        n = self.num_pairs
+        counts = np.zeros(max(n, 1), np.uint32)
+        pts = np.zeros((max(n, 1), 4), S.CONTACT_DTYPE)
+        if n:
+            self._check(library().kzg_download_manifolds(self.h, _p(counts), _p(pts)), "kzg_download_manifolds")
+        return counts[:n], pts[:n]
+
+    def constraints(self):
+        _, nc, _ = self.sizes()
+        hdr = np.zeros((max(nc, 1), 4), np.uint32)
+        pts = np.zeros((max(nc, 1), 8, POINT_FLOATS), np.float32)
+        if nc:
+            self._check(library().kzg_download_constraints(self.h, _p(hdr), _p(pts)), "kzg_download_constraints")
+        return hdr[:nc], pts[:nc]
+
+    def residuals(self) -> np.ndarray:
+        out = np.zeros(64, np.float64)
+        k = self._check(library().kzg_get_residuals(self.h, _p(out), 64), "kzg_get_residuals")
+        return out[:k].copy()
+
+    def timings(self) -> dict:
+        t = np.zeros((), TIMINGS_DTYPE)
+        self._check(library().kzg_get_timings(self.h, _p(t)), "kzg_get_timings")
+        return {k: t[k].item() for k in TIMINGS_DTYPE.names}
+
+    def collide_batch(self, shape_a, shape_b, xf_a, xf_b, max_separation=0.02):
+        sa = np.ascontiguousarray(shape_a, np.uint32)
+        sb = np.ascontiguousarray(shape_b, np.uint32)
+        xa = np.ascontiguousarray(xf_a, np.float32)
+        xb = np.ascontiguousarray(xf_b, np.float32)
+        n = len(sa)
+        counts = np.zeros(max(n, 1), np.uint32)
+        pts = np.zeros((max(n, 1), 4), S.CONTACT_DTYPE)
+        ms = C.c_float()
+        self.num_pairs = n
+        self._check(library().kzg_collide_batch(self.h, _p(sa), _p(sb), _p(xa), _p(xb), n, float(max_separation),
+                                                _p(counts), _p(pts), C.byref(ms)), "kzg_collide_batch")
+        return counts[:n], pts[:n], ms.value

@@ -1247,9 +1247,9 @@ template <class R> struct World {
         } else {
             for (u32 it = 0; it < s.velocity_iterations; ++it) {
                 res_acc = 0; res_cnt = 0;
+                // The scalar island path has no sensor test (PhysicsWorld.cpp:724-728).
                 for (u32 ci : isl)
-                    if (!cons[ci].sensor)
-                        for (u32 k = 0; k < cons[ci].n; ++k) solve_scalar(cons[ci], cons[ci].p[k]);
+                    for (u32 k = 0; k < cons[ci].n; ++k) solve_scalar(cons[ci], cons[ci].p[k]);
                 sweep_sq[it] += res_acc; sweep_n[it] += res_cnt;
             }
         }
