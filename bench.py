@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Contact-path benchmark: contacts solved per second on the 32k-body pile.
+
+Workload: the random convex pile of 32,768 bodies (25 % spheres, boxes,
+capsules, convex hulls of 16-32 points; dropped from y in [2, 60] into a
+64 x 64 m walled bin; density 1000; FP32; 8 velocity + 2 position
+iterations; dt = 1/60; sleeping off).  W warm-up StepFixed calls let the pile
+form, then K StepFixed calls are timed.  A "step" is one StepFixed: host
+broadphase, then narrowphase + constraint build + solve + integration on the
+GPU.  The metric is sum(contact points) / sum(time of narrowphase + build +
+solve), measured on the host clock with the device synchronised (the host
+part of the build - compaction and the solve schedule - is inside it).
+
+Also reported: the roofline of the dominant kernel (HIP events on the
+library's stream) and the reference CPU path (the reference engine built from
+its own sources, oracle/_ref) on a bounded sample of the same pile.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--bodies N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from kizuriphysics3d_amd import scenes as S  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per contact point for one persistent solve launch:
+# warm start 168 B + velocity iteration 312 B + bias iteration 260 B per point.
+WARM_BYTES, VEL_BYTES, BIAS_BYTES = 168, 312, 260
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(scene, states, budget_s: float, threads: int):
+    """The reference engine on the GPU run's pile: restart it from the same
+    body states and time its narrowphase + build + solve phases."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import refharness as R
+    if not R.available():
+        return None
+    workers = R.init_jobs(max(threads - 1, 1))
+    rw = R.RefWorld(scene)
+    rw.set_states(states)
+    t_begin = time.time()
+    pts = 0
+    ms = 0.0
+    n = 0
+    first = True
+    while time.time() - t_begin < budget_s or n == 0:
+        ph = rw.step_phases()
+        hdr, _ = rw.constraints()
+        if first:  # the restart step runs with an empty warm-start cache: not timed
+            first = False
+            if time.time() - t_begin > budget_s:
+                break
+            continue
+        pts += int(hdr[:, 2].sum())
+        ms += float(ph[1] + ph[2] + ph[3] + ph[4])
+        n += 1
+    rw.close()
+    if ms <= 0:
+        return None
+    return {"value": pts / (ms / 1e3), "unit": "contacts/s", "cores": workers + 1, "kind": "reference",
+            "sample": f"{n} StepFixed calls of the reference on the same {len(states)}-body pile restarted from "
+                      f"the GPU run's body state (after one untimed warm-start step); phases Collide + "
+                      f"BuildContactConstraints + IntegrateVelocities + SolveContacts; JobSystem with {workers} "
+                      f"workers + caller on {cpu_model()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--bodies", type=int, default=32768)
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of reference CPU work")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the contact path of one world runs on one MI355X "
+              "(multi-GPU sharding is a design note, not part of this build); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch  # device synchronisation around the timed region
+    from kizuriphysics3d_amd.world import PhysicsWorld, device_timings
+
+    scene = S.pile(args.bodies)
+    t0 = time.time()
+    w = PhysicsWorld(scene)
+    log(f"world: {args.bodies} bodies + 5 static, built in {time.time() - t0:.1f} s")
+    for i in range(args.warmup):
+        w.step_fixed()
+        if (i + 1) % 50 == 0:
+            st = w.stats()
+            log(f"warmup {i + 1}/{args.warmup}: {st['num_contact_points']} points, step {st['step_ms']:.1f} ms "
+                f"(contact path {st['contact_path_ms']:.1f} ms, broadphase {st['broadphase_ms']:.1f} ms)")
+    torch.cuda.synchronize()
+    points = 0
+    path_ms = 0.0
+    solve_kernel_ms = 0.0
+    solve_bytes = 0.0
+    levels = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        w.step_fixed()
+        st = w.stats()
+        tm = device_timings(w)
+        points += st["num_contact_points"]
+        path_ms += st["contact_path_ms"]
+        solve_kernel_ms += tm["solve_kernel_ms"]
+        solve_bytes += st["num_contact_points"] * (WARM_BYTES + 8 * VEL_BYTES + 2 * BIAS_BYTES)
+        levels.append((tm["velocity_levels"], tm["bias_levels"], tm["groups"], tm["islands"]))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    value = points / (path_ms / 1e3)
+    achieved = solve_bytes / (solve_kernel_ms / 1e3) / 1e9
+    lv = np.array(levels)
+    log(f"timed {args.steps} steps: {points / args.steps:.0f} points/step, contact path {path_ms / args.steps:.2f} ms, "
+        f"solve kernel {solve_kernel_ms / args.steps:.2f} ms, levels vel/bias max {lv[:, 0].max()}/{lv[:, 1].max()}, "
+        f"groups {lv[:, 2].mean():.0f}, islands {lv[:, 3].mean():.0f}")
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(scene, w.states(), args.cpu_budget, min(os.cpu_count() or 1, 16))
+        except Exception as e:  # the baseline is reported, never required
+            log(f"cpu baseline unavailable: {e}")
+    line = {
+        "metric": "contacts solved/s at 32k bodies",
+        "value": value,
+        "unit": "contacts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"random convex pile, {args.bodies} bodies, 8+2 iterations, dt 1/60, "
+                               f"steps {args.warmup + 1}-{args.warmup + args.steps}",
+                   "bodies": args.bodies, "contact_points_per_step": points / args.steps,
+                   "contact_path_ms_per_step": path_ms / args.steps},
+        "roofline": {"bound": "hbm", "kernel": "k_solve_groups", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

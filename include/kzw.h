@@ -1,0 +1,75 @@
+/*
+ * kzw.h - C-ABI of the host world around the device contact path
+ * (libkizuri_amd.so): the flat counterpart of the reference's
+ * PhysicsWorld / BodySettings / WorldSettings API
+ * (include/Kizuri/World/PhysicsWorld.h:82-239, include/Kizuri/Body/Body.h:62-90).
+ *
+ * A kzw world owns the shapes (convex hulls built as ConvexHullShape::Create
+ * builds them), the bodies with their mass properties
+ * (BodyManager::RecomputeMassProperties), the dynamic AABB tree broadphase,
+ * and a kzg context that runs the narrowphase, constraint build, solve and
+ * integration on the GPU.  kzw_step is PhysicsWorld::StepFixed.
+ *
+ * device = -1 creates a host-only world (no GPU): shapes, bodies, mass data
+ * and the broadphase work, kzw_step returns KZG_ERR_DEVICE.
+ * Status codes are those of kzg.h.
+ */
+#ifndef KZW_H
+#define KZW_H
+
+#include <stdint.h>
+#include "kz_scene.h"
+#include "kzg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kzw_world kzw_world;
+
+typedef struct kzw_stats {
+    uint32_t num_bodies;
+    uint32_t num_broadphase_pairs;
+    uint32_t num_contact_constraints;
+    uint32_t num_contact_points;     /* WorldStats::numContactPoints */
+    double broadphase_ms;            /* host: broadphase update + pair list */
+    double contact_path_ms;          /* host clock around narrowphase + build + solve (device synchronised) */
+    double step_ms;                  /* whole StepFixed, host clock */
+} kzw_stats;
+
+int kzw_create(kzw_world** out, const kz_world_desc* world, int device);
+void kzw_destroy(kzw_world* w);
+const char* kzw_last_error(const kzw_world* w);
+
+/* Append shapes; hull shapes read hull_count points at hull_offset of
+ * hull_points (float3).  KZG_ERR_ARGUMENT when ConvexHullShape::Create would
+ * reject a point set. */
+int kzw_add_shapes(kzw_world* w, const kz_shape_desc* shapes, uint32_t n, const float* hull_points,
+                   uint32_t num_hull_points);
+/* Append bodies (PhysicsWorld::CreateBody); body i gets packed id = slot. */
+int kzw_add_bodies(kzw_world* w, const kz_body_desc* bodies, uint32_t n);
+
+/* PhysicsWorld::StepFixed(dt). */
+int kzw_step(kzw_world* w, float dt);
+/* The broadphase part of a step alone (UpdateBroadPhase + UpdatePairs). */
+int kzw_update_broadphase(kzw_world* w);
+
+int kzw_get_states(kzw_world* w, kz_body_state* out, uint32_t n);
+int kzw_set_states(kzw_world* w, const kz_body_state* in, uint32_t n);
+/* Pairs of the last broadphase update as (slot A, slot B); returns the count. */
+int kzw_get_pairs(kzw_world* w, uint32_t* out, uint32_t cap);
+int kzw_get_stats(kzw_world* w, kzw_stats* out);
+/* Per body: inverse mass + inverse local inertia (column-major), 10 floats. */
+int kzw_get_mass(kzw_world* w, float* out, uint32_t n);
+/* Built hull of shape `shape`: vertex count / triangle count returned via
+ * the pointers; arrays may be null to query sizes. */
+int kzw_get_hull(kzw_world* w, uint32_t shape, float* verts, uint32_t* tris, uint32_t* num_verts,
+                 uint32_t* num_tris);
+/* The device context of the world (null for a host-only world). */
+kzg_ctx* kzw_device(kzw_world* w);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KZW_H */

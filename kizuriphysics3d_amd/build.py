@@ -17,7 +17,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkizuri_amd.so")
-SOURCES = ["narrowphase.hip", "solver.hip", "context.hip", "schedule.cpp", "world.cpp", "mlp.hip", "batch.hip"]
+SOURCES = ["narrowphase.hip", "solver.hip", "context.hip", "schedule.cpp", "host_geometry.cpp", "world.cpp",
+           "mlp.hip", "batch.hip"]
 
 FLAGS = [
     "-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++20", "-ffp-contract=off", "-fno-fast-math",

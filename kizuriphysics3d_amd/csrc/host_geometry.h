@@ -1,0 +1,86 @@
+// host_geometry.h - host-side geometry of the product library: convex hull
+// construction, mass properties, world bounds and the dynamic AABB tree
+// broadphase.  These run on the CPU when bodies are created and once per step
+// (broadphase), exactly as in the reference engine, and decide the vertex
+// order of hulls and the order of broadphase pairs that the device path
+// consumes - so they follow the reference's arithmetic and tie-breaking.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "kz_math.h"
+
+namespace kz {
+
+struct Aabb {
+    f3 mn{kInf, kInf, kInf};
+    f3 mx{-kInf, -kInf, -kInf};
+};
+
+// Built convex hull: vertices in the order ConvexHullShape::Create emits them
+// and its triangles (src/Geometry/Shapes.cpp:331-471).
+struct Hull {
+    std::vector<f3> verts;
+    std::vector<uint32_t> tris;  // 3 per triangle
+    Aabb bounds;
+    float bounding_radius = 0.0f;
+};
+
+// ConvexHullShape::Create; false when the reference would return null.
+bool build_hull(const float* points, uint32_t n, Hull& out, uint32_t max_points = 256);
+
+struct MassProps {
+    float mass = 0.0f;
+    m3 inertia{};
+    f3 com{0.0f, 0.0f, 0.0f};
+};
+
+// Shape::GetMassProperties for sphere, box, capsule, hull; plane = none.
+MassProps shape_mass(uint32_t type, const float* params, const Hull* hull, float density);
+// Shape::GetBoundingRadius.
+float shape_bounding_radius(uint32_t type, const float* params, const Hull* hull);
+// Shape::GetLocalBounds.
+Aabb shape_local_bounds(uint32_t type, const float* params, const Hull* hull);
+// AABB::Transformed (src/Core/Math.cpp:176-189).
+Aabb transform_aabb(const Aabb& a, f4q q, f3 t);
+// BodyManager::RecomputeMassProperties (src/Body/BodyManager.cpp:63-99):
+// inverse mass and inverse local inertia for a dynamic body.
+void body_mass(uint32_t motion_type, uint32_t type, const float* params, const Hull* hull, float density,
+               float mass_override, float& inv_mass, m3& inv_inertia_local);
+
+// Dynamic AABB tree + layer-filtered pair list (src/BroadPhase/BroadPhase.cpp,
+// include/Kizuri/BroadPhase/BroadPhase.h:61-104).
+class BroadPhase {
+public:
+    uint32_t create_proxy(const Aabb& tight, uint32_t user, uint32_t layer, uint32_t mask);
+    bool move_proxy(uint32_t id, const Aabb& tight, f3 displacement);
+    void update_pairs();
+    const std::vector<uint32_t>& pairs() const { return pairs_; }  // proxy A, proxy B, ...
+    uint32_t user(uint32_t id) const { return nodes_[id].user; }
+
+private:
+    static constexpr uint32_t kNull = 0xFFFFFFFFu;
+    struct Node {
+        Aabb fat, tight;
+        uint32_t parent = kNull, child1 = kNull, child2 = kNull, user = 0, next = kNull;
+        int32_t height = -1;
+        bool leaf() const { return child1 == kNull; }
+    };
+    uint32_t allocate();
+    void free_node(uint32_t id);
+    void insert_leaf(uint32_t leaf);
+    void remove_leaf(uint32_t leaf);
+    uint32_t balance(uint32_t a);
+
+    std::vector<Node> nodes_;
+    uint32_t root_ = kNull, free_ = kNull;
+    std::vector<uint32_t> layers_, masks_, pairs_;
+    struct Entry {
+        uint32_t n1, n2;
+    };
+    std::vector<Entry> stack_;
+};
+
+}  // namespace kz
