@@ -56,7 +56,6 @@ def cpu_model() -> str:
 def cpu_baseline(scene, states, budget_s: float, threads: int):
     """The reference engine on the GPU run's pile: restart it from the same
     body states and time its narrowphase + build + solve phases."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import refharness as R
     if not R.available():
         return None
@@ -122,6 +121,7 @@ def main():
     solve_kernel_ms = 0.0
     solve_bytes = 0.0
     levels = []
+    phases = np.zeros(6)
     t_start = time.perf_counter()
     for i in range(args.steps):
         w.step_fixed()
@@ -132,6 +132,8 @@ def main():
         solve_kernel_ms += tm["solve_kernel_ms"]
         solve_bytes += st["num_contact_points"] * (WARM_BYTES + 8 * VEL_BYTES + 2 * BIAS_BYTES)
         levels.append((tm["velocity_levels"], tm["bias_levels"], tm["groups"], tm["islands"]))
+        phases += [tm["narrowphase_ms"], tm["build_ms"], tm["integrate_velocities_ms"], tm["solve_ms"],
+                   tm["solve_kernel_ms"], st["broadphase_ms"]]
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     value = points / (path_ms / 1e3)
@@ -140,6 +142,9 @@ def main():
     log(f"timed {args.steps} steps: {points / args.steps:.0f} points/step, contact path {path_ms / args.steps:.2f} ms, "
         f"solve kernel {solve_kernel_ms / args.steps:.2f} ms, levels vel/bias max {lv[:, 0].max()}/{lv[:, 1].max()}, "
         f"groups {lv[:, 2].mean():.0f}, islands {lv[:, 3].mean():.0f}")
+    ph = phases / args.steps
+    log(f"per step (ms): narrowphase {ph[0]:.2f}, build (incl. host schedule) {ph[1]:.2f}, integrate velocities "
+        f"{ph[2]:.2f}, solve {ph[3]:.2f} (sweep kernel {ph[4]:.2f}), host broadphase {ph[5]:.2f}")
     cpu = None
     if not args.no_cpu:
         try:

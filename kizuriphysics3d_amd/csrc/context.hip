@@ -149,6 +149,7 @@ struct kzg_ctx {
     // staging and timing
     Pinned stage, stage2;
     hipEvent_t ev[8] = {};
+    uint32_t recorded = 0;  // bit i: ev[i] recorded since the last narrowphase
     kzg_timings timings{};
     double build_host_ms = 0.0;
     int phase = 0;  // 0 idle, 1 narrowphase done, 2 built, 3 solved
@@ -171,6 +172,13 @@ struct kzg_ctx {
 };
 
 namespace {
+
+int fail(kzg_ctx* c, int code, const std::string& msg);
+
+hipError_t record(kzg_ctx* c, int k) {
+    c->recorded |= 1u << k;
+    return hipEventRecord(c->ev[k], c->stream);
+}
 
 int fail(kzg_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -433,12 +441,13 @@ int kzg_narrowphase(kzg_ctx* c, const uint32_t* pairs, uint32_t n) {
     KZG_HIP(c, c->m_n.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
     KZG_HIP(c, c->work.ensure(std::max<uint32_t>(n, 1)));
     int rc = 0;
-    KZG_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    c->recorded = 0;
+    KZG_HIP(c, record(c, 0));
     if ((rc = upload(c, c->d_pairs, reinterpret_cast<const uint2*>(pairs), n))) return rc;
     launch_narrowphase(c->stream, c->d_pairs.p, n, c->bodies(), c->shapes(), c->manifolds(), c->work.p,
                        c->work_count.p, 0.02f, 1, std::min<uint32_t>(n, c->cu_count * 16u));
     KZG_HIP(c, hipGetLastError());
-    KZG_HIP(c, hipEventRecord(c->ev[1], c->stream));
+    KZG_HIP(c, record(c, 1));
     c->phase = 1;
     return KZG_OK;
 }
@@ -495,10 +504,10 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     (void)nc1;
     if ((rc = ensure_cache(c, c->cache_keys.size()))) return rc;
     c->build_host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    KZG_HIP(c, hipEventRecord(c->ev[2], c->stream));
+    KZG_HIP(c, record(c, 2));
     launch_build(c->stream, c->contacts(), c->manifolds(), c->bodies(), c->cache(), c->ncons, warm);
     KZG_HIP(c, hipGetLastError());
-    KZG_HIP(c, hipEventRecord(c->ev[3], c->stream));
+    KZG_HIP(c, record(c, 3));
     c->phase = 2;
     return KZG_OK;
 }
@@ -508,7 +517,7 @@ int kzg_integrate_velocities(kzg_ctx* c, float dt) {
     hipSetDevice(c->device);
     launch_integrate_velocities(c->stream, c->bodies(), c->nbodies, dt, c->world.gravity);
     KZG_HIP(c, hipGetLastError());
-    KZG_HIP(c, hipEventRecord(c->ev[4], c->stream));
+    KZG_HIP(c, record(c, 4));
     return KZG_OK;
 }
 
@@ -529,11 +538,11 @@ int kzg_solve(kzg_ctx* c, float dt) {
     const uint32_t width = std::max(c->sched.ws.max_width, c->sched.vel.max_width);
     uint32_t threads = 64;
     while (threads < width && threads < 1024) threads *= 2;
-    KZG_HIP(c, hipEventRecord(c->ev[5], c->stream));
+    KZG_HIP(c, record(c, 5));
     if (c->ncons) launch_solve_groups(c->stream, c->contacts(), c->bodies(), ws, vel, sp, c->sched.groups, threads,
                                       c->resid.p);
     KZG_HIP(c, hipGetLastError());
-    KZG_HIP(c, hipEventRecord(c->ev[6], c->stream));
+    KZG_HIP(c, record(c, 6));
     launch_cache_store(c->stream, c->contacts(), c->bodies(), c->cache(), c->ncons);
     KZG_HIP(c, hipGetLastError());
     c->resid_points = c->npoints;
@@ -546,7 +555,7 @@ int kzg_integrate_positions(kzg_ctx* c, float dt) {
     hipSetDevice(c->device);
     launch_finish(c->stream, c->bodies(), c->nbodies, dt, c->world.use_velocity_bias ? 0 : 1);
     KZG_HIP(c, hipGetLastError());
-    KZG_HIP(c, hipEventRecord(c->ev[7], c->stream));
+    KZG_HIP(c, record(c, 7));
     c->phase = 0;
     return KZG_OK;
 }
@@ -665,13 +674,21 @@ int kzg_get_timings(kzg_ctx* c, kzg_timings* out) {
     hipSetDevice(c->device);
     KZG_HIP(c, hipStreamSynchronize(c->stream));
     kzg_timings t{};
-    float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) t.narrowphase_ms = ms;
-    if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) t.build_ms = ms + float(c->build_host_ms);
-    if (hipEventElapsedTime(&ms, c->ev[3], c->ev[4]) == hipSuccess) t.integrate_velocities_ms = ms;
-    if (hipEventElapsedTime(&ms, c->ev[4], c->ev[6]) == hipSuccess) t.solve_ms = ms;
-    if (hipEventElapsedTime(&ms, c->ev[5], c->ev[6]) == hipSuccess) t.solve_kernel_ms = ms;
-    if (hipEventElapsedTime(&ms, c->ev[6], c->ev[7]) == hipSuccess) t.integrate_positions_ms = ms;
+    // Only pairs of events recorded since the last narrowphase are read, so
+    // no query fails and leaves a sticky runtime error behind.
+    auto span = [&](int a, int b, float& out) {
+        float ms = 0.0f;
+        if ((c->recorded >> a & 1u) && (c->recorded >> b & 1u) && hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) == hipSuccess)
+            out = ms;
+    };
+    span(0, 1, t.narrowphase_ms);
+    span(2, 3, t.build_ms);
+    t.build_ms += float(c->build_host_ms);
+    span(3, 4, t.integrate_velocities_ms);
+    span(4, 6, t.solve_ms);
+    span(5, 6, t.solve_kernel_ms);
+    span(6, 7, t.integrate_positions_ms);
+    (void)hipGetLastError();
     t.velocity_levels = c->sched.vel.max_depth;
     t.bias_levels = c->sched.ws.max_depth;
     t.groups = c->sched.groups;
@@ -709,11 +726,12 @@ int kzg_collide_batch(kzg_ctx* c, const uint32_t* sa, const uint32_t* sb, const 
     KZG_HIP(c, c->m_n.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
     KZG_HIP(c, c->work.ensure(std::max<uint32_t>(n, 1)));
     BodyBuf bb{c->x_x.p, c->x_q.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->x_meta.p, nullptr};
-    KZG_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    c->recorded = 0;
+    KZG_HIP(c, record(c, 0));
     launch_narrowphase(c->stream, c->x_pairs.p, n, bb, c->shapes(), c->manifolds(), c->work.p, c->work_count.p,
                        max_sep, 0, std::min<uint32_t>(std::max<uint32_t>(n, 1), c->cu_count * 16u));
     KZG_HIP(c, hipGetLastError());
-    KZG_HIP(c, hipEventRecord(c->ev[1], c->stream));
+    KZG_HIP(c, record(c, 1));
     KZG_HIP(c, hipStreamSynchronize(c->stream));
     if (ms) KZG_HIP(c, hipEventElapsedTime(ms, c->ev[0], c->ev[1]));
     c->npairs = n;
