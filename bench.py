@@ -96,6 +96,7 @@ def main():
     ap.add_argument("--bodies", type=int, default=32768)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of reference CPU work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--dump", default="", help="save the last step's contact table (npz) for host-side tuning")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the contact path of one world runs on one MI355X "
@@ -145,6 +146,17 @@ def main():
     ph = phases / args.steps
     log(f"per step (ms): narrowphase {ph[0]:.2f}, build (incl. host schedule) {ph[1]:.2f}, integrate velocities "
         f"{ph[2]:.2f}, solve {ph[3]:.2f} (sweep kernel {ph[4]:.2f}), host broadphase {ph[5]:.2f}")
+    if args.dump:
+        import ctypes as C
+        from kizuriphysics3d_amd import device as D
+        h = w.device_context_handle()
+        a, b, c3 = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        D.library().kzg_query_sizes(h, C.byref(a), C.byref(b), C.byref(c3))
+        hdr = np.zeros((b.value, 4), np.uint32)
+        pts = np.zeros((b.value, 8, 44), np.float32)
+        D.library().kzg_download_constraints(h, hdr.ctypes.data_as(C.c_void_p), pts.ctypes.data_as(C.c_void_p))
+        dyn = (scene.body_array()["motion_type"] == S.MOTION_DYNAMIC).astype(np.uint8)
+        np.savez_compressed(args.dump, hdr=hdr, dynamic=dyn)
     cpu = None
     if not args.no_cpu:
         try:

@@ -125,6 +125,19 @@ int kzg_collide_batch(kzg_ctx* ctx, const uint32_t* shape_a, const uint32_t* sha
                       const float* xf_a, const float* xf_b, uint32_t n, float max_separation,
                       uint32_t* counts, kz_contact_point* points, float* ms);
 
+/* Host only (no GPU): the solve schedule kzg_solve would use for a contact
+ * table, for tests and tuning.  info: 4 u32 per constraint (body A, body B,
+ * first point, num points | sensor << 8), points numbered 0..P-1 in
+ * constraint order.  Outputs per point: vel_level (velocity sweep level,
+ * 0 = not swept) and ws_level (warm-start / bias level), levels counted from
+ * 1 within the point's group; group[p] = its group.  stats = islands, groups,
+ * deepest velocity group, deepest warm-start group.  Islands and the
+ * first-fit batch order are those of PhysicsWorld::BuildIslands and
+ * SolveIsland (src/World/PhysicsWorld.cpp:587-717). */
+int kzg_schedule(uint32_t num_bodies, const uint8_t* dynamic, uint32_t num_constraints, const uint32_t* info,
+                 int use_simd, uint32_t max_groups, uint32_t* vel_level, uint32_t* ws_level, uint32_t* group,
+                 uint32_t* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,10 +23,13 @@ void launch_integrate_velocities(hipStream_t s, const BodyBuf& b, uint32_t n, fl
 void launch_build(hipStream_t s, const ContactBuf& c, const ManifoldBuf& m, const BodyBuf& b, const CacheBuf& cache,
                   uint32_t nc, int warm);
 void launch_prepare(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32_t np, float dt, float rthr,
-                    float baum, float slop, float maxb);
-void launch_solve_groups(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const Schedule& ws,
-                         const Schedule& vel, const SolveParams& sp, uint32_t groups, uint32_t threads,
-                         float* resid);
+                    float baum, float slop, float maxb, int use_simd, int use_vbias, const VelRecBuf& vr,
+                    const WsRecBuf& wr, const uint32_t* vel_pos, const uint32_t* ws_pos);
+void launch_writeback(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const ContactBuf& c,
+                      const uint32_t* vel_pos, const uint32_t* ws_pos, uint32_t np);
+void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b,
+                         const Schedule& ws, const Schedule& vel, const SolveParams& sp, uint32_t groups,
+                         uint32_t threads, float* resid);
 void launch_cache_store(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const CacheBuf& cache, uint32_t nc);
 void launch_cache_rehash(hipStream_t s, const CacheBuf& src, const CacheBuf& dst);
 void launch_finish(hipStream_t s, const BodyBuf& b, uint32_t n, float dt, int split);
@@ -131,23 +134,28 @@ struct kzg_ctx {
     // schedule
     ScheduleBuilder sched_builder;
     ScheduleOutput sched;
-    DevVec<uint32_t> ws_idx, ws_off, vel_idx, vel_off;
+    DevVec<uint32_t> ws_off, vel_off, ws_pos, vel_pos;
     DevVec<uint2> ws_grp, vel_grp;
+    // sweep records
+    DevVec<uint4> vr_bd, wr_bd;
+    DevVec<float4> vr_ra, vr_rb, vr_n, vr_t1, vr_t2, vr_imp;
+    DevVec<float2> vr_fb;
+    DevVec<float4> wr_ra, wr_rb, wr_qa, wr_qb, wr_n, wr_t1, wr_t2, wr_imp;
 
     // warm-start cache
     DevVec<unsigned long long> cache_key;
+    DevVec<uint32_t> cache_used;
     DevVec<uint32_t> cache_count;
     DevVec<float4> cache_p;
     DevVec<float2> cache_t;
     uint32_t cache_cap = 0;
-    std::unordered_set<uint64_t> cache_keys;
 
     // residuals
     DevVec<float> resid;
     uint32_t resid_points = 0;
 
     // staging and timing
-    Pinned stage, stage2;
+    Pinned stage, stage2, stage_a, stage_b;
     hipEvent_t ev[8] = {};
     uint32_t recorded = 0;  // bit i: ev[i] recorded since the last narrowphase
     kzg_timings timings{};
@@ -168,7 +176,11 @@ struct kzg_ctx {
         return ContactBuf{c_info.p, c_pair.p, p_body.p, p_pa.p, p_pb.p, p_n.p, p_t1.p, p_t2.p,
                           p_ra.p, p_rb.p, p_la.p, p_lb.p, p_imp.p, p_nvel.p};
     }
-    CacheBuf cache() { return CacheBuf{cache_key.p, cache_count.p, cache_p.p, cache_t.p, cache_cap}; }
+    CacheBuf cache() { return CacheBuf{cache_key.p, cache_used.p, cache_count.p, cache_p.p, cache_t.p, cache_cap}; }
+    VelRecBuf vel_records() { return VelRecBuf{vr_bd.p, vr_ra.p, vr_rb.p, vr_n.p, vr_t1.p, vr_t2.p, vr_fb.p, vr_imp.p}; }
+    WsRecBuf ws_records() {
+        return WsRecBuf{wr_bd.p, wr_ra.p, wr_rb.p, wr_qa.p, wr_qb.p, wr_n.p, wr_t1.p, wr_t2.p, wr_imp.p};
+    }
 };
 
 namespace {
@@ -220,27 +232,43 @@ int ensure_cache(kzg_ctx* c, size_t needed_keys) {
     while (size_t(want) < 2 * needed_keys) want *= 2u;
     if (want == c->cache_cap) return KZG_OK;
     DevVec<unsigned long long> nk;
-    DevVec<uint32_t> ncnt;
+    DevVec<uint32_t> nused, ncnt;
     DevVec<float4> np;
     DevVec<float2> nt;
     KZG_HIP(c, nk.ensure(want));
+    KZG_HIP(c, nused.ensure(1));
     KZG_HIP(c, ncnt.ensure(want));
     KZG_HIP(c, np.ensure(size_t(want) * kPairPoints));
     KZG_HIP(c, nt.ensure(size_t(want) * kPairPoints));
     KZG_HIP(c, hipMemsetAsync(nk.p, 0, sizeof(unsigned long long) * want, c->stream));
-    CacheBuf dst{nk.p, ncnt.p, np.p, nt.p, want};
+    KZG_HIP(c, hipMemsetAsync(nused.p, 0, sizeof(uint32_t), c->stream));
+    CacheBuf dst{nk.p, nused.p, ncnt.p, np.p, nt.p, want};
     if (c->cache_cap) launch_cache_rehash(c->stream, c->cache(), dst);
     KZG_HIP(c, hipGetLastError());
     KZG_HIP(c, hipStreamSynchronize(c->stream));
     c->cache_key.release();
+    c->cache_used.release();
     c->cache_count.release();
     c->cache_p.release();
     c->cache_t.release();
     c->cache_key = nk;
+    c->cache_used = nused;
     c->cache_count = ncnt;
     c->cache_p = np;
     c->cache_t = nt;
     c->cache_cap = want;
+    return KZG_OK;
+}
+
+// Asynchronous upload through a pinned staging buffer: the caller guarantees
+// that the buffer is not reused before the stream has drained.
+template <class T> int upload_async(kzg_ctx* c, DevVec<T>& d, Pinned& st, size_t& off, const T* src, size_t n) {
+    KZG_HIP(c, d.ensure(std::max<size_t>(n, 1)));
+    if (!n) return KZG_OK;
+    off = (off + 255) & ~size_t(255);
+    std::memcpy(static_cast<char*>(st.p) + off, src, n * sizeof(T));
+    KZG_HIP(c, hipMemcpyAsync(d.p, static_cast<char*>(st.p) + off, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    off += n * sizeof(T);
     return KZG_OK;
 }
 
@@ -287,15 +315,19 @@ void kzg_destroy(kzg_ctx* c) {
     for (auto* d : {&c->b_x, &c->b_q, &c->b_v, &c->b_w, &c->b_pv, &c->b_pw, &c->b_iw, &c->b_il, &c->b_mat, &c->b_lim,
                     &c->s_params, &c->s_hverts, &c->s_hfn, &c->m_pa, &c->m_pb, &c->m_n, &c->p_pa, &c->p_pb, &c->p_n,
                     &c->p_t1, &c->p_t2, &c->p_ra, &c->p_rb, &c->p_la, &c->p_lb, &c->p_imp, &c->cache_p, &c->x_x,
-                    &c->x_q})
+                    &c->x_q, &c->vr_ra, &c->vr_rb, &c->vr_n, &c->vr_t1, &c->vr_t2, &c->vr_imp, &c->wr_ra, &c->wr_rb,
+                    &c->wr_qa, &c->wr_qb, &c->wr_n, &c->wr_t1, &c->wr_t2, &c->wr_imp})
         d->release();
     for (auto* d : {&c->b_meta, &c->b_packed, &c->s_type, &c->s_htris, &c->m_count, &c->work, &c->work_count,
-                    &c->c_pair, &c->ws_idx, &c->ws_off, &c->vel_idx, &c->vel_off, &c->cache_count, &c->x_meta,
-                    &c->x_packed})
+                    &c->c_pair, &c->ws_off, &c->vel_off, &c->ws_pos, &c->vel_pos, &c->cache_count, &c->cache_used,
+                    &c->x_meta, &c->x_packed})
         d->release();
     c->s_hull.release();
     c->c_info.release();
     c->p_body.release();
+    c->vr_bd.release();
+    c->wr_bd.release();
+    c->vr_fb.release();
     c->p_nvel.release();
     c->ws_grp.release();
     c->vel_grp.release();
@@ -306,6 +338,8 @@ void kzg_destroy(kzg_ctx* c) {
     c->resid.release();
     c->stage.release();
     c->stage2.release();
+    c->stage_a.release();
+    c->stage_b.release();
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -458,27 +492,54 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     hipSetDevice(c->device);
     auto t0 = std::chrono::steady_clock::now();
     const uint32_t n = c->npairs;
-    std::vector<uint32_t> counts(n);
-    int rc = download(c, counts.data(), c->m_count, n);
-    if (rc) return rc;
+    // Manifold point counts and the cache occupancy, one synchronisation.
+    KZG_HIP(c, c->stage2.ensure(sizeof(uint32_t) * (size_t(n) + 1)));
+    uint32_t* hc = c->stage2.as<uint32_t>();
+    if (n) KZG_HIP(c, hipMemcpyAsync(hc, c->m_count.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    KZG_HIP(c, hipMemcpyAsync(hc + n, c->cache_used.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    const uint32_t cache_used = hc[n];
     // Compaction in pair order (BuildContactConstraints: empty manifolds and
     // pairs without a dynamic body are skipped; bodies ordered by packed id).
     c->h_info.clear();
     c->h_cpair.clear();
     uint32_t np = 0;
     for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t k = counts[i];
+        const uint32_t k = hc[i];
         if (k == 0) continue;
         uint32_t a = c->h_pairs[2 * i], b = c->h_pairs[2 * i + 1];
         if (c->h_packed[a] > c->h_packed[b]) std::swap(a, b);
         const uint32_t sensor = (c->h_sensor[a] || c->h_sensor[b]) ? 1u : 0u;
         c->h_info.push_back(U4{a, b, np, k | (sensor << 8)});
         c->h_cpair.push_back(i);
-        c->cache_keys.insert(pair_key(c->h_packed[a], c->h_packed[b]));
         np += k;
     }
     c->ncons = uint32_t(c->h_info.size());
     c->npoints = np;
+    int rc;
+    // The store at the end of this step adds at most ncons keys.
+    if ((rc = ensure_cache(c, size_t(cache_used) + c->ncons))) return rc;
+    const size_t np1 = std::max<uint32_t>(np, 1);
+    KZG_HIP(c, c->p_body.ensure(np1));
+    for (auto* d : {&c->p_pa, &c->p_pb, &c->p_n, &c->p_t1, &c->p_t2, &c->p_ra, &c->p_rb, &c->p_la, &c->p_lb, &c->p_imp,
+                    &c->vr_ra, &c->vr_rb, &c->vr_n, &c->vr_t1, &c->vr_t2, &c->vr_imp, &c->wr_ra, &c->wr_rb,
+                    &c->wr_qa, &c->wr_qb, &c->wr_n, &c->wr_t1, &c->wr_t2, &c->wr_imp})
+        KZG_HIP(c, d->ensure(np1));
+    KZG_HIP(c, c->vr_bd.ensure(np1));
+    KZG_HIP(c, c->wr_bd.ensure(np1));
+    KZG_HIP(c, c->vr_fb.ensure(np1));
+    KZG_HIP(c, c->p_nvel.ensure(np1));
+    // Constraint table up, then the build kernel runs while the host derives
+    // the solve schedule.
+    KZG_HIP(c, c->stage_a.ensure(sizeof(U4) * c->ncons + sizeof(uint32_t) * c->ncons + 1024));
+    size_t off = 0;
+    if ((rc = upload_async(c, c->c_info, c->stage_a, off, reinterpret_cast<const uint4*>(c->h_info.data()), c->ncons)) ||
+        (rc = upload_async(c, c->c_pair, c->stage_a, off, c->h_cpair.data(), c->ncons)))
+        return rc;
+    KZG_HIP(c, record(c, 2));
+    launch_build(c->stream, c->contacts(), c->manifolds(), c->bodies(), c->cache(), c->ncons, warm);
+    KZG_HIP(c, hipGetLastError());
+    KZG_HIP(c, record(c, 3));
     ScheduleInput si;
     si.num_bodies = c->nbodies;
     si.dynamic = c->h_dynamic.data();
@@ -487,27 +548,19 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     si.use_simd = c->world.use_simd_solver != 0;
     si.max_groups = c->cu_count * 4u;
     c->sched_builder.build(si, c->sched);
-    const size_t nc1 = std::max<uint32_t>(c->ncons, 1), np1 = std::max<uint32_t>(np, 1);
-    KZG_HIP(c, c->p_body.ensure(np1));
-    for (auto* d : {&c->p_pa, &c->p_pb, &c->p_n, &c->p_t1, &c->p_t2, &c->p_ra, &c->p_rb, &c->p_la, &c->p_lb, &c->p_imp})
-        KZG_HIP(c, d->ensure(np1));
-    KZG_HIP(c, c->p_nvel.ensure(np1));
-    if ((rc = upload(c, c->c_info, reinterpret_cast<const uint4*>(c->h_info.data()), c->ncons)) ||
-        (rc = upload(c, c->c_pair, c->h_cpair.data(), c->ncons)) ||
-        (rc = upload(c, c->ws_idx, c->sched.ws.idx.data(), c->sched.ws.idx.size())) ||
-        (rc = upload(c, c->ws_off, c->sched.ws.off.data(), c->sched.ws.off.size())) ||
-        (rc = upload(c, c->ws_grp, reinterpret_cast<const uint2*>(c->sched.ws.grp.data()), c->sched.ws.grp.size())) ||
-        (rc = upload(c, c->vel_idx, c->sched.vel.idx.data(), c->sched.vel.idx.size())) ||
-        (rc = upload(c, c->vel_off, c->sched.vel.off.data(), c->sched.vel.off.size())) ||
-        (rc = upload(c, c->vel_grp, reinterpret_cast<const uint2*>(c->sched.vel.grp.data()), c->sched.vel.grp.size())))
+    const LevelSchedule &ws = c->sched.ws, &vel = c->sched.vel;
+    const size_t bytes = sizeof(uint32_t) * (ws.off.size() + vel.off.size() + 2 * size_t(np)) +
+                         sizeof(U2) * (ws.grp.size() + vel.grp.size()) + 8 * 256;
+    KZG_HIP(c, c->stage_b.ensure(bytes));
+    off = 0;
+    if ((rc = upload_async(c, c->ws_off, c->stage_b, off, ws.off.data(), ws.off.size())) ||
+        (rc = upload_async(c, c->ws_grp, c->stage_b, off, reinterpret_cast<const uint2*>(ws.grp.data()), ws.grp.size())) ||
+        (rc = upload_async(c, c->ws_pos, c->stage_b, off, ws.pos.data(), ws.pos.size())) ||
+        (rc = upload_async(c, c->vel_off, c->stage_b, off, vel.off.data(), vel.off.size())) ||
+        (rc = upload_async(c, c->vel_grp, c->stage_b, off, reinterpret_cast<const uint2*>(vel.grp.data()), vel.grp.size())) ||
+        (rc = upload_async(c, c->vel_pos, c->stage_b, off, vel.pos.data(), vel.pos.size())))
         return rc;
-    (void)nc1;
-    if ((rc = ensure_cache(c, c->cache_keys.size()))) return rc;
     c->build_host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    KZG_HIP(c, record(c, 2));
-    launch_build(c->stream, c->contacts(), c->manifolds(), c->bodies(), c->cache(), c->ncons, warm);
-    KZG_HIP(c, hipGetLastError());
-    KZG_HIP(c, record(c, 3));
     c->phase = 2;
     return KZG_OK;
 }
@@ -527,22 +580,25 @@ int kzg_solve(kzg_ctx* c, float dt) {
     hipSetDevice(c->device);
     const kz_world_desc& w = c->world;
     launch_prepare(c->stream, c->contacts(), c->bodies(), c->npoints, dt, w.restitution_threshold, w.baumgarte,
-                   w.penetration_slop, w.max_bias_velocity);
+                   w.penetration_slop, w.max_bias_velocity, w.use_simd_solver ? 1 : 0, w.use_velocity_bias ? 1 : 0,
+                   c->vel_records(), c->ws_records(), c->vel_pos.p, c->ws_pos.p);
     KZG_HIP(c, hipGetLastError());
     KZG_HIP(c, c->resid.ensure(std::max<uint32_t>(w.velocity_iterations, 1)));
     KZG_HIP(c, hipMemsetAsync(c->resid.p, 0, sizeof(float) * std::max<uint32_t>(w.velocity_iterations, 1), c->stream));
     SolveParams sp{dt, w.warm_start_factor, w.velocity_iterations, w.position_iterations, w.use_simd_solver,
                    w.use_velocity_bias};
-    Schedule ws{c->ws_idx.p, c->ws_off.p, c->ws_grp.p};
-    Schedule vel{c->vel_idx.p, c->vel_off.p, c->vel_grp.p};
+    Schedule ws{c->ws_off.p, c->ws_grp.p};
+    Schedule vel{c->vel_off.p, c->vel_grp.p};
     const uint32_t width = std::max(c->sched.ws.max_width, c->sched.vel.max_width);
     uint32_t threads = 64;
     while (threads < width && threads < 1024) threads *= 2;
     KZG_HIP(c, record(c, 5));
-    if (c->ncons) launch_solve_groups(c->stream, c->contacts(), c->bodies(), ws, vel, sp, c->sched.groups, threads,
-                                      c->resid.p);
+    if (c->ncons) launch_solve_groups(c->stream, c->vel_records(), c->ws_records(), c->bodies(), ws, vel, sp,
+                                      c->sched.groups, threads, c->resid.p);
     KZG_HIP(c, hipGetLastError());
     KZG_HIP(c, record(c, 6));
+    launch_writeback(c->stream, c->vel_records(), c->ws_records(), c->contacts(), c->vel_pos.p, c->ws_pos.p,
+                     c->npoints);
     launch_cache_store(c->stream, c->contacts(), c->bodies(), c->cache(), c->ncons);
     KZG_HIP(c, hipGetLastError());
     c->resid_points = c->npoints;
@@ -694,6 +750,45 @@ int kzg_get_timings(kzg_ctx* c, kzg_timings* out) {
     t.groups = c->sched.groups;
     t.islands = c->sched.islands;
     *out = t;
+    return KZG_OK;
+}
+
+int kzg_schedule(uint32_t nb, const uint8_t* dyn, uint32_t nc, const uint32_t* info, int use_simd, uint32_t max_groups,
+                 uint32_t* vel_level, uint32_t* ws_level, uint32_t* group, uint32_t* stats) {
+    if ((nb && !dyn) || (nc && !info) || !stats) return KZG_ERR_ARGUMENT;
+    std::vector<U4> in(nc);
+    uint32_t np = 0;
+    for (uint32_t i = 0; i < nc; ++i) {
+        in[i] = U4{info[4 * i], info[4 * i + 1], info[4 * i + 2], info[4 * i + 3]};
+        if (in[i].x >= nb || in[i].y >= nb || in[i].z != np) return KZG_ERR_ARGUMENT;
+        np += in[i].w & 0xFFu;
+    }
+    ScheduleBuilder b;
+    ScheduleOutput out;
+    ScheduleInput si;
+    si.num_bodies = nb;
+    si.dynamic = dyn;
+    si.num_constraints = nc;
+    si.info = in.data();
+    si.use_simd = use_simd != 0;
+    si.max_groups = std::max<uint32_t>(max_groups, 1);
+    b.build(si, out);
+    auto fill = [&](const LevelSchedule& s, uint32_t* lvl, bool with_group) {
+        if (!lvl && !(with_group && group)) return;
+        if (lvl) std::fill(lvl, lvl + np, 0u);
+        for (uint32_t g = 0; g < s.grp.size(); ++g)
+            for (uint32_t l = s.grp[g].x; l < s.grp[g].y; ++l)
+                for (uint32_t e = s.off[l]; e < s.off[l + 1]; ++e) {
+                    if (lvl) lvl[s.idx[e]] = l - s.grp[g].x + 1;
+                    if (with_group && group) group[s.idx[e]] = g;
+                }
+    };
+    fill(out.vel, vel_level, false);
+    fill(out.ws, ws_level, true);
+    stats[0] = out.islands;
+    stats[1] = out.groups;
+    stats[2] = out.vel.max_depth;
+    stats[3] = out.ws.max_depth;
     return KZG_OK;
 }
 

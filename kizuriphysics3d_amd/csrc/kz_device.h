@@ -81,6 +81,7 @@ struct ContactBuf {
 // src/World/PhysicsWorld.cpp:405-452).
 struct CacheBuf {
     unsigned long long* key;  // [cap]
+    uint32_t* used;           // number of occupied slots
     uint32_t* count;          // [cap]
     float4* p;                // [cap*4] position xyz, normal impulse
     float2* t;                // [cap*4] tangent impulses
@@ -88,11 +89,36 @@ struct CacheBuf {
 };
 
 // Level schedule of one solve: per group a range of levels, per level a
-// range of point indices.
+// range of sweep records (records are stored in schedule order).
 struct Schedule {
-    const uint32_t* idx;      // point indices, grouped by level
     const uint32_t* lvl_off;  // [levels + 1]
     const uint2* grp;         // [groups] first level, end level
+};
+
+// Sweep records: the constant per-point data of a sweep, laid out in the
+// order the sweep visits the points so that a level reads contiguous memory.
+// Velocity records (velocity schedule order):
+struct VelRecBuf {
+    uint4* bd;    // body A, body B, flags (1 sensor, 2 dynA, 4 dynB), point index
+    float4* ra;   // rotate(qA, localAnchorA), inverse mass A
+    float4* rb;   // rotate(qB, localAnchorB), inverse mass B
+    float4* n;    // normal, normal mass
+    float4* t1;   // tangent 1, tangent mass 1
+    float4* t2;   // tangent 2, tangent mass 2
+    float2* fb;   // combined friction, normal bias
+    float4* imp;  // normal, tangent 1, tangent 2 impulse
+};
+// Warm-start / bias records (constraint order within each group):
+struct WsRecBuf {
+    uint4* bd;    // body A, body B, flags, point index
+    float4* ra;   // stored rA (pointOnA - position A), inverse mass A
+    float4* rb;   // stored rB, inverse mass B
+    float4* qa;   // rotate(qA, localAnchorA), position bias
+    float4* qb;   // rotate(qB, localAnchorB), 0
+    float4* n;    // normal, normal mass
+    float4* t1;   // tangent 1, 0
+    float4* t2;   // tangent 2, 0
+    float4* imp;  // normal, tangent 1, tangent 2 impulse, bias impulse
 };
 
 struct SolveParams {

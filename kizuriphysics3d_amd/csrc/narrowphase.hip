@@ -191,6 +191,15 @@ struct SP {
     f3 a, b, p;
 };
 
+// Register-resident arrays of up to four support points: the index is
+// uniform across the wave, so these selects keep the arrays out of scratch.
+__device__ __forceinline__ void sp_set(SP* a, int i, const SP& v) {
+    if (i == 0) a[0] = v;
+    else if (i == 1) a[1] = v;
+    else if (i == 2) a[2] = v;
+    else a[3] = v;
+}
+
 struct WaveLds {
     // EPA polytope.
     float vx[kEpaMaxVerts * 9];           // SP records: a, b, p
@@ -198,6 +207,8 @@ struct WaveLds {
     uint32_t fabc[kEpaMaxFaces];          // a | b << 8 | c << 16
     uint16_t vis[kEpaMaxFaces];
     uint16_t hor[kEpaMaxHorizon];         // edge u | v << 8
+    // Exhaustive EPA seed candidates (EPA.cpp:116-146), 9 floats each.
+    float cand16[16][9];
     // Clipping scratch (written by lane 0 only).
     float poly[2][2 * kMaxFaceVerts][3];
     float cand[8][11];
@@ -334,11 +345,11 @@ __device__ bool reduce_simplex(SP* s, int& n, f3& closest) {
     } else if (n == 3) {
         float u, v, w;
         closest = closest_on_triangle(O, s[0].p, s[1].p, s[2].p, u, v, w);
-        SP keep[3] = {};
+        SP keep[4] = {};
         int k = 0;
-        if (u > 0.0f) keep[k++] = s[0];
-        if (v > 0.0f) keep[k++] = s[1];
-        if (w > 0.0f) keep[k++] = s[2];
+        if (u > 0.0f) sp_set(keep, k++, s[0]);
+        if (v > 0.0f) sp_set(keep, k++, s[1]);
+        if (w > 0.0f) sp_set(keep, k++, s[2]);
         if (k == 0) k = 1;
         for (int i = 0; i < 3; ++i)
             if (i < k) s[i] = keep[i];
@@ -355,7 +366,7 @@ __device__ bool reduce_simplex(SP* s, int& n, f3& closest) {
         const bool oBDC = outside(s[1].p, s[3].p, s[2].p, s[0].p);
         if (!oABC && !oACD && !oADB && !oBDC) return true;
         float best = kInf;
-        SP tri[3] = {};
+        SP tri[4] = {};
         int cnt = 0;
         const int fi[4][3] = {{0, 1, 2}, {0, 2, 3}, {0, 3, 1}, {1, 3, 2}};
         const bool fo[4] = {oABC, oACD, oADB, oBDC};
@@ -367,9 +378,9 @@ __device__ bool reduce_simplex(SP* s, int& n, f3& closest) {
             if (d2 < best) {
                 best = d2;
                 int k = 0;
-                if (u > 0.0f) tri[k++] = s[fi[f][0]];
-                if (v > 0.0f) tri[k++] = s[fi[f][1]];
-                if (w > 0.0f) tri[k++] = s[fi[f][2]];
+                if (u > 0.0f) sp_set(tri, k++, s[fi[f][0]]);
+                if (v > 0.0f) sp_set(tri, k++, s[fi[f][1]]);
+                if (w > 0.0f) sp_set(tri, k++, s[fi[f][2]]);
                 cnt = k;
                 closest = p;
             }
@@ -397,7 +408,7 @@ __device__ void gjk(const WShape& A, const WShape& B, float tol, Gjk& r) {
     r.pa = r.pb = r.normal = f3{0.0f, 0.0f, 0.0f};
     SP* s = r.s;
     int n = 0;
-    s[n++] = msupport(A, B, f3{1.0f, 0.0f, 0.0f});
+    sp_set(s, n++, msupport(A, B, f3{1.0f, 0.0f, 0.0f}));
     f3 closest = s[0].p;
     const float tol2 = tol * tol;
     for (int it = 0; it < 32; ++it) {
@@ -410,7 +421,7 @@ __device__ void gjk(const WShape& A, const WShape& B, float tol, Gjk& r) {
         for (int i = 0; i < 4; ++i)
             if (i < n && len2(sub(s[i].p, w.p)) < tol2) dup = true;
         if (dup) break;
-        s[n++] = w;
+        sp_set(s, n++, w);
         if (reduce_simplex(s, n, closest)) { r.intersect = true; r.distance = 0.0f; break; }
     }
     r.n = n;
@@ -454,7 +465,11 @@ __device__ bool origin_in_tetra(const SP* v) {
 }
 
 // BuildInitialTetra (EPA.cpp:58-147).
-__device__ bool initial_tetra(const WShape& A, const WShape& B, const Gjk& g, SP* out) {
+struct WaveLds;
+__device__ bool initial_tetra(const WShape& A, const WShape& B, const Gjk& g, SP* out, WaveLds& L);
+__device__ bool initial_tetra_fallback(const WShape& A, const WShape& B, const Gjk& g, SP* out, WaveLds& L);
+
+__device__ bool initial_tetra(const WShape& A, const WShape& B, const Gjk& g, SP* out, WaveLds& L) {
     const float tiny = 1.0e-12f;
     int n = 0;
     for (int i = 0; i < 4; ++i) {
@@ -462,9 +477,9 @@ __device__ bool initial_tetra(const WShape& A, const WShape& B, const Gjk& g, SP
         bool dup = false;
         for (int j = 0; j < 4; ++j)
             if (j < n && len2(sub(out[j].p, g.s[i].p)) < tiny) dup = true;
-        if (!dup) out[n++] = g.s[i];
+        if (!dup) sp_set(out, n++, g.s[i]);
     }
-    if (n == 0) out[n++] = msupport(A, B, f3{1.0f, 0.0f, 0.0f});
+    if (n == 0) sp_set(out, n++, msupport(A, B, f3{1.0f, 0.0f, 0.0f}));
     if (n == 4 && !origin_in_tetra(out)) n = 3;
     if (n == 1) {
         out[1] = msupport(A, B, neg(out[0].p));
@@ -485,7 +500,7 @@ __device__ bool initial_tetra(const WShape& A, const WShape& B, const Gjk& g, SP
             out[1] = k1;
         }
         const SP sp = msupport(A, B, perp(ab));
-        if (len2(sub(sp.p, out[0].p)) > tiny && len2(sub(sp.p, out[1].p)) > tiny) out[n++] = sp;
+        if (len2(sub(sp.p, out[0].p)) > tiny && len2(sub(sp.p, out[1].p)) > tiny) sp_set(out, n++, sp);
     }
     if (n == 3) {
         f3 tn = cross(sub(out[1].p, out[0].p), sub(out[2].p, out[0].p));
@@ -502,34 +517,7 @@ __device__ bool initial_tetra(const WShape& A, const WShape& B, const Gjk& g, SP
         n = 3;
     }
     if (n == 4 && origin_in_tetra(out)) return true;
-    // Exhaustive fallback (EPA.cpp:116-146).
-    SP cand[16];
-    int nc = 0;
-    auto addc = [&](const SP& sp) {
-        for (int i = 0; i < 16; ++i)
-            if (i < nc && len2(sub(cand[i].p, sp.p)) < tiny) return;
-        if (nc < 16) cand[nc++] = sp;
-    };
-    for (int i = 0; i < 4; ++i)
-        if (i < g.n) addc(g.s[i]);
-    const float s = 0.5773502692f;
-    const f3 dirs[14] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1},
-                         {s, s, s}, {s, s, -s}, {s, -s, s}, {-s, s, s},
-                         {-s, -s, s}, {-s, s, -s}, {s, -s, -s}, {-s, -s, -s}};
-    for (int i = 0; i < 14; ++i) addc(msupport(A, B, dirs[i]));
-    for (int i = 0; i + 3 < nc; ++i)
-        for (int j = i + 1; j + 2 < nc; ++j)
-            for (int k = j + 1; k + 1 < nc; ++k)
-                for (int l = k + 1; l < nc; ++l) {
-                    SP t[4] = {cand[i], cand[j], cand[k], cand[l]};
-                    const float vol = dot(cross(sub(t[1].p, t[0].p), sub(t[2].p, t[0].p)), sub(t[3].p, t[0].p));
-                    if (fabs_(vol) < tiny) continue;
-                    if (origin_in_tetra(t)) {
-                        for (int m = 0; m < 4; ++m) out[m] = t[m];
-                        return true;
-                    }
-                }
-    return false;
+    return initial_tetra_fallback(A, B, g, out, L);
 }
 
 struct Epa {
@@ -607,6 +595,54 @@ __device__ __forceinline__ void lds_write_vert(WaveLds& L, int i, const SP& s) {
     }
 }
 
+
+// Exhaustive fallback of BuildInitialTetra (EPA.cpp:116-146): candidates are
+// the GJK simplex and the supports along 14 fixed directions (duplicates
+// dropped), the first enclosing tetrahedron in i<j<k<l order wins.  The
+// candidate list lives in LDS (written by lane 0, read by all lanes).
+__device__ __noinline__ bool initial_tetra_fallback(const WShape& A, const WShape& B, const Gjk& g, SP* out,
+                                                    WaveLds& L) {
+    const float tiny = 1.0e-12f;
+    int nc = 0;
+    auto cand = [&](int i) {
+        const float* v = L.cand16[i];
+        return SP{f3{v[0], v[1], v[2]}, f3{v[3], v[4], v[5]}, f3{v[6], v[7], v[8]}};
+    };
+    auto addc = [&](const SP& sp) {
+        for (int i = 0; i < nc; ++i)
+            if (len2(sub(cand(i).p, sp.p)) < tiny) return;
+        if (nc < 16) {
+            if (lane_id() == 0) {
+                float* v = L.cand16[nc];
+                v[0] = sp.a.x; v[1] = sp.a.y; v[2] = sp.a.z; v[3] = sp.b.x; v[4] = sp.b.y; v[5] = sp.b.z;
+                v[6] = sp.p.x; v[7] = sp.p.y; v[8] = sp.p.z;
+            }
+            __syncthreads();
+            ++nc;
+        }
+    };
+    for (int i = 0; i < 4; ++i)
+        if (i < g.n) addc(g.s[i]);
+    const float s = 0.5773502692f;
+    const f3 dirs[14] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1},
+                         {s, s, s}, {s, s, -s}, {s, -s, s}, {-s, s, s},
+                         {-s, -s, s}, {-s, s, -s}, {s, -s, -s}, {-s, -s, -s}};
+    for (int i = 0; i < 14; ++i) addc(msupport(A, B, dirs[i]));
+    for (int i = 0; i + 3 < nc; ++i)
+        for (int j = i + 1; j + 2 < nc; ++j)
+            for (int k = j + 1; k + 1 < nc; ++k)
+                for (int l = k + 1; l < nc; ++l) {
+                    SP t[4] = {cand(i), cand(j), cand(k), cand(l)};
+                    const float vol = dot(cross(sub(t[1].p, t[0].p), sub(t[2].p, t[0].p)), sub(t[3].p, t[0].p));
+                    if (fabs_(vol) < tiny) continue;
+                    if (origin_in_tetra(t)) {
+                        for (int m = 0; m < 4; ++m) out[m] = t[m];
+                        return true;
+                    }
+                }
+    return false;
+}
+
 __device__ __forceinline__ uint32_t prefix_count(unsigned long long mask) {
     // number of set bits below this lane
     const int lane = lane_id();
@@ -623,7 +659,7 @@ __device__ void epa(const WShape& A, const WShape& B, const Gjk& g, float tol, W
     r.normal = f3{0.0f, 1.0f, 0.0f};
     r.pa = r.pb = f3{0.0f, 0.0f, 0.0f};
     SP tet[4];
-    if (!initial_tetra(A, B, g, tet)) return;
+    if (!initial_tetra(A, B, g, tet, L)) return;
     for (int i = 0; i < 4; ++i) lds_write_vert(L, i, tet[i]);
     __syncthreads();
     int nv = 4, nf = 0;

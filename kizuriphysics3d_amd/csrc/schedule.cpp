@@ -21,7 +21,6 @@
 
 #include <algorithm>
 #include <queue>
-#include <set>
 #include <unordered_map>
 
 namespace kz {
@@ -40,7 +39,7 @@ struct Batch {
 
 // Bucket (group, level) -> flat arrays.
 void flatten(const std::vector<uint32_t>& pts, const std::vector<uint32_t>& grp_of, const std::vector<uint32_t>& lvl_of,
-             uint32_t groups, LevelSchedule& out) {
+             uint32_t groups, uint32_t num_points, LevelSchedule& out) {
     std::vector<uint32_t> depth(groups, 0);
     for (size_t i = 0; i < pts.size(); ++i) depth[grp_of[i]] = std::max(depth[grp_of[i]], lvl_of[i]);
     out.grp.assign(groups, U2{0, 0});
@@ -63,13 +62,20 @@ void flatten(const std::vector<uint32_t>& pts, const std::vector<uint32_t>& grp_
     }
     out.idx.assign(pts.size(), 0);
     std::vector<uint32_t> fill(out.off.begin(), out.off.end() - 1);
-    for (size_t i = 0; i < pts.size(); ++i) out.idx[fill[base[grp_of[i]] + lvl_of[i] - 1]++] = pts[i];
+    out.pos.assign(num_points, 0xFFFFFFFFu);
+    for (size_t i = 0; i < pts.size(); ++i) {
+        const uint32_t e = fill[base[grp_of[i]] + lvl_of[i] - 1]++;
+        out.idx[e] = pts[i];
+        out.pos[pts[i]] = e;
+    }
 }
 
 }  // namespace
 
 void ScheduleBuilder::build(const ScheduleInput& in, ScheduleOutput& out) {
     const uint32_t nb = in.num_bodies, nc = in.num_constraints;
+    uint32_t num_points = 0;
+    for (uint32_t c = 0; c < nc; ++c) num_points += in.info[c].w & 0xFFu;
     const uint8_t* dyn = in.dynamic;
     // ---- islands -----------------------------------------------------------
     dyn_index_.assign(nb, 0xFFFFFFFFu);
@@ -154,59 +160,84 @@ void ScheduleBuilder::build(const ScheduleInput& in, ScheduleOutput& out) {
             lvl.push_back(l);
         }
     }
-    flatten(pts, grp, lvl, groups, out.ws);
+    flatten(pts, grp, lvl, groups, num_points, out.ws);
     // ---- velocity sequence: first-fit batches per island --------------------
     pts.clear(); grp.clear(); lvl.clear();
     if (in.use_simd) {
         last_.assign(nb, 0);
         std::vector<Batch> batches;
+        // Non-full batches as ordered lists with lazy deletion: `open` holds
+        // every batch in creation order (full ones are skipped and dropped
+        // from the front); open_wo[h] holds, for a non-dynamic body h, the
+        // batches created without h.  Batches h later joins or that fill up
+        // are skipped on the way and compacted away when they dominate.
+        struct OpenList {
+            std::vector<uint32_t> ids;
+            size_t head = 0;
+        };
+        OpenList open;
+        // One list per non-dynamic body met in the island (a handful: the
+        // static floor and walls), searched linearly.
+        std::vector<std::pair<uint32_t, OpenList>> open_wo;
         for (uint32_t is = 0; is < ni; ++is) {
             batches.clear();
-            std::set<uint32_t> open;                                   // non-full batches
-            std::unordered_map<uint32_t, std::set<uint32_t>> open_wo; // non-full batches without body h
-            auto open_without = [&](uint32_t h) -> std::set<uint32_t>& {
-                auto it = open_wo.find(h);
-                if (it != open_wo.end()) return it->second;
-                std::set<uint32_t>& s = open_wo[h];
-                for (uint32_t bi : open)
-                    if (!batches[bi].has(h)) s.insert(bi);
-                return s;
+            open.ids.clear();
+            open.head = 0;
+            open_wo.clear();
+            auto open_without = [&](uint32_t h) -> OpenList& {
+                for (auto& kv : open_wo)
+                    if (kv.first == h) return kv.second;
+                open_wo.emplace_back(h, OpenList{});
+                OpenList& l = open_wo.back().second;
+                for (size_t k = open.head; k < open.ids.size(); ++k) {
+                    const Batch& B = batches[open.ids[k]];
+                    if (B.n < 4 && !B.has(h)) l.ids.push_back(open.ids[k]);
+                }
+                return l;
+            };
+            // First usable batch of a list for (a, b); drops dead entries.
+            auto first_fit = [&](OpenList& l, uint32_t a, uint32_t b) -> uint32_t {
+                while (l.head < l.ids.size() && batches[l.ids[l.head]].n >= 4) ++l.head;
+                size_t dead = 0;
+                uint32_t found = 0xFFFFFFFFu;
+                for (size_t k = l.head; k < l.ids.size(); ++k) {
+                    const Batch& B = batches[l.ids[k]];
+                    if (B.n >= 4) { ++dead; continue; }
+                    if (B.has(a) || B.has(b)) continue;
+                    found = l.ids[k];
+                    break;
+                }
+                if (dead > 64 && dead * 2 > l.ids.size() - l.head) {
+                    size_t w = 0;
+                    for (size_t k = l.head; k < l.ids.size(); ++k)
+                        if (batches[l.ids[k]].n < 4) l.ids[w++] = l.ids[k];
+                    l.ids.resize(w);
+                    l.head = 0;
+                }
+                return found;
             };
             for (uint32_t c : island_cons_[is]) {
                 const U4& ci = in.info[c];
                 if ((ci.w >> 8) & 1u) continue;  // sensor
                 const uint32_t a = ci.x, b = ci.y, np = ci.w & 0xFFu;
                 for (uint32_t k = 0; k < np; ++k) {
-                    const std::set<uint32_t>* cand = &open;
+                    OpenList* cand = &open;
                     if (!dyn[a]) cand = &open_without(a);
                     else if (!dyn[b]) cand = &open_without(b);
-                    uint32_t placed = 0xFFFFFFFFu;
-                    for (uint32_t bi : *cand) {
-                        const Batch& B = batches[bi];
-                        if (B.has(a) || B.has(b)) continue;
-                        placed = bi;
-                        break;
-                    }
+                    uint32_t placed = first_fit(*cand, a, b);
                     if (placed == 0xFFFFFFFFu) {
                         Batch nb2;
                         nb2.a[0] = a; nb2.b[0] = b; nb2.pt[0] = ci.z + k; nb2.n = 1;
                         const uint32_t id = uint32_t(batches.size());
                         batches.push_back(nb2);
-                        open.insert(id);
+                        open.ids.push_back(id);
                         for (auto& kv : open_wo)
-                            if (kv.first != a && kv.first != b) kv.second.insert(id);
+                            if (kv.first != a && kv.first != b) kv.second.ids.push_back(id);
                     } else {
                         Batch& B = batches[placed];
                         B.a[B.n] = a; B.b[B.n] = b; B.pt[B.n] = ci.z + k; ++B.n;
-                        if (B.n >= 4) {
-                            open.erase(placed);
-                            for (auto& kv : open_wo) kv.second.erase(placed);
-                        } else {
-                            for (uint32_t h : {a, b}) {
-                                auto it = open_wo.find(h);
-                                if (it != open_wo.end()) it->second.erase(placed);
-                            }
-                        }
+                        // Entries of open_wo[a] / open_wo[b] now containing a or
+                        // b are skipped by the conflict test on later visits.
                     }
                 }
             }
@@ -223,7 +254,8 @@ void ScheduleBuilder::build(const ScheduleInput& in, ScheduleOutput& out) {
                 }
         }
     }
-    flatten(pts, grp, lvl, groups, out.vel);
+    if (in.use_simd) flatten(pts, grp, lvl, groups, num_points, out.vel);
+    else out.vel = out.ws;  // the scalar path sweeps in constraint order, sensors included
 }
 
 }  // namespace kz

@@ -16,9 +16,11 @@ struct U2 {
 };
 
 // One level-scheduled sequence: idx holds point indices grouped by level,
-// off[l]..off[l+1] is level l, grp[g] = [first level, end level) of group g.
+// off[l]..off[l+1] is level l, grp[g] = [first level, end level) of group g,
+// pos[p] = position of point p in idx (0xFFFFFFFF when absent).
 struct LevelSchedule {
     std::vector<uint32_t> idx;
+    std::vector<uint32_t> pos;
     std::vector<uint32_t> off;
     std::vector<U2> grp;
     uint32_t levels = 0;      // total levels over all groups

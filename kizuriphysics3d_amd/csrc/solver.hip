@@ -118,6 +118,7 @@ __global__ void k_build(ContactBuf c, ManifoldBuf m, BodyBuf b, CacheBuf cache, 
 // ---------------------------------------------------------------------------
 struct PrepParams {
     float dt, restitution_threshold, baumgarte, slop, max_bias;
+    int use_simd, use_velocity_bias;
 };
 __device__ __forceinline__ float eff_mass(float ima, float imb, const m3& ia, const m3& ib, f3 ra, f3 rb, f3 d) {
     f3 xa = cross(ra, d), xb = cross(rb, d);
@@ -126,7 +127,8 @@ __device__ __forceinline__ float eff_mass(float ima, float imb, const m3& ia, co
     k = k + dot(xb, mmul(ib, xb));
     return k > kEps ? 1.0f / k : 0.0f;
 }
-__global__ void k_prepare(ContactBuf c, BodyBuf b, uint32_t np, PrepParams pp) {
+__global__ void k_prepare(ContactBuf c, BodyBuf b, uint32_t np, PrepParams pp, VelRecBuf vr, WsRecBuf wr,
+                          const uint32_t* vel_pos, const uint32_t* ws_pos) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     const uint4 bd = c.body[p];
@@ -176,45 +178,73 @@ __global__ void k_prepare(ContactBuf c, BodyBuf b, uint32_t np, PrepParams pp) {
     c.lb[p] = st3(lb, cr);
     c.imp[p] = imp;
     c.nvel[p] = nvel;
+    // Sweep records.  Body rotations do not change during the solve, so the
+    // anchors the reference rotates on every visit (ContactSolverSIMD.cpp:
+    // 106-107, ContactSolver.cpp:268-269) are rotated once here: same values.
+    const f3 qra = rotate(qa, la), qrb = rotate(qb, lb);
+    const uint32_t e = vel_pos[p];
+    if (e != 0xFFFFFFFFu) {
+        // SIMD lanes add (positionBias or 0) to the restitution bias; the
+        // scalar path adds positionBias only when enabled (ContactSolverSIMD.cpp:128,
+        // ContactSolver.cpp:169-170).
+        const float nbias = pp.use_simd ? rbias + (pp.use_velocity_bias ? pbias : 0.0f)
+                                        : (pp.use_velocity_bias ? rbias + pbias : rbias);
+        vr.bd[e] = bdo;
+        vr.ra[e] = st3(qra, xa.w);
+        vr.rb[e] = st3(qrb, xb.w);
+        vr.n[e] = st3(n, nm);
+        vr.t1[e] = st3(t1, tm1);
+        vr.t2[e] = st3(t2, tm2);
+        vr.fb[e] = make_float2(cf, nbias);
+        vr.imp[e] = make_float4(imp.x, imp.y, imp.z, 0.0f);
+    }
+    const uint32_t w = ws_pos[p];
+    wr.bd[w] = bdo;
+    wr.ra[w] = st3(ra, xa.w);
+    wr.rb[w] = st3(rb, xb.w);
+    wr.qa[w] = st3(qra, pbias);
+    wr.qb[w] = st3(qrb, 0.0f);
+    wr.n[w] = st3(n, nm);
+    wr.t1[w] = st3(t1, 0.0f);
+    wr.t2[w] = st3(t2, 0.0f);
+    wr.imp[w] = imp;
 }
 
 // ---------------------------------------------------------------------------
-// Per-point sweep bodies.
+// Per-point sweep bodies, reading sweep records.
 // ---------------------------------------------------------------------------
 // WarmStart (ContactSolver.cpp:117-132): stored rA / rB.
-__device__ __forceinline__ void warm_point(const ContactBuf& c, const BodyBuf& b, uint32_t p, float wsf) {
-    const uint4 bd = c.body[p];
-    const float4 imp = c.imp[p];
-    const f3 n = ld3(c.n[p]), t1 = ld3(c.t1[p]), t2 = ld3(c.t2[p]);
+__device__ __forceinline__ void warm_point(const WsRecBuf& r, const BodyBuf& b, uint32_t e, uint4 bd, float wsf) {
+    const float4 imp = r.imp[e];
+    const f3 n = ld3(r.n[e]), t1 = ld3(r.t1[e]), t2 = ld3(r.t2[e]);
     f3 j = add(add(mul(n, imp.x), mul(t1, imp.y)), mul(t2, imp.z));
     j = mul(j, wsf);
     if (bd.z & 2u) {
-        const float ima = b.x[bd.x].w;
+        const float4 ra = r.ra[e];
         const m3 ia = ldm(b.iw + 3 * bd.x);
-        b.v[bd.x] = st3(add(ld3(b.v[bd.x]), mul(j, ima)), 0.0f);
-        b.w[bd.x] = st3(add(ld3(b.w[bd.x]), mmul(ia, cross(ld3(c.ra[p]), j))), 0.0f);
+        const f3 v = ld3(b.v[bd.x]), w = ld3(b.w[bd.x]);
+        b.v[bd.x] = st3(add(v, mul(j, ra.w)), 0.0f);
+        b.w[bd.x] = st3(add(w, mmul(ia, cross(ld3(ra), j))), 0.0f);
     }
     if (bd.z & 4u) {
-        const float imb = b.x[bd.y].w;
+        const float4 rb = r.rb[e];
         const m3 ib = ldm(b.iw + 3 * bd.y);
-        b.v[bd.y] = st3(sub(ld3(b.v[bd.y]), mul(j, imb)), 0.0f);
-        b.w[bd.y] = st3(sub(ld3(b.w[bd.y]), mmul(ib, cross(ld3(c.rb[p]), j))), 0.0f);
+        const f3 v = ld3(b.v[bd.y]), w = ld3(b.w[bd.y]);
+        b.v[bd.y] = st3(sub(v, mul(j, rb.w)), 0.0f);
+        b.w[bd.y] = st3(sub(w, mmul(ib, cross(ld3(rb), j))), 0.0f);
     }
 }
 
 // One lane of SolveVelocityBatchSIMD (ContactSolverSIMD.cpp:93-256).
 // _mm_max_ps(a, b) = a > b ? a : b and _mm_min_ps(a, b) = a < b ? a : b.
-__device__ __forceinline__ float velocity_point(const ContactBuf& c, const BodyBuf& b, uint32_t p, bool vbias) {
-    const uint4 bd = c.body[p];
-    const float4 xa = b.x[bd.x], xb = b.x[bd.y];
-    const f3 ra = rotate(ldq(b.q[bd.x]), ld3(c.la[p]));
-    const f3 rb = rotate(ldq(b.q[bd.y]), ld3(c.lb[p]));
+__device__ __forceinline__ float velocity_point(const VelRecBuf& r, const BodyBuf& b, uint32_t e, uint4 bd) {
+    const float4 ra4 = r.ra[e], rb4 = r.rb[e], n4 = r.n[e], t14 = r.t1[e], t24 = r.t2[e];
+    const float2 fb = r.fb[e];
+    float4 imp = r.imp[e];
     const m3 ia = ldm(b.iw + 3 * bd.x), ib = ldm(b.iw + 3 * bd.y);
     f3 va = ld3(b.v[bd.x]), wa = ld3(b.w[bd.x]), vb = ld3(b.v[bd.y]), wb = ld3(b.w[bd.y]);
-    const float4 n4 = c.n[p], t14 = c.t1[p], t24 = c.t2[p];
-    float4 imp = c.imp[p];
-    const float cf = c.la[p].w;
-    const float nbias = c.ra[p].w + (vbias ? c.rb[p].w : 0.0f);
+    const f3 ra = ld3(ra4), rb = ld3(rb4);
+    const float ima = ra4.w, imb = rb4.w, cf = fb.x, nbias = fb.y;
     const float zero = 0.0f;
     float dsq = 0.0f;
     auto rel = [&]() {
@@ -222,9 +252,9 @@ __device__ __forceinline__ float velocity_point(const ContactBuf& c, const BodyB
         return f3{(va.x + xa2.x) - (vb.x + xb2.x), (va.y + xa2.y) - (vb.y + xb2.y), (va.z + xa2.z) - (vb.z + xb2.z)};
     };
     auto apply = [&](f3 j) {
-        va = f3{va.x + j.x * xa.w, va.y + j.y * xa.w, va.z + j.z * xa.w};
+        va = f3{va.x + j.x * ima, va.y + j.y * ima, va.z + j.z * ima};
         wa = add(wa, mmul(ia, cross(ra, j)));
-        vb = f3{vb.x - j.x * xb.w, vb.y - j.y * xb.w, vb.z - j.z * xb.w};
+        vb = f3{vb.x - j.x * imb, vb.y - j.y * imb, vb.z - j.z * imb};
         wb = sub(wb, mmul(ib, cross(rb, j)));
     };
 #pragma unroll
@@ -257,26 +287,23 @@ __device__ __forceinline__ float velocity_point(const ContactBuf& c, const BodyB
     }
     if (bd.z & 2u) { b.v[bd.x] = st3(va, 0.0f); b.w[bd.x] = st3(wa, 0.0f); }
     if (bd.z & 4u) { b.v[bd.y] = st3(vb, 0.0f); b.w[bd.y] = st3(wb, 0.0f); }
-    c.imp[p] = imp;
+    r.imp[e] = imp;
     return dsq;
 }
 
 // SolveContactVelocity, the scalar path (ContactSolver.cpp:137-181).
-__device__ __forceinline__ float scalar_point(const ContactBuf& c, const BodyBuf& b, uint32_t p, bool vbias) {
-    const uint4 bd = c.body[p];
-    const float4 xa = b.x[bd.x], xb = b.x[bd.y];
-    const f3 ra = rotate(ldq(b.q[bd.x]), ld3(c.la[p]));
-    const f3 rb = rotate(ldq(b.q[bd.y]), ld3(c.lb[p]));
+__device__ __forceinline__ float scalar_point(const VelRecBuf& r, const BodyBuf& b, uint32_t e, uint4 bd) {
+    const float4 ra4 = r.ra[e], rb4 = r.rb[e], n4 = r.n[e], t14 = r.t1[e], t24 = r.t2[e];
+    const float2 fb = r.fb[e];
+    float4 imp = r.imp[e];
     const m3 ia = ldm(b.iw + 3 * bd.x), ib = ldm(b.iw + 3 * bd.y);
     f3 va = ld3(b.v[bd.x]), wa = ld3(b.w[bd.x]), vb = ld3(b.v[bd.y]), wb = ld3(b.w[bd.y]);
-    const float4 n4 = c.n[p], t14 = c.t1[p], t24 = c.t2[p];
-    float4 imp = c.imp[p];
-    const float cf = c.la[p].w;
+    const f3 ra = ld3(ra4), rb = ld3(rb4);
     const bool da = bd.z & 2u, db = bd.z & 4u;
     float dsq = 0.0f;
     auto apply = [&](f3 j) {
-        if (da) { va = add(va, mul(j, xa.w)); wa = add(wa, mmul(ia, cross(ra, j))); }
-        if (db) { vb = sub(vb, mul(j, xb.w)); wb = sub(wb, mmul(ib, cross(rb, j))); }
+        if (da) { va = add(va, mul(j, ra4.w)); wa = add(wa, mmul(ia, cross(ra, j))); }
+        if (db) { vb = sub(vb, mul(j, rb4.w)); wb = sub(wb, mmul(ib, cross(rb, j))); }
     };
     f3 vr = sub(add(va, cross(wa, ra)), add(vb, cross(wb, rb)));
 #pragma unroll
@@ -285,7 +312,7 @@ __device__ __forceinline__ float scalar_point(const ContactBuf& c, const BodyBuf
         const float mass = t == 0 ? t14.w : t24.w;
         float acc = t == 0 ? imp.y : imp.z;
         float lam = -mass * dot(vr, tv);
-        const float mf = cf * imp.x;
+        const float mf = fb.x * imp.x;
         const float ni = fclamp(acc + lam, -mf, mf);
         lam = ni - acc;
         if (t == 0) imp.y = ni; else imp.z = ni;
@@ -293,9 +320,7 @@ __device__ __forceinline__ float scalar_point(const ContactBuf& c, const BodyBuf
         apply(mul(tv, lam));
     }
     vr = sub(add(va, cross(wa, ra)), add(vb, cross(wb, rb)));
-    float bias = c.ra[p].w;
-    if (vbias) bias = bias + c.rb[p].w;
-    float lam = -n4.w * (dot(vr, ld3(n4)) - bias);
+    float lam = -n4.w * (dot(vr, ld3(n4)) - fb.y);
     const float ni = fmax_(imp.x + lam, 0.0f);
     lam = ni - imp.x;
     imp.x = ni;
@@ -303,38 +328,61 @@ __device__ __forceinline__ float scalar_point(const ContactBuf& c, const BodyBuf
     apply(mul(ld3(n4), lam));
     if (da) { b.v[bd.x] = st3(va, 0.0f); b.w[bd.x] = st3(wa, 0.0f); }
     if (db) { b.v[bd.y] = st3(vb, 0.0f); b.w[bd.y] = st3(wb, 0.0f); }
-    c.imp[p] = imp;
+    r.imp[e] = imp;
     return dsq;
 }
 
 // SolveBiasConstraint (ContactSolver.cpp:257-290).
-__device__ __forceinline__ void bias_point(const ContactBuf& c, const BodyBuf& b, uint32_t p) {
-    const uint4 bd = c.body[p];
-    const float4 xa = b.x[bd.x], xb = b.x[bd.y];
-    if (xa.w <= 0.0f && xb.w <= 0.0f) return;
-    const float pbias = c.rb[p].w;
+__device__ __forceinline__ void bias_point(const WsRecBuf& r, const BodyBuf& b, uint32_t e, uint4 bd) {
+    const float4 qa4 = r.qa[e];
+    const float pbias = qa4.w;
+    const float ima = r.ra[e].w, imb = r.rb[e].w;
+    if (ima <= 0.0f && imb <= 0.0f) return;
     if (pbias <= 0.0f) return;
-    const f3 ra = rotate(ldq(b.q[bd.x]), ld3(c.la[p]));
-    const f3 rb = rotate(ldq(b.q[bd.y]), ld3(c.lb[p]));
+    const f3 ra = ld3(qa4), rb = ld3(r.qb[e]);
     const f3 pva = ld3(b.pv[bd.x]), pwa = ld3(b.pw[bd.x]), pvb = ld3(b.pv[bd.y]), pwb = ld3(b.pw[bd.y]);
     const f3 vr = sub(add(pva, cross(pwa, ra)), add(pvb, cross(pwb, rb)));
-    const float4 n4 = c.n[p];
-    float4 imp = c.imp[p];
+    const float4 n4 = r.n[e];
+    float4 imp = r.imp[e];
     float lam = -n4.w * (dot(vr, ld3(n4)) - pbias);
     const float ni = fmax_(imp.w + lam, 0.0f);
     lam = ni - imp.w;
     imp.w = ni;
-    c.imp[p] = imp;
+    r.imp[e] = imp;
     const f3 j = mul(ld3(n4), lam);
     if (bd.z & 2u) {
         const m3 ia = ldm(b.iw + 3 * bd.x);
-        b.pv[bd.x] = st3(add(pva, mul(j, xa.w)), 0.0f);
+        b.pv[bd.x] = st3(add(pva, mul(j, ima)), 0.0f);
         b.pw[bd.x] = st3(add(pwa, mmul(ia, cross(ra, j))), 0.0f);
     }
     if (bd.z & 4u) {
         const m3 ib = ldm(b.iw + 3 * bd.y);
-        b.pv[bd.y] = st3(sub(pvb, mul(j, xb.w)), 0.0f);
+        b.pv[bd.y] = st3(sub(pvb, mul(j, imb)), 0.0f);
         b.pw[bd.y] = st3(sub(pwb, mmul(ib, cross(rb, j))), 0.0f);
+    }
+}
+
+// Level-by-level sweep of one group's range of levels.  The body indices of
+// the next level are fetched before the current level is processed (they
+// are constant during the solve), so after each barrier a point waits for
+// one round of loads only.
+template <class F>
+__device__ __forceinline__ void sweep(const Schedule& s, uint2 range, const uint4* bd, F&& f) {
+    const uint32_t t = threadIdx.x, nt = blockDim.x;
+    uint4 nxt = make_uint4(0, 0, 0, 0);
+    if (range.x < range.y) {
+        const uint32_t e = s.lvl_off[range.x] + t;
+        if (e < s.lvl_off[range.x + 1]) nxt = bd[e];
+    }
+    for (uint32_t l = range.x; l < range.y; ++l) {
+        const uint32_t e0 = s.lvl_off[l] + t, e1 = s.lvl_off[l + 1];
+        const uint4 cur = nxt;
+        if (l + 1 < range.y) {
+            const uint32_t en = e1 + t;
+            if (en < s.lvl_off[l + 2]) nxt = bd[en];
+        }
+        for (uint32_t e = e0; e < e1; e += nt) f(e, e == e0 ? cur : bd[e]);
+        __syncthreads();
     }
 }
 
@@ -347,40 +395,40 @@ __device__ __forceinline__ void bias_point(const ContactBuf& c, const BodyBuf& b
 // needed between levels).  `resid` (optional) receives the sum of squared
 // impulse increments of each velocity sweep.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_solve_groups(ContactBuf c, BodyBuf b, Schedule ws, Schedule vel,
-                                                       SolveParams sp, float* resid) {
-    const uint2 wr = ws.grp[blockIdx.x];
-    const uint2 vr = vel.grp[blockIdx.x];
-    for (uint32_t l = wr.x; l < wr.y; ++l) {
-        for (uint32_t e = ws.lvl_off[l] + threadIdx.x; e < ws.lvl_off[l + 1]; e += blockDim.x)
-            warm_point(c, b, ws.idx[e], sp.warm_start_factor);
-        __syncthreads();
-    }
-    const bool vbias = sp.use_velocity_bias != 0;
-    const Schedule& vs = sp.use_simd ? vel : ws;
-    const uint2 vrange = sp.use_simd ? vr : wr;
+__global__ void __launch_bounds__(1024) k_solve_groups(VelRecBuf vr, WsRecBuf wr, BodyBuf b, Schedule ws,
+                                                       Schedule vel, SolveParams sp, float* resid) {
+    const uint2 wrange = ws.grp[blockIdx.x];
+    const uint2 vrange = vel.grp[blockIdx.x];
+    const float wsf = sp.warm_start_factor;
+    sweep(ws, wrange, wr.bd, [&](uint32_t e, uint4 bd) { warm_point(wr, b, e, bd, wsf); });
     for (uint32_t it = 0; it < sp.velocity_iterations; ++it) {
         float dsq = 0.0f;
-        for (uint32_t l = vrange.x; l < vrange.y; ++l) {
-            for (uint32_t e = vs.lvl_off[l] + threadIdx.x; e < vs.lvl_off[l + 1]; e += blockDim.x) {
-                // Sensor points are absent from the SIMD schedule (the batch
-                // builder skips them) and present in the scalar one.
-                const uint32_t p = vs.idx[e];
-                dsq += sp.use_simd ? velocity_point(c, b, p, vbias) : scalar_point(c, b, p, vbias);
-            }
-            __syncthreads();
-        }
+        if (sp.use_simd)
+            sweep(vel, vrange, vr.bd, [&](uint32_t e, uint4 bd) { dsq += velocity_point(vr, b, e, bd); });
+        else
+            sweep(vel, vrange, vr.bd, [&](uint32_t e, uint4 bd) { dsq += scalar_point(vr, b, e, bd); });
         if (resid && dsq != 0.0f) atomicAdd(resid + it, dsq);
     }
-    if (!vbias) {
-        for (uint32_t it = 0; it < sp.position_iterations; ++it) {
-            for (uint32_t l = wr.x; l < wr.y; ++l) {
-                for (uint32_t e = ws.lvl_off[l] + threadIdx.x; e < ws.lvl_off[l + 1]; e += blockDim.x)
-                    bias_point(c, b, ws.idx[e]);
-                __syncthreads();
-            }
-        }
+    if (!sp.use_velocity_bias)
+        for (uint32_t it = 0; it < sp.position_iterations; ++it)
+            sweep(ws, wrange, wr.bd, [&](uint32_t e, uint4 bd) { bias_point(wr, b, e, bd); });
+}
+
+// Final impulses back to constraint-point order (for the cache store and
+// read-back): velocity impulses from the velocity records, the bias impulse
+// from the warm-start/bias records.
+__global__ void k_writeback(VelRecBuf vr, WsRecBuf wr, ContactBuf c, const uint32_t* vel_pos,
+                            const uint32_t* ws_pos, uint32_t np) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    float4 imp = c.imp[p];
+    const uint32_t e = vel_pos[p];
+    if (e != 0xFFFFFFFFu) {
+        const float4 v = vr.imp[e];
+        imp.x = v.x; imp.y = v.y; imp.z = v.z;
     }
+    imp.w = wr.imp[ws_pos[p]].w;
+    c.imp[p] = imp;
 }
 
 // ---------------------------------------------------------------------------
@@ -395,6 +443,7 @@ __global__ void k_cache_store(ContactBuf c, BodyBuf b, CacheBuf cache, uint32_t 
     uint32_t mask = cache.cap - 1u, s = cache_hash(key) & mask;
     for (uint32_t probe = 0; probe < cache.cap; ++probe) {
         unsigned long long prev = atomicCAS(cache.key + s, 0ull, key);
+        if (prev == 0ull) atomicAdd(cache.used, 1u);
         if (prev == 0ull || prev == key) break;
         s = (s + 1u) & mask;
     }
@@ -416,7 +465,10 @@ __global__ void k_cache_rehash(CacheBuf src, CacheBuf dst) {
     if (key == 0ull) return;
     uint32_t mask = dst.cap - 1u, s = cache_hash(key) & mask;
     for (uint32_t probe = 0; probe < dst.cap; ++probe) {
-        if (atomicCAS(dst.key + s, 0ull, key) == 0ull) break;
+        if (atomicCAS(dst.key + s, 0ull, key) == 0ull) {
+            atomicAdd(dst.used, 1u);
+            break;
+        }
         s = (s + 1u) & mask;
     }
     dst.count[s] = src.count[i];
@@ -485,14 +537,19 @@ void launch_build(hipStream_t s, const ContactBuf& c, const ManifoldBuf& m, cons
     if (nc) k_build<<<blocks(nc, 256), 256, 0, s>>>(c, m, b, cache, nc, warm);
 }
 void launch_prepare(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32_t np, float dt, float rthr,
-                    float baum, float slop, float maxb) {
-    PrepParams pp{dt, rthr, baum, slop, maxb};
-    if (np) k_prepare<<<blocks(np, 256), 256, 0, s>>>(c, b, np, pp);
+                    float baum, float slop, float maxb, int use_simd, int use_vbias, const VelRecBuf& vr,
+                    const WsRecBuf& wr, const uint32_t* vel_pos, const uint32_t* ws_pos) {
+    PrepParams pp{dt, rthr, baum, slop, maxb, use_simd, use_vbias};
+    if (np) k_prepare<<<blocks(np, 256), 256, 0, s>>>(c, b, np, pp, vr, wr, vel_pos, ws_pos);
 }
-void launch_solve_groups(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const Schedule& ws,
-                         const Schedule& vel, const SolveParams& sp, uint32_t groups, uint32_t threads,
-                         float* resid) {
-    if (groups) k_solve_groups<<<groups, threads, 0, s>>>(c, b, ws, vel, sp, resid);
+void launch_writeback(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const ContactBuf& c,
+                      const uint32_t* vel_pos, const uint32_t* ws_pos, uint32_t np) {
+    if (np) k_writeback<<<blocks(np, 256), 256, 0, s>>>(vr, wr, c, vel_pos, ws_pos, np);
+}
+void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b,
+                         const Schedule& ws, const Schedule& vel, const SolveParams& sp, uint32_t groups,
+                         uint32_t threads, float* resid) {
+    if (groups) k_solve_groups<<<groups, threads, 0, s>>>(vr, wr, b, ws, vel, sp, resid);
 }
 void launch_cache_store(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const CacheBuf& cache, uint32_t nc) {
     if (nc) k_cache_store<<<blocks(nc, 256), 256, 0, s>>>(c, b, cache, nc);

@@ -35,7 +35,7 @@ EXPORTS = [
     "kzg_create", "kzg_destroy", "kzg_last_error", "kzg_upload_shapes", "kzg_upload_bodies", "kzg_upload_states",
     "kzg_narrowphase", "kzg_build_constraints", "kzg_integrate_velocities", "kzg_solve", "kzg_integrate_positions",
     "kzg_step", "kzg_query_sizes", "kzg_download_states", "kzg_download_manifolds", "kzg_download_constraints",
-    "kzg_get_residuals", "kzg_get_timings", "kzg_collide_batch",
+    "kzg_get_residuals", "kzg_get_timings", "kzg_collide_batch", "kzg_schedule",
 ]
 
 _lib = None
@@ -66,6 +66,7 @@ def library():
             "kzg_download_manifolds": ([vp, vp, vp], i32), "kzg_download_constraints": ([vp, vp, vp], i32),
             "kzg_get_residuals": ([vp, vp, u32], i32), "kzg_get_timings": ([vp, vp], i32),
             "kzg_collide_batch": ([vp, vp, vp, vp, vp, u32, f32, vp, vp, vp], i32),
+            "kzg_schedule": ([u32, vp, u32, vp, i32, u32, vp, vp, vp, vp], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -106,6 +107,23 @@ def device_bodies(bodies: np.ndarray, mass: np.ndarray) -> np.ndarray:
     out["inverse_mass"] = mass[:, 0]
     out["inverse_inertia_local"] = mass[:, 1:10]
     return out
+
+
+def schedule(dynamic: np.ndarray, info: np.ndarray, use_simd: bool = True, max_groups: int = 1024):
+    """Host-only solve schedule (kzg_schedule): per-point velocity level,
+    warm-start level, group, and (islands, groups, vel depth, ws depth)."""
+    dyn = np.ascontiguousarray(dynamic, np.uint8)
+    info = np.ascontiguousarray(info, np.uint32).reshape(-1, 4)
+    npts = int((info[:, 3] & 0xFF).sum()) if len(info) else 0
+    vl = np.zeros(max(npts, 1), np.uint32)
+    wl = np.zeros(max(npts, 1), np.uint32)
+    gr = np.zeros(max(npts, 1), np.uint32)
+    st = np.zeros(4, np.uint32)
+    rc = library().kzg_schedule(len(dyn), _p(dyn), len(info), _p(info), 1 if use_simd else 0, max_groups,
+                                _p(vl), _p(wl), _p(gr), _p(st))
+    if rc != 0:
+        raise KzgError(f"kzg_schedule failed ({rc})")
+    return vl[:npts], wl[:npts], gr[:npts], st
 
 
 class DeviceContext:
