@@ -97,6 +97,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of reference CPU work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dump", default="", help="save the last step's contact table (npz) for host-side tuning")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
+    ap.add_argument("--learn-collect", type=int, default=100, help="last warm-up steps that collect training rows")
+    ap.add_argument("--learn-steps", type=int, default=2000, help="Adam steps on the device")
+    ap.add_argument("--learn-batch", type=int, default=16384)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the contact path of one world runs on one MI355X "
@@ -110,12 +114,31 @@ def main():
     t0 = time.time()
     w = PhysicsWorld(scene)
     log(f"world: {args.bodies} bodies + 5 static, built in {time.time() - t0:.1f} s")
+    learn = not args.no_learned and args.learn_collect > 0
+    ctx = w.device_context()
+    if learn:
+        ctx.mlp_init(0x4B5A0005)
     for i in range(args.warmup):
+        if learn and i == max(args.warmup - args.learn_collect, 0):
+            ctx.mlp_collect(1 << 24)  # rows of (features, impulse corrections) from these steps
         w.step_fixed()
         if (i + 1) % 50 == 0:
             st = w.stats()
             log(f"warmup {i + 1}/{args.warmup}: {st['num_contact_points']} points, step {st['step_ms']:.1f} ms "
                 f"(contact path {st['contact_path_ms']:.1f} ms, broadphase {st['broadphase_ms']:.1f} ms)")
+    train = None
+    if learn:
+        t1 = time.perf_counter()
+        written, held = ctx.mlp_num_samples()
+        ctx.mlp_fit_normalization()
+        losses = ctx.mlp_train(args.learn_steps, batch=args.learn_batch, lr=1e-3, seed=0x4B5A0005)
+        train_s = time.perf_counter() - t1
+        ctx.mlp_collect(0)  # the timed steps run the plain path
+        train = {"rows": int(held), "adam_steps": args.learn_steps, "batch": args.learn_batch,
+                 "seconds": train_s, "loss_first": float(losses[:10].mean()), "loss_last": float(losses[-100:].mean()),
+                 "collected_over_steps": f"{args.warmup - args.learn_collect + 1}-{args.warmup}"}
+        log(f"learned warm start: {held} rows, {args.learn_steps} Adam steps of {args.learn_batch} in {train_s:.2f} s, "
+            f"loss {train['loss_first']:.3f} -> {train['loss_last']:.3f}")
     torch.cuda.synchronize()
     points = 0
     path_ms = 0.0
@@ -157,6 +180,20 @@ def main():
         D.library().kzg_download_constraints(h, hdr.ctypes.data_as(C.c_void_p), pts.ctypes.data_as(C.c_void_p))
         dyn = (scene.body_array()["motion_type"] == S.MOTION_DYNAMIC).astype(np.uint8)
         np.savez_compressed(args.dump, hdr=hdr, dynamic=dyn)
+    residual = None
+    if learn:
+        from kizuriphysics3d_amd import learned as Lw
+        res = Lw.compare_warm_starts(w, penetration_iterations=(4, 8))
+        cache8 = float(res["cache"]["residual"][-1])
+        residual = {"definition": "RMS over contact points of the impulse increments of sweep k (N s)",
+                    "state": f"after step {args.warmup + args.steps}", "points": res["cache"]["points"],
+                    "k": list(range(1, len(res["cache"]["residual"]) + 1)), "train": train}
+        for name, r in res.items():
+            residual[name] = [float(x) for x in r["residual"]]
+            residual[name + "_penetration"] = r["penetration"]
+            residual[name + "_sweeps_to_cache8"] = Lw.iterations_to_reach(r["residual"], cache8)
+            log(f"residual {name:13s} k=1..8 {np.array2string(r['residual'], precision=3)} "
+                f"penetration k=4 {r['penetration'][4]}")
     cpu = None
     if not args.no_cpu:
         try:
@@ -164,7 +201,7 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline unavailable: {e}")
     line = {
-        "metric": "contacts solved/s at 32k bodies",
+        "metric": "contacts solved/s at 32k bodies; PGS residual at fixed iters, learned vs plain",
         "value": value,
         "unit": "contacts/s",
         "n_gpus": 1,
@@ -183,6 +220,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_solve_groups", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,
+        "residual": residual,
     }
     print(json.dumps(line), flush=True)
 

@@ -65,13 +65,18 @@ typedef struct kzg_hull_range {
     uint32_t vert_offset, vert_count, tri_offset, tri_count;
 } kzg_hull_range;
 
-/* Per-step phase timings measured with HIP events on the context's stream
- * (milliseconds): narrowphase, build (incl. the host schedule), integrate
- * velocities, solve, integrate positions. */
+/* Per-step phase timings (milliseconds).  Device phases are measured with
+ * HIP events on the context's stream: narrowphase (pair upload + kernels),
+ * integrate velocities, solve (prepare + sweeps + write-back), the sweep
+ * kernel alone, integrate positions and the constraint build kernel.
+ * build_ms is host wall time of kzg_build_constraints once the manifold
+ * counts have arrived: compaction, the solve schedule and its upload; the
+ * build kernel runs on the device meanwhile. */
 typedef struct kzg_timings {
     float narrowphase_ms, build_ms, integrate_velocities_ms, solve_ms, integrate_positions_ms;
     float solve_kernel_ms;   /* the persistent sweep kernel alone */
     uint32_t velocity_levels, bias_levels, groups, islands;
+    float build_kernel_ms;   /* the constraint build kernel alone */
 } kzg_timings;
 
 /* Create a context on HIP device `device` for a world with settings `world`. */
@@ -124,6 +129,74 @@ int kzg_get_timings(kzg_ctx* ctx, kzg_timings* out);
 int kzg_collide_batch(kzg_ctx* ctx, const uint32_t* shape_a, const uint32_t* shape_b,
                       const float* xf_a, const float* xf_b, uint32_t n, float max_separation,
                       uint32_t* counts, kz_contact_point* points, float* ms);
+
+/* Warm-start modes of kzg_build_constraints / kzg_set_warm_start.  The
+ * reference initialises accumulated impulses from its manifold cache
+ * (PhysicsWorld.cpp:492-530) or, under KZ_NOWS, with zeros (:493).  The
+ * learned modes add the contact-point network's correction (kzg_mlp_*) to
+ * the cache's impulses: at every point, or only where the cache has no
+ * match.  The sums are clamped as Prepare clamps cached impulses
+ * (ContactSolver.cpp:107-110).  KZG_WARM_NO_STORE, or-ed into a mode,
+ * makes the following kzg_solve leave the cache untouched (for comparing
+ * modes on one state). */
+enum kzg_warm_start {
+    KZG_WARM_CACHE = 0,
+    KZG_WARM_COLD = 1,
+    KZG_WARM_LEARNED = 2,
+    KZG_WARM_LEARNED_MISS = 3,
+    KZG_WARM_NO_STORE = 0x100
+};
+/* Mode kzg_step uses (default KZG_WARM_CACHE). */
+int kzg_set_warm_start(kzg_ctx* ctx, int mode);
+/* Override SolverSettings::velocityIterations / positionIterations. */
+int kzg_set_iterations(kzg_ctx* ctx, uint32_t velocity, uint32_t position);
+
+/* ---- Learned warm start (no reference counterpart) ---------------------
+ * A per-contact-point MLP 14 -> 64 -> 64 -> 3 with ReLU hidden layers,
+ * BF16 weights and FP32 accumulation on the matrix cores; it replaces the
+ * impulse initialisation at PhysicsWorld.cpp:514-530 and is trained on the
+ * impulses the solver stores at :568-581.
+ * Feature row, 16 floats per point: 0 normal relative velocity, 1-2
+ * relative velocity along t1 / t2, 3-5 normal, 6 depth, 7-8 inverse masses
+ * A / B, 9-10 (r x n).I^-1(r x n) for A / B, 11-12 the mean of that along
+ * t1 and t2 for A / B, 13 cache hit (0/1), 14 impulse scale s = normal mass
+ * * |gravity| * dt, 15 zero.  Inputs are features 0-13, normalised with a
+ * per-feature mean and inverse standard deviation.  Outputs y are impulse
+ * corrections in units of s on top of the cache's clamped impulses c (zero
+ * where the cache has no match): lambda_n = max(c_n + y0 s, 0),
+ * lambda_t = clamp(c_t + y1,2 s, +-mu lambda_n).  KZG_WARM_LEARNED applies
+ * the correction everywhere, KZG_WARM_LEARNED_MISS only where c is zero by
+ * a miss.
+ * Parameters (FP32 master copy), in this order: W1[64][14], b1[64],
+ * W2[64][64], b2[64], W3[3][64], b3[3] (row = output unit). */
+#define KZG_MLP_PARAMS 5315
+#define KZG_MLP_FEATURES 14
+#define KZG_MLP_FEATURE_ROW 16
+#define KZG_MLP_SAMPLE_ROW 20   /* feature row + target correction y (3) + 0 */
+/* He-uniform weights from a SplitMix64 seed, zero biases, Adam reset. */
+int kzg_mlp_init(kzg_ctx* ctx, uint64_t seed);
+int kzg_mlp_set_params(kzg_ctx* ctx, const float* params, uint32_t n);
+int kzg_mlp_get_params(kzg_ctx* ctx, float* params, uint32_t n);
+int kzg_mlp_set_normalization(kzg_ctx* ctx, const float* mean, const float* inv_std);
+int kzg_mlp_get_normalization(kzg_ctx* ctx, float* mean, float* inv_std);
+/* Training data: every kzg_solve appends, per point, its feature row and
+ * (final accumulated impulses - c) / s to a device ring of `capacity` rows
+ * (0 switches collection off; the call empties the ring). */
+int kzg_mlp_collect(kzg_ctx* ctx, uint32_t capacity);
+int kzg_mlp_num_samples(kzg_ctx* ctx, uint64_t* written, uint32_t* held);
+/* Rows first .. first+count-1 of the ring, KZG_MLP_SAMPLE_ROW floats each. */
+int kzg_mlp_get_samples(kzg_ctx* ctx, uint32_t first, uint32_t count, float* out);
+/* Feature mean / inverse standard deviation from up to 65536 ring rows. */
+int kzg_mlp_fit_normalization(kzg_ctx* ctx);
+/* `steps` Adam steps (beta 0.9 / 0.999, eps 1e-8) on minibatches of `batch`
+ * rows drawn from the ring by a hash of (seed, step, row); loss = mean
+ * squared error over the batch and the 3 outputs; losses[steps] optional. */
+int kzg_mlp_train(kzg_ctx* ctx, uint32_t steps, uint32_t batch, float lr, uint32_t seed, float* losses);
+/* Inference on feature rows (n x 16 floats): out = (y0, y1, y2, 0) per row. */
+int kzg_mlp_forward(kzg_ctx* ctx, const float* features, uint32_t n, float* out);
+/* Gradient of the mean squared error over sample rows (n x 20 floats) with
+ * respect to the parameters (KZG_MLP_PARAMS floats); loss = summed squared error. */
+int kzg_mlp_gradients(kzg_ctx* ctx, const float* samples, uint32_t n, float* grad, float* loss);
 
 /* Host only (no GPU): the solve schedule kzg_solve would use for a contact
  * table, for tests and tuning.  info: 4 u32 per constraint (body A, body B,

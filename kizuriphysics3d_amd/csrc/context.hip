@@ -13,6 +13,7 @@
 #include "kz_device.h"
 #include "kz_math.h"
 #include "kzg.h"
+#include "mlp.h"
 #include "schedule.h"
 
 namespace kz {
@@ -24,7 +25,8 @@ void launch_build(hipStream_t s, const ContactBuf& c, const ManifoldBuf& m, cons
                   uint32_t nc, int warm);
 void launch_prepare(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32_t np, float dt, float rthr,
                     float baum, float slop, float maxb, int use_simd, int use_vbias, const VelRecBuf& vr,
-                    const WsRecBuf& wr, const uint32_t* vel_pos, const uint32_t* ws_pos);
+                    const WsRecBuf& wr, const uint32_t* vel_pos, const uint32_t* ws_pos, float gdt, float4* feat,
+                    float4* base);
 void launch_writeback(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const ContactBuf& c,
                       const uint32_t* vel_pos, const uint32_t* ws_pos, uint32_t np);
 void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b,
@@ -156,11 +158,27 @@ struct kzg_ctx {
 
     // staging and timing
     Pinned stage, stage2, stage_a, stage_b;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[9] = {};
     uint32_t recorded = 0;  // bit i: ev[i] recorded since the last narrowphase
     kzg_timings timings{};
     double build_host_ms = 0.0;
     int phase = 0;  // 0 idle, 1 narrowphase done, 2 built, 3 solved
+
+    // learned warm start
+    bool mlp_ready = false;
+    DevVec<float> mlp_master, mlp_m, mlp_v, mlp_norm, mlp_partial, mlp_out, mlp_loss;
+    DevVec<uint16_t> mlp_w1b, mlp_w2b, mlp_w3b;
+    uint32_t adam_t = 0;
+    DevVec<float4> feat;       // feature rows of the current step, 4 float4 per point
+    DevVec<float4> base;       // the cache's clamped impulses of the current step, 1 float4 per point
+    DevVec<float4> samples;    // training ring, 5 float4 per sample
+    DevVec<float4> x_samples;  // caller-supplied rows (forward / gradient entry points)
+    DevVec<float4> x_out;      // forward outputs
+    uint32_t sample_cap = 0;
+    uint64_t sample_count = 0;
+    int warm_mode = KZG_WARM_CACHE;  // kzg_step's mode
+    int cur_mode = KZG_WARM_CACHE;   // mode of the step being built and solved
+    bool no_store = false;
 
     // standalone batch scratch
     DevVec<float4> x_x, x_q;
@@ -177,6 +195,9 @@ struct kzg_ctx {
                           p_ra.p, p_rb.p, p_la.p, p_lb.p, p_imp.p, p_nvel.p};
     }
     CacheBuf cache() { return CacheBuf{cache_key.p, cache_used.p, cache_count.p, cache_p.p, cache_t.p, cache_cap}; }
+    MlpBuf mlp() {
+        return MlpBuf{mlp_master.p, mlp_w1b.p, mlp_w2b.p, mlp_w3b.p, mlp_norm.p, mlp_m.p, mlp_v.p};
+    }
     VelRecBuf vel_records() { return VelRecBuf{vr_bd.p, vr_ra.p, vr_rb.p, vr_n.p, vr_t1.p, vr_t2.p, vr_fb.p, vr_imp.p}; }
     WsRecBuf ws_records() {
         return WsRecBuf{wr_bd.p, wr_ra.p, wr_rb.p, wr_qa.p, wr_qb.p, wr_n.p, wr_t1.p, wr_t2.p, wr_imp.p};
@@ -336,6 +357,16 @@ void kzg_destroy(kzg_ctx* c) {
     c->cache_key.release();
     c->cache_t.release();
     c->resid.release();
+    for (auto* d : {&c->mlp_master, &c->mlp_m, &c->mlp_v, &c->mlp_norm, &c->mlp_partial, &c->mlp_out, &c->mlp_loss})
+        d->release();
+    c->mlp_w1b.release();
+    c->mlp_w2b.release();
+    c->mlp_w3b.release();
+    c->feat.release();
+    c->base.release();
+    c->samples.release();
+    c->x_samples.release();
+    c->x_out.release();
     c->stage.release();
     c->stage2.release();
     c->stage_a.release();
@@ -489,8 +520,14 @@ int kzg_narrowphase(kzg_ctx* c, const uint32_t* pairs, uint32_t n) {
 int kzg_build_constraints(kzg_ctx* c, int warm) {
     if (!c) return KZG_ERR_ARGUMENT;
     if (c->phase != 1) return fail(c, KZG_ERR_STATE, "kzg_build_constraints needs a preceding kzg_narrowphase");
+    const int mode = warm & 0xFF;
+    if ((warm & ~(0xFF | KZG_WARM_NO_STORE)) || mode > KZG_WARM_LEARNED_MISS)
+        return fail(c, KZG_ERR_ARGUMENT, "unknown warm-start mode " + std::to_string(warm));
+    if ((mode == KZG_WARM_LEARNED || mode == KZG_WARM_LEARNED_MISS) && !c->mlp_ready)
+        return fail(c, KZG_ERR_STATE, "learned warm start needs network weights (kzg_mlp_init or kzg_mlp_set_params)");
+    c->cur_mode = mode;
+    c->no_store = (warm & KZG_WARM_NO_STORE) != 0;
     hipSetDevice(c->device);
-    auto t0 = std::chrono::steady_clock::now();
     const uint32_t n = c->npairs;
     // Manifold point counts and the cache occupancy, one synchronisation.
     KZG_HIP(c, c->stage2.ensure(sizeof(uint32_t) * (size_t(n) + 1)));
@@ -498,6 +535,9 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     if (n) KZG_HIP(c, hipMemcpyAsync(hc, c->m_count.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
     KZG_HIP(c, hipMemcpyAsync(hc + n, c->cache_used.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     KZG_HIP(c, hipStreamSynchronize(c->stream));
+    // Host time of the build proper starts once the counts are here (the
+    // wait above is the narrowphase still running).
+    auto t0 = std::chrono::steady_clock::now();
     const uint32_t cache_used = hc[n];
     // Compaction in pair order (BuildContactConstraints: empty manifolds and
     // pairs without a dynamic body are skipped; bodies ordered by packed id).
@@ -537,7 +577,8 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
         (rc = upload_async(c, c->c_pair, c->stage_a, off, c->h_cpair.data(), c->ncons)))
         return rc;
     KZG_HIP(c, record(c, 2));
-    launch_build(c->stream, c->contacts(), c->manifolds(), c->bodies(), c->cache(), c->ncons, warm);
+    launch_build(c->stream, c->contacts(), c->manifolds(), c->bodies(), c->cache(), c->ncons,
+                 mode == KZG_WARM_COLD ? 1 : 0);
     KZG_HIP(c, hipGetLastError());
     KZG_HIP(c, record(c, 3));
     ScheduleInput si;
@@ -568,6 +609,7 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
 int kzg_integrate_velocities(kzg_ctx* c, float dt) {
     if (!c) return KZG_ERR_ARGUMENT;
     hipSetDevice(c->device);
+    KZG_HIP(c, record(c, 8));
     launch_integrate_velocities(c->stream, c->bodies(), c->nbodies, dt, c->world.gravity);
     KZG_HIP(c, hipGetLastError());
     KZG_HIP(c, record(c, 4));
@@ -579,10 +621,23 @@ int kzg_solve(kzg_ctx* c, float dt) {
     if (c->phase != 2) return fail(c, KZG_ERR_STATE, "kzg_solve needs a preceding kzg_build_constraints");
     hipSetDevice(c->device);
     const kz_world_desc& w = c->world;
+    const bool learned = c->cur_mode == KZG_WARM_LEARNED || c->cur_mode == KZG_WARM_LEARNED_MISS;
+    float4 *feat = nullptr, *base = nullptr;
+    if ((learned || c->sample_cap) && c->npoints) {
+        KZG_HIP(c, c->feat.ensure(4 * size_t(c->npoints)));
+        KZG_HIP(c, c->base.ensure(c->npoints));
+        feat = c->feat.p;
+        base = c->base.p;
+    }
+    const float g = std::sqrt(w.gravity[0] * w.gravity[0] + w.gravity[1] * w.gravity[1] + w.gravity[2] * w.gravity[2]);
     launch_prepare(c->stream, c->contacts(), c->bodies(), c->npoints, dt, w.restitution_threshold, w.baumgarte,
                    w.penetration_slop, w.max_bias_velocity, w.use_simd_solver ? 1 : 0, w.use_velocity_bias ? 1 : 0,
-                   c->vel_records(), c->ws_records(), c->vel_pos.p, c->ws_pos.p);
+                   c->vel_records(), c->ws_records(), c->vel_pos.p, c->ws_pos.p, (g > 0.0f ? g : 1.0f) * dt, feat,
+                   base);
     KZG_HIP(c, hipGetLastError());
+    if (learned)
+        launch_mlp_warm(c->stream, c->mlp(), feat, base, c->npoints, c->cur_mode == KZG_WARM_LEARNED_MISS ? 1 : 0,
+                        c->p_imp.p, c->p_la.p, c->vr_imp.p, c->wr_imp.p, c->vel_pos.p, c->ws_pos.p);
     KZG_HIP(c, c->resid.ensure(std::max<uint32_t>(w.velocity_iterations, 1)));
     KZG_HIP(c, hipMemsetAsync(c->resid.p, 0, sizeof(float) * std::max<uint32_t>(w.velocity_iterations, 1), c->stream));
     SolveParams sp{dt, w.warm_start_factor, w.velocity_iterations, w.position_iterations, w.use_simd_solver,
@@ -599,7 +654,14 @@ int kzg_solve(kzg_ctx* c, float dt) {
     KZG_HIP(c, record(c, 6));
     launch_writeback(c->stream, c->vel_records(), c->ws_records(), c->contacts(), c->vel_pos.p, c->ws_pos.p,
                      c->npoints);
-    launch_cache_store(c->stream, c->contacts(), c->bodies(), c->cache(), c->ncons);
+    if (c->sample_cap && c->npoints) {
+        // Training samples: this step's features with its final impulses.
+        const uint32_t n = std::min(c->npoints, c->sample_cap);
+        launch_mlp_collect(c->stream, feat, base, c->p_imp.p, n, c->samples.p, c->sample_cap,
+                           uint32_t(c->sample_count % c->sample_cap));
+        c->sample_count += n;
+    }
+    if (!c->no_store) launch_cache_store(c->stream, c->contacts(), c->bodies(), c->cache(), c->ncons);
     KZG_HIP(c, hipGetLastError());
     c->resid_points = c->npoints;
     c->phase = 3;
@@ -619,7 +681,7 @@ int kzg_integrate_positions(kzg_ctx* c, float dt) {
 int kzg_step(kzg_ctx* c, const uint32_t* pairs, uint32_t n, float dt) {
     int rc;
     if ((rc = kzg_narrowphase(c, pairs, n))) return rc;
-    if ((rc = kzg_build_constraints(c, 0))) return rc;
+    if ((rc = kzg_build_constraints(c, c->warm_mode))) return rc;
     if ((rc = kzg_integrate_velocities(c, dt))) return rc;
     if ((rc = kzg_solve(c, dt))) return rc;
     return kzg_integrate_positions(c, dt);
@@ -738,9 +800,9 @@ int kzg_get_timings(kzg_ctx* c, kzg_timings* out) {
             out = ms;
     };
     span(0, 1, t.narrowphase_ms);
-    span(2, 3, t.build_ms);
-    t.build_ms += float(c->build_host_ms);
-    span(3, 4, t.integrate_velocities_ms);
+    span(2, 3, t.build_kernel_ms);
+    t.build_ms = float(c->build_host_ms);
+    span(8, 4, t.integrate_velocities_ms);
     span(4, 6, t.solve_ms);
     span(5, 6, t.solve_kernel_ms);
     span(6, 7, t.integrate_positions_ms);
@@ -832,6 +894,253 @@ int kzg_collide_batch(kzg_ctx* c, const uint32_t* sa, const uint32_t* sb, const 
     c->npairs = n;
     c->phase = 0;
     return kzg_download_manifolds(c, counts, pts);
+}
+
+
+// ---------------------------------------------------------------------------
+// Warm-start mode, iteration counts and the learned warm start.
+// ---------------------------------------------------------------------------
+int kzg_set_warm_start(kzg_ctx* c, int mode) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if ((mode & ~(0xFF | KZG_WARM_NO_STORE)) || (mode & 0xFF) > KZG_WARM_LEARNED_MISS)
+        return fail(c, KZG_ERR_ARGUMENT, "unknown warm-start mode " + std::to_string(mode));
+    c->warm_mode = mode;
+    return KZG_OK;
+}
+
+int kzg_set_iterations(kzg_ctx* c, uint32_t velocity, uint32_t position) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (velocity > 1024 || position > 1024) return fail(c, KZG_ERR_ARGUMENT, "iteration count above 1024");
+    c->world.velocity_iterations = velocity;
+    c->world.position_iterations = position;
+    return KZG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int mlp_alloc(kzg_ctx* c) {
+    if (c->mlp_master.p) return KZG_OK;
+    hipSetDevice(c->device);
+    KZG_HIP(c, c->mlp_master.ensure(kMlpParams));
+    KZG_HIP(c, c->mlp_m.ensure(kMlpParams));
+    KZG_HIP(c, c->mlp_v.ensure(kMlpParams));
+    KZG_HIP(c, c->mlp_norm.ensure(32));
+    KZG_HIP(c, c->mlp_out.ensure(kMlpParams + 1));
+    KZG_HIP(c, c->mlp_w1b.ensure(64 * 16));
+    KZG_HIP(c, c->mlp_w2b.ensure(64 * 64));
+    KZG_HIP(c, c->mlp_w3b.ensure(16 * 64));
+    // Zero padding of the BF16 operands (columns 14-15 of W1, rows 3-15 of W3).
+    KZG_HIP(c, hipMemsetAsync(c->mlp_w1b.p, 0, 64 * 16 * sizeof(uint16_t), c->stream));
+    KZG_HIP(c, hipMemsetAsync(c->mlp_w3b.p, 0, 16 * 64 * sizeof(uint16_t), c->stream));
+    std::vector<float> norm(32, 0.0f);
+    for (int f = 0; f < kMlpIn; ++f) norm[16 + f] = 1.0f;
+    return upload(c, c->mlp_norm, norm.data(), 32);
+}
+
+// Upload master weights, reset Adam and refresh the BF16 copies.
+int mlp_load(kzg_ctx* c, const float* params) {
+    int rc;
+    if ((rc = mlp_alloc(c)) || (rc = upload(c, c->mlp_master, params, kMlpParams))) return rc;
+    KZG_HIP(c, hipMemsetAsync(c->mlp_m.p, 0, kMlpParams * sizeof(float), c->stream));
+    KZG_HIP(c, hipMemsetAsync(c->mlp_v.p, 0, kMlpParams * sizeof(float), c->stream));
+    launch_mlp_sync(c->stream, c->mlp());
+    KZG_HIP(c, hipGetLastError());
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    c->adam_t = 0;
+    c->mlp_ready = true;
+    return KZG_OK;
+}
+
+int mlp_check_rows(kzg_ctx* c, const float* rows, uint32_t n) {
+    if (!rows && n) return fail(c, KZG_ERR_ARGUMENT, "null row array");
+    if (!c->mlp_ready) return fail(c, KZG_ERR_STATE, "no network weights (kzg_mlp_init or kzg_mlp_set_params)");
+    return KZG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kzg_mlp_init(kzg_ctx* c, uint64_t seed) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    // He-uniform weights, zero biases; SplitMix64 stream.
+    uint64_t st = seed;
+    auto uni = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return double(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    std::vector<float> p(kMlpParams, 0.0f);
+    auto fill = [&](int off, int count, int fan_in) {
+        const double bound = std::sqrt(6.0 / fan_in);
+        for (int k = 0; k < count; ++k) p[off + k] = float((2.0 * uni() - 1.0) * bound);
+    };
+    fill(kMlpW1, kMlpHid * kMlpIn, kMlpIn);
+    fill(kMlpW2, kMlpHid * kMlpHid, kMlpHid);
+    fill(kMlpW3, kMlpOut * kMlpHid, kMlpHid);
+    return mlp_load(c, p.data());
+}
+
+int kzg_mlp_set_params(kzg_ctx* c, const float* params, uint32_t n) {
+    if (!c || !params) return KZG_ERR_ARGUMENT;
+    if (n != uint32_t(kMlpParams)) return fail(c, KZG_ERR_ARGUMENT, "expected " + std::to_string(kMlpParams) + " parameters");
+    return mlp_load(c, params);
+}
+
+int kzg_mlp_get_params(kzg_ctx* c, float* params, uint32_t n) {
+    if (!c || !params) return KZG_ERR_ARGUMENT;
+    if (n != uint32_t(kMlpParams)) return fail(c, KZG_ERR_ARGUMENT, "expected " + std::to_string(kMlpParams) + " parameters");
+    if (!c->mlp_ready) return fail(c, KZG_ERR_STATE, "no network weights");
+    hipSetDevice(c->device);
+    return download(c, params, c->mlp_master, kMlpParams);
+}
+
+int kzg_mlp_set_normalization(kzg_ctx* c, const float* mean, const float* inv_std) {
+    if (!c || !mean || !inv_std) return KZG_ERR_ARGUMENT;
+    int rc;
+    if ((rc = mlp_alloc(c))) return rc;
+    std::vector<float> norm(32, 0.0f);
+    for (int f = 0; f < kMlpIn; ++f) {
+        norm[f] = mean[f];
+        norm[16 + f] = inv_std[f];
+    }
+    return upload(c, c->mlp_norm, norm.data(), 32);
+}
+
+int kzg_mlp_get_normalization(kzg_ctx* c, float* mean, float* inv_std) {
+    if (!c || !mean || !inv_std) return KZG_ERR_ARGUMENT;
+    int rc;
+    if ((rc = mlp_alloc(c))) return rc;
+    std::vector<float> norm(32);
+    if ((rc = download(c, norm.data(), c->mlp_norm, 32))) return rc;
+    for (int f = 0; f < kMlpIn; ++f) {
+        mean[f] = norm[f];
+        inv_std[f] = norm[16 + f];
+    }
+    return KZG_OK;
+}
+
+int kzg_mlp_collect(kzg_ctx* c, uint32_t capacity) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    hipSetDevice(c->device);
+    if (capacity) KZG_HIP(c, c->samples.ensure(5 * size_t(capacity)));
+    c->sample_cap = capacity;
+    c->sample_count = 0;
+    return KZG_OK;
+}
+
+int kzg_mlp_num_samples(kzg_ctx* c, uint64_t* written, uint32_t* held) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (written) *written = c->sample_count;
+    if (held) *held = uint32_t(std::min<uint64_t>(c->sample_count, c->sample_cap));
+    return KZG_OK;
+}
+
+int kzg_mlp_get_samples(kzg_ctx* c, uint32_t first, uint32_t count, float* out) {
+    if (!c || (count && !out)) return KZG_ERR_ARGUMENT;
+    const uint64_t held = std::min<uint64_t>(c->sample_count, c->sample_cap);
+    if (uint64_t(first) + count > held) return fail(c, KZG_ERR_ARGUMENT, "sample range beyond the collected rows");
+    if (!count) return KZG_OK;
+    hipSetDevice(c->device);
+    KZG_HIP(c, c->stage2.ensure(size_t(count) * 5 * sizeof(float4)));
+    KZG_HIP(c, hipMemcpyAsync(c->stage2.p, c->samples.p + 5 * size_t(first), size_t(count) * 5 * sizeof(float4),
+                              hipMemcpyDeviceToHost, c->stream));
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    std::memcpy(out, c->stage2.p, size_t(count) * 5 * sizeof(float4));
+    return KZG_OK;
+}
+
+int kzg_mlp_fit_normalization(kzg_ctx* c) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    const uint32_t held = uint32_t(std::min<uint64_t>(c->sample_count, c->sample_cap));
+    if (!held) return fail(c, KZG_ERR_STATE, "no collected samples");
+    // Mean and standard deviation of every feature over at most 65536 rows
+    // spread evenly over the ring (double accumulation on the host).
+    const uint32_t take = std::min<uint32_t>(held, 65536), stride = held / take;
+    const size_t row_bytes = kSampleRow * sizeof(float);
+    hipSetDevice(c->device);
+    KZG_HIP(c, c->stage2.ensure(size_t(take) * row_bytes));
+    KZG_HIP(c, hipMemcpy2DAsync(c->stage2.p, row_bytes, c->samples.p, row_bytes * stride, row_bytes, take,
+                                hipMemcpyDeviceToHost, c->stream));
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    const float* rows = c->stage2.as<float>();
+    double sum[kMlpIn] = {}, sq[kMlpIn] = {};
+    for (uint32_t k = 0; k < take; ++k) {
+        const float* r = rows + size_t(k) * kSampleRow;
+        for (int f = 0; f < kMlpIn; ++f) {
+            sum[f] += r[f];
+            sq[f] += double(r[f]) * r[f];
+        }
+    }
+    float mean[kMlpIn], istd[kMlpIn];
+    for (int f = 0; f < kMlpIn; ++f) {
+        const double m = sum[f] / take, var = std::max(0.0, sq[f] / take - m * m), sd = std::sqrt(var);
+        mean[f] = float(m);
+        istd[f] = sd > 1e-6 ? float(1.0 / sd) : 0.0f;  // a constant feature is switched off
+    }
+    return kzg_mlp_set_normalization(c, mean, istd);
+}
+
+int kzg_mlp_train(kzg_ctx* c, uint32_t steps, uint32_t batch, float lr, uint32_t seed, float* losses) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (!c->mlp_ready) return fail(c, KZG_ERR_STATE, "no network weights");
+    const uint32_t held = uint32_t(std::min<uint64_t>(c->sample_count, c->sample_cap));
+    if (!held) return fail(c, KZG_ERR_STATE, "no collected samples");
+    if (batch < 16 || batch > (1u << 20)) return fail(c, KZG_ERR_ARGUMENT, "batch outside [16, 2^20]");
+    if (!(lr > 0.0f)) return fail(c, KZG_ERR_ARGUMENT, "learning rate must be positive");
+    hipSetDevice(c->device);
+    const uint32_t rows = mlp_grad_blocks(batch);
+    KZG_HIP(c, c->mlp_partial.ensure(size_t(rows) * (kMlpParams + 1)));
+    KZG_HIP(c, c->mlp_loss.ensure(std::max<uint32_t>(steps, 1)));
+    const float b1 = 0.9f, b2 = 0.999f;
+    for (uint32_t s = 0; s < steps; ++s) {
+        ++c->adam_t;
+        AdamParams ap{lr, b1, b2, 1e-8f, float(1.0 - std::pow(double(b1), double(c->adam_t))),
+                      float(1.0 - std::pow(double(b2), double(c->adam_t))), 1.0f / (3.0f * float(batch)), rows};
+        launch_mlp_grad(c->stream, c->mlp(), c->samples.p, held, batch, 0, seed + 0x9E3779B9u * s, c->mlp_partial.p);
+        launch_mlp_adam(c->stream, c->mlp(), c->mlp_partial.p, ap, c->mlp_loss.p + s);
+    }
+    KZG_HIP(c, hipGetLastError());
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    if (losses && steps) return download(c, losses, c->mlp_loss, steps);
+    return KZG_OK;
+}
+
+int kzg_mlp_forward(kzg_ctx* c, const float* features, uint32_t n, float* out) {
+    if (!c || (n && !out)) return KZG_ERR_ARGUMENT;
+    int rc;
+    if ((rc = mlp_check_rows(c, features, n))) return rc;
+    if (!n) return KZG_OK;
+    hipSetDevice(c->device);
+    if ((rc = upload(c, c->x_samples, reinterpret_cast<const float4*>(features), 4 * size_t(n)))) return rc;
+    KZG_HIP(c, c->x_out.ensure(size_t(n)));
+    launch_mlp_infer(c->stream, c->mlp(), c->x_samples.p, n, c->x_out.p);
+    KZG_HIP(c, hipGetLastError());
+    return download(c, reinterpret_cast<float4*>(out), c->x_out, n);
+}
+
+int kzg_mlp_gradients(kzg_ctx* c, const float* samples, uint32_t n, float* grad, float* loss) {
+    if (!c || !grad) return KZG_ERR_ARGUMENT;
+    int rc;
+    if ((rc = mlp_check_rows(c, samples, n))) return rc;
+    if (n < 1 || n > (1u << 20)) return fail(c, KZG_ERR_ARGUMENT, "sample count outside [1, 2^20]");
+    hipSetDevice(c->device);
+    if ((rc = upload(c, c->x_samples, reinterpret_cast<const float4*>(samples), 5 * size_t(n)))) return rc;
+    const uint32_t rows = mlp_grad_blocks(n);
+    KZG_HIP(c, c->mlp_partial.ensure(size_t(rows) * (kMlpParams + 1)));
+    launch_mlp_grad(c->stream, c->mlp(), c->x_samples.p, n, n, 1, 0, c->mlp_partial.p);
+    launch_mlp_reduce(c->stream, c->mlp_partial.p, rows, c->mlp_out.p);
+    KZG_HIP(c, hipGetLastError());
+    std::vector<float> g(kMlpParams + 1);
+    if ((rc = download(c, g.data(), c->mlp_out, kMlpParams + 1))) return rc;
+    std::memcpy(grad, g.data(), kMlpParams * sizeof(float));
+    if (loss) *loss = g[kMlpParams];
+    return KZG_OK;
 }
 
 }  // extern "C"

@@ -98,9 +98,11 @@ __global__ void k_build(ContactBuf c, ManifoldBuf m, BodyBuf b, CacheBuf cache, 
                 if (d < best) { best = d; bi = int(j); }
             }
             if (bi >= 0) {
+                // w = 1 marks a cache hit for Prepare's feature row; Prepare
+                // resets it to the bias impulse 0.
                 const float4 cp = cache.p[slot * kPairPoints + bi];
                 const float2 ct = cache.t[slot * kPairPoints + bi];
-                imp = make_float4(cp.w, ct.x, ct.y, 0.0f);
+                imp = make_float4(cp.w, ct.x, ct.y, 1.0f);
             }
         }
         c.body[p] = make_uint4(info.x, info.y, flags, ci);
@@ -119,7 +121,14 @@ __global__ void k_build(ContactBuf c, ManifoldBuf m, BodyBuf b, CacheBuf cache, 
 struct PrepParams {
     float dt, restitution_threshold, baumgarte, slop, max_bias;
     int use_simd, use_velocity_bias;
+    float gdt;      // |gravity| * dt (or dt without gravity): the impulse scale per unit normal mass
+    float4* feat;   // optional: learned warm-start feature rows, 4 float4 per point
+    float4* base;   // with feat: the clamped cache impulses (zero on a miss), 1 float4 per point
 };
+__device__ __forceinline__ float proj_inertia(const m3& inv_i, f3 r, f3 d) {
+    const f3 x = cross(r, d);
+    return dot(x, mmul(inv_i, x));
+}
 __device__ __forceinline__ float eff_mass(float ima, float imb, const m3& ia, const m3& ib, f3 ra, f3 rb, f3 d) {
     f3 xa = cross(ra, d), xb = cross(rb, d);
     float k = ima + imb;
@@ -160,11 +169,22 @@ __global__ void k_prepare(ContactBuf c, BodyBuf b, uint32_t np, PrepParams pp, V
         pbias = pp.baumgarte / pp.dt * (-sep - pp.slop);
         pbias = fmin_(pbias, pp.max_bias);
     }
+    if (pp.feat) {
+        // Feature row of the learned warm start (layout in mlp.h).
+        const f3 vrel = sub(pva, pvb);
+        float4* row = pp.feat + 4 * size_t(p);
+        row[0] = make_float4(nvel, dot(vrel, t1), dot(vrel, t2), n.x);
+        row[1] = make_float4(n.y, n.z, -sep, xa.w);
+        row[2] = make_float4(xb.w, proj_inertia(ia, ra, n), proj_inertia(ib, rb, n),
+                             0.5f * (proj_inertia(ia, ra, t1) + proj_inertia(ia, ra, t2)));
+        row[3] = make_float4(0.5f * (proj_inertia(ib, rb, t1) + proj_inertia(ib, rb, t2)), imp.w, nm * pp.gdt, 0.0f);
+    }
     const float mf = cf * imp.x;
     imp.y = fclamp(imp.y, -mf, mf);
     imp.z = fclamp(imp.z, -mf, mf);
     imp.x = fmax_(imp.x, 0.0f);
     imp.w = 0.0f;
+    if (pp.feat) pp.base[p] = imp;
     uint4 bdo = bd;
     const uint32_t ma = meta_motion(b.meta[bd.x]), mb = meta_motion(b.meta[bd.y]);
     bdo.z = (bd.z & 1u) | (ma == 2u ? 2u : 0u) | (mb == 2u ? 4u : 0u);
@@ -538,8 +558,9 @@ void launch_build(hipStream_t s, const ContactBuf& c, const ManifoldBuf& m, cons
 }
 void launch_prepare(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32_t np, float dt, float rthr,
                     float baum, float slop, float maxb, int use_simd, int use_vbias, const VelRecBuf& vr,
-                    const WsRecBuf& wr, const uint32_t* vel_pos, const uint32_t* ws_pos) {
-    PrepParams pp{dt, rthr, baum, slop, maxb, use_simd, use_vbias};
+                    const WsRecBuf& wr, const uint32_t* vel_pos, const uint32_t* ws_pos, float gdt, float4* feat,
+                    float4* base) {
+    PrepParams pp{dt, rthr, baum, slop, maxb, use_simd, use_vbias, gdt, feat, base};
     if (np) k_prepare<<<blocks(np, 256), 256, 0, s>>>(c, b, np, pp, vr, wr, vel_pos, ws_pos);
 }
 void launch_writeback(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const ContactBuf& c,

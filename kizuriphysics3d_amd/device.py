@@ -27,7 +27,7 @@ HULL_RANGE_DTYPE = np.dtype([("vert_offset", "<u4"), ("vert_count", "<u4"), ("tr
 TIMINGS_DTYPE = np.dtype([
     ("narrowphase_ms", "<f4"), ("build_ms", "<f4"), ("integrate_velocities_ms", "<f4"), ("solve_ms", "<f4"),
     ("integrate_positions_ms", "<f4"), ("solve_kernel_ms", "<f4"), ("velocity_levels", "<u4"),
-    ("bias_levels", "<u4"), ("groups", "<u4"), ("islands", "<u4"),
+    ("bias_levels", "<u4"), ("groups", "<u4"), ("islands", "<u4"), ("build_kernel_ms", "<f4"),
 ])
 assert KZG_BODY_DTYPE.itemsize == 136
 
@@ -35,8 +35,15 @@ EXPORTS = [
     "kzg_create", "kzg_destroy", "kzg_last_error", "kzg_upload_shapes", "kzg_upload_bodies", "kzg_upload_states",
     "kzg_narrowphase", "kzg_build_constraints", "kzg_integrate_velocities", "kzg_solve", "kzg_integrate_positions",
     "kzg_step", "kzg_query_sizes", "kzg_download_states", "kzg_download_manifolds", "kzg_download_constraints",
-    "kzg_get_residuals", "kzg_get_timings", "kzg_collide_batch", "kzg_schedule",
+    "kzg_get_residuals", "kzg_get_timings", "kzg_collide_batch", "kzg_schedule", "kzg_set_warm_start",
+    "kzg_set_iterations", "kzg_mlp_init", "kzg_mlp_set_params", "kzg_mlp_get_params", "kzg_mlp_set_normalization",
+    "kzg_mlp_get_normalization", "kzg_mlp_collect", "kzg_mlp_num_samples", "kzg_mlp_get_samples",
+    "kzg_mlp_fit_normalization", "kzg_mlp_train", "kzg_mlp_forward", "kzg_mlp_gradients",
 ]
+
+# Warm-start modes (kzg_warm_start in include/kzg.h).
+WARM_CACHE, WARM_COLD, WARM_LEARNED, WARM_LEARNED_MISS, WARM_NO_STORE = 0, 1, 2, 3, 0x100
+MLP_PARAMS, MLP_FEATURES, MLP_FEATURE_ROW, MLP_SAMPLE_ROW = 5315, 14, 16, 20
 
 _lib = None
 
@@ -67,6 +74,13 @@ def library():
             "kzg_get_residuals": ([vp, vp, u32], i32), "kzg_get_timings": ([vp, vp], i32),
             "kzg_collide_batch": ([vp, vp, vp, vp, vp, u32, f32, vp, vp, vp], i32),
             "kzg_schedule": ([u32, vp, u32, vp, i32, u32, vp, vp, vp, vp], i32),
+            "kzg_set_warm_start": ([vp, i32], i32), "kzg_set_iterations": ([vp, u32, u32], i32),
+            "kzg_mlp_init": ([vp, C.c_uint64], i32), "kzg_mlp_set_params": ([vp, vp, u32], i32),
+            "kzg_mlp_get_params": ([vp, vp, u32], i32), "kzg_mlp_set_normalization": ([vp, vp, vp], i32),
+            "kzg_mlp_get_normalization": ([vp, vp, vp], i32), "kzg_mlp_collect": ([vp, u32], i32),
+            "kzg_mlp_num_samples": ([vp, vp, vp], i32), "kzg_mlp_get_samples": ([vp, u32, u32, vp], i32),
+            "kzg_mlp_fit_normalization": ([vp], i32), "kzg_mlp_train": ([vp, u32, u32, f32, u32, vp], i32),
+            "kzg_mlp_forward": ([vp, vp, u32, vp], i32), "kzg_mlp_gradients": ([vp, vp, u32, vp, vp], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -140,10 +154,20 @@ class DeviceContext:
         self.num_bodies = 0
         self.num_pairs = 0
 
+    @classmethod
+    def borrow(cls, handle, num_bodies: int) -> "DeviceContext":
+        """Wrap a context owned elsewhere (a PhysicsWorld's); never destroyed here."""
+        self = cls.__new__(cls)
+        self.h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+        self._borrowed = True
+        self.num_bodies = num_bodies
+        self.num_pairs = 0
+        return self
+
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and not getattr(self, "_borrowed", False):
             library().kzg_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -236,6 +260,73 @@ class DeviceContext:
         t = np.zeros((), TIMINGS_DTYPE)
         self._check(library().kzg_get_timings(self.h, _p(t)), "kzg_get_timings")
         return {k: t[k].item() for k in TIMINGS_DTYPE.names}
+
+    # ---- warm-start mode, iterations, learned warm start --------------------
+    def set_warm_start(self, mode: int):
+        self._check(library().kzg_set_warm_start(self.h, int(mode)), "kzg_set_warm_start")
+
+    def set_iterations(self, velocity: int, position: int):
+        self._check(library().kzg_set_iterations(self.h, int(velocity), int(position)), "kzg_set_iterations")
+
+    def mlp_init(self, seed: int = 0x4B5A0005):
+        self._check(library().kzg_mlp_init(self.h, C.c_uint64(seed)), "kzg_mlp_init")
+
+    def mlp_params(self) -> np.ndarray:
+        out = np.zeros(MLP_PARAMS, np.float32)
+        self._check(library().kzg_mlp_get_params(self.h, _p(out), MLP_PARAMS), "kzg_mlp_get_params")
+        return out
+
+    def mlp_set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        self._check(library().kzg_mlp_set_params(self.h, _p(p), len(p)), "kzg_mlp_set_params")
+
+    def mlp_normalization(self):
+        mean, istd = np.zeros(MLP_FEATURES, np.float32), np.zeros(MLP_FEATURES, np.float32)
+        self._check(library().kzg_mlp_get_normalization(self.h, _p(mean), _p(istd)), "kzg_mlp_get_normalization")
+        return mean, istd
+
+    def mlp_set_normalization(self, mean, inv_std):
+        m = np.ascontiguousarray(mean, np.float32)
+        s = np.ascontiguousarray(inv_std, np.float32)
+        assert len(m) == len(s) == MLP_FEATURES
+        self._check(library().kzg_mlp_set_normalization(self.h, _p(m), _p(s)), "kzg_mlp_set_normalization")
+
+    def mlp_collect(self, capacity: int):
+        self._check(library().kzg_mlp_collect(self.h, int(capacity)), "kzg_mlp_collect")
+
+    def mlp_num_samples(self):
+        w, h = C.c_uint64(), C.c_uint32()
+        self._check(library().kzg_mlp_num_samples(self.h, C.byref(w), C.byref(h)), "kzg_mlp_num_samples")
+        return w.value, h.value
+
+    def mlp_samples(self, first: int = 0, count: int | None = None) -> np.ndarray:
+        if count is None:
+            count = self.mlp_num_samples()[1] - first
+        out = np.zeros((max(count, 1), MLP_SAMPLE_ROW), np.float32)
+        self._check(library().kzg_mlp_get_samples(self.h, int(first), int(count), _p(out)), "kzg_mlp_get_samples")
+        return out[:count]
+
+    def mlp_fit_normalization(self):
+        self._check(library().kzg_mlp_fit_normalization(self.h), "kzg_mlp_fit_normalization")
+
+    def mlp_train(self, steps: int, batch: int = 4096, lr: float = 1e-3, seed: int = 1) -> np.ndarray:
+        losses = np.zeros(max(steps, 1), np.float32)
+        self._check(library().kzg_mlp_train(self.h, int(steps), int(batch), float(lr), int(seed) & 0xFFFFFFFF,
+                                            _p(losses)), "kzg_mlp_train")
+        return losses[:steps]
+
+    def mlp_forward(self, features) -> np.ndarray:
+        f = np.ascontiguousarray(features, np.float32).reshape(-1, MLP_FEATURE_ROW)
+        out = np.zeros((max(len(f), 1), 4), np.float32)
+        self._check(library().kzg_mlp_forward(self.h, _p(f), len(f), _p(out)), "kzg_mlp_forward")
+        return out[:len(f)]
+
+    def mlp_gradients(self, samples):
+        s = np.ascontiguousarray(samples, np.float32).reshape(-1, MLP_SAMPLE_ROW)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        self._check(library().kzg_mlp_gradients(self.h, _p(s), len(s), _p(g), C.byref(loss)), "kzg_mlp_gradients")
+        return g, loss.value
 
     def collide_batch(self, shape_a, shape_b, xf_a, xf_b, max_separation=0.02):
         sa = np.ascontiguousarray(shape_a, np.uint32)

@@ -139,6 +139,13 @@ class PhysicsWorld:
     def device_context_handle(self):
         return _lib().kzw_device(self.h)
 
+    def device_context(self) -> D.DeviceContext:
+        """The world's kzg context (borrowed: it lives as long as the world)."""
+        h = self.device_context_handle()
+        if not h:
+            raise D.KzgError("the world has no device context")
+        return D.DeviceContext.borrow(h, self.num_bodies)
+
 
 def device_timings(world: PhysicsWorld) -> dict:
     """Per-phase device timings of the last step (HIP events, see kzg.h)."""
