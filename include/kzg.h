@@ -65,13 +65,14 @@ typedef struct kzg_hull_range {
     uint32_t vert_offset, vert_count, tri_offset, tri_count;
 } kzg_hull_range;
 
-/* Per-step phase timings (milliseconds).  Device phases are measured with
- * HIP events on the context's stream: narrowphase (pair upload + kernels),
- * integrate velocities, solve (prepare + sweeps + write-back), the sweep
- * kernel alone, integrate positions and the constraint build kernel.
- * build_ms is host wall time of kzg_build_constraints once the manifold
- * counts have arrived: compaction, the solve schedule and its upload; the
- * build kernel runs on the device meanwhile. */
+/* Per-step phase timings (milliseconds), host-device transfers excluded.
+ * Device phases are measured with HIP events on the context's stream:
+ * narrowphase (kernels, after the pair upload), integrate velocities, solve
+ * (prepare, sweeps, write-back, cache store), the sweep kernel alone,
+ * integrate positions and the constraint build kernel.  build_ms is host
+ * compute of kzg_build_constraints once the manifold counts have arrived:
+ * compaction and the solve schedule (the build kernel runs meanwhile); the
+ * staging and upload of the schedule are not included. */
 typedef struct kzg_timings {
     float narrowphase_ms, build_ms, integrate_velocities_ms, solve_ms, integrate_positions_ms;
     float solve_kernel_ms;   /* the persistent sweep kernel alone */

@@ -8,6 +8,7 @@ import pytest
 
 from kizuriphysics3d_amd import scenes as S
 from kizuriphysics3d_amd import device as D
+from oracle import oracle as O
 from oracle import refharness as R
 from tests.helpers import first_mismatch, reference_hulls
 
@@ -69,3 +70,28 @@ def test_pyramid_trajectory_bit_exact():
 
 def test_pile_trajectory_bit_exact():
     _trajectory(S.pile(512, seed=0x4B5A0102), 240)
+
+
+def test_residuals_match_oracle():
+    """The PGS residual (RMS over points of each velocity sweep's impulse
+    increments) of the device solve against the CPU oracle on the same
+    trajectory.  The increments are bit-identical; only the order of the
+    FP32 sums differs."""
+    scene = S.pile(512, seed=0x4B5A0102)
+    R.init_jobs(0)
+    rw = R.RefWorld(scene)
+    ctx = _ctx_for(scene, rw)
+    shapes = scene.shape_array()
+    ow = O.OracleWorld(scene, O.Derived.pack(shapes, reference_hulls(shapes, scene.hull_array()), rw.mass()[:, :10]))
+    checked = 0
+    for i in range(160):
+        rw.step_phases()
+        pairs = rw.pairs()
+        ctx.step(pairs)
+        ow.step(pairs)
+        if i % 20 == 19:
+            g, o = ctx.residuals(), ow.residuals()
+            assert len(g) == len(o) == 8
+            assert np.allclose(g, o, rtol=2e-5, atol=1e-9), (i, g, o)
+            checked += int(o[0] > 0)
+    assert checked >= 4

@@ -169,8 +169,8 @@ def test_near_touching_boxes_reproduce_the_reference_normal_sign():
     assert np.allclose(p[:4]["point_on_b"][:, 1], 3.01, atol=1e-6)
 
 
-def _single_box_world(gravity=-9.81):
-    sc = S.Scene("box", S.world_settings(10, 2, gravity=(0.0, gravity, 0.0)))
+def _single_box_world(gravity=-9.81, velocity_iterations=10):
+    sc = S.Scene("box", S.world_settings(velocity_iterations, 2, gravity=(0.0, gravity, 0.0)))
     plane = sc.add_shape(S.SHAPE_PLANE, (0, 1, 0, 0))
     sc.add_body(S.default_body(plane, S.MOTION_STATIC))
     box = sc.add_shape(S.SHAPE_BOX, (0.5, 0.5, 0.5))
@@ -199,6 +199,30 @@ def test_resting_box_normal_impulse_equals_weight():
     # Coulomb box: |lambda_t| <= mu lambda_n per point (mu = sqrt(0.5 * 0.5)).
     mu = pts[0, :4, 37]
     assert np.all(np.abs(pts[0, :4, 31:33]) <= mu[:, None] * pts[0, :4, 30:31] + 1e-7)
+
+
+def test_residual_of_a_cold_first_sweep_is_the_rms_impulse():
+    # Cold start and one velocity iteration: each point is visited once and
+    # its increments are its whole accumulated impulses, so the residual
+    # (RMS over points of the sweep's impulse increments, all three axes)
+    # equals the RMS of the final impulses.
+    sc, d = _single_box_world(velocity_iterations=1)
+    ow = O.OracleWorld(sc, d)
+    ow.set_warm_start(1)
+    ow.step(np.array([[0, 1]], np.uint32))
+    hdr, pts = ow.constraints()
+    n = int(hdr[0, 2])
+    lam = pts[0, :n, 30:33].astype(np.float64)
+    r = ow.residuals()
+    assert len(r) == 1 and n == 4 and lam[:, 0].sum() > 0
+    assert r[0] == pytest.approx(math.sqrt((lam ** 2).sum() / n), rel=1e-12)
+    # with more sweeps the increments shrink: a resting box converges
+    sc, d = _single_box_world(velocity_iterations=8)
+    ow = O.OracleWorld(sc, d)
+    ow.set_warm_start(1)
+    ow.step(np.array([[0, 1]], np.uint32))
+    r = ow.residuals()
+    assert len(r) == 8 and r[-1] < 0.2 * r[0]
 
 
 def test_free_pair_conserves_momentum():
