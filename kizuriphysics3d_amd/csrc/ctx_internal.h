@@ -1,0 +1,303 @@
+// ctx_internal.h - the kzg_ctx object shared by the single-world C-ABI
+// (context.hip) and the batched-worlds C-ABI (batch.hip): device buffers,
+// staging, events and the upload / download helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <unordered_set>
+#include <vector>
+
+#include "kz_device.h"
+#include "kz_math.h"
+#include "kzg.h"
+#include "mlp.h"
+#include "schedule.h"
+
+namespace kz {
+void launch_narrowphase(hipStream_t s, const uint2* pairs, uint32_t n, const BodyBuf& b, const ShapeBuf& sb,
+                        const ManifoldBuf& m, uint32_t* work, uint32_t* work_count, float max_sep, int world_mode,
+                        uint32_t wave_blocks);
+void launch_integrate_velocities(hipStream_t s, const BodyBuf& b, uint32_t n, float dt, const float* g);
+void launch_build(hipStream_t s, const ContactBuf& c, const ManifoldBuf& m, const BodyBuf& b, const CacheBuf& cache,
+                  uint32_t nc, int warm);
+void launch_prepare(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32_t np, float dt, float rthr,
+                    float baum, float slop, float maxb, int use_simd, int use_vbias, const VelRecBuf& vr,
+                    const WsRecBuf& wr, const uint32_t* vel_pos, const uint32_t* ws_pos, float gdt, float4* feat,
+                    float4* base);
+void launch_writeback(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const ContactBuf& c,
+                      const uint32_t* vel_pos, const uint32_t* ws_pos, uint32_t np);
+void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b,
+                         const Schedule& ws, const Schedule& vel, const SolveParams& sp, uint32_t groups,
+                         uint32_t threads, float* resid);
+void launch_cache_store(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const CacheBuf& cache, uint32_t nc);
+void launch_cache_rehash(hipStream_t s, const CacheBuf& src, const CacheBuf& dst);
+void launch_finish(hipStream_t s, const BodyBuf& b, uint32_t n, float dt, int split);
+void launch_world_inertia(hipStream_t s, const BodyBuf& b, uint32_t n);
+}  // namespace kz
+
+using namespace kz;
+
+namespace kzctx {
+
+// Device array that grows geometrically and never copies on growth.
+template <class T> struct DevVec {
+    T* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= cap && p) return hipSuccess;
+        size_t nc = std::max<size_t>(n, cap + cap / 2);
+        nc = std::max<size_t>(nc, 64);
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, nc * sizeof(T));
+        if (e == hipSuccess) cap = nc;
+        return e;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Pinned host staging buffer.
+struct Pinned {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        size_t nc = std::max<size_t>(bytes, cap + cap / 2);
+        nc = std::max<size_t>(nc, 4096);
+        if (p) hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipHostMalloc(&p, nc, hipHostMallocDefault);
+        if (e == hipSuccess) cap = nc;
+        return e;
+    }
+    template <class T> T* as() { return static_cast<T*>(p); }
+    void release() {
+        if (p) hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+constexpr int kPointFloats = 44;
+
+}  // namespace kzctx
+
+using namespace kzctx;
+
+struct kzg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    kz_world_desc world{};
+    uint32_t cu_count = 256;
+
+    // shapes
+    uint32_t nshapes = 0;
+    std::vector<uint32_t> h_shape_type;
+    DevVec<uint32_t> s_type;
+    DevVec<float4> s_params;
+    DevVec<uint4> s_hull;
+    DevVec<float4> s_hverts;
+    DevVec<uint32_t> s_htris;
+    DevVec<float4> s_hfn;
+
+    // bodies
+    uint32_t nbodies = 0;
+    std::vector<uint32_t> h_packed;
+    std::vector<uint8_t> h_dynamic, h_sensor, h_motion;
+    DevVec<float4> b_x, b_q, b_v, b_w, b_pv, b_pw, b_iw, b_il, b_mat, b_lim;
+    DevVec<uint32_t> b_meta, b_packed;
+
+    // pairs and manifolds
+    uint32_t npairs = 0;
+    std::vector<uint32_t> h_pairs;
+    DevVec<uint2> d_pairs;
+    DevVec<uint32_t> m_count;
+    DevVec<float4> m_pa, m_pb, m_n;
+    DevVec<uint32_t> work, work_count;
+
+    // constraints and points
+    uint32_t ncons = 0, npoints = 0;
+    std::vector<U4> h_info;
+    std::vector<uint32_t> h_cpair;
+    DevVec<uint4> c_info;
+    DevVec<uint32_t> c_pair;
+    DevVec<uint4> p_body;
+    DevVec<float4> p_pa, p_pb, p_n, p_t1, p_t2, p_ra, p_rb, p_la, p_lb, p_imp;
+    DevVec<float> p_nvel;
+
+    // schedule
+    ScheduleBuilder sched_builder;
+    ScheduleOutput sched;
+    DevVec<uint32_t> ws_off, vel_off, ws_pos, vel_pos;
+    DevVec<uint2> ws_grp, vel_grp;
+    // sweep records
+    DevVec<uint4> vr_bd, wr_bd;
+    DevVec<float4> vr_ra, vr_rb, vr_n, vr_t1, vr_t2, vr_imp;
+    DevVec<float2> vr_fb;
+    DevVec<float4> wr_ra, wr_rb, wr_qa, wr_qb, wr_n, wr_t1, wr_t2, wr_imp;
+
+    // warm-start cache
+    DevVec<unsigned long long> cache_key;
+    DevVec<uint32_t> cache_used;
+    DevVec<uint32_t> cache_count;
+    DevVec<float4> cache_p;
+    DevVec<float2> cache_t;
+    uint32_t cache_cap = 0;
+
+    // residuals
+    DevVec<float> resid;
+    uint32_t resid_points = 0;
+
+    // staging and timing
+    Pinned stage, stage2, stage_a, stage_b;
+    hipEvent_t ev[10] = {};
+    uint32_t recorded = 0;  // bit i: ev[i] recorded since the last narrowphase
+    kzg_timings timings{};
+    double build_host_ms = 0.0;
+    int phase = 0;  // 0 idle, 1 narrowphase done, 2 built, 3 solved
+
+    // learned warm start
+    bool mlp_ready = false;
+    DevVec<float> mlp_master, mlp_m, mlp_v, mlp_norm, mlp_partial, mlp_out, mlp_loss;
+    DevVec<uint16_t> mlp_w1b, mlp_w2b, mlp_w3b;
+    uint32_t adam_t = 0;
+    DevVec<float4> feat;       // feature rows of the current step, 4 float4 per point
+    DevVec<float4> base;       // the cache's clamped impulses of the current step, 1 float4 per point
+    DevVec<float4> samples;    // training ring, 5 float4 per sample
+    DevVec<float4> x_samples;  // caller-supplied rows (forward / gradient entry points)
+    DevVec<float4> x_out;      // forward outputs
+    uint32_t sample_cap = 0;
+    uint64_t sample_count = 0;
+    int warm_mode = KZG_WARM_CACHE;  // kzg_step's mode
+    int cur_mode = KZG_WARM_CACHE;   // mode of the step being built and solved
+    bool no_store = false;
+
+    // standalone batch scratch
+    DevVec<float4> x_x, x_q;
+    DevVec<uint32_t> x_meta, x_packed;
+    DevVec<uint2> x_pairs;
+
+    BodyBuf bodies() {
+        return BodyBuf{b_x.p, b_q.p, b_v.p, b_w.p, b_pv.p, b_pw.p, b_iw.p, b_il.p, b_mat.p, b_lim.p, b_meta.p, b_packed.p};
+    }
+    ShapeBuf shapes() { return ShapeBuf{s_type.p, s_params.p, s_hull.p, s_hverts.p, s_htris.p, s_hfn.p}; }
+    ManifoldBuf manifolds() { return ManifoldBuf{m_count.p, m_pa.p, m_pb.p, m_n.p}; }
+    ContactBuf contacts() {
+        return ContactBuf{c_info.p, c_pair.p, p_body.p, p_pa.p, p_pb.p, p_n.p, p_t1.p, p_t2.p,
+                          p_ra.p, p_rb.p, p_la.p, p_lb.p, p_imp.p, p_nvel.p};
+    }
+    CacheBuf cache() { return CacheBuf{cache_key.p, cache_used.p, cache_count.p, cache_p.p, cache_t.p, cache_cap}; }
+    MlpBuf mlp() {
+        return MlpBuf{mlp_master.p, mlp_w1b.p, mlp_w2b.p, mlp_w3b.p, mlp_norm.p, mlp_m.p, mlp_v.p};
+    }
+    VelRecBuf vel_records() { return VelRecBuf{vr_bd.p, vr_ra.p, vr_rb.p, vr_n.p, vr_t1.p, vr_t2.p, vr_fb.p, vr_imp.p}; }
+    WsRecBuf ws_records() {
+        return WsRecBuf{wr_bd.p, wr_ra.p, wr_rb.p, wr_qa.p, wr_qb.p, wr_n.p, wr_t1.p, wr_t2.p, wr_imp.p};
+    }
+};
+
+
+namespace kzctx {
+
+inline int fail(kzg_ctx* c, int code, const std::string& msg);
+
+inline hipError_t record(kzg_ctx* c, int k) {
+    c->recorded |= 1u << k;
+    return hipEventRecord(c->ev[k], c->stream);
+}
+
+inline int fail(kzg_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define KZG_HIP(ctx, call)                                                                            \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            return fail(ctx, KZG_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+template <class T> inline int upload(kzg_ctx* c, DevVec<T>& d, const T* src, size_t n) {
+    KZG_HIP(c, d.ensure(std::max<size_t>(n, 1)));
+    if (n) {
+        KZG_HIP(c, c->stage.ensure(n * sizeof(T)));
+        std::memcpy(c->stage.p, src, n * sizeof(T));
+        KZG_HIP(c, hipMemcpyAsync(d.p, c->stage.p, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+        KZG_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return KZG_OK;
+}
+
+template <class T> inline int download(kzg_ctx* c, T* dst, const DevVec<T>& d, size_t n) {
+    if (!n) return KZG_OK;
+    KZG_HIP(c, c->stage2.ensure(n * sizeof(T)));
+    KZG_HIP(c, hipMemcpyAsync(c->stage2.p, d.p, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    std::memcpy(dst, c->stage2.p, n * sizeof(T));
+    return KZG_OK;
+}
+
+inline float4 f4(float a, float b, float cc, float d) { return make_float4(a, b, cc, d); }
+
+inline int ensure_cache(kzg_ctx* c, size_t needed_keys) {
+    // Load factor at most 1/2 so that probing always ends quickly.
+    uint32_t want = c->cache_cap ? c->cache_cap : 1024u;
+    while (size_t(want) < 2 * needed_keys) want *= 2u;
+    if (want == c->cache_cap) return KZG_OK;
+    DevVec<unsigned long long> nk;
+    DevVec<uint32_t> nused, ncnt;
+    DevVec<float4> np;
+    DevVec<float2> nt;
+    KZG_HIP(c, nk.ensure(want));
+    KZG_HIP(c, nused.ensure(1));
+    KZG_HIP(c, ncnt.ensure(want));
+    KZG_HIP(c, np.ensure(size_t(want) * kPairPoints));
+    KZG_HIP(c, nt.ensure(size_t(want) * kPairPoints));
+    KZG_HIP(c, hipMemsetAsync(nk.p, 0, sizeof(unsigned long long) * want, c->stream));
+    KZG_HIP(c, hipMemsetAsync(nused.p, 0, sizeof(uint32_t), c->stream));
+    CacheBuf dst{nk.p, nused.p, ncnt.p, np.p, nt.p, want};
+    if (c->cache_cap) launch_cache_rehash(c->stream, c->cache(), dst);
+    KZG_HIP(c, hipGetLastError());
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    c->cache_key.release();
+    c->cache_used.release();
+    c->cache_count.release();
+    c->cache_p.release();
+    c->cache_t.release();
+    c->cache_key = nk;
+    c->cache_used = nused;
+    c->cache_count = ncnt;
+    c->cache_p = np;
+    c->cache_t = nt;
+    c->cache_cap = want;
+    return KZG_OK;
+}
+
+// Asynchronous upload through a pinned staging buffer: the caller guarantees
+// that the buffer is not reused before the stream has drained.
+template <class T> inline int upload_async(kzg_ctx* c, DevVec<T>& d, Pinned& st, size_t& off, const T* src, size_t n) {
+    KZG_HIP(c, d.ensure(std::max<size_t>(n, 1)));
+    if (!n) return KZG_OK;
+    off = (off + 255) & ~size_t(255);
+    std::memcpy(static_cast<char*>(st.p) + off, src, n * sizeof(T));
+    KZG_HIP(c, hipMemcpyAsync(d.p, static_cast<char*>(st.p) + off, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    off += n * sizeof(T);
+    return KZG_OK;
+}
+
+}  // namespace kzctx
+
+using namespace kzctx;

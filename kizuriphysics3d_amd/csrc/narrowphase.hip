@@ -137,9 +137,10 @@ __device__ __forceinline__ ShapeView load_shape(const ShapeBuf& s, uint32_t i) {
 
 // Pair inputs: body indices.  For the world path body = slot; for the
 // standalone batch every pair side is its own "body" carrying a transform.
-__global__ void k_collide_simple(const uint2* pairs, uint32_t n, BodyBuf b, ShapeBuf s, ManifoldBuf m,
-                                 uint32_t* work, uint32_t* work_count, float max_sep, int world_mode) {
+__global__ void k_collide_simple(const uint2* pairs, uint32_t n, const uint32_t* n_dev, BodyBuf b, ShapeBuf s,
+                                 ManifoldBuf m, uint32_t* work, uint32_t* work_count, float max_sep, int world_mode) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n_dev) n = *n_dev;  // count kept on the device (graph-captured steps)
     if (i >= n) return;
     const uint2 pr = pairs[i];
     uint32_t ia = pr.x, ib = pr.y;
@@ -1048,9 +1049,20 @@ void launch_narrowphase(hipStream_t s, const uint2* pairs, uint32_t n, const Bod
                         uint32_t wave_blocks) {
     if (!n) return;
     hipMemsetAsync(work_count, 0, sizeof(uint32_t), s);
-    k_collide_simple<<<nblocks(n, 256), 256, 0, s>>>(pairs, n, b, sb, m, work, work_count, max_sep, world_mode);
+    k_collide_simple<<<nblocks(n, 256), 256, 0, s>>>(pairs, n, nullptr, b, sb, m, work, work_count, max_sep,
+                                                      world_mode);
     uint32_t grid = wave_blocks ? wave_blocks : (n < 65536u ? n : 65536u);
     k_collide_wave<<<grid, 64, 0, s>>>(pairs, work, work_count, b, sb, m, max_sep);
+}
+
+// Capacity-sized launch with the pair count read on the device; work_count
+// must already be zero (no memset: the launch may be graph-captured).
+void launch_narrowphase_counted(hipStream_t s, const uint2* pairs, uint32_t cap, const uint32_t* n_dev,
+                                const BodyBuf& b, const ShapeBuf& sb, const ManifoldBuf& m, uint32_t* work,
+                                uint32_t* work_count, float max_sep, uint32_t wave_blocks) {
+    if (!cap) return;
+    k_collide_simple<<<nblocks(cap, 256), 256, 0, s>>>(pairs, cap, n_dev, b, sb, m, work, work_count, max_sep, 1);
+    k_collide_wave<<<wave_blocks ? wave_blocks : 1024u, 64, 0, s>>>(pairs, work, work_count, b, sb, m, max_sep);
 }
 
 }  // namespace kz

@@ -60,8 +60,10 @@ __global__ void k_integrate_velocities(BodyBuf b, uint32_t n, float dt, float gx
 // constraint.  The host has already compacted the non-empty manifolds in pair
 // order and ordered each constraint's bodies by packed id (info).
 // ---------------------------------------------------------------------------
-__global__ void k_build(ContactBuf c, ManifoldBuf m, BodyBuf b, CacheBuf cache, uint32_t nc, int warm) {
+__global__ void k_build(ContactBuf c, ManifoldBuf m, BodyBuf b, CacheBuf cache, uint32_t nc, const uint32_t* nc_dev,
+                        int warm) {
     uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nc_dev) nc = *nc_dev;
     if (ci >= nc) return;
     const uint4 info = c.info[ci];
     const uint32_t pair = c.pair[ci];
@@ -136,9 +138,10 @@ __device__ __forceinline__ float eff_mass(float ima, float imb, const m3& ia, co
     k = k + dot(xb, mmul(ib, xb));
     return k > kEps ? 1.0f / k : 0.0f;
 }
-__global__ void k_prepare(ContactBuf c, BodyBuf b, uint32_t np, PrepParams pp, VelRecBuf vr, WsRecBuf wr,
-                          const uint32_t* vel_pos, const uint32_t* ws_pos) {
+__global__ void k_prepare(ContactBuf c, BodyBuf b, uint32_t np, const uint32_t* np_dev, PrepParams pp, VelRecBuf vr,
+                          WsRecBuf wr, const uint32_t* vel_pos, const uint32_t* ws_pos) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (np_dev) np = *np_dev;
     if (p >= np) return;
     const uint4 bd = c.body[p];
     const float4 xa = b.x[bd.x], xb = b.x[bd.y];
@@ -438,8 +441,9 @@ __global__ void __launch_bounds__(1024) k_solve_groups(VelRecBuf vr, WsRecBuf wr
 // read-back): velocity impulses from the velocity records, the bias impulse
 // from the warm-start/bias records.
 __global__ void k_writeback(VelRecBuf vr, WsRecBuf wr, ContactBuf c, const uint32_t* vel_pos,
-                            const uint32_t* ws_pos, uint32_t np) {
+                            const uint32_t* ws_pos, uint32_t np, const uint32_t* np_dev) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (np_dev) np = *np_dev;
     if (p >= np) return;
     float4 imp = c.imp[p];
     const uint32_t e = vel_pos[p];
@@ -455,8 +459,9 @@ __global__ void k_writeback(VelRecBuf vr, WsRecBuf wr, ContactBuf c, const uint3
 // Cache store (PhysicsWorld.cpp:568-581), one thread per constraint.  The
 // host guarantees free capacity (load factor <= 1/2), so probing ends.
 // ---------------------------------------------------------------------------
-__global__ void k_cache_store(ContactBuf c, BodyBuf b, CacheBuf cache, uint32_t nc) {
+__global__ void k_cache_store(ContactBuf c, BodyBuf b, CacheBuf cache, uint32_t nc, const uint32_t* nc_dev) {
     uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nc_dev) nc = *nc_dev;
     if (ci >= nc) return;
     const uint4 info = c.info[ci];
     const unsigned long long key = pair_key(b.packed[info.x], b.packed[info.y]);
@@ -554,18 +559,18 @@ void launch_integrate_velocities(hipStream_t s, const BodyBuf& b, uint32_t n, fl
 }
 void launch_build(hipStream_t s, const ContactBuf& c, const ManifoldBuf& m, const BodyBuf& b, const CacheBuf& cache,
                   uint32_t nc, int warm) {
-    if (nc) k_build<<<blocks(nc, 256), 256, 0, s>>>(c, m, b, cache, nc, warm);
+    if (nc) k_build<<<blocks(nc, 256), 256, 0, s>>>(c, m, b, cache, nc, nullptr, warm);
 }
 void launch_prepare(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32_t np, float dt, float rthr,
                     float baum, float slop, float maxb, int use_simd, int use_vbias, const VelRecBuf& vr,
                     const WsRecBuf& wr, const uint32_t* vel_pos, const uint32_t* ws_pos, float gdt, float4* feat,
                     float4* base) {
     PrepParams pp{dt, rthr, baum, slop, maxb, use_simd, use_vbias, gdt, feat, base};
-    if (np) k_prepare<<<blocks(np, 256), 256, 0, s>>>(c, b, np, pp, vr, wr, vel_pos, ws_pos);
+    if (np) k_prepare<<<blocks(np, 256), 256, 0, s>>>(c, b, np, nullptr, pp, vr, wr, vel_pos, ws_pos);
 }
 void launch_writeback(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const ContactBuf& c,
                       const uint32_t* vel_pos, const uint32_t* ws_pos, uint32_t np) {
-    if (np) k_writeback<<<blocks(np, 256), 256, 0, s>>>(vr, wr, c, vel_pos, ws_pos, np);
+    if (np) k_writeback<<<blocks(np, 256), 256, 0, s>>>(vr, wr, c, vel_pos, ws_pos, np, nullptr);
 }
 void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b,
                          const Schedule& ws, const Schedule& vel, const SolveParams& sp, uint32_t groups,
@@ -573,7 +578,7 @@ void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr,
     if (groups) k_solve_groups<<<groups, threads, 0, s>>>(vr, wr, b, ws, vel, sp, resid);
 }
 void launch_cache_store(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const CacheBuf& cache, uint32_t nc) {
-    if (nc) k_cache_store<<<blocks(nc, 256), 256, 0, s>>>(c, b, cache, nc);
+    if (nc) k_cache_store<<<blocks(nc, 256), 256, 0, s>>>(c, b, cache, nc, nullptr);
 }
 void launch_cache_rehash(hipStream_t s, const CacheBuf& src, const CacheBuf& dst) {
     if (src.cap) k_cache_rehash<<<blocks(src.cap, 256), 256, 0, s>>>(src, dst);
@@ -583,6 +588,27 @@ void launch_finish(hipStream_t s, const BodyBuf& b, uint32_t n, float dt, int sp
 }
 void launch_world_inertia(hipStream_t s, const BodyBuf& b, uint32_t n) {
     if (n) k_world_inertia<<<blocks(n, 256), 256, 0, s>>>(b, n);
+}
+
+// Capacity-sized launches whose counts are read on the device (graph-captured
+// steps of the batched worlds).
+void launch_build_counted(hipStream_t s, const ContactBuf& c, const ManifoldBuf& m, const BodyBuf& b,
+                          const CacheBuf& cache, uint32_t cap, const uint32_t* nc_dev, int warm) {
+    if (cap) k_build<<<blocks(cap, 256), 256, 0, s>>>(c, m, b, cache, cap, nc_dev, warm);
+}
+void launch_prepare_counted(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32_t cap, const uint32_t* np_dev,
+                            float dt, float rthr, float baum, float slop, float maxb, int use_simd, int use_vbias,
+                            const VelRecBuf& vr, const WsRecBuf& wr, const uint32_t* vel_pos, const uint32_t* ws_pos) {
+    PrepParams pp{dt, rthr, baum, slop, maxb, use_simd, use_vbias, 0.0f, nullptr, nullptr};
+    if (cap) k_prepare<<<blocks(cap, 256), 256, 0, s>>>(c, b, cap, np_dev, pp, vr, wr, vel_pos, ws_pos);
+}
+void launch_writeback_counted(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const ContactBuf& c,
+                              const uint32_t* vel_pos, const uint32_t* ws_pos, uint32_t cap, const uint32_t* np_dev) {
+    if (cap) k_writeback<<<blocks(cap, 256), 256, 0, s>>>(vr, wr, c, vel_pos, ws_pos, cap, np_dev);
+}
+void launch_cache_store_counted(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const CacheBuf& cache,
+                                uint32_t cap, const uint32_t* nc_dev) {
+    if (cap) k_cache_store<<<blocks(cap, 256), 256, 0, s>>>(c, b, cache, cap, nc_dev);
 }
 
 }  // namespace kz

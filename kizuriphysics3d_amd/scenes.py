@@ -271,6 +271,77 @@ def pile(num_bodies: int = 32768, seed: int = 0x4B5A0002, width: float = 64.0,
     return sc
 
 
+def _ground_plane(sc: Scene) -> None:
+    plane = sc.add_shape(SHAPE_PLANE, (0.0, 1.0, 0.0, 0.0))
+    sc.add_body(default_body(plane, MOTION_STATIC))
+
+
+def box_stack(num_bodies: int, rng: SplitMix64, name: str = "stack") -> Scene:
+    """A random box stack on its own ground plane: ``num_bodies`` boxes of one
+    random half extent h ~ U[0.3, 0.6] in pyramid rows (the widest row at the
+    bottom, each row centred on the one below, a gap of 1% of a box between
+    neighbours), rows starting exactly touching."""
+    sc = Scene(name, world_settings(8, 2))
+    _ground_plane(sc)
+    h = rng.uniform(0.3, 0.6)
+    box = sc.add_shape(SHAPE_BOX, (h, h, h))
+    base = 1
+    while base * (base + 1) // 2 < num_bodies:
+        base += 1
+    pitch = 2.0 * h * 1.01
+    placed, row = 0, 0
+    while placed < num_bodies:
+        n = min(base - row, num_bodies - placed)
+        x0 = -0.5 * (n - 1) * pitch
+        for i in range(n):
+            b = default_body(box)
+            b["position"] = (x0 + i * pitch, h + row * 2.0 * h, 0.0)
+            sc.add_body(b)
+        placed += n
+        row += 1
+    return sc
+
+
+def small_pile(num_bodies: int, rng: SplitMix64, name: str = "pile") -> Scene:
+    """The random convex pile's body mix on its own ground plane, inside four
+    static box walls; the bin width grows with the body count so that the
+    drop column is about 4 bodies deep."""
+    sc = Scene(name, world_settings(8, 2))
+    _ground_plane(sc)
+    width = max(4.0, math.sqrt(num_bodies / 4.0) * 1.6)
+    half = 0.5 * width
+    hy = 8.0
+    for pos, he in (((-half - 0.5, hy, 0.0), (0.5, hy, half)), ((half + 0.5, hy, 0.0), (0.5, hy, half)),
+                    ((0.0, hy, -half - 0.5), (half, hy, 0.5)), ((0.0, hy, half + 0.5), (half, hy, 0.5))):
+        s = sc.add_shape(SHAPE_BOX, he)
+        b = default_body(s, MOTION_STATIC)
+        b["position"] = pos
+        sc.add_body(b)
+    top = 1.0 + 1.6 * num_bodies / (width * width) * 4.0
+    for i in range(num_bodies):
+        b = random_convex_body(sc, rng, i % 4)
+        b["position"] = (rng.uniform(-half + 0.8, half - 0.8), rng.uniform(1.0, top), rng.uniform(-half + 0.8, half - 0.8))
+        sc.add_body(b)
+    return sc
+
+
+def batched_worlds(num_worlds: int = 4096, seed: int = 0x4B5A0004, min_bodies: int = 16,
+                   max_bodies: int = 512) -> list:
+    """Independent worlds of uneven size: body counts log-uniform in
+    [min_bodies, max_bodies], each world a random box stack or a convex pile
+    with its own ground plane (chosen with equal odds)."""
+    rng = SplitMix64(seed)
+    out = []
+    for w in range(num_worlds):
+        n = int(round(min_bodies * (max_bodies / min_bodies) ** rng.uniform(0.0, 1.0)))
+        n = max(min_bodies, min(max_bodies, n))
+        if rng.uniform(0.0, 1.0) < 0.5:
+            out.append(box_stack(n, rng, f"stack{w}"))
+        else:
+            out.append(small_pile(n, rng, f"pile{w}"))
+    return out
+
+
 def _quat_rotate(q: np.ndarray, v: np.ndarray, inverse: bool = False) -> np.ndarray:
     """Rotate each row of v by the matching quaternion row of q (or its inverse)."""
     qv = -q[:, :3] if inverse else q[:, :3]
