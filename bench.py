@@ -1,21 +1,32 @@
 #!/usr/bin/env python3
-"""Contact-path benchmark: contacts solved per second on the 32k-body pile.
+"""Contact-path benchmarks on one MI355X.
 
-Workload: the random convex pile of 32,768 bodies (25 % spheres, boxes,
-capsules, convex hulls of 16-32 points; dropped from y in [2, 60] into a
-64 x 64 m walled bin; density 1000; FP32; 8 velocity + 2 position
-iterations; dt = 1/60; sleeping off).  W warm-up StepFixed calls let the pile
-form, then K StepFixed calls are timed.  A "step" is one StepFixed: host
-broadphase, then narrowphase + constraint build + solve + integration on the
-GPU.  The metric is sum(contact points) / sum(time of narrowphase + build +
-solve), measured on the host clock with the device synchronised (the host
-part of the build - compaction and the solve schedule - is inside it).
+Default workload (the headline metric): contacts solved per second on the
+random convex pile of 32,768 bodies (25 % spheres, boxes, capsules, convex
+hulls of 16-32 points; dropped from y in [2, 60] into a 64 x 64 m walled bin;
+density 1000; FP32; 8 velocity + 2 position iterations; dt = 1/60; sleeping
+off).  W warm-up StepFixed calls let the pile form, then K StepFixed calls
+are timed.  A step is one StepFixed: host broadphase, then narrowphase,
+constraint build, solve and integration on the GPU.  value = sum(contact
+points) / sum(narrowphase kernels + host build compute + integrate
+velocities + solve), with inputs in HBM; the rate including the PCIe
+transfers at the boundary is reported beside it.  The learned warm start is
+trained during the last warm-up steps and compared with the cache and cold
+warm starts on the final state (PGS residual after k sweeps).
 
-Also reported: the roofline of the dominant kernel (HIP events on the
-library's stream) and the reference CPU path (the reference engine built from
-its own sources, oracle/_ref) on a bounded sample of the same pile.
+Other workloads:
+  --workload batched      4,096 independent worlds of 16-512 bodies (box
+                          stacks and convex piles, each on its own ground
+                          plane), one captured HIP graph per step, launch
+                          configuration chosen by the autotuner
+  --workload narrowphase  1,048,576 convex pairs (hull-hull 8-64 vertices,
+                          box-hull, capsule-box), penetration U[0, 0.05] m,
+                          10 % separated by U[0, 0.02] m
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--bodies N]
+Every workload reports the reference CPU path (the reference engine built
+from its own sources, oracle/_ref) on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--bodies N] [--workload pile|batched|narrowphase]
 """
 from __future__ import annotations
 
@@ -88,6 +99,176 @@ def cpu_baseline(scene, states, budget_s: float, threads: int):
                       f"workers + caller on {cpu_model()}"}
 
 
+def bench_batched(args):
+    """4,096 independent worlds, one captured graph per step."""
+    import torch
+    from kizuriphysics3d_amd.batch import BatchedWorlds
+    t0 = time.time()
+    scenes = S.batched_worlds(args.worlds)
+    bw = BatchedWorlds(scenes)
+    nb = sum(len(sc.bodies) for sc in scenes)
+    log(f"batch: {len(scenes)} worlds, {nb} bodies, built in {time.time() - t0:.1f} s")
+    tuned_steps = 0
+    for i in range(args.warmup):
+        bw.step()
+        if (i + 1) % 50 == 0:
+            st = bw.stats()
+            log(f"warmup {i + 1}/{args.warmup}: {st['num_points']} points, device {st['device_ms']:.2f} ms, "
+                f"host broadphase {st['broadphase_ms']:.1f} ms, step {st['step_ms']:.1f} ms")
+    cands = bw.candidates()
+    best, times = bw.autotune(repeats=2)
+    tuned_steps = 2 * len(cands)
+    log(f"autotune over {len(cands)} configurations ({tuned_steps} steps): best {cands[best]}, "
+        f"medians ms {np.round(times, 3).tolist()}")
+    torch.cuda.synchronize()
+    points = 0
+    dev_ms = copies_ms = bp_ms = 0.0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        bw.step()
+        st = bw.stats()
+        points += st["num_points"]
+        dev_ms += st["device_ms"]
+        copies_ms += st["device_with_copies_ms"]
+        bp_ms += st["broadphase_ms"]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    st = bw.stats()
+    log(f"timed {args.steps} steps: {points / args.steps:.0f} points/step, device {dev_ms / args.steps:.2f} ms "
+        f"({copies_ms / args.steps:.2f} ms with the state copies), host broadphase {bp_ms / args.steps:.1f} ms, "
+        f"levels vel/bias max {st['max_velocity_levels']}/{st['max_bias_levels']}")
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline_worlds(scenes, bw, args.cpu_budget, min(os.cpu_count() or 1, 16))
+        except Exception as e:
+            log(f"cpu baseline unavailable: {e}")
+    total_steps = args.warmup + tuned_steps + args.steps
+    return {
+        "metric": "contacts solved/s, 4096 batched worlds",
+        "value": points / (dev_ms / 1e3),
+        "unit": "contacts/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup + tuned_steps,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{len(scenes)} worlds of 16-512 bodies (log-uniform; box stacks and convex piles, "
+                               f"each on its own ground plane), 8+2 iterations, dt 1/60, steps "
+                               f"{total_steps - args.steps + 1}-{total_steps}",
+                   "worlds": len(scenes), "bodies": nb, "contact_points_per_step": points / args.steps,
+                   "device_ms_per_step": dev_ms / args.steps,
+                   "device_ms_per_step_with_state_copies": copies_ms / args.steps,
+                   "host_broadphase_ms_per_step": bp_ms / args.steps,
+                   "world_steps_per_s_device": len(scenes) * args.steps / (dev_ms / 1e3),
+                   "autotuned": cands[best]},
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+
+
+def cpu_baseline_worlds(scenes, bw, budget_s: float, threads: int):
+    """The reference engine stepping worlds of the batch one after another,
+    restarted from the GPU run's states (one untimed warm-start step each)."""
+    from oracle import refharness as R
+    if not R.available():
+        return None
+    workers = R.init_jobs(max(threads - 1, 1))
+    t_begin = time.time()
+    pts, ms, n, worlds = 0, 0.0, 0, 0
+    for k, sc in enumerate(scenes):
+        if time.time() - t_begin > budget_s:
+            break
+        rw = R.RefWorld(sc)
+        rw.set_states(bw.states(k))
+        rw.step_phases()
+        for _ in range(3):
+            ph = rw.step_phases()
+            hdr, _ = rw.constraints()
+            pts += int(hdr[:, 2].sum())
+            ms += float(ph[1] + ph[2] + ph[3] + ph[4])
+            n += 1
+        rw.close()
+        worlds += 1
+    if ms <= 0:
+        return None
+    return {"value": pts / (ms / 1e3), "unit": "contacts/s", "cores": workers + 1, "kind": "reference",
+            "sample": f"{worlds} of the batch's worlds, each restarted from the GPU run's state and stepped 3 times "
+                      f"after one untimed warm-start step ({n} StepFixed calls); phases Collide + "
+                      f"BuildContactConstraints + IntegrateVelocities + SolveContacts; JobSystem with {workers} "
+                      f"workers + caller on {cpu_model()}"}
+
+
+def bench_narrowphase(args):
+    """1,048,576 convex pairs through the standalone narrowphase."""
+    import torch
+    from kizuriphysics3d_amd import device as D
+    from kizuriphysics3d_amd.world import build_hulls
+    t0 = time.time()
+    pb = S.pair_batch(args.pairs)
+    hulls = build_hulls(pb.shapes, pb.hull_points)
+    ctx = D.DeviceContext(S.world_settings())
+    ctx.upload_shapes(pb.shapes, hulls)
+    log(f"narrowphase: {args.pairs} pairs over {len(pb.shapes)} shapes, prepared in {time.time() - t0:.1f} s")
+    for _ in range(max(args.warmup, 1)):
+        counts, pts, ms = ctx.collide_batch(pb.shape_a, pb.shape_b, pb.xf_a, pb.xf_b)
+    torch.cuda.synchronize()
+    total_ms = 0.0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        counts, pts, ms = ctx.collide_batch(pb.shape_a, pb.shape_b, pb.xf_a, pb.xf_b)
+        total_ms += ms
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    per = total_ms / args.steps
+    # Algorithmic bytes per pair: 2 transforms (2 x 32 B) + shape records
+    # (2 x 28 B) + hull vertices (16 B each) and, for box/hull pairs, their
+    # triangles with face normals (20 B each), + count (4 B) + points (48 B each).
+    nverts = np.zeros(len(pb.shapes), np.int64)
+    ntris = np.zeros(len(pb.shapes), np.int64)
+    for i, (v, t) in hulls.items():
+        nverts[i], ntris[i] = len(v), len(t)
+    poly = (pb.shapes["type"][pb.shape_a] != S.SHAPE_CAPSULE)
+    bytes_pair = (64 + 56 + 16 * (nverts[pb.shape_a] + nverts[pb.shape_b]) +
+                  20 * np.where(poly, ntris[pb.shape_a] + ntris[pb.shape_b], 0) + 4 + 48 * counts.astype(np.int64))
+    achieved = float(bytes_pair.sum()) / (per / 1e3) / 1e9
+    log(f"timed {args.steps} batches: {per:.2f} ms per {args.pairs} pairs, {int(counts.sum())} points, "
+        f"points/pair histogram {np.bincount(counts, minlength=5).tolist()}")
+    cpu = None
+    if not args.no_cpu:
+        try:
+            from oracle import refharness as R
+            if R.available():
+                workers = R.init_jobs(max(min(os.cpu_count() or 1, 16) - 1, 1))
+                n = min(args.pairs, 1 << 16)
+                sample = S.PairBatch(pb.shapes, pb.hull_points, pb.shape_a[:n], pb.shape_b[:n], pb.xf_a[:n],
+                                     pb.xf_b[:n])
+                R.collide_batch(sample)
+                t1 = time.perf_counter()
+                reps = 0
+                while reps == 0 or time.perf_counter() - t1 < args.cpu_budget * 0.5:
+                    R.collide_batch(sample)
+                    reps += 1
+                cpu_s = (time.perf_counter() - t1) / reps
+                cpu = {"value": n / cpu_s, "unit": "pairs/s", "cores": workers + 1, "kind": "reference",
+                       "sample": f"the first {n} pairs of the batch through NarrowPhase::Collide, {reps} passes, "
+                                 f"ParallelFor with {workers} workers + caller on {cpu_model()}"}
+        except Exception as e:
+            log(f"cpu baseline unavailable: {e}")
+    return {
+        "metric": "narrowphase pairs/s, 1M convex pairs",
+        "value": args.pairs / (per / 1e3),
+        "unit": "pairs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": max(args.warmup, 1),
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{args.pairs} convex pairs: hull-hull (8-64 vertices), box-hull, capsule-box; "
+                               "penetration U[0, 0.05] m, 10 % separated by U[0, 0.02] m",
+                   "pairs": args.pairs, "contact_points": int(counts.sum()), "kernel_ms_per_batch": per},
+        "roofline": {"bound": "hbm", "kernel": "k_collide_simple + k_collide_wave", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,11 +282,20 @@ def main():
     ap.add_argument("--learn-collect", type=int, default=100, help="last warm-up steps that collect training rows")
     ap.add_argument("--learn-steps", type=int, default=2000, help="Adam steps on the device")
     ap.add_argument("--learn-batch", type=int, default=16384)
+    ap.add_argument("--workload", choices=("pile", "batched", "narrowphase"), default="pile")
+    ap.add_argument("--worlds", type=int, default=4096, help="batched workload: number of worlds")
+    ap.add_argument("--pairs", type=int, default=1 << 20, help="narrowphase workload: number of pairs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the contact path of one world runs on one MI355X "
               "(multi-GPU sharding is a design note, not part of this build); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        print(json.dumps(bench_batched(args)), flush=True)
+        return
+    if args.workload == "narrowphase":
+        print(json.dumps(bench_narrowphase(args)), flush=True)
+        return
 
     import torch  # device synchronisation around the timed region
     from kizuriphysics3d_amd.world import PhysicsWorld, device_timings

@@ -16,7 +16,7 @@
  *     reference's first-fit batches, the level schedule, prepare, the
  *     persistent solve kernel (one workgroup per world: warm start,
  *     velocity and bias sweeps), write-back, cache store and integration;
- *   - the body states are copied back inside the graph.
+ *   - the body states are copied back to the host after the graph.
  *
  * The launch configuration (waves for the iterative narrowphase, threads
  * per world in the solve) never changes results; kzb_autotune times
@@ -51,8 +51,9 @@ typedef struct kzb_stats {
     uint32_t max_velocity_levels, max_bias_levels;
     uint32_t reserved;
     double broadphase_ms;   /* host: every world's broadphase */
-    double device_ms;       /* HIP events around the graph launch: the whole device step */
+    double device_ms;       /* HIP events around the graph: the device step */
     double step_ms;         /* host clock around kzb_step */
+    double device_with_copies_ms;  /* the graph and the copies of the body states to the host */
 } kzb_stats;
 
 int kzb_create(kzb_batch** out, const kz_world_desc* settings, int device);

@@ -14,7 +14,7 @@ CONFIG_DTYPE = np.dtype([("narrowphase_waves", "<u4"), ("solve_threads", "<u4"),
 STATS_DTYPE = np.dtype([
     ("num_worlds", "<u4"), ("num_bodies", "<u4"), ("num_pairs", "<u4"), ("num_constraints", "<u4"),
     ("num_points", "<u4"), ("max_velocity_levels", "<u4"), ("max_bias_levels", "<u4"), ("reserved", "<u4"),
-    ("broadphase_ms", "<f8"), ("device_ms", "<f8"), ("step_ms", "<f8"),
+    ("broadphase_ms", "<f8"), ("device_ms", "<f8"), ("step_ms", "<f8"), ("device_with_copies_ms", "<f8"),
 ])
 KZB_EXPORTS = ["kzb_create", "kzb_destroy", "kzb_last_error", "kzb_add_world", "kzb_step", "kzb_num_worlds",
                "kzb_get_states", "kzb_get_stats", "kzb_set_config", "kzb_get_config", "kzb_autotune",

@@ -187,19 +187,37 @@ __global__ void k_collide_simple(const uint2* pairs, uint32_t n, const uint32_t*
 constexpr int kEpaMaxVerts = 72;    // 4 seed vertices + one per iteration (64)
 constexpr int kEpaMaxFaces = 512;   // EPA.cpp:206
 constexpr int kEpaMaxHorizon = 256; // EPA.cpp:253
+constexpr int kEdgeStride = 72;     // > 4 + 64 vertices
+constexpr int kEdgeWords = (kEdgeStride * kEdgeStride + 31) / 32;
 
 struct SP {
     f3 a, b, p;
 };
 
-// Register-resident arrays of up to four support points: the index is
-// uniform across the wave, so these selects keep the arrays out of scratch.
-__device__ __forceinline__ void sp_set(SP* a, int i, const SP& v) {
-    if (i == 0) a[0] = v;
-    else if (i == 1) a[1] = v;
-    else if (i == 2) a[2] = v;
-    else a[3] = v;
+// Up to four support points in named registers.  Indices are uniform
+// across the wave; the accessors are select chains, so no private array
+// (and no scratch memory) is involved.
+struct Simplex {
+    SP v0, v1, v2, v3;
+};
+// Component-wise value selects: no address is ever selected, so the
+// simplex stays in registers.
+__device__ __forceinline__ f3 sel3(bool c, f3 a, f3 b) {
+    return f3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z};
 }
+__device__ __forceinline__ SP selsp(bool c, const SP& a, const SP& b) {
+    return SP{sel3(c, a.a, b.a), sel3(c, a.b, b.b), sel3(c, a.p, b.p)};
+}
+__device__ __forceinline__ SP sget(const Simplex& s, int i) {
+    return selsp(i == 0, s.v0, selsp(i == 1, s.v1, selsp(i == 2, s.v2, s.v3)));
+}
+__device__ __forceinline__ void sset(Simplex& s, int i, const SP& v) {
+    s.v0 = selsp(i == 0, v, s.v0);
+    s.v1 = selsp(i == 1, v, s.v1);
+    s.v2 = selsp(i == 2, v, s.v2);
+    s.v3 = selsp(i == 3, v, s.v3);
+}
+__device__ __forceinline__ f3 pick(bool c, f3 a, f3 b) { return sel3(c, a, b); }
 
 struct WaveLds {
     // EPA polytope.
@@ -208,11 +226,18 @@ struct WaveLds {
     uint32_t fabc[kEpaMaxFaces];          // a | b << 8 | c << 16
     uint16_t vis[kEpaMaxFaces];
     uint16_t hor[kEpaMaxHorizon];         // edge u | v << 8
+    // Directed edges (u, v) of the visible faces, bit u * 72 + v (at most
+    // 4 + 64 polytope vertices).
+    uint32_t edges[kEdgeWords];
     // Exhaustive EPA seed candidates (EPA.cpp:116-146), 9 floats each.
     float cand16[16][9];
     // Clipping scratch (written by lane 0 only).
     float poly[2][2 * kMaxFaceVerts][3];
     float cand[8][11];
+    // Face polygons of A and B (GetShapeFace, at most 4 vertices).
+    float face[2][4][3];
+    // Result contacts of the pair: pa, pb, normal, separation, feature id.
+    float out[4][11];
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
@@ -282,7 +307,10 @@ __device__ __forceinline__ f3 wsupport(const WShape& w, f3 d) {
     wave_argmax(best, bi);
     if (bi == 0x7FFFFFFF) bi = 0;
     const int slot = bi >> 6, src = bi & 63;
-    f3 mine = slot == 0 ? w.hv[0] : (slot == 1 ? w.hv[1] : (slot == 2 ? w.hv[2] : w.hv[3]));
+    f3 mine = w.hv[3];
+    if (slot == 0) mine = w.hv[0];
+    else if (slot == 1) mine = w.hv[1];
+    else if (slot == 2) mine = w.hv[2];
     return f3{__shfl(mine.x, src, 64), __shfl(mine.y, src, 64), __shfl(mine.z, src, 64)};
 }
 
@@ -298,7 +326,7 @@ __device__ __forceinline__ SP msupport(const WShape& A, const WShape& B, f3 d) {
 }
 
 // --------------------------- GJK (src/Collision/GJK.cpp) -------------------
-__device__ f3 closest_on_triangle(f3 p, f3 a, f3 b, f3 c, float& u, float& v, float& w) {
+__device__ __forceinline__ f3 closest_on_triangle(f3 p, f3 a, f3 b, f3 c, float& u, float& v, float& w) {
     const f3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
     const float d1 = dot(ab, ap), d2 = dot(ac, ap);
     if (d1 <= 0.0f && d2 <= 0.0f) { u = 1; v = 0; w = 0; return a; }
@@ -333,27 +361,32 @@ __device__ f3 closest_on_triangle(f3 p, f3 a, f3 b, f3 c, float& u, float& v, fl
     return add(add(a, mul(ab, v)), mul(ac, w));
 }
 
-__device__ bool reduce_simplex(SP* s, int& n, f3& closest) {
+__device__ __forceinline__ bool reduce_simplex(Simplex& s, int& n, f3& closest) {
     const f3 O{0.0f, 0.0f, 0.0f};
     if (n == 1) {
-        closest = s[0].p;
+        closest = s.v0.p;
     } else if (n == 2) {
-        const f3 a = s[0].p, b = s[1].p, ab = sub(b, a);
+        const f3 a = s.v0.p, b = s.v1.p, ab = sub(b, a);
         const float t = -dot(a, ab);
         if (t <= 0.0f) { closest = a; n = 1; }
-        else if (t >= len2(ab)) { s[0] = s[1]; closest = b; n = 1; }
+        else if (t >= len2(ab)) { s.v0 = s.v1; closest = b; n = 1; }
         else closest = add(a, mul(ab, t / len2(ab)));
     } else if (n == 3) {
         float u, v, w;
-        closest = closest_on_triangle(O, s[0].p, s[1].p, s[2].p, u, v, w);
-        SP keep[4] = {};
+        closest = closest_on_triangle(O, s.v0.p, s.v1.p, s.v2.p, u, v, w);
+        Simplex keep = s;
         int k = 0;
-        if (u > 0.0f) sp_set(keep, k++, s[0]);
-        if (v > 0.0f) sp_set(keep, k++, s[1]);
-        if (w > 0.0f) sp_set(keep, k++, s[2]);
-        if (k == 0) k = 1;
-        for (int i = 0; i < 3; ++i)
-            if (i < k) s[i] = keep[i];
+        if (u > 0.0f) sset(keep, k++, s.v0);
+        if (v > 0.0f) sset(keep, k++, s.v1);
+        if (w > 0.0f) sset(keep, k++, s.v2);
+        if (k == 0) {
+            // no positive weight: the reference keeps one default (zero) point
+            keep.v0 = SP{f3{0.0f, 0.0f, 0.0f}, f3{0.0f, 0.0f, 0.0f}, f3{0.0f, 0.0f, 0.0f}};
+            k = 1;
+        }
+        s.v0 = keep.v0;
+        if (k > 1) s.v1 = keep.v1;
+        if (k > 2) s.v2 = keep.v2;
         n = k;
     } else if (n == 4) {
         auto outside = [&](f3 p0, f3 p1, f3 p2, f3 opp) {
@@ -361,33 +394,38 @@ __device__ bool reduce_simplex(SP* s, int& n, f3& closest) {
             if (dot(nn, sub(opp, p0)) > 0.0f) nn = neg(nn);
             return dot(nn, sub(O, p0)) > 0.0f;
         };
-        const bool oABC = outside(s[0].p, s[1].p, s[2].p, s[3].p);
-        const bool oACD = outside(s[0].p, s[2].p, s[3].p, s[1].p);
-        const bool oADB = outside(s[0].p, s[3].p, s[1].p, s[2].p);
-        const bool oBDC = outside(s[1].p, s[3].p, s[2].p, s[0].p);
+        const bool oABC = outside(s.v0.p, s.v1.p, s.v2.p, s.v3.p);
+        const bool oACD = outside(s.v0.p, s.v2.p, s.v3.p, s.v1.p);
+        const bool oADB = outside(s.v0.p, s.v3.p, s.v1.p, s.v2.p);
+        const bool oBDC = outside(s.v1.p, s.v3.p, s.v2.p, s.v0.p);
         if (!oABC && !oACD && !oADB && !oBDC) return true;
         float best = kInf;
-        SP tri[4] = {};
+        Simplex tri = s;
         int cnt = 0;
-        const int fi[4][3] = {{0, 1, 2}, {0, 2, 3}, {0, 3, 1}, {1, 3, 2}};
-        const bool fo[4] = {oABC, oACD, oADB, oBDC};
-        for (int f = 0; f < 4; ++f) {
-            if (!fo[f]) continue;
+        // The faces ABC, ACD, ADB, BDC in this order; strict < keeps the first.
+        auto consider = [&](bool out, const SP& x, const SP& y, const SP& z) {
+            if (!out) return;
             float u, v, w;
-            const f3 p = closest_on_triangle(O, s[fi[f][0]].p, s[fi[f][1]].p, s[fi[f][2]].p, u, v, w);
+            const f3 p = closest_on_triangle(O, x.p, y.p, z.p, u, v, w);
             const float d2 = len2(p);
             if (d2 < best) {
                 best = d2;
                 int k = 0;
-                if (u > 0.0f) sp_set(tri, k++, s[fi[f][0]]);
-                if (v > 0.0f) sp_set(tri, k++, s[fi[f][1]]);
-                if (w > 0.0f) sp_set(tri, k++, s[fi[f][2]]);
+                if (u > 0.0f) sset(tri, k++, x);
+                if (v > 0.0f) sset(tri, k++, y);
+                if (w > 0.0f) sset(tri, k++, z);
                 cnt = k;
                 closest = p;
             }
-        }
-        for (int i = 0; i < 3; ++i)
-            if (i < cnt) s[i] = tri[i];
+        };
+        const Simplex o = s;
+        consider(oABC, o.v0, o.v1, o.v2);
+        consider(oACD, o.v0, o.v2, o.v3);
+        consider(oADB, o.v0, o.v3, o.v1);
+        consider(oBDC, o.v1, o.v3, o.v2);
+        if (cnt > 0) s.v0 = tri.v0;
+        if (cnt > 1) s.v1 = tri.v1;
+        if (cnt > 2) s.v2 = tri.v2;
         n = cnt;
     }
     return false;
@@ -398,19 +436,21 @@ struct Gjk {
     float distance;
     f3 pa, pb, normal;
     int n;
-    SP s[4];
+    Simplex s;
 };
 
 // GJK::GetClosestPoints (GJK.cpp:155-244).  Uniform control flow: every lane
 // runs the same scalar code, the support queries are wave-cooperative.
-__device__ void gjk(const WShape& A, const WShape& B, float tol, Gjk& r) {
+__device__ __forceinline__ void gjk(const WShape& A, const WShape& B, float tol, Gjk& r) {
     r.intersect = false;
     r.distance = 0.0f;
     r.pa = r.pb = r.normal = f3{0.0f, 0.0f, 0.0f};
-    SP* s = r.s;
+    Simplex& s = r.s;
     int n = 0;
-    sp_set(s, n++, msupport(A, B, f3{1.0f, 0.0f, 0.0f}));
-    f3 closest = s[0].p;
+    s.v0 = msupport(A, B, f3{1.0f, 0.0f, 0.0f});
+    s.v1 = s.v2 = s.v3 = s.v0;
+    n = 1;
+    f3 closest = s.v0.p;
     const float tol2 = tol * tol;
     for (int it = 0; it < 32; ++it) {
         if (len2(closest) <= tol2) { r.intersect = true; r.distance = 0.0f; break; }
@@ -419,10 +459,11 @@ __device__ void gjk(const WShape& A, const WShape& B, float tol, Gjk& r) {
         const float progress = len2(closest) - dot(closest, w.p);
         if (progress <= tol * len2(closest)) break;
         bool dup = false;
+#pragma unroll
         for (int i = 0; i < 4; ++i)
-            if (i < n && len2(sub(s[i].p, w.p)) < tol2) dup = true;
+            if (i < n && len2(sub(sget(s, i).p, w.p)) < tol2) dup = true;
         if (dup) break;
-        sp_set(s, n++, w);
+        sset(s, n++, w);
         if (reduce_simplex(s, n, closest)) { r.intersect = true; r.distance = 0.0f; break; }
     }
     r.n = n;
@@ -432,18 +473,18 @@ __device__ void gjk(const WShape& A, const WShape& B, float tol, Gjk& r) {
         if (dist < kEps) r.intersect = true;
         else r.normal = divs(neg(closest), dist);
         if (n == 2) {
-            const f3 a = s[0].p, b = s[1].p, ab = sub(b, a);
+            const f3 a = s.v0.p, b = s.v1.p, ab = sub(b, a);
             const float t = fclamp(-dot(a, ab) / fmax_(len2(ab), kEps), 0.0f, 1.0f);
-            r.pa = add(s[0].a, mul(sub(s[1].a, s[0].a), t));
-            r.pb = add(s[0].b, mul(sub(s[1].b, s[0].b), t));
+            r.pa = add(s.v0.a, mul(sub(s.v1.a, s.v0.a), t));
+            r.pb = add(s.v0.b, mul(sub(s.v1.b, s.v0.b), t));
         } else if (n == 3) {
             float u, v, w;
-            closest_on_triangle(f3{0.0f, 0.0f, 0.0f}, s[0].p, s[1].p, s[2].p, u, v, w);
-            r.pa = add(add(mul(s[0].a, u), mul(s[1].a, v)), mul(s[2].a, w));
-            r.pb = add(add(mul(s[0].b, u), mul(s[1].b, v)), mul(s[2].b, w));
+            closest_on_triangle(f3{0.0f, 0.0f, 0.0f}, s.v0.p, s.v1.p, s.v2.p, u, v, w);
+            r.pa = add(add(mul(s.v0.a, u), mul(s.v1.a, v)), mul(s.v2.a, w));
+            r.pb = add(add(mul(s.v0.b, u), mul(s.v1.b, v)), mul(s.v2.b, w));
         } else {
-            r.pa = s[0].a;
-            r.pb = s[0].b;
+            r.pa = s.v0.a;
+            r.pb = s.v0.b;
         }
     }
 }
@@ -454,65 +495,72 @@ __device__ __forceinline__ f3 perp(f3 v) {
     if (fabs_(v.y) <= fabs_(v.z)) return f3{-v.z, 0.0f, v.x};
     return f3{-v.y, v.x, 0.0f};
 }
-__device__ bool origin_in_tetra(const SP* v) {
+__device__ __forceinline__ bool origin_in_tetra(const Simplex& t) {
     const f3 O{0.0f, 0.0f, 0.0f};
     auto ok = [&](f3 a, f3 b, f3 c, f3 d) {
         f3 n = cross(sub(b, a), sub(c, a));
         if (dot(n, sub(d, a)) > 0.0f) n = neg(n);
         return dot(n, sub(O, a)) < -1.0e-9f;
     };
-    return ok(v[0].p, v[1].p, v[2].p, v[3].p) && ok(v[0].p, v[2].p, v[3].p, v[1].p) &&
-           ok(v[0].p, v[3].p, v[1].p, v[2].p) && ok(v[1].p, v[3].p, v[2].p, v[0].p);
+    return ok(t.v0.p, t.v1.p, t.v2.p, t.v3.p) && ok(t.v0.p, t.v2.p, t.v3.p, t.v1.p) &&
+           ok(t.v0.p, t.v3.p, t.v1.p, t.v2.p) && ok(t.v1.p, t.v3.p, t.v2.p, t.v0.p);
 }
 
 // BuildInitialTetra (EPA.cpp:58-147).
 struct WaveLds;
-__device__ bool initial_tetra(const WShape& A, const WShape& B, const Gjk& g, SP* out, WaveLds& L);
-__device__ bool initial_tetra_fallback(const WShape& A, const WShape& B, const Gjk& g, SP* out, WaveLds& L);
+__device__ __forceinline__ bool initial_tetra(const WShape& A, const WShape& B, const Gjk& g, Simplex& out,
+                                              WaveLds& L);
+__device__ __forceinline__ bool initial_tetra_fallback(const WShape& A, const WShape& B, const Gjk& g, Simplex& out,
+                                                       WaveLds& L);
 
-__device__ bool initial_tetra(const WShape& A, const WShape& B, const Gjk& g, SP* out, WaveLds& L) {
+__device__ __forceinline__ bool initial_tetra(const WShape& A, const WShape& B, const Gjk& g, Simplex& out,
+                                              WaveLds& L) {
     const float tiny = 1.0e-12f;
     int n = 0;
+    out = g.s;
+#pragma unroll
     for (int i = 0; i < 4; ++i) {
         if (i >= g.n || n >= 4) continue;
+        const SP gi = sget(g.s, i);
         bool dup = false;
+#pragma unroll
         for (int j = 0; j < 4; ++j)
-            if (j < n && len2(sub(out[j].p, g.s[i].p)) < tiny) dup = true;
-        if (!dup) sp_set(out, n++, g.s[i]);
+            if (j < n && len2(sub(sget(out, j).p, gi.p)) < tiny) dup = true;
+        if (!dup) sset(out, n++, gi);
     }
-    if (n == 0) sp_set(out, n++, msupport(A, B, f3{1.0f, 0.0f, 0.0f}));
+    if (n == 0) sset(out, n++, msupport(A, B, f3{1.0f, 0.0f, 0.0f}));
     if (n == 4 && !origin_in_tetra(out)) n = 3;
     if (n == 1) {
-        out[1] = msupport(A, B, neg(out[0].p));
-        if (len2(sub(out[1].p, out[0].p)) > tiny) ++n;
+        out.v1 = msupport(A, B, neg(out.v0.p));
+        if (len2(sub(out.v1.p, out.v0.p)) > tiny) ++n;
     }
     if (n == 2) {
-        const f3 ab = sub(out[1].p, out[0].p);
+        const f3 ab = sub(out.v1.p, out.v0.p);
         if (len2(ab) > tiny) {
             const f3 d1 = normalized_or(perp(ab), f3{0.0f, 0.0f, 0.0f});
             const f3 d2 = normalized_or(cross(ab, d1), f3{0.0f, 0.0f, 0.0f});
-            const SP k0 = out[0], k1 = out[1];
-            out[0] = msupport(A, B, d1);
-            out[1] = msupport(A, B, neg(d1));
-            out[2] = msupport(A, B, d2);
-            out[3] = msupport(A, B, neg(d2));
+            const SP k0 = out.v0, k1 = out.v1;
+            out.v0 = msupport(A, B, d1);
+            out.v1 = msupport(A, B, neg(d1));
+            out.v2 = msupport(A, B, d2);
+            out.v3 = msupport(A, B, neg(d2));
             if (origin_in_tetra(out)) return true;
-            out[0] = k0;
-            out[1] = k1;
+            out.v0 = k0;
+            out.v1 = k1;
         }
         const SP sp = msupport(A, B, perp(ab));
-        if (len2(sub(sp.p, out[0].p)) > tiny && len2(sub(sp.p, out[1].p)) > tiny) sp_set(out, n++, sp);
+        if (len2(sub(sp.p, out.v0.p)) > tiny && len2(sub(sp.p, out.v1.p)) > tiny) sset(out, n++, sp);
     }
     if (n == 3) {
-        f3 tn = cross(sub(out[1].p, out[0].p), sub(out[2].p, out[0].p));
+        f3 tn = cross(sub(out.v1.p, out.v0.p), sub(out.v2.p, out.v0.p));
         const float l = len(tn);
         if (l > tiny) {
             tn = divs(tn, l);
-            const float sd = dot(tn, sub(f3{0.0f, 0.0f, 0.0f}, out[0].p));
+            const float sd = dot(tn, sub(f3{0.0f, 0.0f, 0.0f}, out.v0.p));
             const SP a = msupport(A, B, sd > 0.0f ? neg(tn) : tn);
             const SP b = msupport(A, B, sd > 0.0f ? tn : neg(tn));
-            out[3] = a;
-            if (!origin_in_tetra(out)) out[3] = b;
+            out.v3 = a;
+            if (!origin_in_tetra(out)) out.v3 = b;
             if (origin_in_tetra(out)) return true;
         }
         n = 3;
@@ -551,7 +599,7 @@ __device__ __forceinline__ bool build_face(const WaveLds& L, uint32_t& a, uint32
 }
 
 // FinishFace (EPA.cpp:149-172).
-__device__ void finish_face(const WaveLds& L, int f, Epa& r) {
+__device__ __forceinline__ void finish_face(const WaveLds& L, int f, Epa& r) {
     const f3 n{L.fnx[f], L.fny[f], L.fnz[f]};
     const float dist = L.fd[f];
     const uint32_t abc = L.fabc[f];
@@ -601,8 +649,27 @@ __device__ __forceinline__ void lds_write_vert(WaveLds& L, int i, const SP& s) {
 // the GJK simplex and the supports along 14 fixed directions (duplicates
 // dropped), the first enclosing tetrahedron in i<j<k<l order wins.  The
 // candidate list lives in LDS (written by lane 0, read by all lanes).
-__device__ __noinline__ bool initial_tetra_fallback(const WShape& A, const WShape& B, const Gjk& g, SP* out,
-                                                    WaveLds& L) {
+__device__ __forceinline__ f3 fallback_dir(int i) {
+    const float s = 0.5773502692f;
+    switch (i) {
+        case 0: return f3{1, 0, 0};
+        case 1: return f3{-1, 0, 0};
+        case 2: return f3{0, 1, 0};
+        case 3: return f3{0, -1, 0};
+        case 4: return f3{0, 0, 1};
+        case 5: return f3{0, 0, -1};
+        case 6: return f3{s, s, s};
+        case 7: return f3{s, s, -s};
+        case 8: return f3{s, -s, s};
+        case 9: return f3{-s, s, s};
+        case 10: return f3{-s, -s, s};
+        case 11: return f3{-s, s, -s};
+        case 12: return f3{s, -s, -s};
+        default: return f3{-s, -s, -s};
+    }
+}
+__device__ __forceinline__ bool initial_tetra_fallback(const WShape& A, const WShape& B, const Gjk& g, Simplex& out,
+                                                       WaveLds& L) {
     const float tiny = 1.0e-12f;
     int nc = 0;
     auto cand = [&](int i) {
@@ -622,22 +689,19 @@ __device__ __noinline__ bool initial_tetra_fallback(const WShape& A, const WShap
             ++nc;
         }
     };
+#pragma unroll
     for (int i = 0; i < 4; ++i)
-        if (i < g.n) addc(g.s[i]);
-    const float s = 0.5773502692f;
-    const f3 dirs[14] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1},
-                         {s, s, s}, {s, s, -s}, {s, -s, s}, {-s, s, s},
-                         {-s, -s, s}, {-s, s, -s}, {s, -s, -s}, {-s, -s, -s}};
-    for (int i = 0; i < 14; ++i) addc(msupport(A, B, dirs[i]));
+        if (i < g.n) addc(sget(g.s, i));
+    for (int i = 0; i < 14; ++i) addc(msupport(A, B, fallback_dir(i)));
     for (int i = 0; i + 3 < nc; ++i)
         for (int j = i + 1; j + 2 < nc; ++j)
             for (int k = j + 1; k + 1 < nc; ++k)
                 for (int l = k + 1; l < nc; ++l) {
-                    SP t[4] = {cand(i), cand(j), cand(k), cand(l)};
-                    const float vol = dot(cross(sub(t[1].p, t[0].p), sub(t[2].p, t[0].p)), sub(t[3].p, t[0].p));
+                    const Simplex t{cand(i), cand(j), cand(k), cand(l)};
+                    const float vol = dot(cross(sub(t.v1.p, t.v0.p), sub(t.v2.p, t.v0.p)), sub(t.v3.p, t.v0.p));
                     if (fabs_(vol) < tiny) continue;
                     if (origin_in_tetra(t)) {
-                        for (int m = 0; m < 4; ++m) out[m] = t[m];
+                        out = t;
                         return true;
                     }
                 }
@@ -654,17 +718,21 @@ __device__ __forceinline__ uint32_t prefix_count(unsigned long long mask) {
 // EPA::Compute (EPA.cpp:176-296).  All faces in [0, nf) are alive at the start
 // of every iteration (dead faces are compacted away), so the alive tests of
 // the reference reduce to index bounds.
-__device__ void epa(const WShape& A, const WShape& B, const Gjk& g, float tol, WaveLds& L, Epa& r) {
+__device__ __forceinline__ void epa(const WShape& A, const WShape& B, const Gjk& g, float tol, WaveLds& L, Epa& r) {
     r.valid = false;
     r.depth = 0.0f;
     r.normal = f3{0.0f, 1.0f, 0.0f};
     r.pa = r.pb = f3{0.0f, 0.0f, 0.0f};
-    SP tet[4];
+    Simplex tet;
     if (!initial_tetra(A, B, g, tet, L)) return;
-    for (int i = 0; i < 4; ++i) lds_write_vert(L, i, tet[i]);
+    lds_write_vert(L, 0, tet.v0);
+    lds_write_vert(L, 1, tet.v1);
+    lds_write_vert(L, 2, tet.v2);
+    lds_write_vert(L, 3, tet.v3);
     __syncthreads();
     int nv = 4, nf = 0;
     const uint32_t kT[4][3] = {{0, 1, 2}, {0, 2, 3}, {0, 3, 1}, {1, 3, 2}};
+#pragma unroll
     for (int t = 0; t < 4; ++t) {
         uint32_t a = kT[t][0], b = kT[t][1], c = kT[t][2];
         f3 n;
@@ -706,7 +774,20 @@ __device__ void epa(const WShape& A, const WShape& B, const Gjk& g, float tol, W
         }
         __syncthreads();
         // Horizon: edges (ab, bc, ca) of each visible face, in list order,
-        // that no other visible face shares reversed (EPA.cpp:253-270).
+        // that no other visible face shares reversed (EPA.cpp:253-270).  A
+        // face never holds both directions of one edge, so "another visible
+        // face has (v, u)" is "(v, u) is among the visible faces' edges".
+        for (int wd = lane; wd < kEdgeWords; wd += 64) L.edges[wd] = 0u;
+        __syncthreads();
+        for (int e = lane; e < nvis; e += 64) {
+            const uint32_t abc = L.fabc[L.vis[e]];
+            const uint32_t fa = abc & 0xFF, fb = (abc >> 8) & 0xFF, fc = abc >> 16;
+            const uint32_t k0 = fa * kEdgeStride + fb, k1 = fb * kEdgeStride + fc, k2 = fc * kEdgeStride + fa;
+            atomicOr(&L.edges[k0 >> 5], 1u << (k0 & 31u));
+            atomicOr(&L.edges[k1 >> 5], 1u << (k1 & 31u));
+            atomicOr(&L.edges[k2 >> 5], 1u << (k2 & 31u));
+        }
+        __syncthreads();
         int nh = 0;
         for (int base = 0; base < nvis; base += 64) {
             const int e = base + lane;
@@ -717,18 +798,17 @@ __device__ void epa(const WShape& A, const WShape& B, const Gjk& g, float tol, W
                 const uint32_t abc = L.fabc[fi];
                 const uint32_t fa = abc & 0xFF, fb = (abc >> 8) & 0xFF, fc = abc >> 16;
                 const uint32_t es[3][2] = {{fa, fb}, {fb, fc}, {fc, fa}};
+#pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    bool shared = false;
-                    for (int vj = 0; vj < nvis && !shared; ++vj) {
-                        const int fj = L.vis[vj];
-                        if (fj == fi) continue;
-                        const uint32_t g = L.fabc[fj];
-                        const uint32_t ga = g & 0xFF, gb = (g >> 8) & 0xFF, gc = g >> 16;
-                        if ((ga == es[k][1] && gb == es[k][0]) || (gb == es[k][1] && gc == es[k][0]) ||
-                            (gc == es[k][1] && ga == es[k][0]))
-                            shared = true;
+                    const uint32_t rk = es[k][1] * kEdgeStride + es[k][0];
+                    const bool shared = (L.edges[rk >> 5] >> (rk & 31u)) & 1u;
+                    if (!shared) {
+                        // select-chain stores keep eu / ev in registers
+                        if (keep == 0) { eu[0] = es[k][0]; ev[0] = es[k][1]; }
+                        else if (keep == 1) { eu[1] = es[k][0]; ev[1] = es[k][1]; }
+                        else { eu[2] = es[k][0]; ev[2] = es[k][1]; }
+                        ++keep;
                     }
-                    if (!shared) { eu[keep] = es[k][0]; ev[keep] = es[k][1]; ++keep; }
                 }
             }
             // Exclusive prefix sum of `keep` over the lanes, in order.
@@ -739,6 +819,7 @@ __device__ void epa(const WShape& A, const WShape& B, const Gjk& g, float tol, W
                 if (lane >= off) incl += o;
             }
             const int pos = nh + incl - keep;
+#pragma unroll
             for (int k = 0; k < 3; ++k)
                 if (k < keep && pos + k < kEpaMaxHorizon) L.hor[pos + k] = uint16_t(eu[k] | (ev[k] << 8));
             nh += __shfl(incl, 63, 64);
@@ -802,7 +883,15 @@ __device__ void epa(const WShape& A, const WShape& B, const Gjk& g, float tol, W
 // ---------------------- Polyhedral clipping (CollideShape.cpp) -------------
 // GetShapeFace (CollideShape.cpp:30-98); the hull triangle scan is a wave
 // arg-max over precomputed local face normals.
-__device__ bool shape_face(const WShape& w, const ShapeBuf& sb, f3 dir, f3* out, int& cnt, f3& nw) {
+__device__ __forceinline__ void put3(float* dst, f3 v) {
+    if (lane_id() == 0) {
+        dst[0] = v.x;
+        dst[1] = v.y;
+        dst[2] = v.z;
+    }
+}
+__device__ __forceinline__ bool shape_face(const WShape& w, const ShapeBuf& sb, f3 dir, float (*out)[3], int& cnt,
+                                           f3& nw) {
     if (w.s.type == kBox) {
         const f3 ld = inv_rotate(w.q, dir);
         const int ax = max_index(vabs(ld));
@@ -813,12 +902,15 @@ __device__ bool shape_face(const WShape& w, const ShapeBuf& sb, f3 dir, f3* out,
         const f3 c{he.x * n.x, he.y * n.y, he.z * n.z};
         const int a0 = (ax + 1) % 3, a1 = (ax + 2) % 3;
         const float s0 = comp(he, a0), s1 = comp(he, a1);
-        f3 k[4] = {c, c, c, c};
-        set_comp(k[0], a0, s0); set_comp(k[0], a1, s1);
-        set_comp(k[1], a0, -s0); set_comp(k[1], a1, s1);
-        set_comp(k[2], a0, -s0); set_comp(k[2], a1, -s1);
-        set_comp(k[3], a0, s0); set_comp(k[3], a1, -s1);
-        for (int i = 0; i < 4; ++i) out[i] = xform(w.q, w.t, k[i]);
+        f3 k0 = c, k1 = c, k2 = c, k3 = c;
+        set_comp(k0, a0, s0); set_comp(k0, a1, s1);
+        set_comp(k1, a0, -s0); set_comp(k1, a1, s1);
+        set_comp(k2, a0, -s0); set_comp(k2, a1, -s1);
+        set_comp(k3, a0, s0); set_comp(k3, a1, -s1);
+        put3(out[0], xform(w.q, w.t, k0));
+        put3(out[1], xform(w.q, w.t, k1));
+        put3(out[2], xform(w.q, w.t, k2));
+        put3(out[3], xform(w.q, w.t, k3));
         cnt = 4;
         nw = rotate(w.q, n);
         return true;
@@ -835,8 +927,9 @@ __device__ bool shape_face(const WShape& w, const ShapeBuf& sb, f3 dir, f3* out,
         wave_argmax(best, bi);
         if (bi == 0x7FFFFFFF) return false;
         const uint32_t abc = sb.htris[w.s.hull.z + bi];
-        const uint32_t vi[3] = {abc & 0xFF, (abc >> 8) & 0xFF, abc >> 16};
-        for (int i = 0; i < 3; ++i) out[i] = xform(w.q, w.t, l3(sb.hverts[w.s.hull.x + vi[i]]));
+        put3(out[0], xform(w.q, w.t, l3(sb.hverts[w.s.hull.x + (abc & 0xFF)])));
+        put3(out[1], xform(w.q, w.t, l3(sb.hverts[w.s.hull.x + ((abc >> 8) & 0xFF)])));
+        put3(out[2], xform(w.q, w.t, l3(sb.hverts[w.s.hull.x + (abc >> 16)])));
         cnt = 3;
         nw = rotate(w.q, l3(sb.hfn[w.s.hull.z + bi]));
         return true;
@@ -846,7 +939,7 @@ __device__ bool shape_face(const WShape& w, const ShapeBuf& sb, f3 dir, f3* out,
 
 // Sutherland-Hodgman against one plane, keeping d <= 0 (CollideShape.cpp:111-126).
 // Run by lane 0 on the LDS polygons.
-__device__ int clip_plane(const float (*in)[3], int nin, f3 pn, float pd, float (*out)[3]) {
+__device__ __forceinline__ int clip_plane(const float (*in)[3], int nin, f3 pn, float pd, float (*out)[3]) {
     int n = 0;
     if (nin == 0) return 0;
     for (int i = 0; i < nin; ++i) {
@@ -863,32 +956,43 @@ __device__ int clip_plane(const float (*in)[3], int nin, f3 pn, float pd, float 
     return n;
 }
 
+// One result contact into the wave's LDS output slot k (lane 0 writes).
+__device__ __forceinline__ void put_contact(WaveLds& L, int k, f3 pa, f3 pb, f3 n, float sep, uint32_t feature) {
+    if (lane_id() == 0) {
+        float* c = L.out[k];
+        c[0] = pa.x; c[1] = pa.y; c[2] = pa.z; c[3] = pb.x; c[4] = pb.y; c[5] = pb.z;
+        c[6] = n.x; c[7] = n.y; c[8] = n.z; c[9] = sep; c[10] = __uint_as_float(feature);
+    }
+}
+
 // CollidePolyhedral (CollideShape.cpp:131-228).  Returns the point count;
-// points are written to `res`.
-__device__ int collide_polyhedral(const WShape& A, const WShape& B, const ShapeBuf& sb, f3 normal, float max_sep,
-                                  WaveLds& L, Contact* res) {
-    f3 fa[4], fb[4], na, nb;
+// the points are left in L.out.
+__device__ __forceinline__ int collide_polyhedral(const WShape& A, const WShape& B, const ShapeBuf& sb, f3 normal,
+                                                  float max_sep, WaveLds& L) {
+    f3 na, nb;
     int ca = 0, cb = 0;
-    const bool ha = shape_face(A, sb, neg(normal), fa, ca, na);
-    const bool hb = shape_face(B, sb, normal, fb, cb, nb);
+    const bool ha = shape_face(A, sb, neg(normal), L.face[0], ca, na);
+    const bool hb = shape_face(B, sb, normal, L.face[1], cb, nb);
+    __syncthreads();
     if (!ha || !hb) return 0;
     const float align_a = dot(na, neg(normal)), align_b = dot(nb, normal);
     const bool ref_a = align_a >= align_b;
-    const f3* rv = ref_a ? fa : fb;
+    const float (*rv)[3] = L.face[ref_a ? 0 : 1];
+    const float (*iv)[3] = L.face[ref_a ? 1 : 0];
     const int rc = ref_a ? ca : cb;
-    const f3* iv = ref_a ? fb : fa;
     const int ic = ref_a ? cb : ca;
-    const f3 rn = ref_a ? na : nb;
-    const float rd = dot(rn, rv[0]);
+    const f3 rn = pick(ref_a, na, nb);
+    const float rd = dot(rn, f3{rv[0][0], rv[0][1], rv[0][2]});
     int count = 0;
     if (lane_id() == 0) {
         int cur = 0, np = 0;
-        for (int i = 0; i < ic; ++i) { L.poly[0][np][0] = iv[i].x; L.poly[0][np][1] = iv[i].y; L.poly[0][np][2] = iv[i].z; ++np; }
+        for (int i = 0; i < ic; ++i) { L.poly[0][np][0] = iv[i][0]; L.poly[0][np][1] = iv[i][1]; L.poly[0][np][2] = iv[i][2]; ++np; }
         f3 cen{0.0f, 0.0f, 0.0f};
-        for (int i = 0; i < rc; ++i) cen = add(cen, rv[i]);
+        for (int i = 0; i < rc; ++i) cen = add(cen, f3{rv[i][0], rv[i][1], rv[i][2]});
         cen = divs(cen, float(rc));
         for (int i = 0; i < rc; ++i) {
-            const f3 v0 = rv[i], v1 = rv[(i + 1) % rc];
+            const int i1 = (i + 1) % rc;
+            const f3 v0{rv[i][0], rv[i][1], rv[i][2]}, v1{rv[i1][0], rv[i1][1], rv[i1][2]};
             f3 sn = cross(sub(v1, v0), rn);
             const float l = len(sn);
             if (l < 1.0e-9f) continue;
@@ -904,7 +1008,7 @@ __device__ int collide_polyhedral(const WShape& A, const WShape& B, const ShapeB
             const float sep = dot(rn, p) - rd;
             if (sep > max_sep) continue;
             const f3 proj = sub(p, mul(rn, sep));
-            const f3 pa = ref_a ? proj : p, pb = ref_a ? p : proj;
+            const f3 pa = pick(ref_a, proj, p), pb = pick(ref_a, p, proj);
             float* c = L.cand[k];
             c[0] = pa.x; c[1] = pa.y; c[2] = pa.z; c[3] = pb.x; c[4] = pb.y; c[5] = pb.z;
             c[6] = normal.x; c[7] = normal.y; c[8] = normal.z; c[9] = sep; c[10] = __uint_as_float(uint32_t(i));
@@ -915,14 +1019,12 @@ __device__ int collide_polyhedral(const WShape& A, const WShape& B, const ShapeB
     count = __shfl(count, 0, 64);
     __syncthreads();
     const int outn = count > 4 ? 4 : count;
-    for (int j = 0; j < outn; ++j) {
+    // Reduction to four points: cand[j * (count - 1) / 3] (CollideShape.cpp:216-224).
+    const int j = lane_id();
+    if (j < outn) {
         const int src = count > 4 ? j * (count - 1) / 3 : j;
-        const float* c = L.cand[src];
-        res[j].pa = f3{c[0], c[1], c[2]};
-        res[j].pb = f3{c[3], c[4], c[5]};
-        res[j].n = f3{c[6], c[7], c[8]};
-        res[j].sep = c[9];
-        res[j].feature = __float_as_uint(c[10]);
+#pragma unroll
+        for (int q = 0; q < 11; ++q) L.out[j][q] = L.cand[src][q];
     }
     __syncthreads();
     return outn;
@@ -930,47 +1032,44 @@ __device__ int collide_polyhedral(const WShape& A, const WShape& B, const ShapeB
 
 __device__ __forceinline__ bool polyhedral(const WShape& w) { return w.s.type == kBox || w.s.type == kHull; }
 
-// CollideConvexGeneric (CollideShape.cpp:233-299).
-__device__ int collide_generic(const WShape& A, const WShape& B, const ShapeBuf& sb, float max_sep, WaveLds& L,
-                               Contact* res) {
+// CollideConvexGeneric (CollideShape.cpp:233-299).  Results in L.out.
+__device__ __forceinline__ int collide_generic(const WShape& A, const WShape& B, const ShapeBuf& sb, float max_sep,
+                                               WaveLds& L) {
     const float tol = 1.0e-4f;
     Gjk g;
     gjk(A, B, tol, g);
     if (!g.intersect) {
         if (g.distance > max_sep) return 0;
         if (polyhedral(A) && polyhedral(B) && len2(g.normal) > 0.0f) {
-            const int k = collide_polyhedral(A, B, sb, g.normal, max_sep, L, res);
+            const int k = collide_polyhedral(A, B, sb, g.normal, max_sep, L);
             if (k > 0) return k;
         }
-        res[0].pa = g.pa; res[0].pb = g.pb; res[0].n = g.normal; res[0].sep = g.distance; res[0].feature = 0;
+        put_contact(L, 0, g.pa, g.pb, g.normal, g.distance, 0);
         return 1;
     }
     if (polyhedral(A) && polyhedral(B)) {
         Epa e;
         epa(A, B, g, tol, L, e);
         if (e.valid) {
-            const int k = collide_polyhedral(A, B, sb, e.normal, max_sep, L, res);
+            const int k = collide_polyhedral(A, B, sb, e.normal, max_sep, L);
             if (k > 0) return k;
-            res[0].pa = e.pa; res[0].pb = e.pb; res[0].n = e.normal; res[0].sep = -e.depth; res[0].feature = 0;
+            put_contact(L, 0, e.pa, e.pb, e.normal, -e.depth, 0);
             return 1;
         }
     }
     Epa e;
     epa(A, B, g, tol, L, e);
     if (e.valid) {
-        res[0].pa = e.pa; res[0].pb = e.pb; res[0].n = e.normal; res[0].sep = -e.depth; res[0].feature = 0;
+        put_contact(L, 0, e.pa, e.pb, e.normal, -e.depth, 0);
         return 1;
     }
-    res[0].pa = g.pa; res[0].pb = g.pb;
-    res[0].n = len2(g.normal) > 0.0f ? g.normal : f3{0.0f, 1.0f, 0.0f};
-    res[0].sep = 0.0f;
-    res[0].feature = 0;
+    put_contact(L, 0, g.pa, g.pb, len2(g.normal) > 0.0f ? g.normal : f3{0.0f, 1.0f, 0.0f}, 0.0f, 0);
     return 1;
 }
 
 // CollideSphereConvex (CollideShape.cpp:304-344): GJK against a 1e-4 proxy
 // sphere at the sphere centre.
-__device__ int collide_sphere_convex(const WShape& Sph, const WShape& B, float max_sep, WaveLds& L, Contact* res) {
+__device__ __forceinline__ int collide_sphere_convex(const WShape& Sph, const WShape& B, float max_sep, WaveLds& L) {
     const float tol = 1.0e-4f;
     WShape P = Sph;
     P.s.type = kSphere;
@@ -983,11 +1082,8 @@ __device__ int collide_sphere_convex(const WShape& Sph, const WShape& B, float m
         Epa e;
         epa(P, B, g, tol, L, e);
         if (e.valid) {
-            res[0].n = neg(e.normal);
-            res[0].pb = e.pb;
-            res[0].pa = add(c, mul(res[0].n, r));
-            res[0].sep = -(e.depth + r);
-            res[0].feature = 0;
+            const f3 n = neg(e.normal);
+            put_contact(L, 0, add(c, mul(n, r)), e.pb, n, -(e.depth + r), 0);
             return 1;
         }
         return 0;
@@ -996,11 +1092,7 @@ __device__ int collide_sphere_convex(const WShape& Sph, const WShape& B, float m
     const float sep = dist - r;
     if (sep > max_sep) return 0;
     const f3 n = dist > kEps ? divs(sub(c, g.pb), dist) : f3{0.0f, 1.0f, 0.0f};
-    res[0].n = n;
-    res[0].pa = sub(c, mul(n, r));
-    res[0].pb = g.pb;
-    res[0].sep = sep;
-    res[0].feature = 0;
+    put_contact(L, 0, sub(c, mul(n, r)), g.pb, n, sep, 0);
     return 1;
 }
 
@@ -1018,24 +1110,29 @@ __global__ void __launch_bounds__(64) k_collide_wave(const uint2* pairs, const u
         WShape A, B;
         load_wshape(A, sb, meta_shape(b.meta[ia]), b.x[ia], b.q[ia]);
         load_wshape(B, sb, meta_shape(b.meta[ib]), b.x[ib], b.q[ib]);
-        Contact res[4];
         int k;
         bool swap_result = swapped;
         if (A.s.type == kSphere) {
-            k = collide_sphere_convex(A, B, max_sep, L, res);
+            k = collide_sphere_convex(A, B, max_sep, L);
         } else if (B.s.type == kSphere) {
-            k = collide_sphere_convex(B, A, max_sep, L, res);
+            k = collide_sphere_convex(B, A, max_sep, L);
             if (k > 0) swap_result = !swap_result;
         } else {
-            k = collide_generic(A, B, sb, max_sep, L, res);
+            k = collide_generic(A, B, sb, max_sep, L);
         }
+        __syncthreads();
+        // Lane j writes result point j.
         const int lane = lane_id();
-        for (int j = 0; j < 4; ++j) {
-            if (j < k && lane == j) {
-                Contact c = res[j];
-                if (swap_result) swap_contact(c);
-                write_contact(m, pi, j, c);
-            }
+        if (lane < k) {
+            const float* o = L.out[lane];
+            Contact c;
+            c.pa = f3{o[0], o[1], o[2]};
+            c.pb = f3{o[3], o[4], o[5]};
+            c.n = f3{o[6], o[7], o[8]};
+            c.sep = o[9];
+            c.feature = __float_as_uint(o[10]);
+            if (swap_result) swap_contact(c);
+            write_contact(m, pi, lane, c);
         }
         if (lane == 0) m.count[pi] = uint32_t(k);
         __syncthreads();

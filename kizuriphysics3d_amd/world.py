@@ -157,3 +157,16 @@ def device_timings(world: PhysicsWorld) -> dict:
     if rc != 0:
         raise D.KzgError(f"kzg_get_timings failed ({rc})")
     return {k: t[k].item() for k in D.TIMINGS_DTYPE.names}
+
+
+def build_hulls(shapes: np.ndarray, hull_points: np.ndarray) -> dict:
+    """ConvexHullShape::Create of every hull shape, on the host (no GPU):
+    {shape index: (vertices (V, 3), triangles (T, 3))}."""
+    sc = S.Scene("hulls", S.world_settings())
+    sc.shapes = list(np.asarray(shapes, dtype=S.SHAPE_DTYPE))
+    sc.hull_points = [tuple(p) for p in np.asarray(hull_points, np.float32).reshape(-1, 3)]
+    w = PhysicsWorld(sc, device=-1)
+    try:
+        return w.hulls(sc.shape_array())
+    finally:
+        w.close()
