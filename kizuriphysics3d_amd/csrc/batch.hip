@@ -605,6 +605,7 @@ int upload_tables(kzb_batch* B) {
     KZB_HIP(B, c->vel_grp.ensure(std::max<size_t>(W, 1)));
     KZB_HIP(B, c->ws_grp.ensure(std::max<size_t>(W, 1)));
     KZB_HIP(B, B->h_out.ensure(4 * sizeof(float4) * size_t(B->nbodies) + 64));
+    KZB_HIP(B, c->b_packed_solver.ensure(8 * size_t(std::max<uint32_t>(B->nbodies, 1))));
     B->tables_dirty = false;
     drop_graph(B);
     return KZG_OK;
@@ -674,8 +675,9 @@ int capture(kzb_batch* B) {
     launch_prepare_counted(s, c->contacts(), c->bodies(), pts, B->cnt.p + kCntPts, dt, ws.restitution_threshold,
                            ws.baumgarte, ws.penetration_slop, ws.max_bias_velocity, use_simd,
                            ws.use_velocity_bias ? 1 : 0, c->vel_records(), c->ws_records(), c->vel_pos.p, c->ws_pos.p);
-    launch_solve_groups(s, c->vel_records(), c->ws_records(), c->bodies(), Schedule{c->ws_off.p, c->ws_grp.p},
-                        Schedule{c->vel_off.p, c->vel_grp.p}, sp, W, threads, c->resid.p);
+    launch_solve_groups(s, c->vel_records(), c->ws_records(), c->bodies(), nb, c->b_packed_solver.p,
+                        Schedule{c->ws_off.p, c->ws_grp.p}, Schedule{c->vel_off.p, c->vel_grp.p}, sp, W, threads,
+                        c->resid.p);
     launch_writeback_counted(s, c->vel_records(), c->ws_records(), c->contacts(), c->vel_pos.p, c->ws_pos.p, pts,
                              B->cnt.p + kCntPts);
     launch_cache_store_counted(s, c->contacts(), c->bodies(), c->cache(), cap, B->cnt.p + kCntCons);

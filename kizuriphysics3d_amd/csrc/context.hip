@@ -69,6 +69,7 @@ void kzg_destroy(kzg_ctx* c) {
     c->cache_key.release();
     c->cache_t.release();
     c->resid.release();
+    c->b_packed_solver.release();
     for (auto* d : {&c->mlp_master, &c->mlp_m, &c->mlp_v, &c->mlp_norm, &c->mlp_partial, &c->mlp_out, &c->mlp_loss})
         d->release();
     c->mlp_w1b.release();
@@ -362,8 +363,11 @@ int kzg_solve(kzg_ctx* c, float dt) {
     uint32_t threads = 64;
     while (threads < width && threads < 1024) threads *= 2;
     KZG_HIP(c, record(c, 5));
-    if (c->ncons) launch_solve_groups(c->stream, c->vel_records(), c->ws_records(), c->bodies(), ws, vel, sp,
-                                      c->sched.groups, threads, c->resid.p);
+    if (c->ncons) {
+        KZG_HIP(c, c->b_packed_solver.ensure(8 * size_t(std::max<uint32_t>(c->nbodies, 1))));
+        launch_solve_groups(c->stream, c->vel_records(), c->ws_records(), c->bodies(), c->nbodies,
+                            c->b_packed_solver.p, ws, vel, sp, c->sched.groups, threads, c->resid.p);
+    }
     KZG_HIP(c, hipGetLastError());
     KZG_HIP(c, record(c, 6));
     launch_writeback(c->stream, c->vel_records(), c->ws_records(), c->contacts(), c->vel_pos.p, c->ws_pos.p,

@@ -31,9 +31,9 @@ void launch_prepare(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32
                     float4* base);
 void launch_writeback(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const ContactBuf& c,
                       const uint32_t* vel_pos, const uint32_t* ws_pos, uint32_t np);
-void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b,
-                         const Schedule& ws, const Schedule& vel, const SolveParams& sp, uint32_t groups,
-                         uint32_t threads, float* resid);
+void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b, uint32_t nbodies,
+                         float4* packed, const Schedule& ws, const Schedule& vel, const SolveParams& sp,
+                         uint32_t groups, uint32_t threads, float* resid);
 void launch_cache_store(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const CacheBuf& cache, uint32_t nc);
 void launch_cache_rehash(hipStream_t s, const CacheBuf& src, const CacheBuf& dst);
 void launch_finish(hipStream_t s, const BodyBuf& b, uint32_t n, float dt, int split);
@@ -117,6 +117,7 @@ struct kzg_ctx {
     std::vector<uint32_t> h_packed;
     std::vector<uint8_t> h_dynamic, h_sensor, h_motion;
     DevVec<float4> b_x, b_q, b_v, b_w, b_pv, b_pw, b_iw, b_il, b_mat, b_lim;
+    DevVec<float4> b_packed_solver;  // 8 float4 per body during the solve
     DevVec<uint32_t> b_meta, b_packed;
 
     // pairs and manifolds

@@ -234,38 +234,83 @@ __global__ void k_prepare(ContactBuf c, BodyBuf b, uint32_t np, const uint32_t* 
 }
 
 // ---------------------------------------------------------------------------
+// Solver bodies.  During the sweeps the state a point reads of a body lives in
+// one 64-byte line per body (instead of five separate arrays): velocities,
+// or pseudo velocities, and the inverse world inertia columns c0, c1, c2:
+//   (vx, vy, vz, wx) (wy, wz, c0x, c0y) (c0z, c1x, c1y, c1z) (c2x, c2y, c2z, 0)
+// ---------------------------------------------------------------------------
+struct PackedBody {
+    f3 v, w;
+    m3 ii;
+};
+__device__ __forceinline__ PackedBody load_pb(const float4* p, uint32_t i) {
+    const float4 a = p[4 * i], b = p[4 * i + 1], c = p[4 * i + 2], d = p[4 * i + 3];
+    PackedBody r;
+    r.v = f3{a.x, a.y, a.z};
+    r.w = f3{a.w, b.x, b.y};
+    r.ii = m3{f3{b.z, b.w, c.x}, f3{c.y, c.z, c.w}, f3{d.x, d.y, d.z}};
+    return r;
+}
+__device__ __forceinline__ void store_vw(float4* p, uint32_t i, f3 v, f3 w, const m3& ii) {
+    p[4 * i] = make_float4(v.x, v.y, v.z, w.x);
+    p[4 * i + 1] = make_float4(w.y, w.z, ii.c0.x, ii.c0.y);
+}
+
+__global__ void k_pack_bodies(BodyBuf b, uint32_t n, float4* vel, float4* pse) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 c0 = b.iw[3 * i], c1 = b.iw[3 * i + 1], c2 = b.iw[3 * i + 2];
+    const float4 v = b.v[i], w = b.w[i], pv = b.pv[i], pw = b.pw[i];
+    vel[4 * i] = make_float4(v.x, v.y, v.z, w.x);
+    vel[4 * i + 1] = make_float4(w.y, w.z, c0.x, c0.y);
+    vel[4 * i + 2] = make_float4(c0.z, c1.x, c1.y, c1.z);
+    vel[4 * i + 3] = make_float4(c2.x, c2.y, c2.z, 0.0f);
+    pse[4 * i] = make_float4(pv.x, pv.y, pv.z, pw.x);
+    pse[4 * i + 1] = make_float4(pw.y, pw.z, c0.x, c0.y);
+    pse[4 * i + 2] = vel[4 * i + 2];
+    pse[4 * i + 3] = vel[4 * i + 3];
+}
+
+__global__ void k_unpack_bodies(BodyBuf b, uint32_t n, const float4* vel, const float4* pse) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = vel[4 * i], c = vel[4 * i + 1], pa = pse[4 * i], pc = pse[4 * i + 1];
+    b.v[i] = make_float4(a.x, a.y, a.z, 0.0f);
+    b.w[i] = make_float4(a.w, c.x, c.y, 0.0f);
+    b.pv[i] = make_float4(pa.x, pa.y, pa.z, 0.0f);
+    b.pw[i] = make_float4(pa.w, pc.x, pc.y, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
 // Per-point sweep bodies, reading sweep records.
 // ---------------------------------------------------------------------------
 // WarmStart (ContactSolver.cpp:117-132): stored rA / rB.
-__device__ __forceinline__ void warm_point(const WsRecBuf& r, const BodyBuf& b, uint32_t e, uint4 bd, float wsf) {
+__device__ __forceinline__ void warm_point(const WsRecBuf& r, float4* sb, uint32_t e, uint4 bd, float wsf) {
     const float4 imp = r.imp[e];
     const f3 n = ld3(r.n[e]), t1 = ld3(r.t1[e]), t2 = ld3(r.t2[e]);
     f3 j = add(add(mul(n, imp.x), mul(t1, imp.y)), mul(t2, imp.z));
     j = mul(j, wsf);
     if (bd.z & 2u) {
         const float4 ra = r.ra[e];
-        const m3 ia = ldm(b.iw + 3 * bd.x);
-        const f3 v = ld3(b.v[bd.x]), w = ld3(b.w[bd.x]);
-        b.v[bd.x] = st3(add(v, mul(j, ra.w)), 0.0f);
-        b.w[bd.x] = st3(add(w, mmul(ia, cross(ld3(ra), j))), 0.0f);
+        const PackedBody A = load_pb(sb, bd.x);
+        store_vw(sb, bd.x, add(A.v, mul(j, ra.w)), add(A.w, mmul(A.ii, cross(ld3(ra), j))), A.ii);
     }
     if (bd.z & 4u) {
         const float4 rb = r.rb[e];
-        const m3 ib = ldm(b.iw + 3 * bd.y);
-        const f3 v = ld3(b.v[bd.y]), w = ld3(b.w[bd.y]);
-        b.v[bd.y] = st3(sub(v, mul(j, rb.w)), 0.0f);
-        b.w[bd.y] = st3(sub(w, mmul(ib, cross(ld3(rb), j))), 0.0f);
+        const PackedBody B = load_pb(sb, bd.y);
+        store_vw(sb, bd.y, sub(B.v, mul(j, rb.w)), sub(B.w, mmul(B.ii, cross(ld3(rb), j))), B.ii);
     }
 }
 
 // One lane of SolveVelocityBatchSIMD (ContactSolverSIMD.cpp:93-256).
 // _mm_max_ps(a, b) = a > b ? a : b and _mm_min_ps(a, b) = a < b ? a : b.
-__device__ __forceinline__ float velocity_point(const VelRecBuf& r, const BodyBuf& b, uint32_t e, uint4 bd) {
+__device__ __forceinline__ float velocity_point(const VelRecBuf& r, float4* sb, uint32_t e, uint4 bd) {
     const float4 ra4 = r.ra[e], rb4 = r.rb[e], n4 = r.n[e], t14 = r.t1[e], t24 = r.t2[e];
     const float2 fb = r.fb[e];
     float4 imp = r.imp[e];
-    const m3 ia = ldm(b.iw + 3 * bd.x), ib = ldm(b.iw + 3 * bd.y);
-    f3 va = ld3(b.v[bd.x]), wa = ld3(b.w[bd.x]), vb = ld3(b.v[bd.y]), wb = ld3(b.w[bd.y]);
+    const PackedBody A = load_pb(sb, bd.x), B = load_pb(sb, bd.y);
+    const m3 ia = A.ii, ib = B.ii;
+    f3 va = A.v, wa = A.w, vb = B.v, wb = B.w;
     const f3 ra = ld3(ra4), rb = ld3(rb4);
     const float ima = ra4.w, imb = rb4.w, cf = fb.x, nbias = fb.y;
     const float zero = 0.0f;
@@ -308,19 +353,20 @@ __device__ __forceinline__ float velocity_point(const VelRecBuf& r, const BodyBu
         dsq += dl * dl;
         apply(mul(nv, dl));
     }
-    if (bd.z & 2u) { b.v[bd.x] = st3(va, 0.0f); b.w[bd.x] = st3(wa, 0.0f); }
-    if (bd.z & 4u) { b.v[bd.y] = st3(vb, 0.0f); b.w[bd.y] = st3(wb, 0.0f); }
+    if (bd.z & 2u) store_vw(sb, bd.x, va, wa, ia);
+    if (bd.z & 4u) store_vw(sb, bd.y, vb, wb, ib);
     r.imp[e] = imp;
     return dsq;
 }
 
 // SolveContactVelocity, the scalar path (ContactSolver.cpp:137-181).
-__device__ __forceinline__ float scalar_point(const VelRecBuf& r, const BodyBuf& b, uint32_t e, uint4 bd) {
+__device__ __forceinline__ float scalar_point(const VelRecBuf& r, float4* sb, uint32_t e, uint4 bd) {
     const float4 ra4 = r.ra[e], rb4 = r.rb[e], n4 = r.n[e], t14 = r.t1[e], t24 = r.t2[e];
     const float2 fb = r.fb[e];
     float4 imp = r.imp[e];
-    const m3 ia = ldm(b.iw + 3 * bd.x), ib = ldm(b.iw + 3 * bd.y);
-    f3 va = ld3(b.v[bd.x]), wa = ld3(b.w[bd.x]), vb = ld3(b.v[bd.y]), wb = ld3(b.w[bd.y]);
+    const PackedBody A = load_pb(sb, bd.x), B = load_pb(sb, bd.y);
+    const m3 ia = A.ii, ib = B.ii;
+    f3 va = A.v, wa = A.w, vb = B.v, wb = B.w;
     const f3 ra = ld3(ra4), rb = ld3(rb4);
     const bool da = bd.z & 2u, db = bd.z & 4u;
     float dsq = 0.0f;
@@ -349,21 +395,22 @@ __device__ __forceinline__ float scalar_point(const VelRecBuf& r, const BodyBuf&
     imp.x = ni;
     dsq += lam * lam;
     apply(mul(ld3(n4), lam));
-    if (da) { b.v[bd.x] = st3(va, 0.0f); b.w[bd.x] = st3(wa, 0.0f); }
-    if (db) { b.v[bd.y] = st3(vb, 0.0f); b.w[bd.y] = st3(wb, 0.0f); }
+    if (da) store_vw(sb, bd.x, va, wa, ia);
+    if (db) store_vw(sb, bd.y, vb, wb, ib);
     r.imp[e] = imp;
     return dsq;
 }
 
 // SolveBiasConstraint (ContactSolver.cpp:257-290).
-__device__ __forceinline__ void bias_point(const WsRecBuf& r, const BodyBuf& b, uint32_t e, uint4 bd) {
+__device__ __forceinline__ void bias_point(const WsRecBuf& r, float4* ps, uint32_t e, uint4 bd) {
     const float4 qa4 = r.qa[e];
     const float pbias = qa4.w;
     const float ima = r.ra[e].w, imb = r.rb[e].w;
     if (ima <= 0.0f && imb <= 0.0f) return;
     if (pbias <= 0.0f) return;
     const f3 ra = ld3(qa4), rb = ld3(r.qb[e]);
-    const f3 pva = ld3(b.pv[bd.x]), pwa = ld3(b.pw[bd.x]), pvb = ld3(b.pv[bd.y]), pwb = ld3(b.pw[bd.y]);
+    const PackedBody A = load_pb(ps, bd.x), B = load_pb(ps, bd.y);
+    const f3 pva = A.v, pwa = A.w, pvb = B.v, pwb = B.w;
     const f3 vr = sub(add(pva, cross(pwa, ra)), add(pvb, cross(pwb, rb)));
     const float4 n4 = r.n[e];
     float4 imp = r.imp[e];
@@ -373,16 +420,26 @@ __device__ __forceinline__ void bias_point(const WsRecBuf& r, const BodyBuf& b, 
     imp.w = ni;
     r.imp[e] = imp;
     const f3 j = mul(ld3(n4), lam);
-    if (bd.z & 2u) {
-        const m3 ia = ldm(b.iw + 3 * bd.x);
-        b.pv[bd.x] = st3(add(pva, mul(j, ima)), 0.0f);
-        b.pw[bd.x] = st3(add(pwa, mmul(ia, cross(ra, j))), 0.0f);
-    }
-    if (bd.z & 4u) {
-        const m3 ib = ldm(b.iw + 3 * bd.y);
-        b.pv[bd.y] = st3(sub(pvb, mul(j, imb)), 0.0f);
-        b.pw[bd.y] = st3(sub(pwb, mmul(ib, cross(rb, j))), 0.0f);
-    }
+    if (bd.z & 2u) store_vw(ps, bd.x, add(pva, mul(j, ima)), add(pwa, mmul(A.ii, cross(ra, j))), A.ii);
+    if (bd.z & 4u) store_vw(ps, bd.y, sub(pvb, mul(j, imb)), sub(pwb, mmul(B.ii, cross(rb, j))), B.ii);
+}
+
+// Level offsets of one group, staged in LDS when they fit (the sweep reads
+// them after every barrier).
+constexpr uint32_t kLevelsLds = 2048;
+struct LevelOffsets {
+    const uint32_t* glob;
+    const uint32_t* lds;
+    uint32_t base;
+    bool in_lds;
+    __device__ __forceinline__ uint32_t operator()(uint32_t l) const { return in_lds ? lds[l - base] : glob[l]; }
+};
+__device__ __forceinline__ LevelOffsets stage_offsets(const Schedule& s, uint2 range, uint32_t* lds) {
+    const uint32_t n = range.y - range.x + 1;
+    LevelOffsets o{s.lvl_off, lds, range.x, n <= kLevelsLds};
+    if (o.in_lds)
+        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) lds[k] = s.lvl_off[range.x + k];
+    return o;
 }
 
 // Level-by-level sweep of one group's range of levels.  The body indices of
@@ -390,19 +447,19 @@ __device__ __forceinline__ void bias_point(const WsRecBuf& r, const BodyBuf& b, 
 // are constant during the solve), so after each barrier a point waits for
 // one round of loads only.
 template <class F>
-__device__ __forceinline__ void sweep(const Schedule& s, uint2 range, const uint4* bd, F&& f) {
+__device__ __forceinline__ void sweep(const LevelOffsets& off, uint2 range, const uint4* bd, F&& f) {
     const uint32_t t = threadIdx.x, nt = blockDim.x;
     uint4 nxt = make_uint4(0, 0, 0, 0);
     if (range.x < range.y) {
-        const uint32_t e = s.lvl_off[range.x] + t;
-        if (e < s.lvl_off[range.x + 1]) nxt = bd[e];
+        const uint32_t e = off(range.x) + t;
+        if (e < off(range.x + 1)) nxt = bd[e];
     }
     for (uint32_t l = range.x; l < range.y; ++l) {
-        const uint32_t e0 = s.lvl_off[l] + t, e1 = s.lvl_off[l + 1];
+        const uint32_t e0 = off(l) + t, e1 = off(l + 1);
         const uint4 cur = nxt;
         if (l + 1 < range.y) {
             const uint32_t en = e1 + t;
-            if (en < s.lvl_off[l + 2]) nxt = bd[en];
+            if (en < off(l + 2)) nxt = bd[en];
         }
         for (uint32_t e = e0; e < e1; e += nt) f(e, e == e0 ? cur : bd[e]);
         __syncthreads();
@@ -418,23 +475,27 @@ __device__ __forceinline__ void sweep(const Schedule& s, uint2 range, const uint
 // needed between levels).  `resid` (optional) receives the sum of squared
 // impulse increments of each velocity sweep.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_solve_groups(VelRecBuf vr, WsRecBuf wr, BodyBuf b, Schedule ws,
+__global__ void __launch_bounds__(1024) k_solve_groups(VelRecBuf vr, WsRecBuf wr, float4* sbv, float4* sbp, Schedule ws,
                                                        Schedule vel, SolveParams sp, float* resid) {
+    __shared__ uint32_t ws_lds[kLevelsLds], vel_lds[kLevelsLds];
     const uint2 wrange = ws.grp[blockIdx.x];
     const uint2 vrange = vel.grp[blockIdx.x];
+    const LevelOffsets wo = stage_offsets(ws, wrange, ws_lds);
+    const LevelOffsets vo = stage_offsets(vel, vrange, vel_lds);
+    __syncthreads();
     const float wsf = sp.warm_start_factor;
-    sweep(ws, wrange, wr.bd, [&](uint32_t e, uint4 bd) { warm_point(wr, b, e, bd, wsf); });
+    sweep(wo, wrange, wr.bd, [&](uint32_t e, uint4 bd) { warm_point(wr, sbv, e, bd, wsf); });
     for (uint32_t it = 0; it < sp.velocity_iterations; ++it) {
         float dsq = 0.0f;
         if (sp.use_simd)
-            sweep(vel, vrange, vr.bd, [&](uint32_t e, uint4 bd) { dsq += velocity_point(vr, b, e, bd); });
+            sweep(vo, vrange, vr.bd, [&](uint32_t e, uint4 bd) { dsq += velocity_point(vr, sbv, e, bd); });
         else
-            sweep(vel, vrange, vr.bd, [&](uint32_t e, uint4 bd) { dsq += scalar_point(vr, b, e, bd); });
+            sweep(vo, vrange, vr.bd, [&](uint32_t e, uint4 bd) { dsq += scalar_point(vr, sbv, e, bd); });
         if (resid && dsq != 0.0f) atomicAdd(resid + it, dsq);
     }
     if (!sp.use_velocity_bias)
         for (uint32_t it = 0; it < sp.position_iterations; ++it)
-            sweep(ws, wrange, wr.bd, [&](uint32_t e, uint4 bd) { bias_point(wr, b, e, bd); });
+            sweep(wo, wrange, wr.bd, [&](uint32_t e, uint4 bd) { bias_point(wr, sbp, e, bd); });
 }
 
 // Final impulses back to constraint-point order (for the cache store and
@@ -572,10 +633,16 @@ void launch_writeback(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, co
                       const uint32_t* vel_pos, const uint32_t* ws_pos, uint32_t np) {
     if (np) k_writeback<<<blocks(np, 256), 256, 0, s>>>(vr, wr, c, vel_pos, ws_pos, np, nullptr);
 }
-void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b,
-                         const Schedule& ws, const Schedule& vel, const SolveParams& sp, uint32_t groups,
-                         uint32_t threads, float* resid) {
-    if (groups) k_solve_groups<<<groups, threads, 0, s>>>(vr, wr, b, ws, vel, sp, resid);
+// packed: 8 float4 per body (velocity lines, then pseudo-velocity lines).
+void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b, uint32_t nbodies,
+                         float4* packed, const Schedule& ws, const Schedule& vel, const SolveParams& sp,
+                         uint32_t groups, uint32_t threads, float* resid) {
+    if (!groups || !nbodies) return;
+    float4* sbv = packed;
+    float4* sbp = packed + 4 * size_t(nbodies);
+    k_pack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
+    k_solve_groups<<<groups, threads, 0, s>>>(vr, wr, sbv, sbp, ws, vel, sp, resid);
+    k_unpack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
 }
 void launch_cache_store(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const CacheBuf& cache, uint32_t nc) {
     if (nc) k_cache_store<<<blocks(nc, 256), 256, 0, s>>>(c, b, cache, nc, nullptr);
