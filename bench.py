@@ -282,6 +282,8 @@ def main():
     ap.add_argument("--learn-collect", type=int, default=100, help="last warm-up steps that collect training rows")
     ap.add_argument("--learn-steps", type=int, default=2000, help="Adam steps on the device")
     ap.add_argument("--learn-batch", type=int, default=16384)
+    ap.add_argument("--cooperative", type=int, default=0,
+                    help="workgroups sharing the largest island's levels (0 auto, 1 one per group)")
     ap.add_argument("--workload", choices=("pile", "batched", "narrowphase"), default="pile")
     ap.add_argument("--worlds", type=int, default=4096, help="batched workload: number of worlds")
     ap.add_argument("--pairs", type=int, default=1 << 20, help="narrowphase workload: number of pairs")
@@ -306,6 +308,7 @@ def main():
     log(f"world: {args.bodies} bodies + 5 static, built in {time.time() - t0:.1f} s")
     learn = not args.no_learned and args.learn_collect > 0
     ctx = w.device_context()
+    ctx.set_cooperative(args.cooperative)
     if learn:
         ctx.mlp_init(0x4B5A0005)
     for i in range(args.warmup):
@@ -335,6 +338,9 @@ def main():
     resident_ms = 0.0
     solve_kernel_ms = 0.0
     solve_bytes = 0.0
+    coop_ms = 0.0
+    coop_bytes = 0.0
+    coop_blocks = set()
     levels = []
     phases = np.zeros(6)
     t_start = time.perf_counter()
@@ -347,6 +353,11 @@ def main():
         resident_ms += tm["narrowphase_ms"] + tm["build_ms"] + tm["integrate_velocities_ms"] + tm["solve_ms"]
         solve_kernel_ms += tm["solve_kernel_ms"]
         solve_bytes += st["num_contact_points"] * (WARM_BYTES + 8 * VEL_BYTES + 2 * BIAS_BYTES)
+        coop_blocks.add(tm["cooperative_blocks"])
+        if tm["cooperative_blocks"] > 1:
+            coop_ms += tm["cooperative_kernel_ms"]
+            coop_bytes += (tm["cooperative_ws_points"] * (WARM_BYTES + 2 * BIAS_BYTES)
+                           + tm["cooperative_vel_points"] * 8 * VEL_BYTES)
         levels.append((tm["velocity_levels"], tm["bias_levels"], tm["groups"], tm["islands"]))
         phases += [tm["narrowphase_ms"], tm["build_ms"], tm["integrate_velocities_ms"], tm["solve_ms"],
                    tm["solve_kernel_ms"], st["broadphase_ms"]]
@@ -358,11 +369,18 @@ def main():
     # beside it.
     value = points / (resident_ms / 1e3)
     pcie_value = points / (path_ms / 1e3)
-    achieved = solve_bytes / (solve_kernel_ms / 1e3) / 1e9
+    # Roofline of the dominant kernel: the cooperative sweep of the largest
+    # island's group when it ran, else the per-group sweep kernel (the span
+    # then also holds the body pack/unpack kernels).
+    if coop_ms > 0:
+        rl_kernel, achieved = "k_solve_coop", coop_bytes / (coop_ms / 1e3) / 1e9
+    else:
+        rl_kernel, achieved = "k_solve_groups", solve_bytes / (solve_kernel_ms / 1e3) / 1e9
     lv = np.array(levels)
     log(f"timed {args.steps} steps: {points / args.steps:.0f} points/step, contact path {resident_ms / args.steps:.2f} ms "
         f"resident, {path_ms / args.steps:.2f} ms with transfers, "
-        f"solve kernel {solve_kernel_ms / args.steps:.2f} ms, levels vel/bias max {lv[:, 0].max()}/{lv[:, 1].max()}, "
+        f"solve kernels {solve_kernel_ms / args.steps:.2f} ms (cooperative {coop_ms / args.steps:.2f} ms, "
+        f"workgroups {sorted(coop_blocks)}), levels vel/bias max {lv[:, 0].max()}/{lv[:, 1].max()}, "
         f"groups {lv[:, 2].mean():.0f}, islands {lv[:, 3].mean():.0f}")
     ph = phases / args.steps
     log(f"per step (ms): narrowphase {ph[0]:.2f}, build (incl. host schedule) {ph[1]:.2f}, integrate velocities "
@@ -417,7 +435,7 @@ def main():
                    "contact_path_ms_per_step": resident_ms / args.steps,
                    "contact_path_ms_per_step_with_pcie": path_ms / args.steps,
                    "contacts_per_s_with_pcie": pcie_value},
-        "roofline": {"bound": "hbm", "kernel": "k_solve_groups", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": rl_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,
         "residual": residual,

@@ -78,6 +78,11 @@ typedef struct kzg_timings {
     float solve_kernel_ms;   /* the persistent sweep kernel alone */
     uint32_t velocity_levels, bias_levels, groups, islands;
     float build_kernel_ms;   /* the constraint build kernel alone */
+    uint32_t cooperative_blocks;  /* workgroups that shared the largest group's levels */
+    /* with cooperative_blocks > 1: that group's points in the warm-start/bias
+     * and in the velocity sequences, and its sweep kernel alone */
+    uint32_t cooperative_ws_points, cooperative_vel_points;
+    float cooperative_kernel_ms;
 } kzg_timings;
 
 /* Create a context on HIP device `device` for a world with settings `world`. */
@@ -151,6 +156,12 @@ enum kzg_warm_start {
 int kzg_set_warm_start(kzg_ctx* ctx, int mode);
 /* Override SolverSettings::velocityIterations / positionIterations. */
 int kzg_set_iterations(kzg_ctx* ctx, uint32_t velocity, uint32_t position);
+/* Workgroups that share the levels of the largest group of islands (the
+ * one holding the largest island) in the sweeps, with a device-wide barrier
+ * between levels: 0 = chosen from the level widths (default), 1 = one
+ * workgroup per group, n = n workgroups (at most 64).  Results are the
+ * same whatever the setting: a level's points share no dynamic body. */
+int kzg_set_cooperative(kzg_ctx* ctx, uint32_t blocks);
 
 /* ---- Learned warm start (no reference counterpart) ---------------------
  * A per-contact-point MLP 14 -> 64 -> 64 -> 3 with ReLU hidden layers,

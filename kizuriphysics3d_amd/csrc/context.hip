@@ -14,6 +14,37 @@ float4 hull_face_normal(const float* v, uint32_t a, uint32_t b, uint32_t cc) {
     return make_float4(n.x, n.y, n.z, 0.0f);
 }
 
+constexpr uint32_t kMaxCoopBlocks = 64;
+constexpr uint32_t kAutoCoopBlocks = 8;   // auto mode's ceiling (fastest measured on the 32k pile)
+constexpr uint32_t kCoopSlice = 64;       // auto mode: points per workgroup and level, on average
+
+struct CoopPlan {
+    uint32_t blocks = 1, threads = 64;
+    uint32_t ws_points = 0, vel_points = 0;  // group 0's
+};
+
+// Workgroups and threads for group 0 (the group holding the largest
+// island).  `request` 0 sizes the split from the group's mean level width.
+CoopPlan plan_cooperative(const ScheduleOutput& s, uint32_t request) {
+    CoopPlan p;
+    if (request == 1 || s.groups == 0 || s.ws.grp.empty()) return p;
+    uint32_t points = 0, levels = 0, widest = 0;
+    for (const LevelSchedule* ls : {&s.ws, &s.vel}) {
+        if (ls->grp.empty()) continue;
+        const U2 g = ls->grp[0];
+        for (uint32_t l = g.x; l < g.y; ++l) widest = std::max(widest, ls->off[l + 1] - ls->off[l]);
+        points += ls->off[g.y] - ls->off[g.x];
+        levels += g.y - g.x;
+        (ls == &s.ws ? p.ws_points : p.vel_points) = ls->off[g.y] - ls->off[g.x];
+    }
+    if (!levels) return CoopPlan{};
+    p.blocks = request ? request : std::min(kAutoCoopBlocks, std::max(1u, points / levels / kCoopSlice));
+    if (p.blocks <= 1) return CoopPlan{};
+    const uint32_t slice = (widest + p.blocks - 1) / p.blocks;
+    while (p.threads < slice && p.threads < 1024) p.threads *= 2;
+    return p;
+}
+
 }  // namespace
 
 extern "C" {
@@ -362,11 +393,23 @@ int kzg_solve(kzg_ctx* c, float dt) {
     const uint32_t width = std::max(c->sched.ws.max_width, c->sched.vel.max_width);
     uint32_t threads = 64;
     while (threads < width && threads < 1024) threads *= 2;
+    const CoopPlan coop = plan_cooperative(c->sched, c->coop_blocks);
+    c->coop_used = coop.blocks;
+    c->coop_ws_points = coop.ws_points;
+    c->coop_vel_points = coop.vel_points;
     KZG_HIP(c, record(c, 5));
     if (c->ncons) {
         KZG_HIP(c, c->b_packed_solver.ensure(8 * size_t(std::max<uint32_t>(c->nbodies, 1))));
-        launch_solve_groups(c->stream, c->vel_records(), c->ws_records(), c->bodies(), c->nbodies,
-                            c->b_packed_solver.p, ws, vel, sp, c->sched.groups, threads, c->resid.p);
+        if (coop.blocks > 1) {
+            KZG_HIP(c, c->coop_counter.ensure(1));
+            KZG_HIP(c, launch_solve_split(c->stream, c->vel_records(), c->ws_records(), c->bodies(), c->nbodies,
+                                          c->b_packed_solver.p, ws, vel, sp, c->sched.groups, threads, c->resid.p,
+                                          coop.blocks, coop.threads, c->coop_counter.p, c->ev[10], c->ev[11]));
+            c->recorded |= 3u << 10;
+        } else {
+            launch_solve_groups(c->stream, c->vel_records(), c->ws_records(), c->bodies(), c->nbodies,
+                                c->b_packed_solver.p, ws, vel, sp, c->sched.groups, threads, c->resid.p);
+        }
     }
     KZG_HIP(c, hipGetLastError());
     KZG_HIP(c, record(c, 6));
@@ -529,6 +572,12 @@ int kzg_get_timings(kzg_ctx* c, kzg_timings* out) {
     t.velocity_levels = c->sched.vel.max_depth;
     t.bias_levels = c->sched.ws.max_depth;
     t.groups = c->sched.groups;
+    t.cooperative_blocks = c->coop_used;
+    if (c->coop_used > 1) {
+        t.cooperative_ws_points = c->coop_ws_points;
+        t.cooperative_vel_points = c->coop_vel_points;
+        span(10, 11, t.cooperative_kernel_ms);
+    }
     t.islands = c->sched.islands;
     *out = t;
     return KZG_OK;
@@ -632,6 +681,13 @@ int kzg_set_iterations(kzg_ctx* c, uint32_t velocity, uint32_t position) {
     if (velocity > 1024 || position > 1024) return fail(c, KZG_ERR_ARGUMENT, "iteration count above 1024");
     c->world.velocity_iterations = velocity;
     c->world.position_iterations = position;
+    return KZG_OK;
+}
+
+int kzg_set_cooperative(kzg_ctx* c, uint32_t blocks) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (blocks > kMaxCoopBlocks) return fail(c, KZG_ERR_ARGUMENT, "more than 64 cooperative workgroups");
+    c->coop_blocks = blocks;
     return KZG_OK;
 }
 

@@ -34,6 +34,11 @@ void launch_writeback(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, co
 void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b, uint32_t nbodies,
                          float4* packed, const Schedule& ws, const Schedule& vel, const SolveParams& sp,
                          uint32_t groups, uint32_t threads, float* resid);
+hipError_t launch_solve_split(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b,
+                              uint32_t nbodies, float4* packed, const Schedule& ws, const Schedule& vel,
+                              const SolveParams& sp, uint32_t groups, uint32_t threads, float* resid,
+                              uint32_t coop_blocks, uint32_t coop_threads, uint32_t* counter,
+                              hipEvent_t coop_begin, hipEvent_t coop_end);
 void launch_cache_store(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const CacheBuf& cache, uint32_t nc);
 void launch_cache_rehash(hipStream_t s, const CacheBuf& src, const CacheBuf& dst);
 void launch_finish(hipStream_t s, const BodyBuf& b, uint32_t n, float dt, int split);
@@ -118,6 +123,10 @@ struct kzg_ctx {
     std::vector<uint8_t> h_dynamic, h_sensor, h_motion;
     DevVec<float4> b_x, b_q, b_v, b_w, b_pv, b_pw, b_iw, b_il, b_mat, b_lim;
     DevVec<float4> b_packed_solver;  // 8 float4 per body during the solve
+    DevVec<uint32_t> coop_counter;   // device-wide barrier of the cooperative solve
+    uint32_t coop_blocks = 0;        // kzg_set_cooperative: 0 auto, 1 off, n workgroups
+    uint32_t coop_used = 1;          // workgroups the last solve gave group 0
+    uint32_t coop_ws_points = 0, coop_vel_points = 0;
     DevVec<uint32_t> b_meta, b_packed;
 
     // pairs and manifolds
@@ -163,7 +172,7 @@ struct kzg_ctx {
 
     // staging and timing
     Pinned stage, stage2, stage_a, stage_b;
-    hipEvent_t ev[10] = {};
+    hipEvent_t ev[12] = {};
     uint32_t recorded = 0;  // bit i: ev[i] recorded since the last narrowphase
     kzg_timings timings{};
     double build_host_ms = 0.0;

@@ -446,24 +446,58 @@ __device__ __forceinline__ LevelOffsets stage_offsets(const Schedule& s, uint2 r
 // the next level are fetched before the current level is processed (they
 // are constant during the solve), so after each barrier a point waits for
 // one round of loads only.
-template <class F>
-__device__ __forceinline__ void sweep(const LevelOffsets& off, uint2 range, const uint4* bd, F&& f) {
+// With `parts` workgroups sharing the levels, workgroup `part` takes the
+// part-th contiguous slice of every level (its threads stride through it)
+// and `sync` is a device-wide barrier; one workgroup alone uses
+// __syncthreads().
+struct Slice {
+    uint32_t lo, hi;
+};
+__device__ __forceinline__ Slice level_slice(const LevelOffsets& off, uint32_t l, uint32_t parts, uint32_t part) {
+    const uint32_t e0 = off(l), e1 = off(l + 1);
+    if (parts == 1) return {e0, e1};
+    const uint32_t chunk = (e1 - e0 + parts - 1) / parts;
+    const uint32_t lo = min(e0 + part * chunk, e1);
+    return {lo, min(lo + chunk, e1)};
+}
+template <class F, class Sync>
+__device__ __forceinline__ void sweep(const LevelOffsets& off, uint2 range, const uint4* bd, uint32_t parts,
+                                      uint32_t part, F&& f, Sync&& sync) {
     const uint32_t t = threadIdx.x, nt = blockDim.x;
     uint4 nxt = make_uint4(0, 0, 0, 0);
+    Slice cur_s = {0, 0};
     if (range.x < range.y) {
-        const uint32_t e = off(range.x) + t;
-        if (e < off(range.x + 1)) nxt = bd[e];
+        cur_s = level_slice(off, range.x, parts, part);
+        if (cur_s.lo + t < cur_s.hi) nxt = bd[cur_s.lo + t];
     }
     for (uint32_t l = range.x; l < range.y; ++l) {
-        const uint32_t e0 = off(l) + t, e1 = off(l + 1);
+        const Slice sl = cur_s;
         const uint4 cur = nxt;
         if (l + 1 < range.y) {
-            const uint32_t en = e1 + t;
-            if (en < off(l + 2)) nxt = bd[en];
+            cur_s = level_slice(off, l + 1, parts, part);
+            if (cur_s.lo + t < cur_s.hi) nxt = bd[cur_s.lo + t];
         }
-        for (uint32_t e = e0; e < e1; e += nt) f(e, e == e0 ? cur : bd[e]);
-        __syncthreads();
+        for (uint32_t e = sl.lo + t; e < sl.hi; e += nt) f(e, e == sl.lo + t ? cur : bd[e]);
+        sync();
     }
+}
+template <class F>
+__device__ __forceinline__ void sweep(const LevelOffsets& off, uint2 range, const uint4* bd, F&& f) {
+    sweep(off, range, bd, 1u, 0u, f, [] { __syncthreads(); });
+}
+
+// Device-wide barrier of the cooperating workgroups: monotonic counter,
+// agent-scope fences publish the body updates of a level before the next.
+__device__ __forceinline__ void grid_sync(uint32_t* counter, uint32_t target) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(counter, 1u);
+        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
+            __builtin_amdgcn_s_sleep(1);
+        __threadfence();
+    }
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -476,10 +510,10 @@ __device__ __forceinline__ void sweep(const LevelOffsets& off, uint2 range, cons
 // impulse increments of each velocity sweep.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(1024) k_solve_groups(VelRecBuf vr, WsRecBuf wr, float4* sbv, float4* sbp, Schedule ws,
-                                                       Schedule vel, SolveParams sp, float* resid) {
+                                                       Schedule vel, SolveParams sp, float* resid, uint32_t group0) {
     __shared__ uint32_t ws_lds[kLevelsLds], vel_lds[kLevelsLds];
-    const uint2 wrange = ws.grp[blockIdx.x];
-    const uint2 vrange = vel.grp[blockIdx.x];
+    const uint2 wrange = ws.grp[group0 + blockIdx.x];
+    const uint2 vrange = vel.grp[group0 + blockIdx.x];
     const LevelOffsets wo = stage_offsets(ws, wrange, ws_lds);
     const LevelOffsets vo = stage_offsets(vel, vrange, vel_lds);
     __syncthreads();
@@ -497,6 +531,39 @@ __global__ void __launch_bounds__(1024) k_solve_groups(VelRecBuf vr, WsRecBuf wr
         for (uint32_t it = 0; it < sp.position_iterations; ++it)
             sweep(wo, wrange, wr.bd, [&](uint32_t e, uint4 bd) { bias_point(wr, sbp, e, bd); });
 }
+
+// One group solved by gridDim.x cooperating workgroups (the launch makes
+// them co-resident): the same sweeps, each level split over all of them.
+__global__ void __launch_bounds__(1024) k_solve_coop(VelRecBuf vr, WsRecBuf wr, float4* sbv, float4* sbp, Schedule ws,
+                                                     Schedule vel, SolveParams sp, float* resid, uint32_t group,
+                                                     uint32_t* counter) {
+    __shared__ uint32_t ws_lds[kLevelsLds], vel_lds[kLevelsLds];
+    const uint2 wrange = ws.grp[group];
+    const uint2 vrange = vel.grp[group];
+    const LevelOffsets wo = stage_offsets(ws, wrange, ws_lds);
+    const LevelOffsets vo = stage_offsets(vel, vrange, vel_lds);
+    __syncthreads();
+    const uint32_t parts = gridDim.x, part = blockIdx.x;
+    uint32_t phase = 0;
+    auto gsync = [&] { grid_sync(counter, gridDim.x * ++phase); };
+    const float wsf = sp.warm_start_factor;
+    sweep(wo, wrange, wr.bd, parts, part, [&](uint32_t e, uint4 bd) { warm_point(wr, sbv, e, bd, wsf); }, gsync);
+    for (uint32_t it = 0; it < sp.velocity_iterations; ++it) {
+        float dsq = 0.0f;
+        if (sp.use_simd)
+            sweep(vo, vrange, vr.bd, parts, part,
+                  [&](uint32_t e, uint4 bd) { dsq += velocity_point(vr, sbv, e, bd); }, gsync);
+        else
+            sweep(vo, vrange, vr.bd, parts, part,
+                  [&](uint32_t e, uint4 bd) { dsq += scalar_point(vr, sbv, e, bd); }, gsync);
+        if (resid && dsq != 0.0f) atomicAdd(resid + it, dsq);
+    }
+    if (!sp.use_velocity_bias)
+        for (uint32_t it = 0; it < sp.position_iterations; ++it)
+            sweep(wo, wrange, wr.bd, parts, part, [&](uint32_t e, uint4 bd) { bias_point(wr, sbp, e, bd); }, gsync);
+}
+
+__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
 
 // Final impulses back to constraint-point order (for the cache store and
 // read-back): velocity impulses from the velocity records, the bias impulse
@@ -641,8 +708,36 @@ void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr,
     float4* sbv = packed;
     float4* sbp = packed + 4 * size_t(nbodies);
     k_pack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
-    k_solve_groups<<<groups, threads, 0, s>>>(vr, wr, sbv, sbp, ws, vel, sp, resid);
+    k_solve_groups<<<groups, threads, 0, s>>>(vr, wr, sbv, sbp, ws, vel, sp, resid, 0u);
     k_unpack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
+}
+
+// Group 0 by `coop_blocks` cooperating workgroups of `coop_threads`, the
+// other groups one workgroup each.
+hipError_t launch_solve_split(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b,
+                              uint32_t nbodies, float4* packed, const Schedule& ws, const Schedule& vel,
+                              const SolveParams& sp, uint32_t groups, uint32_t threads, float* resid,
+                              uint32_t coop_blocks, uint32_t coop_threads, uint32_t* counter,
+                              hipEvent_t coop_begin, hipEvent_t coop_end) {
+    if (!groups || !nbodies) return hipSuccess;
+    float4* sbv = packed;
+    float4* sbp = packed + 4 * size_t(nbodies);
+    k_pack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
+    k_zero_u32<<<1, 1, 0, s>>>(counter);
+    VelRecBuf vr_ = vr;
+    WsRecBuf wr_ = wr;
+    Schedule ws_ = ws, vel_ = vel;
+    SolveParams sp_ = sp;
+    uint32_t group = 0;
+    void* args[] = {&vr_, &wr_, &sbv, &sbp, &ws_, &vel_, &sp_, &resid, &group, &counter};
+    if (coop_begin) hipEventRecord(coop_begin, s);
+    hipError_t e = hipLaunchCooperativeKernel((const void*)k_solve_coop, dim3(coop_blocks), dim3(coop_threads), args,
+                                              0, s);
+    if (e != hipSuccess) return e;
+    if (coop_end) hipEventRecord(coop_end, s);
+    if (groups > 1) k_solve_groups<<<groups - 1, threads, 0, s>>>(vr, wr, sbv, sbp, ws, vel, sp, resid, 1u);
+    k_unpack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
+    return hipGetLastError();
 }
 void launch_cache_store(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const CacheBuf& cache, uint32_t nc) {
     if (nc) k_cache_store<<<blocks(nc, 256), 256, 0, s>>>(c, b, cache, nc, nullptr);

@@ -28,6 +28,8 @@ TIMINGS_DTYPE = np.dtype([
     ("narrowphase_ms", "<f4"), ("build_ms", "<f4"), ("integrate_velocities_ms", "<f4"), ("solve_ms", "<f4"),
     ("integrate_positions_ms", "<f4"), ("solve_kernel_ms", "<f4"), ("velocity_levels", "<u4"),
     ("bias_levels", "<u4"), ("groups", "<u4"), ("islands", "<u4"), ("build_kernel_ms", "<f4"),
+    ("cooperative_blocks", "<u4"), ("cooperative_ws_points", "<u4"), ("cooperative_vel_points", "<u4"),
+    ("cooperative_kernel_ms", "<f4"),
 ])
 assert KZG_BODY_DTYPE.itemsize == 136
 
@@ -36,7 +38,7 @@ EXPORTS = [
     "kzg_narrowphase", "kzg_build_constraints", "kzg_integrate_velocities", "kzg_solve", "kzg_integrate_positions",
     "kzg_step", "kzg_query_sizes", "kzg_download_states", "kzg_download_manifolds", "kzg_download_constraints",
     "kzg_get_residuals", "kzg_get_timings", "kzg_collide_batch", "kzg_schedule", "kzg_set_warm_start",
-    "kzg_set_iterations", "kzg_mlp_init", "kzg_mlp_set_params", "kzg_mlp_get_params", "kzg_mlp_set_normalization",
+    "kzg_set_iterations", "kzg_set_cooperative", "kzg_mlp_init", "kzg_mlp_set_params", "kzg_mlp_get_params", "kzg_mlp_set_normalization",
     "kzg_mlp_get_normalization", "kzg_mlp_collect", "kzg_mlp_num_samples", "kzg_mlp_get_samples",
     "kzg_mlp_fit_normalization", "kzg_mlp_train", "kzg_mlp_forward", "kzg_mlp_gradients",
 ]
@@ -75,6 +77,7 @@ def library():
             "kzg_collide_batch": ([vp, vp, vp, vp, vp, u32, f32, vp, vp, vp], i32),
             "kzg_schedule": ([u32, vp, u32, vp, i32, u32, vp, vp, vp, vp], i32),
             "kzg_set_warm_start": ([vp, i32], i32), "kzg_set_iterations": ([vp, u32, u32], i32),
+            "kzg_set_cooperative": ([vp, u32], i32),
             "kzg_mlp_init": ([vp, C.c_uint64], i32), "kzg_mlp_set_params": ([vp, vp, u32], i32),
             "kzg_mlp_get_params": ([vp, vp, u32], i32), "kzg_mlp_set_normalization": ([vp, vp, vp], i32),
             "kzg_mlp_get_normalization": ([vp, vp, vp], i32), "kzg_mlp_collect": ([vp, u32], i32),
@@ -267,6 +270,10 @@ class DeviceContext:
 
     def set_iterations(self, velocity: int, position: int):
         self._check(library().kzg_set_iterations(self.h, int(velocity), int(position)), "kzg_set_iterations")
+
+    def set_cooperative(self, blocks: int):
+        """Workgroups sharing the largest group's levels: 0 auto, 1 off."""
+        self._check(library().kzg_set_cooperative(self.h, int(blocks)), "kzg_set_cooperative")
 
     def mlp_init(self, seed: int = 0x4B5A0005):
         self._check(library().kzg_mlp_init(self.h, C.c_uint64(seed)), "kzg_mlp_init")

@@ -41,10 +41,11 @@ def test_narrowphase_batch_bit_exact(n):
         assert i is None, f"point {k} differs at pair {np.nonzero(live)[0][i]}"
 
 
-def _trajectory(scene, steps):
+def _trajectory(scene, steps, cooperative=0):
     R.init_jobs(0)
     rw = R.RefWorld(scene)
     ctx = _ctx_for(scene, rw)
+    ctx.set_cooperative(cooperative)
     for i in range(steps):
         rw.step_phases()
         pairs = rw.pairs()
@@ -70,6 +71,36 @@ def test_pyramid_trajectory_bit_exact():
 
 def test_pile_trajectory_bit_exact():
     _trajectory(S.pile(512, seed=0x4B5A0102), 240)
+
+
+@pytest.mark.parametrize("blocks", [2, 7, 64])
+def test_pile_trajectory_cooperative_bit_exact(blocks):
+    """The largest group's levels split over `blocks` workgroups with a
+    device-wide barrier between levels (forced even where levels are narrow,
+    so some workgroups get empty slices)."""
+    _trajectory(S.pile(512, seed=0x4B5A0102), 240, cooperative=blocks)
+
+
+def test_cooperative_split_invariant_large_pile():
+    """On a 4096-body pile the state after every step is the same with one
+    workgroup, the automatic split, 4 and 16 workgroups."""
+    from kizuriphysics3d_amd.world import PhysicsWorld
+    scene = S.pile(4096, seed=0x4B5A0104)
+    worlds = []
+    for blocks in (1, 0, 4, 16):
+        w = PhysicsWorld(scene)
+        w.device_context().set_cooperative(blocks)
+        worlds.append(w)
+    used = set()
+    for i in range(300):
+        for w in worlds:
+            w.step_fixed()
+        base = worlds[0].states()
+        for w in worlds[1:]:
+            j = first_mismatch(base, w.states())
+            assert j is None, f"step {i}: body {j} differs between cooperative splits"
+        used.add(int(worlds[1].device_context().timings()["cooperative_blocks"]))
+    print(f"automatic split used {sorted(used)} workgroups", flush=True)
 
 
 def test_residuals_match_oracle():
