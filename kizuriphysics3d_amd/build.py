@@ -45,7 +45,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = ["hipcc", "-shared", "--offload-arch=gfx950", "-fPIC", "-o", tmp] + objs
+    cmd = ["hipcc", "-shared", "--offload-arch=gfx950", "-fPIC", "-pthread", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

@@ -4,6 +4,11 @@
 
 #include <stdint.h>
 
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 namespace kz {
@@ -44,15 +49,48 @@ struct ScheduleOutput {
     uint32_t groups = 0;
 };
 
+// A first-fit batch: bodies A of lanes 0-3, then bodies B (unused lanes hold
+// 0xFFFFFFFF), the points, the lane count.
+struct FitBatch {
+    uint32_t body[8];
+    uint32_t pt[4];
+    uint32_t n;
+};
+
 class ScheduleBuilder {
 public:
+    ScheduleBuilder() = default;
+    ScheduleBuilder(const ScheduleBuilder&) = delete;
+    ScheduleBuilder& operator=(const ScheduleBuilder&) = delete;
+    ~ScheduleBuilder();
     void build(const ScheduleInput& in, ScheduleOutput& out);
 
 private:
+    // A sequence of points with their group and level, before bucketing.
+    struct Seq {
+        std::vector<uint32_t> pts, grp, lvl, last, depth, base, fill;
+        uint32_t n = 0;
+    };
+    void islands(const ScheduleInput& in, ScheduleOutput& out);
+    void ws_levels(const ScheduleInput& in);
+    void vel_sequence(const ScheduleInput& in);
+    void flatten(Seq& s, uint32_t groups, uint32_t num_points, LevelSchedule& out);
+    void run_helper(std::function<void()> job);
+    void wait_helper();
+
     std::vector<uint32_t> dyn_index_, parent_, rank_, root_island_, island_of_con_, island_points_;
-    std::vector<uint32_t> group_of_island_, last_;
-    std::vector<uint32_t> pt_group_, pt_level_;
-    std::vector<std::vector<uint32_t>> island_cons_;
+    std::vector<uint32_t> island_off_, island_cons_, group_of_island_, order_;
+    std::vector<uint32_t> ws_con_;  // per point of the warm-start sequence: its constraint
+    Seq ws_, vel_;
+    std::vector<FitBatch> batches_;
+    uint32_t groups_ = 0;
+    // One helper thread runs the warm-start sequence beside the velocity one.
+    std::thread helper_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::function<void()> job_;
+    bool stop_ = false, busy_ = false;
+    std::atomic<bool> groups_ready_{false};
 };
 
 }  // namespace kz
