@@ -269,6 +269,32 @@ def bench_narrowphase(args):
     }
 
 
+def bench_copy(args):
+    """A plain device-to-device copy of known size: the reference point that
+    scales the HBM byte counters (FETCH_SIZE, WRITE_SIZE) of other kernels."""
+    import torch
+    n = args.copy_mib << 18  # float32 elements
+    a = torch.rand(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(max(args.warmup, 1)):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(args.steps):
+        b.copy_(a)
+    t1.record()
+    torch.cuda.synchronize()
+    per = t0.elapsed_time(t1) / args.steps
+    moved = 2 * 4 * n
+    return {"metric": "device copy bandwidth", "value": moved / (per / 1e3) / 1e9, "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": max(args.warmup, 1), "ms_per_step": per, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"copy of {args.copy_mib} MiB (read {args.copy_mib} MiB, write {args.copy_mib} MiB)",
+                       "bytes_read": 4 * n, "bytes_written": 4 * n},
+            "roofline": None, "cpu_baseline": None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -284,7 +310,8 @@ def main():
     ap.add_argument("--learn-batch", type=int, default=16384)
     ap.add_argument("--cooperative", type=int, default=0,
                     help="workgroups sharing the largest island's levels (0 auto, 1 one per group)")
-    ap.add_argument("--workload", choices=("pile", "batched", "narrowphase"), default="pile")
+    ap.add_argument("--workload", choices=("pile", "batched", "narrowphase", "copy"), default="pile")
+    ap.add_argument("--copy-mib", type=int, default=1024, help="copy workload: buffer size")
     ap.add_argument("--worlds", type=int, default=4096, help="batched workload: number of worlds")
     ap.add_argument("--pairs", type=int, default=1 << 20, help="narrowphase workload: number of pairs")
     args = ap.parse_args()
@@ -297,6 +324,9 @@ def main():
         return
     if args.workload == "narrowphase":
         print(json.dumps(bench_narrowphase(args)), flush=True)
+        return
+    if args.workload == "copy":
+        print(json.dumps(bench_copy(args)), flush=True)
         return
 
     import torch  # device synchronisation around the timed region

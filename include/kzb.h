@@ -43,8 +43,12 @@ typedef struct kzb_config {
     uint32_t narrowphase_waves;  /* workgroups (one wave each) of the iterative narrowphase; 0 = 16 per CU */
     uint32_t solve_threads;      /* threads per world in the solve kernel: 64..1024, power of two */
     uint32_t broadphase_threads; /* host threads for the per-world broadphase; 0 = hardware */
-    uint32_t reserved;
+    uint32_t flags;              /* KZB_DIRECT_LAUNCH */
 } kzb_config;
+
+/* kzb_config.flags: launch the step's kernels one by one instead of replaying
+ * the captured graph, with events between the phases (kzb_stats.phase_ms). */
+#define KZB_DIRECT_LAUNCH 1u
 
 typedef struct kzb_stats {
     uint32_t num_worlds, num_bodies, num_pairs, num_constraints, num_points;
@@ -54,6 +58,10 @@ typedef struct kzb_stats {
     double device_ms;       /* HIP events around the graph: the device step */
     double step_ms;         /* host clock around kzb_step */
     double device_with_copies_ms;  /* the graph and the copies of the body states to the host */
+    double launch_ms;       /* host clock around the graph launch (or the direct launches) */
+    /* KZB_DIRECT_LAUNCH only: narrowphase, compaction + constraint build,
+     * schedule, integration + prepare, sweeps, write-back to the end */
+    float phase_ms[6];
 } kzb_stats;
 
 int kzb_create(kzb_batch** out, const kz_world_desc* settings, int device);

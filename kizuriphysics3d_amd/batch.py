@@ -10,12 +10,14 @@ from . import device as D
 from . import scenes as S
 
 CONFIG_DTYPE = np.dtype([("narrowphase_waves", "<u4"), ("solve_threads", "<u4"), ("broadphase_threads", "<u4"),
-                         ("reserved", "<u4")])
+                         ("flags", "<u4")])
 STATS_DTYPE = np.dtype([
     ("num_worlds", "<u4"), ("num_bodies", "<u4"), ("num_pairs", "<u4"), ("num_constraints", "<u4"),
     ("num_points", "<u4"), ("max_velocity_levels", "<u4"), ("max_bias_levels", "<u4"), ("reserved", "<u4"),
     ("broadphase_ms", "<f8"), ("device_ms", "<f8"), ("step_ms", "<f8"), ("device_with_copies_ms", "<f8"),
+    ("launch_ms", "<f8"), ("phase_ms", "<f4", (6,)),
 ])
+DIRECT_LAUNCH = 1  # kzb_config.flags: kernels one by one with phase events, no graph
 KZB_EXPORTS = ["kzb_create", "kzb_destroy", "kzb_last_error", "kzb_add_world", "kzb_step", "kzb_num_worlds",
                "kzb_get_states", "kzb_get_stats", "kzb_set_config", "kzb_get_config", "kzb_autotune",
                "kzb_autotune_candidates", "kzb_world_schedule"]
@@ -96,17 +98,17 @@ class BatchedWorlds:
     def stats(self) -> dict:
         s = np.zeros((), STATS_DTYPE)
         self._check(_lib().kzb_get_stats(self.h, _p(s)), "kzb_get_stats")
-        return {k: s[k].item() for k in STATS_DTYPE.names}
+        return {k: s[k].tolist() for k in STATS_DTYPE.names}
 
     def config(self) -> dict:
         c = np.zeros((), CONFIG_DTYPE)
         self._check(_lib().kzb_get_config(self.h, _p(c)), "kzb_get_config")
         return {k: c[k].item() for k in CONFIG_DTYPE.names}
 
-    def set_config(self, narrowphase_waves=0, solve_threads=0, broadphase_threads=0):
+    def set_config(self, narrowphase_waves=0, solve_threads=0, broadphase_threads=0, flags=0):
         c = np.zeros((), CONFIG_DTYPE)
-        c["narrowphase_waves"], c["solve_threads"], c["broadphase_threads"] = narrowphase_waves, solve_threads, \
-            broadphase_threads
+        c["narrowphase_waves"], c["solve_threads"], c["broadphase_threads"], c["flags"] = narrowphase_waves, \
+            solve_threads, broadphase_threads, flags
         self._check(_lib().kzb_set_config(self.h, _p(c)), "kzb_set_config")
 
     def candidates(self) -> list:

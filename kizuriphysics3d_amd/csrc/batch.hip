@@ -63,6 +63,8 @@ struct BatchDev {
     uint32_t* werr;            // [W] 1 when the world exceeded a schedule table
     uint32_t* lvb_ws;          // [W] first level of the world (each world also owns a sentinel)
     uint32_t* lvb_vel;         // [W]
+    const uint32_t* order;     // [W] worlds by body count, largest first (schedule kernel)
+    uint16_t* lane_pt;         // [4 * points] world-local point of each first-fit batch lane
 };
 
 __device__ __forceinline__ void wave_sync() {
@@ -201,7 +203,7 @@ struct alignas(16) SchedLds {
     uint16_t last_ws[kWorldBodies], last_vel[kWorldBodies], hint[kWorldBodies];
     uint32_t dyn[kWorldBodies / 32];
     uint4 stage[64];
-    uint16_t ba[kWorldBatches * 4], bb[kWorldBatches * 4], bp[kWorldBatches * 4];
+    uint16_t ba[kWorldBatches * 4], bb[kWorldBatches * 4];
     uint16_t bm[kWorldBatches];  // lanes used | island root << 3
 };
 
@@ -215,12 +217,12 @@ __device__ __forceinline__ uint32_t lds_find(const volatile uint32_t* parent, ui
     return x;
 }
 
-__global__ void __launch_bounds__(128) k_world_schedule(BatchDev d, ContactBuf c, BodyBuf b, int use_simd) {
-    __shared__ SchedLds LL[2];
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t w = blockIdx.x * 2u + wave;
-    if (w >= d.W) return;  // the whole wave; no block barrier below
-    SchedLds& L = LL[wave];
+// One wave per workgroup; workgroups take the worlds largest first so that
+// the small worlds fill the tail.
+__global__ void __launch_bounds__(64) k_world_schedule(BatchDev d, ContactBuf c, BodyBuf b, int use_simd) {
+    __shared__ SchedLds L;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t w = d.order[blockIdx.x];
     const uint32_t b0 = d.body_off[w], nb = d.body_off[w + 1] - b0;
     const uint32_t c0 = d.con_base[w], nc = d.wc[w], p0 = d.pt_base[w], np = d.wp[w];
     uint32_t err = 0;
@@ -284,8 +286,10 @@ __global__ void __launch_bounds__(128) k_world_schedule(BatchDev d, ContactBuf c
         err = __shfl(err, 0);
         wave_sync();
     }
-    // First-fit batches (SIMD solver).
+    // First-fit batches (SIMD solver).  A batch's lanes take points of the
+    // world only, so its lane table fits in 4 entries per point.
     uint32_t nbat = 0;
+    uint16_t* lane_pt = d.lane_pt + 4 * size_t(p0);
     if (use_simd) {
         for (uint32_t base = 0; !err && base < nc; base += 64) {
             const uint32_t cnt = nc - base < 64u ? nc - base : 64u;
@@ -334,7 +338,7 @@ __global__ void __launch_bounds__(128) k_world_schedule(BatchDev d, ContactBuf c
                         const uint32_t m = L.bm[found], n = m & 7u;
                         L.ba[found * 4 + n] = uint16_t(x);
                         L.bb[found * 4 + n] = uint16_t(y);
-                        L.bp[found * 4 + n] = uint16_t(first + q);
+                        lane_pt[found * 4 + n] = uint16_t(first + q);
                         L.bm[found] = uint16_t(m + 1u);
                         L.hint[isl] = uint16_t(open);
                     }
@@ -344,13 +348,21 @@ __global__ void __launch_bounds__(128) k_world_schedule(BatchDev d, ContactBuf c
             wave_sync();
         }
     }
-    // Velocity levels in batch order.
+    // Velocity levels in batch order; the lane points come through LDS,
+    // 128 batches at a time.
     uint32_t depth_vel = 0;
-    if (use_simd && !err && lane == 0) {
-        for (uint32_t jb = 0; jb < nbat; ++jb) {
+    uint16_t* stage_pt = reinterpret_cast<uint16_t*>(L.stage);
+    for (uint32_t jb0 = 0; use_simd && !err && jb0 < nbat; jb0 += 128) {
+        const uint32_t nj = nbat - jb0 < 128u ? nbat - jb0 : 128u;
+        wave_sync();
+        for (uint32_t k = lane; k < 4 * nj; k += 64)
+            if ((L.bm[jb0 + k / 4] & 7u) > k % 4) stage_pt[k] = lane_pt[4 * jb0 + k];
+        wave_sync();
+        if (lane != 0) continue;
+        for (uint32_t jb = jb0; jb < jb0 + nj; ++jb) {
             const uint32_t n = L.bm[jb] & 7u;
             for (uint32_t t = 0; t < n; ++t) {
-                const uint32_t x = L.ba[jb * 4 + t], y = L.bb[jb * 4 + t], p = L.bp[jb * 4 + t];
+                const uint32_t x = L.ba[jb * 4 + t], y = L.bb[jb * 4 + t], p = stage_pt[(jb - jb0) * 4 + t];
                 const bool dx = is_dyn(L, x), dy = is_dyn(L, y);
                 const uint32_t lx = dx ? L.last_vel[x] : 0u, ly = dy ? L.last_vel[y] : 0u;
                 uint32_t l = 1u + (lx > ly ? lx : ly);
@@ -493,6 +505,8 @@ struct kzb_batch {
     bool tables_dirty = true;
     std::vector<uint32_t> pair_off;
     // device-side batch arrays
+    DevVec<uint32_t> world_order;
+    DevVec<uint16_t> lane_pt;
     DevVec<uint32_t> d_body_off, d_pair_off, wc, wp, con_base, pt_base, cnt, ws_level, vel_level, depth_ws,
         depth_vel, vel_slots, werr, lvb_ws, lvb_vel;
     Pinned h_pairs, h_out;
@@ -504,10 +518,12 @@ struct kzb_batch {
     kzb_config cfg{};
     kzb_stats stats{};
     hipEvent_t e0 = nullptr, e1 = nullptr, emid = nullptr;
+    hipEvent_t phase_ev[7] = {};  // direct launches: phase boundaries
 
     BatchDev dev() {
         return BatchDev{uint32_t(worlds.size()), d_body_off.p, d_pair_off.p, wc.p, wp.p, con_base.p, pt_base.p, cnt.p,
-                        ws_level.p, vel_level.p, depth_ws.p, depth_vel.p, vel_slots.p, werr.p, lvb_ws.p, lvb_vel.p};
+                        ws_level.p, vel_level.p, depth_ws.p, depth_vel.p, vel_slots.p, werr.p, lvb_ws.p, lvb_vel.p,
+                        world_order.p, lane_pt.p};
     }
 };
 
@@ -597,6 +613,13 @@ int upload_tables(kzb_batch* B) {
     B->nbodies = uint32_t(bodies.size());
     const size_t W = B->worlds.size();
     if ((rc = upload(c, B->d_body_off, B->body_off.data(), W + 1))) return bfail(B, rc, kzg_last_error(c));
+    std::vector<uint32_t> order(W);
+    for (uint32_t i = 0; i < W; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return B->body_off[a + 1] - B->body_off[a] > B->body_off[b + 1] - B->body_off[b];
+    });
+    KZB_HIP(B, B->world_order.ensure(std::max<size_t>(W, 1)));
+    if (W && (rc = upload(c, B->world_order, order.data(), W))) return bfail(B, rc, kzg_last_error(c));
     for (auto* v : {&B->wc, &B->wp, &B->con_base, &B->pt_base, &B->depth_ws, &B->depth_vel, &B->vel_slots, &B->werr,
                     &B->lvb_ws, &B->lvb_vel})
         KZB_HIP(B, v->ensure(std::max<size_t>(W, 1)));
@@ -640,6 +663,7 @@ int ensure_capacity(kzb_batch* B, uint32_t pairs) {
     KZB_HIP(B, c->vel_off.ensure(pts + W + 1));
     KZB_HIP(B, B->ws_level.ensure(pts));
     KZB_HIP(B, B->vel_level.ensure(pts));
+    KZB_HIP(B, B->lane_pt.ensure(4 * pts));
     KZB_HIP(B, B->h_pairs.ensure(2 * sizeof(uint32_t) * size_t(cap) + sizeof(uint32_t) * (W + 1) + 512));
     B->cap_pairs = cap;
     drop_graph(B);
@@ -647,10 +671,16 @@ int ensure_capacity(kzb_batch* B, uint32_t pairs) {
 }
 
 // The device step of every world, captured once per buffer layout.
-int capture(kzb_batch* B) {
-    drop_graph(B);
+// The device step of every world, in stream order.  `ev` (direct launches
+// only) receives events between the phases: narrowphase, compaction and
+// constraint build, schedule, integration and prepare, sweeps, write-back
+// to the end.
+void enqueue_step(kzb_batch* B, hipEvent_t* ev) {
     kzg_ctx* c = B->ctx;
     hipStream_t s = c->stream;
+    auto mark = [&](int k) {
+        if (ev) hipEventRecord(ev[k], s);
+    };
     const kz_world_desc& ws = B->settings;
     const uint32_t W = uint32_t(B->worlds.size()), cap = B->cap_pairs, pts = 4 * cap, nb = B->nbodies;
     const int use_simd = ws.use_simd_solver ? 1 : 0;
@@ -660,28 +690,41 @@ int capture(kzb_batch* B) {
     const uint32_t threads = B->cfg.solve_threads ? B->cfg.solve_threads : 256u;
     SolveParams sp{dt, ws.warm_start_factor, ws.velocity_iterations, ws.position_iterations, ws.use_simd_solver,
                    ws.use_velocity_bias};
-    KZB_HIP(B, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    mark(0);
     k_batch_begin<<<1, 256, 0, s>>>(d, c->work_count.p, c->resid.p, ws.velocity_iterations);
     launch_narrowphase_counted(s, c->d_pairs.p, cap, B->d_pair_off.p + W, c->bodies(), c->shapes(), c->manifolds(),
                                c->work.p, c->work_count.p, 0.02f, waves);
+    mark(1);
     k_world_count<<<W, 256, 0, s>>>(d, c->m_count.p);
     k_world_scan<<<1, 1024, 0, s>>>(d);
     k_world_emit<<<W, 256, 0, s>>>(d, c->contacts(), c->bodies(), c->d_pairs.p, c->m_count.p);
     launch_build_counted(s, c->contacts(), c->manifolds(), c->bodies(), c->cache(), cap, B->cnt.p + kCntCons, 0);
-    k_world_schedule<<<cdiv(W, 2), 128, 0, s>>>(d, c->contacts(), c->bodies(), use_simd);
+    mark(2);
+    k_world_schedule<<<W, 64, 0, s>>>(d, c->contacts(), c->bodies(), use_simd);
     k_level_scan<<<1, 1024, 0, s>>>(d, c->ws_grp.p, c->vel_grp.p);
     k_world_bucket<<<W, 256, 0, s>>>(d, c->ws_off.p, c->vel_off.p, c->ws_pos.p, c->vel_pos.p, use_simd);
+    mark(3);
     launch_integrate_velocities(s, c->bodies(), nb, dt, ws.gravity);
     launch_prepare_counted(s, c->contacts(), c->bodies(), pts, B->cnt.p + kCntPts, dt, ws.restitution_threshold,
                            ws.baumgarte, ws.penetration_slop, ws.max_bias_velocity, use_simd,
                            ws.use_velocity_bias ? 1 : 0, c->vel_records(), c->ws_records(), c->vel_pos.p, c->ws_pos.p);
+    mark(4);
     launch_solve_groups(s, c->vel_records(), c->ws_records(), c->bodies(), nb, c->b_packed_solver.p,
                         Schedule{c->ws_off.p, c->ws_grp.p}, Schedule{c->vel_off.p, c->vel_grp.p}, sp, W, threads,
                         c->resid.p);
+    mark(5);
     launch_writeback_counted(s, c->vel_records(), c->ws_records(), c->contacts(), c->vel_pos.p, c->ws_pos.p, pts,
                              B->cnt.p + kCntPts);
     launch_cache_store_counted(s, c->contacts(), c->bodies(), c->cache(), cap, B->cnt.p + kCntCons);
     launch_finish(s, c->bodies(), nb, dt, ws.use_velocity_bias ? 0 : 1);
+    mark(6);
+}
+
+int capture(kzb_batch* B) {
+    drop_graph(B);
+    hipStream_t s = B->ctx->stream;
+    KZB_HIP(B, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    enqueue_step(B, nullptr);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s, &g);
     if (e != hipSuccess) return bfail(B, KZG_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
@@ -731,6 +774,7 @@ int kzb_create(kzb_batch** out, const kz_world_desc* settings, int device) {
     hipEventCreate(&B->e0);
     hipEventCreate(&B->e1);
     hipEventCreate(&B->emid);
+    for (auto& e : B->phase_ev) hipEventCreate(&e);
     *out = B;
     return KZG_OK;
 }
@@ -742,14 +786,17 @@ void kzb_destroy(kzb_batch* B) {
         hipStreamSynchronize(B->ctx->stream);
     }
     drop_graph(B);
-    for (auto* v : {&B->d_body_off, &B->d_pair_off, &B->wc, &B->wp, &B->con_base, &B->pt_base, &B->cnt, &B->ws_level,
+    for (auto* v : {&B->world_order, &B->d_body_off, &B->d_pair_off, &B->wc, &B->wp, &B->con_base, &B->pt_base, &B->cnt, &B->ws_level,
                     &B->vel_level, &B->depth_ws, &B->depth_vel, &B->vel_slots, &B->werr, &B->lvb_ws, &B->lvb_vel})
         v->release();
+    B->lane_pt.release();
     B->h_pairs.release();
     B->h_out.release();
     if (B->e0) hipEventDestroy(B->e0);
     if (B->e1) hipEventDestroy(B->e1);
     if (B->emid) hipEventDestroy(B->emid);
+    for (auto& e : B->phase_ev)
+        if (e) hipEventDestroy(e);
     for (auto& W : B->worlds) kzw_destroy(W.w);
     kzg_destroy(B->ctx);
     delete B;
@@ -830,9 +877,17 @@ int kzb_step(kzb_batch* B, float dt) {
                                   c->stream));
     KZB_HIP(B, hipMemcpyAsync(B->d_pair_off.p, hoff, sizeof(uint32_t) * (W + 1), hipMemcpyHostToDevice, c->stream));
     // 4. The captured device step.
-    if (!B->graph_valid && (rc = capture(B))) return rc;
+    const bool direct = (B->cfg.flags & KZB_DIRECT_LAUNCH) != 0;
+    if (!direct && !B->graph_valid && (rc = capture(B))) return rc;
     KZB_HIP(B, hipEventRecord(B->e0, c->stream));
-    KZB_HIP(B, hipGraphLaunch(B->exec, c->stream));
+    const auto tl0 = clock::now();
+    if (direct) {
+        enqueue_step(B, B->phase_ev);
+        KZB_HIP(B, hipGetLastError());
+    } else {
+        KZB_HIP(B, hipGraphLaunch(B->exec, c->stream));
+    }
+    const auto tl1 = clock::now();
     KZB_HIP(B, hipEventRecord(B->emid, c->stream));
     {
         // Body states and counters back to the host (after the graph).
@@ -890,6 +945,13 @@ int kzb_step(kzb_batch* B, float dt) {
     S.device_ms = dev_ms;
     S.device_with_copies_ms = all_ms;
     S.step_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
+    S.launch_ms = std::chrono::duration<double, std::milli>(tl1 - tl0).count();
+    for (int k = 0; k < 6; ++k) {
+        float ms = 0.0f;
+        if (!direct || hipEventElapsedTime(&ms, B->phase_ev[k], B->phase_ev[k + 1]) != hipSuccess) ms = 0.0f;
+        S.phase_ms[k] = ms;
+    }
+    (void)hipGetLastError();
     return KZG_OK;
 }
 
@@ -909,6 +971,7 @@ int kzb_set_config(kzb_batch* B, const kzb_config* cfg) {
     const uint32_t t = cfg->solve_threads;
     if (t && (t < 64 || t > 1024 || (t & (t - 1))))
         return bfail(B, KZG_ERR_ARGUMENT, "solve_threads must be a power of two in [64, 1024]");
+    if (cfg->flags & ~uint32_t(KZB_DIRECT_LAUNCH)) return bfail(B, KZG_ERR_ARGUMENT, "unknown config flags");
     B->cfg = *cfg;
     drop_graph(B);
     return KZG_OK;
@@ -928,7 +991,7 @@ int kzb_autotune_candidates(kzb_batch* B, kzb_config* out, uint32_t cap) {
     uint32_t n = 0;
     for (uint32_t waves : {cu * 8u, cu * 16u, cu * 32u})
         for (uint32_t threads : {64u, 128u, 256u, 512u}) {
-            if (out && n < cap) out[n] = kzb_config{waves, threads, B->cfg.broadphase_threads, 0};
+            if (out && n < cap) out[n] = kzb_config{waves, threads, B->cfg.broadphase_threads, B->cfg.flags};
             ++n;
         }
     return int(n);

@@ -94,17 +94,21 @@ def test_batched_schedule_matches_host_schedule():
 
 @pytest.mark.gpu
 def test_launch_configuration_does_not_change_results():
-    from kizuriphysics3d_amd.batch import BatchedWorlds
+    from kizuriphysics3d_amd.batch import DIRECT_LAUNCH, BatchedWorlds
     from tests.helpers import first_mismatch
     scenes = _worlds(16, seed=0x4B5A0406)
     a, b = BatchedWorlds(scenes), BatchedWorlds(scenes)
     a.set_config(narrowphase_waves=256, solve_threads=64)
-    b.set_config(narrowphase_waves=8192, solve_threads=512)
+    b.set_config(narrowphase_waves=8192, solve_threads=512, flags=DIRECT_LAUNCH)
     for _ in range(40):
         a.step()
         b.step()
     for k in range(len(scenes)):
         assert first_mismatch(a.states(k), b.states(k)) is None, k
+    # direct launches time the phases with events; the graph does not
+    ph = np.array(b.stats()["phase_ms"])
+    assert np.all(ph > 0) and np.all(np.array(a.stats()["phase_ms"]) == 0), ph
+    b.set_config()
     # the autotuner steps the batch with every candidate; results stay identical
     n = len(b.candidates())
     bi, times = b.autotune(repeats=1)
