@@ -242,25 +242,49 @@ struct WaveLds {
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
-// Wave arg-max of (value, index): larger value wins, ties go to the lower
-// index; indices never set stay at INT_MAX.  Equivalent to the sequential
-// "strict > from -inf, first wins" scans of the reference.
-__device__ __forceinline__ void wave_argmax(float& v, int& i) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        float ov = __shfl_xor(v, off, 64);
-        int oi = __shfl_xor(i, off, 64);
-        if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
-    }
+// ---- Wave arg-max / arg-min -------------------------------------------
+// The reference scans candidates in index order keeping the first strict
+// best ("v > best" from -inf, or "v < best" from +inf).  Here candidate
+// index i sits in lane i % 64, slot i / 64; each lane keeps its first best
+// slot, turns the value into an unsigned key whose order is the value order
+// (-0 and +0 equal; NaN and the start value never win: key 0), and the wave
+// takes the largest key with DPP row reductions and four lane reads.  The
+// lowest index among the lanes holding that key wins, as in the reference.
+// Every lane of the wave must be active.
+__device__ __forceinline__ uint32_t orderable(float v) {
+    const uint32_t u = v == 0.0f ? 0u : __float_as_uint(v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
-// Wave arg-min with the same tie rule ("strict <, first wins").
-__device__ __forceinline__ void wave_argmin(float& v, int& i) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        float ov = __shfl_xor(v, off, 64);
-        int oi = __shfl_xor(i, off, 64);
-        if (ov < v || (ov == v && oi < i)) { v = ov; i = oi; }
+__device__ __forceinline__ uint32_t max_key(float v) { return v > -kInf ? orderable(v) : 0u; }
+__device__ __forceinline__ uint32_t min_key(float v) { return v < kInf ? ~orderable(v) : 0u; }
+
+template <int Ctrl> __device__ __forceinline__ uint32_t dpp_max(uint32_t v) {
+    const uint32_t o = uint32_t(__builtin_amdgcn_update_dpp(0, int(v), Ctrl, 0xF, 0xF, false));
+    return o > v ? o : v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    v = dpp_max<0xB1>(v);   // quad_perm [1,0,3,2]
+    v = dpp_max<0x4E>(v);   // quad_perm [2,3,0,1]
+    v = dpp_max<0x141>(v);  // row_half_mirror
+    v = dpp_max<0x140>(v);  // row_mirror: every lane holds its 16-lane row's maximum
+    const uint32_t r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const uint32_t r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    const uint32_t a = r0 > r1 ? r0 : r1, b = r2 > r3 ? r2 : r3;
+    return a > b ? a : b;
+}
+// Winning index (lane + 64 slot) of per-lane candidates (key, first best
+// slot), or INT_MAX when no lane has a candidate.  Wave-uniform.
+__device__ __forceinline__ int wave_arg_by_key(uint32_t key, int slot) {
+    const uint32_t mk = wave_max_u32(key);
+    if (mk == 0u) return 0x7FFFFFFF;
+    for (int k = 0; k < 64; ++k) {
+        const unsigned long long m = __ballot(key == mk && slot == k);
+        if (m) return int(__builtin_ctzll(m)) + 64 * k;
     }
+    return 0x7FFFFFFF;
+}
+__device__ __forceinline__ float read_lane(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
 // A shape with its transform, as seen by the wave; hull vertices are held
@@ -294,24 +318,24 @@ __device__ __forceinline__ void load_wshape(WShape& w, const ShapeBuf& sb, uint3
 __device__ __forceinline__ f3 wsupport(const WShape& w, f3 d) {
     if (w.s.type != kHull) return support_closed(w.s, d);
     float best = -kInf;
-    int bi = 0x7FFFFFFF;
+    int bj = 0;
     const int lane = lane_id();
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int vi = lane + 64 * j;
         if (vi < w.nverts) {
             float v = dot(w.hv[j], d);
-            if (v > best) { best = v; bi = vi; }
+            if (v > best) { best = v; bj = j; }
         }
     }
-    wave_argmax(best, bi);
+    int bi = wave_arg_by_key(max_key(best), bj);
     if (bi == 0x7FFFFFFF) bi = 0;
     const int slot = bi >> 6, src = bi & 63;
     f3 mine = w.hv[3];
     if (slot == 0) mine = w.hv[0];
     else if (slot == 1) mine = w.hv[1];
     else if (slot == 2) mine = w.hv[2];
-    return f3{__shfl(mine.x, src, 64), __shfl(mine.y, src, 64), __shfl(mine.z, src, 64)};
+    return f3{read_lane(mine.x, src), read_lane(mine.y, src), read_lane(mine.z, src)};
 }
 
 // SupportMap::GetSupport (include/Kizuri/Collision/GJK.h:31-40).
@@ -626,13 +650,12 @@ __device__ __forceinline__ void finish_face(const WaveLds& L, int f, Epa& r) {
 // Wave-parallel closest alive face: strict <, first wins.
 __device__ __forceinline__ int closest_face(const WaveLds& L, int nf) {
     float best = kInf;
-    int bi = 0x7FFFFFFF;
-    for (int f = lane_id(); f < nf; f += 64) {
+    int bk = 0, k = 0;
+    for (int f = lane_id(); f < nf; f += 64, ++k) {
         const float d = L.fd[f];
-        if (d < best) { best = d; bi = f; }
+        if (d < best) { best = d; bk = k; }
     }
-    wave_argmin(best, bi);
-    return bi;
+    return wave_arg_by_key(min_key(best), bk);
 }
 
 __device__ __forceinline__ void lds_write_vert(WaveLds& L, int i, const SP& s) {
@@ -710,9 +733,7 @@ __device__ __forceinline__ bool initial_tetra_fallback(const WShape& A, const WS
 
 __device__ __forceinline__ uint32_t prefix_count(unsigned long long mask) {
     // number of set bits below this lane
-    const int lane = lane_id();
-    const unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
-    return uint32_t(__popcll(below));
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
 }
 
 // EPA::Compute (EPA.cpp:176-296).  All faces in [0, nf) are alive at the start
@@ -811,18 +832,14 @@ __device__ __forceinline__ void epa(const WShape& A, const WShape& B, const Gjk&
                     }
                 }
             }
-            // Exclusive prefix sum of `keep` over the lanes, in order.
-            int incl = keep;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                int o = __shfl_up(incl, off, 64);
-                if (lane >= off) incl += o;
-            }
-            const int pos = nh + incl - keep;
+            // Exclusive prefix sum of `keep` (0..3) over the lanes, in order,
+            // from the ballots of its two bits.
+            const unsigned long long m0 = __ballot(keep & 1), m1 = __ballot(keep & 2);
+            const int pos = nh + int(prefix_count(m0)) + 2 * int(prefix_count(m1));
 #pragma unroll
             for (int k = 0; k < 3; ++k)
                 if (k < keep && pos + k < kEpaMaxHorizon) L.hor[pos + k] = uint16_t(eu[k] | (ev[k] << 8));
-            nh += __shfl(incl, 63, 64);
+            nh += __popcll(m0) + 2 * __popcll(m1);
         }
         if (nh > kEpaMaxHorizon) nh = kEpaMaxHorizon;
         __syncthreads();
@@ -919,12 +936,12 @@ __device__ __forceinline__ bool shape_face(const WShape& w, const ShapeBuf& sb, 
         const f3 ld = inv_rotate(w.q, dir);
         const int nt = int(w.s.hull.w);
         float best = -kInf;
-        int bi = 0x7FFFFFFF;
-        for (int f = lane_id(); f < nt; f += 64) {
+        int bk = 0, k = 0;
+        for (int f = lane_id(); f < nt; f += 64, ++k) {
             const float d = dot(l3(sb.hfn[w.s.hull.z + f]), ld);
-            if (d > best) { best = d; bi = f; }
+            if (d > best) { best = d; bk = k; }
         }
-        wave_argmax(best, bi);
+        const int bi = wave_arg_by_key(max_key(best), bk);
         if (bi == 0x7FFFFFFF) return false;
         const uint32_t abc = sb.htris[w.s.hull.z + bi];
         put3(out[0], xform(w.q, w.t, l3(sb.hverts[w.s.hull.x + (abc & 0xFF)])));
