@@ -23,7 +23,7 @@
 namespace kz {
 void launch_narrowphase_counted(hipStream_t s, const uint2* pairs, uint32_t cap, const uint32_t* n_dev,
                                 const BodyBuf& b, const ShapeBuf& sb, const ManifoldBuf& m, uint32_t* work,
-                                uint32_t* work_count, float max_sep, uint32_t wave_blocks);
+                                uint32_t* work_count, void* gjk, float max_sep, uint32_t wave_blocks);
 void launch_build_counted(hipStream_t s, const ContactBuf& c, const ManifoldBuf& m, const BodyBuf& b,
                           const CacheBuf& cache, uint32_t cap, const uint32_t* nc_dev, int warm);
 void launch_prepare_counted(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32_t cap, const uint32_t* np_dev,
@@ -103,7 +103,7 @@ __device__ uint32_t block_scan(uint32_t v, uint32_t& total, uint32_t* sh) {
 __global__ void k_batch_begin(BatchDev d, uint32_t* work_count, float* resid, uint32_t nres) {
     const uint32_t i = threadIdx.x;
     if (i < kCntN) d.cnt[i] = 0;
-    if (i == 0) *work_count = 0;
+    if (i < 2) work_count[i] = 0;
     for (uint32_t k = i; k < nres; k += blockDim.x) resid[k] = 0.0f;
 }
 
@@ -645,7 +645,8 @@ int ensure_capacity(kzb_batch* B, uint32_t pairs) {
     KZB_HIP(B, c->m_pa.ensure(pts));
     KZB_HIP(B, c->m_pb.ensure(pts));
     KZB_HIP(B, c->m_n.ensure(pts));
-    KZB_HIP(B, c->work.ensure(cap));
+    KZB_HIP(B, c->work.ensure(2 * size_t(cap)));
+    KZB_HIP(B, c->gjk.ensure(size_t(cap) * kGjkRecBytes));
     KZB_HIP(B, c->c_info.ensure(cap));
     KZB_HIP(B, c->c_pair.ensure(cap));
     KZB_HIP(B, c->p_body.ensure(pts));
@@ -693,7 +694,7 @@ void enqueue_step(kzb_batch* B, hipEvent_t* ev) {
     mark(0);
     k_batch_begin<<<1, 256, 0, s>>>(d, c->work_count.p, c->resid.p, ws.velocity_iterations);
     launch_narrowphase_counted(s, c->d_pairs.p, cap, B->d_pair_off.p + W, c->bodies(), c->shapes(), c->manifolds(),
-                               c->work.p, c->work_count.p, 0.02f, waves);
+                               c->work.p, c->work_count.p, c->gjk.p, 0.02f, waves);
     mark(1);
     k_world_count<<<W, 256, 0, s>>>(d, c->m_count.p);
     k_world_scan<<<1, 1024, 0, s>>>(d);

@@ -64,7 +64,7 @@ int kzg_create(kzg_ctx** out, const kz_world_desc* world, int device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = uint32_t(prop.multiProcessorCount);
     for (auto& e : c->ev) hipEventCreate(&e);
-    if (c->work_count.ensure(1) != hipSuccess || ensure_cache(c, 512) != KZG_OK) {
+    if (c->work_count.ensure(2) != hipSuccess || ensure_cache(c, 512) != KZG_OK) {
         kzg_destroy(c);
         return KZG_ERR_DEVICE;
     }
@@ -87,6 +87,7 @@ void kzg_destroy(kzg_ctx* c) {
                     &c->x_meta, &c->x_packed})
         d->release();
     c->s_hull.release();
+    c->gjk.release();
     c->c_info.release();
     c->p_body.release();
     c->vr_bd.release();
@@ -248,13 +249,14 @@ int kzg_narrowphase(kzg_ctx* c, const uint32_t* pairs, uint32_t n) {
     KZG_HIP(c, c->m_pa.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
     KZG_HIP(c, c->m_pb.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
     KZG_HIP(c, c->m_n.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
-    KZG_HIP(c, c->work.ensure(std::max<uint32_t>(n, 1)));
+    KZG_HIP(c, c->work.ensure(2 * size_t(std::max<uint32_t>(n, 1))));
+    KZG_HIP(c, c->gjk.ensure(size_t(std::max<uint32_t>(n, 1)) * kGjkRecBytes));
     int rc = 0;
     c->recorded = 0;
     if ((rc = upload(c, c->d_pairs, reinterpret_cast<const uint2*>(pairs), n))) return rc;
     KZG_HIP(c, record(c, 0));
     launch_narrowphase(c->stream, c->d_pairs.p, n, c->bodies(), c->shapes(), c->manifolds(), c->work.p,
-                       c->work_count.p, 0.02f, 1, std::min<uint32_t>(n, c->cu_count * 16u));
+                       c->work_count.p, c->gjk.p, 0.02f, 1, std::min<uint32_t>(n, c->cu_count * 16u));
     KZG_HIP(c, hipGetLastError());
     KZG_HIP(c, record(c, 1));
     c->phase = 1;
@@ -649,12 +651,13 @@ int kzg_collide_batch(kzg_ctx* c, const uint32_t* sa, const uint32_t* sb, const 
     KZG_HIP(c, c->m_pa.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
     KZG_HIP(c, c->m_pb.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
     KZG_HIP(c, c->m_n.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
-    KZG_HIP(c, c->work.ensure(std::max<uint32_t>(n, 1)));
+    KZG_HIP(c, c->work.ensure(2 * size_t(std::max<uint32_t>(n, 1))));
+    KZG_HIP(c, c->gjk.ensure(size_t(std::max<uint32_t>(n, 1)) * kGjkRecBytes));
     BodyBuf bb{c->x_x.p, c->x_q.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->x_meta.p, nullptr};
     c->recorded = 0;
     KZG_HIP(c, record(c, 0));
     launch_narrowphase(c->stream, c->x_pairs.p, n, bb, c->shapes(), c->manifolds(), c->work.p, c->work_count.p,
-                       max_sep, 0, std::min<uint32_t>(std::max<uint32_t>(n, 1), c->cu_count * 16u));
+                       c->gjk.p, max_sep, 0, std::min<uint32_t>(std::max<uint32_t>(n, 1), c->cu_count * 16u));
     KZG_HIP(c, hipGetLastError());
     KZG_HIP(c, record(c, 1));
     KZG_HIP(c, hipStreamSynchronize(c->stream));

@@ -19,9 +19,10 @@
 #include "schedule.h"
 
 namespace kz {
+// work: 2 n entries; work_count: 2 counters; gjk: n records of kGjkRecBytes.
 void launch_narrowphase(hipStream_t s, const uint2* pairs, uint32_t n, const BodyBuf& b, const ShapeBuf& sb,
-                        const ManifoldBuf& m, uint32_t* work, uint32_t* work_count, float max_sep, int world_mode,
-                        uint32_t wave_blocks);
+                        const ManifoldBuf& m, uint32_t* work, uint32_t* work_count, void* gjk, float max_sep,
+                        int world_mode, uint32_t wave_blocks);
 void launch_integrate_velocities(hipStream_t s, const BodyBuf& b, uint32_t n, float dt, const float* g);
 void launch_build(hipStream_t s, const ContactBuf& c, const ManifoldBuf& m, const BodyBuf& b, const CacheBuf& cache,
                   uint32_t nc, int warm);
@@ -136,6 +137,7 @@ struct kzg_ctx {
     DevVec<uint32_t> m_count;
     DevVec<float4> m_pa, m_pb, m_n;
     DevVec<uint32_t> work, work_count;
+    DevVec<uint8_t> gjk;  // GJK results handed from the lane kernel to the wave kernel
 
     // constraints and points
     uint32_t ncons = 0, npoints = 0;

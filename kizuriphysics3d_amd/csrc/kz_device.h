@@ -26,6 +26,9 @@ constexpr int kPairPoints = 4;
 constexpr int kMaxHullVerts = 256;
 constexpr int kMaxHullTris = 512;
 
+// Bytes of one GJK result handed from the lane kernel to the wave kernel.
+constexpr size_t kGjkRecBytes = 256;
+
 struct ShapeBuf {
     uint32_t* type;   // [S]
     float4* params;   // [S] sphere r | box hx hy hz | capsule hh r | plane nx ny nz d

@@ -462,16 +462,43 @@ struct Gjk {
     int n;
     Simplex s;
 };
+// A GJK result in memory (16 x 16 B), handed from k_collide_lanes to the
+// wave kernel.
+struct GjkRec {
+    float4 w[16];
+};
+static_assert(sizeof(GjkRec) == kGjkRecBytes, "GJK record size");
+__device__ __forceinline__ Gjk load_gjk(const GjkRec& o) {
+    Gjk g;
+    const float4 w0 = o.w[0], w1 = o.w[1], w2 = o.w[2], w3 = o.w[3];
+    g.intersect = w0.x != 0.0f;
+    g.distance = w0.y;
+    g.n = __float_as_int(w0.z);
+    g.pa = f3{w1.x, w1.y, w1.z};
+    g.pb = f3{w1.w, w2.x, w2.y};
+    g.normal = f3{w2.z, w2.w, w3.x};
+    SP* v[4] = {&g.s.v0, &g.s.v1, &g.s.v2, &g.s.v3};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float4 a = o.w[4 + 2 * k], bq = o.w[5 + 2 * k], c = o.w[12 + k];
+        v[k]->a = f3{a.x, a.y, a.z};
+        v[k]->b = f3{a.w, bq.x, bq.y};
+        v[k]->p = f3{bq.z, bq.w, c.x};
+    }
+    return g;
+}
 
-// GJK::GetClosestPoints (GJK.cpp:155-244).  Uniform control flow: every lane
-// runs the same scalar code, the support queries are wave-cooperative.
-__device__ __forceinline__ void gjk(const WShape& A, const WShape& B, float tol, Gjk& r) {
+// GJK::GetClosestPoints (GJK.cpp:155-244) over a support map `ms` (d -> the
+// support point of A - B).  In the wave kernel every lane runs the same
+// scalar code and the support queries are wave-cooperative; in the
+// thread-per-pair kernel each lane runs its own pair.
+template <class MS> __device__ __forceinline__ void gjk_core(MS&& ms, float tol, Gjk& r) {
     r.intersect = false;
     r.distance = 0.0f;
     r.pa = r.pb = r.normal = f3{0.0f, 0.0f, 0.0f};
     Simplex& s = r.s;
     int n = 0;
-    s.v0 = msupport(A, B, f3{1.0f, 0.0f, 0.0f});
+    s.v0 = ms(f3{1.0f, 0.0f, 0.0f});
     s.v1 = s.v2 = s.v3 = s.v0;
     n = 1;
     f3 closest = s.v0.p;
@@ -479,7 +506,7 @@ __device__ __forceinline__ void gjk(const WShape& A, const WShape& B, float tol,
     for (int it = 0; it < 32; ++it) {
         if (len2(closest) <= tol2) { r.intersect = true; r.distance = 0.0f; break; }
         const f3 d = neg(closest);
-        const SP w = msupport(A, B, d);
+        const SP w = ms(d);
         const float progress = len2(closest) - dot(closest, w.p);
         if (progress <= tol * len2(closest)) break;
         bool dup = false;
@@ -511,6 +538,10 @@ __device__ __forceinline__ void gjk(const WShape& A, const WShape& B, float tol,
             r.pb = s.v0.b;
         }
     }
+}
+
+__device__ __forceinline__ void gjk(const WShape& A, const WShape& B, float tol, Gjk& r) {
+    gjk_core([&](f3 d) { return msupport(A, B, d); }, tol, r);
 }
 
 // --------------------------- EPA (src/Collision/EPA.cpp) -------------------
@@ -1051,10 +1082,11 @@ __device__ __forceinline__ bool polyhedral(const WShape& w) { return w.s.type ==
 
 // CollideConvexGeneric (CollideShape.cpp:233-299).  Results in L.out.
 __device__ __forceinline__ int collide_generic(const WShape& A, const WShape& B, const ShapeBuf& sb, float max_sep,
-                                               WaveLds& L) {
+                                               WaveLds& L, bool have_pre, const Gjk& pre) {
     const float tol = 1.0e-4f;
     Gjk g;
-    gjk(A, B, tol, g);
+    if (have_pre) g = pre;
+    else gjk(A, B, tol, g);
     if (!g.intersect) {
         if (g.distance > max_sep) return 0;
         if (polyhedral(A) && polyhedral(B) && len2(g.normal) > 0.0f) {
@@ -1086,13 +1118,15 @@ __device__ __forceinline__ int collide_generic(const WShape& A, const WShape& B,
 
 // CollideSphereConvex (CollideShape.cpp:304-344): GJK against a 1e-4 proxy
 // sphere at the sphere centre.
-__device__ __forceinline__ int collide_sphere_convex(const WShape& Sph, const WShape& B, float max_sep, WaveLds& L) {
+__device__ __forceinline__ int collide_sphere_convex(const WShape& Sph, const WShape& B, float max_sep, WaveLds& L,
+                                                     bool have_pre, const Gjk& pre) {
     const float tol = 1.0e-4f;
     WShape P = Sph;
     P.s.type = kSphere;
     P.s.prm.x = 1.0e-4f;
     Gjk g;
-    gjk(P, B, tol, g);
+    if (have_pre) g = pre;
+    else gjk(P, B, tol, g);
     const float r = Sph.s.prm.x;
     const f3 c = Sph.t;
     if (g.intersect) {
@@ -1113,12 +1147,16 @@ __device__ __forceinline__ int collide_sphere_convex(const WShape& Sph, const WS
     return 1;
 }
 
+// `pre` (optional): the GJK result of each work item, computed by
+// k_collide_lanes; the wave kernel then continues from it.
 __global__ void __launch_bounds__(64) k_collide_wave(const uint2* pairs, const uint32_t* work,
                                                      const uint32_t* work_count, BodyBuf b, ShapeBuf sb,
-                                                     ManifoldBuf m, float max_sep) {
+                                                     ManifoldBuf m, float max_sep, const GjkRec* pre) {
     __shared__ WaveLds L;
     const uint32_t total = *work_count;
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
+        Gjk pg{};
+        if (pre) pg = load_gjk(pre[item]);
         const uint32_t wv = work[item];
         const uint32_t pi = wv & 0x7FFFFFFFu;
         const bool swapped = (wv >> 31) != 0u;
@@ -1129,13 +1167,14 @@ __global__ void __launch_bounds__(64) k_collide_wave(const uint2* pairs, const u
         load_wshape(B, sb, meta_shape(b.meta[ib]), b.x[ib], b.q[ib]);
         int k;
         bool swap_result = swapped;
+        const bool hp = pre != nullptr;
         if (A.s.type == kSphere) {
-            k = collide_sphere_convex(A, B, max_sep, L);
+            k = collide_sphere_convex(A, B, max_sep, L, hp, pg);
         } else if (B.s.type == kSphere) {
-            k = collide_sphere_convex(B, A, max_sep, L);
+            k = collide_sphere_convex(B, A, max_sep, L, hp, pg);
             if (k > 0) swap_result = !swap_result;
         } else {
-            k = collide_generic(A, B, sb, max_sep, L);
+            k = collide_generic(A, B, sb, max_sep, L, hp, pg);
         }
         __syncthreads();
         // Lane j writes result point j.
@@ -1156,27 +1195,150 @@ __global__ void __launch_bounds__(64) k_collide_wave(const uint2* pairs, const u
     }
 }
 
-static inline uint32_t nblocks(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
-
-void launch_narrowphase(hipStream_t s, const uint2* pairs, uint32_t n, const BodyBuf& b, const ShapeBuf& sb,
-                        const ManifoldBuf& m, uint32_t* work, uint32_t* work_count, float max_sep, int world_mode,
-                        uint32_t wave_blocks) {
-    if (!n) return;
-    hipMemsetAsync(work_count, 0, sizeof(uint32_t), s);
-    k_collide_simple<<<nblocks(n, 256), 256, 0, s>>>(pairs, n, nullptr, b, sb, m, work, work_count, max_sep,
-                                                      world_mode);
-    uint32_t grid = wave_blocks ? wave_blocks : (n < 65536u ? n : 65536u);
-    k_collide_wave<<<grid, 64, 0, s>>>(pairs, work, work_count, b, sb, m, max_sep);
+// ===========================================================================
+// Thread-per-pair GJK: one lane per pair runs the reference GJK (hull
+// supports as sequential scans: the same first strict maximum as the wave
+// arg-max, so the same arithmetic and results as the wave kernel).  Pairs
+// whose result needs nothing more are finished here: separated beyond
+// maxSeparation (no point), or one GJK point (CollideSphereConvex
+// separated; CollideConvexGeneric separated with a non-polyhedral shape).
+// Intersecting pairs (EPA) and separated polyhedral pairs within
+// maxSeparation (clipping) go to the wave kernel's list with their GJK
+// result, from which the wave kernel continues.
+// ===========================================================================
+struct TShape {
+    ShapeView s;
+    f4q q;
+    f3 t;
+};
+__device__ __forceinline__ f3 tsupport(const TShape& w, const float4* hverts, f3 d) {
+    if (w.s.type != kHull) return support_closed(w.s, d);
+    float best = -kInf;
+    uint32_t bi = 0;
+    const float4* v = hverts + w.s.hull.x;
+    for (uint32_t i = 0; i < w.s.hull.y; ++i) {
+        const float x = dot(l3(v[i]), d);
+        if (x > best) { best = x; bi = i; }
+    }
+    return l3(v[bi]);
 }
 
-// Capacity-sized launch with the pair count read on the device; work_count
-// must already be zero (no memset: the launch may be graph-captured).
+__device__ __forceinline__ void store_gjk(GjkRec& o, const Gjk& g) {
+    o.w[0] = make_float4(g.intersect ? 1.0f : 0.0f, g.distance, __int_as_float(g.n), 0.0f);
+    o.w[1] = make_float4(g.pa.x, g.pa.y, g.pa.z, g.pb.x);
+    o.w[2] = make_float4(g.pb.y, g.pb.z, g.normal.x, g.normal.y);
+    o.w[3] = make_float4(g.normal.z, 0.0f, 0.0f, 0.0f);
+    const SP* v[4] = {&g.s.v0, &g.s.v1, &g.s.v2, &g.s.v3};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        o.w[4 + 2 * k] = make_float4(v[k]->a.x, v[k]->a.y, v[k]->a.z, v[k]->b.x);
+        o.w[5 + 2 * k] = make_float4(v[k]->b.y, v[k]->b.z, v[k]->p.x, v[k]->p.y);
+        o.w[12 + k] = make_float4(v[k]->p.z, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_collide_lanes(const uint2* pairs, const uint32_t* work,
+                                                       const uint32_t* work_count, uint32_t* fwd,
+                                                       uint32_t* fwd_count, GjkRec* fwd_gjk, BodyBuf b, ShapeBuf sb,
+                                                       ManifoldBuf m, float max_sep) {
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= *work_count) return;
+    const uint32_t wv = work[item];
+    const uint32_t pi = wv & 0x7FFFFFFFu;
+    const bool swapped = (wv >> 31) != 0u;
+    uint32_t ia = pairs[pi].x, ib = pairs[pi].y;
+    if (swapped) { uint32_t t = ia; ia = ib; ib = t; }
+    // The dispatch of k_collide_wave: a sphere side goes through
+    // CollideSphereConvex (GJK of a 1e-4 proxy sphere at its centre against
+    // the other shape), everything else through CollideConvexGeneric.
+    const ShapeView sa = load_shape(sb, meta_shape(b.meta[ia])), sbv = load_shape(sb, meta_shape(b.meta[ib]));
+    const bool sph_a = sa.type == kSphere, sph_b = !sph_a && sbv.type == kSphere;
+    const bool both_poly = (sa.type == kBox || sa.type == kHull) && (sbv.type == kBox || sbv.type == kHull);
+    const uint32_t ix = sph_b ? ib : ia, iy = sph_b ? ia : ib;
+    TShape X{sph_b ? sbv : sa, lq(b.q[ix]), l3(b.x[ix])};
+    const TShape Y{sph_b ? sa : sbv, lq(b.q[iy]), l3(b.x[iy])};
+    const float r = X.s.prm.x;
+    if (sph_a || sph_b) {
+        X.s.type = kSphere;
+        X.s.prm.x = 1.0e-4f;
+    }
+    Gjk g;
+    gjk_core(
+        [&](f3 d) {
+            SP sp;
+            sp.a = xform(X.q, X.t, tsupport(X, sb.hverts, inv_rotate(X.q, d)));
+            sp.b = xform(Y.q, Y.t, tsupport(Y, sb.hverts, inv_rotate(Y.q, neg(d))));
+            sp.p = sub(sp.a, sp.b);
+            return sp;
+        },
+        1.0e-4f, g);
+    int k = -1;  // -1: to the wave kernel
+    bool swap_result = swapped;
+    Contact c;
+    if (sph_a || sph_b) {
+        if (!g.intersect) {
+            const float dist = g.distance;
+            const float sep = dist - r;
+            if (sep > max_sep) {
+                k = 0;
+            } else {
+                const f3 n = dist > kEps ? divs(sub(X.t, g.pb), dist) : f3{0.0f, 1.0f, 0.0f};
+                c = Contact{sub(X.t, mul(n, r)), g.pb, n, sep, 0u};
+                k = 1;
+                if (sph_b) swap_result = !swap_result;
+            }
+        }
+    } else if (!g.intersect) {
+        if (g.distance > max_sep) {
+            k = 0;
+        } else if (!(both_poly && len2(g.normal) > 0.0f)) {
+            c = Contact{g.pa, g.pb, g.normal, g.distance, 0u};
+            k = 1;
+        }
+    }
+    if (k < 0) {
+        const uint32_t slot = atomicAdd(fwd_count, 1u);
+        fwd[slot] = wv;
+        store_gjk(fwd_gjk[slot], g);
+        return;
+    }
+    if (k == 1) {
+        if (swap_result) swap_contact(c);
+        write_contact(m, pi, 0, c);
+    }
+    m.count[pi] = uint32_t(k);
+}
+
+static inline uint32_t nblocks(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
+
+// `work` holds 2 n entries, `work_count` two counters, `gjk` n records.
+void launch_narrowphase(hipStream_t s, const uint2* pairs, uint32_t n, const BodyBuf& b, const ShapeBuf& sb,
+                        const ManifoldBuf& m, uint32_t* work, uint32_t* work_count, void* gjk, float max_sep,
+                        int world_mode, uint32_t wave_blocks) {
+    if (!n) return;
+    hipMemsetAsync(work_count, 0, 2 * sizeof(uint32_t), s);
+    k_collide_simple<<<nblocks(n, 256), 256, 0, s>>>(pairs, n, nullptr, b, sb, m, work, work_count, max_sep,
+                                                      world_mode);
+    GjkRec* rec = static_cast<GjkRec*>(gjk);
+    k_collide_lanes<<<nblocks(n, 256), 256, 0, s>>>(pairs, work, work_count, work + n, work_count + 1, rec, b, sb, m,
+                                                     max_sep);
+    uint32_t grid = wave_blocks ? wave_blocks : (n < 65536u ? n : 65536u);
+    k_collide_wave<<<grid, 64, 0, s>>>(pairs, work + n, work_count + 1, b, sb, m, max_sep, rec);
+}
+
+// Capacity-sized launch with the pair count read on the device; both work
+// counters must already be zero (no memset: the launch may be
+// graph-captured).  work: 2 cap entries; gjk: cap records.
 void launch_narrowphase_counted(hipStream_t s, const uint2* pairs, uint32_t cap, const uint32_t* n_dev,
                                 const BodyBuf& b, const ShapeBuf& sb, const ManifoldBuf& m, uint32_t* work,
-                                uint32_t* work_count, float max_sep, uint32_t wave_blocks) {
+                                uint32_t* work_count, void* gjk, float max_sep, uint32_t wave_blocks) {
     if (!cap) return;
     k_collide_simple<<<nblocks(cap, 256), 256, 0, s>>>(pairs, cap, n_dev, b, sb, m, work, work_count, max_sep, 1);
-    k_collide_wave<<<wave_blocks ? wave_blocks : 1024u, 64, 0, s>>>(pairs, work, work_count, b, sb, m, max_sep);
+    GjkRec* rec = static_cast<GjkRec*>(gjk);
+    k_collide_lanes<<<nblocks(cap, 256), 256, 0, s>>>(pairs, work, work_count, work + cap, work_count + 1, rec, b, sb,
+                                                       m, max_sep);
+    k_collide_wave<<<wave_blocks ? wave_blocks : 1024u, 64, 0, s>>>(pairs, work + cap, work_count + 1, b, sb, m,
+                                                                    max_sep, rec);
 }
 
 }  // namespace kz
