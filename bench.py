@@ -45,6 +45,24 @@ sys.path.insert(0, ROOT)
 from kizuriphysics3d_amd import scenes as S  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+
+
+def pmc_traffic(summary: str):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary in
+    profiles/: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by
+    the copy-kernel calibration recorded in the same file.  None if absent."""
+    path = os.path.join(ROOT, "profiles", summary)
+    try:
+        kv = {}
+        for line in open(path):
+            if "=" in line and line.split("=")[0].strip() in ("fetch_kib_per_dispatch", "write_kib_per_dispatch",
+                                                              "fetch_scale", "write_scale"):
+                k, v = line.split("=")
+                kv[k.strip()] = float(v)
+        return 1024.0 * (kv["fetch_scale"] * kv["fetch_kib_per_dispatch"] +
+                         kv["write_scale"] * kv["write_kib_per_dispatch"])
+    except (OSError, KeyError, ValueError):
+        return None
 # Algorithmic HBM bytes per contact point for one persistent solve launch:
 # warm start 168 B + velocity iteration 312 B + bias iteration 260 B per point.
 WARM_BYTES, VEL_BYTES, BIAS_BYTES = 168, 312, 260
@@ -263,8 +281,10 @@ def bench_narrowphase(args):
         "config": {"workload": f"{args.pairs} convex pairs: hull-hull (8-64 vertices), box-hull, capsule-box; "
                                "penetration U[0, 0.05] m, 10 % separated by U[0, 0.02] m",
                    "pairs": args.pairs, "contact_points": int(counts.sum()), "kernel_ms_per_batch": per},
-        "roofline": {"bound": "hbm", "kernel": "k_collide_simple + k_collide_wave", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "roofline": {"bound": "hbm", "kernel": "k_collide_simple + k_collide_lanes + k_collide_wave", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": pmc_traffic("narrowphase1m_pmc.md") if args.pairs == 1 << 20 else None,
+                     "algorithmic_bytes_per_launch": float(bytes_pair.sum())},
         "cpu_baseline": cpu,
     }
 
@@ -372,7 +392,7 @@ def main():
     coop_bytes = 0.0
     coop_blocks = set()
     levels = []
-    phases = np.zeros(6)
+    phases = np.zeros(7)
     t_start = time.perf_counter()
     for i in range(args.steps):
         w.step_fixed()
@@ -390,7 +410,7 @@ def main():
                            + tm["cooperative_vel_points"] * 8 * VEL_BYTES)
         levels.append((tm["velocity_levels"], tm["bias_levels"], tm["groups"], tm["islands"]))
         phases += [tm["narrowphase_ms"], tm["build_ms"], tm["integrate_velocities_ms"], tm["solve_ms"],
-                   tm["solve_kernel_ms"], st["broadphase_ms"]]
+                   tm["solve_kernel_ms"], st["broadphase_ms"], tm["schedule_ms"]]
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     # value: the path's phases with inputs in HBM (kernel times, host build
@@ -413,7 +433,7 @@ def main():
         f"workgroups {sorted(coop_blocks)}), levels vel/bias max {lv[:, 0].max()}/{lv[:, 1].max()}, "
         f"groups {lv[:, 2].mean():.0f}, islands {lv[:, 3].mean():.0f}")
     ph = phases / args.steps
-    log(f"per step (ms): narrowphase {ph[0]:.2f}, build (incl. host schedule) {ph[1]:.2f}, integrate velocities "
+    log(f"per step (ms): narrowphase {ph[0]:.2f}, build {ph[1]:.2f} (host schedule {ph[6]:.2f}), integrate velocities "
         f"{ph[2]:.2f}, solve {ph[3]:.2f} (sweep kernel {ph[4]:.2f}), host broadphase {ph[5]:.2f}")
     if args.dump:
         import ctypes as C
@@ -466,7 +486,10 @@ def main():
                    "contact_path_ms_per_step_with_pcie": path_ms / args.steps,
                    "contacts_per_s_with_pcie": pcie_value},
         "roofline": {"bound": "hbm", "kernel": rl_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": pmc_traffic("pile32k_k_solve_coop_pmc.md") if rl_kernel == "k_solve_coop" else None,
+                     "algorithmic_bytes_per_launch": (coop_bytes / max(len(levels), 1)) if coop_ms > 0 else
+                     solve_bytes / max(len(levels), 1)},
         "cpu_baseline": cpu,
         "residual": residual,
     }

@@ -83,6 +83,7 @@ typedef struct kzg_timings {
      * and in the velocity sequences, and its sweep kernel alone */
     uint32_t cooperative_ws_points, cooperative_vel_points;
     float cooperative_kernel_ms;
+    float schedule_ms;       /* the solve schedule alone (host, part of build_ms) */
 } kzg_timings;
 
 /* Create a context on HIP device `device` for a world with settings `world`. */

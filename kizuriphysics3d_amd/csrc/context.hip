@@ -334,10 +334,13 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     si.info = c->h_info.data();
     si.use_simd = c->world.use_simd_solver != 0;
     si.max_groups = c->cu_count * 4u;
+    const auto ts = std::chrono::steady_clock::now();
     c->sched_builder.build(si, c->sched);
     // Host compute of the build ends here; what follows stages the
     // schedule for the device.
-    c->build_host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    const auto te = std::chrono::steady_clock::now();
+    c->build_host_ms = std::chrono::duration<double, std::milli>(te - t0).count();
+    c->schedule_host_ms = std::chrono::duration<double, std::milli>(te - ts).count();
     const LevelSchedule &ws = c->sched.ws, &vel = c->sched.vel;
     const size_t bytes = sizeof(uint32_t) * (ws.off.size() + vel.off.size() + 2 * size_t(np)) +
                          sizeof(U2) * (ws.grp.size() + vel.grp.size()) + 8 * 256;
@@ -566,6 +569,7 @@ int kzg_get_timings(kzg_ctx* c, kzg_timings* out) {
     span(0, 1, t.narrowphase_ms);
     span(2, 3, t.build_kernel_ms);
     t.build_ms = float(c->build_host_ms);
+    t.schedule_ms = float(c->schedule_host_ms);
     span(8, 4, t.integrate_velocities_ms);
     span(4, 9, t.solve_ms);
     span(5, 6, t.solve_kernel_ms);

@@ -177,7 +177,7 @@ struct kzg_ctx {
     hipEvent_t ev[12] = {};
     uint32_t recorded = 0;  // bit i: ev[i] recorded since the last narrowphase
     kzg_timings timings{};
-    double build_host_ms = 0.0;
+    double build_host_ms = 0.0, schedule_host_ms = 0.0;
     int phase = 0;  // 0 idle, 1 narrowphase done, 2 built, 3 solved
 
     // learned warm start
