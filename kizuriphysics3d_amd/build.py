@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkizuri_amd.so")
-SOURCES = ["narrowphase.hip", "solver.hip", "context.hip", "schedule.cpp", "host_geometry.cpp", "world.cpp",
+SOURCES = ["narrowphase.hip", "compact.hip", "solver.hip", "context.hip", "schedule.cpp", "host_geometry.cpp", "world.cpp",
            "mlp.hip", "batch.hip"]
 
 FLAGS = [

@@ -116,6 +116,10 @@ void kzg_destroy(kzg_ctx* c) {
     c->stage2.release();
     c->stage_a.release();
     c->stage_b.release();
+    c->pin_totals.release();
+    c->pin_info.release();
+    c->compact_tile.release();
+    c->compact_tot.release();
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -243,7 +247,6 @@ int kzg_narrowphase(kzg_ctx* c, const uint32_t* pairs, uint32_t n) {
     hipSetDevice(c->device);
     for (uint32_t i = 0; i < 2 * n; ++i)
         if (pairs[i] >= c->nbodies) return fail(c, KZG_ERR_ARGUMENT, "pair " + std::to_string(i / 2) + ": bad body index");
-    c->h_pairs.assign(pairs, pairs + 2 * size_t(n));
     c->npairs = n;
     KZG_HIP(c, c->m_count.ensure(std::max<uint32_t>(n, 1)));
     KZG_HIP(c, c->m_pa.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
@@ -251,14 +254,26 @@ int kzg_narrowphase(kzg_ctx* c, const uint32_t* pairs, uint32_t n) {
     KZG_HIP(c, c->m_n.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
     KZG_HIP(c, c->work.ensure(2 * size_t(std::max<uint32_t>(n, 1))));
     KZG_HIP(c, c->gjk.ensure(size_t(std::max<uint32_t>(n, 1)) * kGjkRecBytes));
+    KZG_HIP(c, c->c_info.ensure(std::max<uint32_t>(n, 1)));
+    KZG_HIP(c, c->c_pair.ensure(std::max<uint32_t>(n, 1)));
+    KZG_HIP(c, c->compact_tile.ensure((std::max<uint32_t>(n, 1) + 1023) / 1024));
+    KZG_HIP(c, c->compact_tot.ensure(2));
+    KZG_HIP(c, c->pin_totals.ensure(4 * sizeof(uint32_t)));
     int rc = 0;
     c->recorded = 0;
     if ((rc = upload(c, c->d_pairs, reinterpret_cast<const uint2*>(pairs), n))) return rc;
     KZG_HIP(c, record(c, 0));
     launch_narrowphase(c->stream, c->d_pairs.p, n, c->bodies(), c->shapes(), c->manifolds(), c->work.p,
                        c->work_count.p, c->gjk.p, 0.02f, 1, std::min<uint32_t>(n, c->cu_count * 16u));
+    // The constraint table (BuildContactConstraints' compaction, :454-471)
+    // is built on the device right behind the narrowphase.
+    launch_compact(c->stream, c->m_count.p, n, c->d_pairs.p, c->bodies(), c->contacts(), c->compact_tile.p,
+                   c->compact_tot.p);
     KZG_HIP(c, hipGetLastError());
     KZG_HIP(c, record(c, 1));
+    uint32_t* tot = c->pin_totals.as<uint32_t>();
+    KZG_HIP(c, hipMemcpyAsync(tot, c->compact_tot.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    KZG_HIP(c, hipMemcpyAsync(tot + 2, c->cache_used.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     c->phase = 1;
     return KZG_OK;
 }
@@ -274,34 +289,14 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     c->cur_mode = mode;
     c->no_store = (warm & KZG_WARM_NO_STORE) != 0;
     hipSetDevice(c->device);
-    const uint32_t n = c->npairs;
-    // Manifold point counts and the cache occupancy, one synchronisation.
-    KZG_HIP(c, c->stage2.ensure(sizeof(uint32_t) * (size_t(n) + 1)));
-    uint32_t* hc = c->stage2.as<uint32_t>();
-    if (n) KZG_HIP(c, hipMemcpyAsync(hc, c->m_count.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream));
-    KZG_HIP(c, hipMemcpyAsync(hc + n, c->cache_used.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    // Constraint and point totals and the cache occupancy (the wait is the
+    // narrowphase and the compaction still running).
     KZG_HIP(c, hipStreamSynchronize(c->stream));
-    // Host time of the build proper starts once the counts are here (the
-    // wait above is the narrowphase still running).
-    auto t0 = std::chrono::steady_clock::now();
-    const uint32_t cache_used = hc[n];
-    // Compaction in pair order (BuildContactConstraints: empty manifolds and
-    // pairs without a dynamic body are skipped; bodies ordered by packed id).
-    c->h_info.clear();
-    c->h_cpair.clear();
-    uint32_t np = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t k = hc[i];
-        if (k == 0) continue;
-        uint32_t a = c->h_pairs[2 * i], b = c->h_pairs[2 * i + 1];
-        if (c->h_packed[a] > c->h_packed[b]) std::swap(a, b);
-        const uint32_t sensor = (c->h_sensor[a] || c->h_sensor[b]) ? 1u : 0u;
-        c->h_info.push_back(U4{a, b, np, k | (sensor << 8)});
-        c->h_cpair.push_back(i);
-        np += k;
-    }
-    c->ncons = uint32_t(c->h_info.size());
-    c->npoints = np;
+    const uint32_t* tot = c->pin_totals.as<uint32_t>();
+    c->ncons = tot[0];
+    c->npoints = tot[1];
+    const uint32_t cache_used = tot[2];
+    const uint32_t np = c->npoints;
     int rc;
     // The store at the end of this step adds at most ncons keys.
     if ((rc = ensure_cache(c, size_t(cache_used) + c->ncons))) return rc;
@@ -315,23 +310,26 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     KZG_HIP(c, c->wr_bd.ensure(np1));
     KZG_HIP(c, c->vr_fb.ensure(np1));
     KZG_HIP(c, c->p_nvel.ensure(np1));
-    // Constraint table up, then the build kernel runs while the host derives
-    // the solve schedule.
-    KZG_HIP(c, c->stage_a.ensure(sizeof(U4) * c->ncons + sizeof(uint32_t) * c->ncons + 1024));
-    size_t off = 0;
-    if ((rc = upload_async(c, c->c_info, c->stage_a, off, reinterpret_cast<const uint4*>(c->h_info.data()), c->ncons)) ||
-        (rc = upload_async(c, c->c_pair, c->stage_a, off, c->h_cpair.data(), c->ncons)))
-        return rc;
+    // The constraint table comes down while the build kernel runs; the host
+    // derives the solve schedule from it.
+    KZG_HIP(c, c->pin_info.ensure(sizeof(U4) * std::max<uint32_t>(c->ncons, 1)));
+    c->h_info = c->pin_info.as<U4>();
+    if (c->ncons)
+        KZG_HIP(c, hipMemcpyAsync(c->pin_info.p, c->c_info.p, sizeof(U4) * c->ncons, hipMemcpyDeviceToHost, c->stream));
+    KZG_HIP(c, hipEventRecord(c->ev[12], c->stream));
     KZG_HIP(c, record(c, 2));
     launch_build(c->stream, c->contacts(), c->manifolds(), c->bodies(), c->cache(), c->ncons,
                  mode == KZG_WARM_COLD ? 1 : 0);
     KZG_HIP(c, hipGetLastError());
     KZG_HIP(c, record(c, 3));
+    KZG_HIP(c, hipEventSynchronize(c->ev[12]));
+    // Host time of the build proper starts once the table is here.
+    const auto t0 = std::chrono::steady_clock::now();
     ScheduleInput si;
     si.num_bodies = c->nbodies;
     si.dynamic = c->h_dynamic.data();
     si.num_constraints = c->ncons;
-    si.info = c->h_info.data();
+    si.info = c->h_info;
     si.use_simd = c->world.use_simd_solver != 0;
     si.max_groups = c->cu_count * 4u;
     const auto ts = std::chrono::steady_clock::now();
@@ -345,7 +343,7 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     const size_t bytes = sizeof(uint32_t) * (ws.off.size() + vel.off.size() + 2 * size_t(np)) +
                          sizeof(U2) * (ws.grp.size() + vel.grp.size()) + 8 * 256;
     KZG_HIP(c, c->stage_b.ensure(bytes));
-    off = 0;
+    size_t off = 0;
     if ((rc = upload_async(c, c->ws_off, c->stage_b, off, ws.off.data(), ws.off.size())) ||
         (rc = upload_async(c, c->ws_grp, c->stage_b, off, reinterpret_cast<const uint2*>(ws.grp.data()), ws.grp.size())) ||
         (rc = upload_async(c, c->ws_pos, c->stage_b, off, ws.pos.data(), ws.pos.size())) ||

@@ -40,6 +40,8 @@ hipError_t launch_solve_split(hipStream_t s, const VelRecBuf& vr, const WsRecBuf
                               const SolveParams& sp, uint32_t groups, uint32_t threads, float* resid,
                               uint32_t coop_blocks, uint32_t coop_threads, uint32_t* counter,
                               hipEvent_t coop_begin, hipEvent_t coop_end);
+void launch_compact(hipStream_t s, const uint32_t* m_count, uint32_t n, const uint2* pairs, const BodyBuf& b,
+                    const ContactBuf& c, uint2* tile, uint32_t* totals);
 void launch_cache_store(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const CacheBuf& cache, uint32_t nc);
 void launch_cache_rehash(hipStream_t s, const CacheBuf& src, const CacheBuf& dst);
 void launch_finish(hipStream_t s, const BodyBuf& b, uint32_t n, float dt, int split);
@@ -132,7 +134,6 @@ struct kzg_ctx {
 
     // pairs and manifolds
     uint32_t npairs = 0;
-    std::vector<uint32_t> h_pairs;
     DevVec<uint2> d_pairs;
     DevVec<uint32_t> m_count;
     DevVec<float4> m_pa, m_pb, m_n;
@@ -141,8 +142,7 @@ struct kzg_ctx {
 
     // constraints and points
     uint32_t ncons = 0, npoints = 0;
-    std::vector<U4> h_info;
-    std::vector<uint32_t> h_cpair;
+    const U4* h_info = nullptr;  // pin_info, valid after kzg_build_constraints
     DevVec<uint4> c_info;
     DevVec<uint32_t> c_pair;
     DevVec<uint4> p_body;
@@ -174,7 +174,11 @@ struct kzg_ctx {
 
     // staging and timing
     Pinned stage, stage2, stage_a, stage_b;
-    hipEvent_t ev[12] = {};
+    Pinned pin_totals;  // constraints, points and cache occupancy of the step
+    Pinned pin_info;    // the step's constraint table (the schedule's input)
+    DevVec<uint2> compact_tile;
+    DevVec<uint32_t> compact_tot;
+    hipEvent_t ev[13] = {};  // 12: the constraint table has reached the host
     uint32_t recorded = 0;  // bit i: ev[i] recorded since the last narrowphase
     kzg_timings timings{};
     double build_host_ms = 0.0, schedule_host_ms = 0.0;
