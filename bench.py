@@ -392,7 +392,7 @@ def main():
     coop_bytes = 0.0
     coop_blocks = set()
     levels = []
-    phases = np.zeros(7)
+    phases = np.zeros(8)
     t_start = time.perf_counter()
     for i in range(args.steps):
         w.step_fixed()
@@ -400,7 +400,8 @@ def main():
         tm = device_timings(w)
         points += st["num_contact_points"]
         path_ms += st["contact_path_ms"]
-        resident_ms += tm["narrowphase_ms"] + tm["build_ms"] + tm["integrate_velocities_ms"] + tm["solve_ms"]
+        resident_ms += (tm["narrowphase_ms"] + tm["build_device_ms"] + tm["build_ms"] + tm["integrate_velocities_ms"] +
+                        tm["solve_ms"])
         solve_kernel_ms += tm["solve_kernel_ms"]
         solve_bytes += st["num_contact_points"] * (WARM_BYTES + 8 * VEL_BYTES + 2 * BIAS_BYTES)
         coop_blocks.add(tm["cooperative_blocks"])
@@ -410,7 +411,7 @@ def main():
                            + tm["cooperative_vel_points"] * 8 * VEL_BYTES)
         levels.append((tm["velocity_levels"], tm["bias_levels"], tm["groups"], tm["islands"]))
         phases += [tm["narrowphase_ms"], tm["build_ms"], tm["integrate_velocities_ms"], tm["solve_ms"],
-                   tm["solve_kernel_ms"], st["broadphase_ms"], tm["schedule_ms"]]
+                   tm["solve_kernel_ms"], st["broadphase_ms"], tm["schedule_ms"], tm["build_device_ms"]]
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     # value: the path's phases with inputs in HBM (kernel times, host build
@@ -433,7 +434,8 @@ def main():
         f"workgroups {sorted(coop_blocks)}), levels vel/bias max {lv[:, 0].max()}/{lv[:, 1].max()}, "
         f"groups {lv[:, 2].mean():.0f}, islands {lv[:, 3].mean():.0f}")
     ph = phases / args.steps
-    log(f"per step (ms): narrowphase {ph[0]:.2f}, build {ph[1]:.2f} (host schedule {ph[6]:.2f}), integrate velocities "
+    log(f"per step (ms): narrowphase {ph[0]:.2f}, build on the device {ph[7]:.2f} (compaction, islands), "
+        f"build on the host {ph[1]:.2f} (schedule {ph[6]:.2f}), integrate velocities "
         f"{ph[2]:.2f}, solve {ph[3]:.2f} (sweep kernel {ph[4]:.2f}), host broadphase {ph[5]:.2f}")
     if args.dump:
         import ctypes as C

@@ -67,12 +67,13 @@ typedef struct kzg_hull_range {
 
 /* Per-step phase timings (milliseconds), host-device transfers excluded.
  * Device phases are measured with HIP events on the context's stream:
- * narrowphase (kernels, after the pair upload), integrate velocities, solve
- * (prepare, sweeps, write-back, cache store), the sweep kernel alone,
+ * narrowphase (kernels, after the pair upload), the device part of the
+ * build (constraint-table compaction and islands), integrate velocities,
+ * solve (prepare, sweeps, write-back, cache store), the sweep kernel alone,
  * integrate positions and the constraint build kernel.  build_ms is host
- * compute of kzg_build_constraints once the manifold counts have arrived:
- * compaction and the solve schedule (the build kernel runs meanwhile); the
- * staging and upload of the schedule are not included. */
+ * compute of kzg_build_constraints once the constraint table has arrived:
+ * the solve schedule (the build kernel runs meanwhile); the staging and
+ * upload of the schedule are not included. */
 typedef struct kzg_timings {
     float narrowphase_ms, build_ms, integrate_velocities_ms, solve_ms, integrate_positions_ms;
     float solve_kernel_ms;   /* the persistent sweep kernel alone */
@@ -84,6 +85,7 @@ typedef struct kzg_timings {
     uint32_t cooperative_ws_points, cooperative_vel_points;
     float cooperative_kernel_ms;
     float schedule_ms;       /* the solve schedule alone (host, part of build_ms) */
+    float build_device_ms;   /* device part of the build before the host schedule: compaction, islands */
 } kzg_timings;
 
 /* Create a context on HIP device `device` for a world with settings `world`. */

@@ -96,7 +96,95 @@ __global__ void __launch_bounds__(1024) k_compact_emit(const uint32_t* m_count, 
     c.pair[ci] = i;
 }
 
+// ---- Islands (BuildIslands, PhysicsWorld.cpp:587-664) -----------------------
+// Lock-free union-find over the dynamic-dynamic constraints: a root is only
+// ever linked under a smaller root (parent[x] <= x), so every find ends and
+// each component's root is its smallest body slot.  The reference numbers
+// islands by the first appearance of their root in slot order, i.e. by that
+// smallest slot; island ids are therefore the exclusive count of roots below.
+__device__ __forceinline__ bool dyn_body(const BodyBuf& b, uint32_t x) { return meta_motion(b.meta[x]) == 2u; }
+__device__ __forceinline__ uint32_t uf_find(const uint32_t* parent, uint32_t x) {
+    uint32_t p = __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (p != x) {
+        x = p;
+        p = __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return x;
+}
+
+__global__ void k_uf_init(uint32_t* parent, uint32_t nb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nb) parent[i] = i;
+}
+
+__global__ void k_uf_union(ContactBuf c, const uint32_t* totals, BodyBuf b, uint32_t* parent) {
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= totals[0]) return;
+    const uint4 in = c.info[ci];
+    if (!dyn_body(b, in.x) || !dyn_body(b, in.y)) return;
+    uint32_t x = in.x, y = in.y;
+    for (;;) {
+        x = uf_find(parent, x);
+        y = uf_find(parent, y);
+        if (x == y) return;
+        const uint32_t hi = x > y ? x : y, lo = x > y ? y : x;
+        if (atomicCAS(parent + hi, hi, lo) == hi) return;
+    }
+}
+
+// parent[i] = root of i; flag[i] = i is a dynamic root.
+__global__ void __launch_bounds__(1024) k_uf_roots(uint32_t* parent, uint32_t nb, BodyBuf b, uint2* tile) {
+    __shared__ uint32_t sh[32];
+    const uint32_t i = blockIdx.x * kTile + threadIdx.x;
+    uint32_t root = 0;
+    if (i < nb) {
+        root = uf_find(parent, i);
+        parent[i] = root;
+    }
+    const uint32_t f = (i < nb && root == i && dyn_body(b, i)) ? 1u : 0u;
+    uint2 tot;
+    tile_scan(f, 0u, tot, sh);
+    if (threadIdx.x == 0) tile[blockIdx.x] = tot;
+}
+
+// island[i] = island id of root i (other entries unused).
+__global__ void __launch_bounds__(1024) k_island_ids(const uint32_t* parent, uint32_t nb, BodyBuf b,
+                                                     const uint2* tile, uint32_t* island) {
+    __shared__ uint32_t sh[32];
+    const uint32_t i = blockIdx.x * kTile + threadIdx.x;
+    const uint32_t f = (i < nb && parent[i] == i && dyn_body(b, i)) ? 1u : 0u;
+    uint2 tot;
+    const uint2 e = tile_scan(f, 0u, tot, sh);
+    if (f) island[i] = tile[blockIdx.x].x + e.x;
+}
+
+// Island of each constraint: that of body A when dynamic, else of body B.
+__global__ void k_con_island(ContactBuf c, const uint32_t* totals, BodyBuf b, const uint32_t* parent,
+                             const uint32_t* island, uint32_t* con_island) {
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= totals[0]) return;
+    const uint4 in = c.info[ci];
+    const uint32_t x = dyn_body(b, in.x) ? in.x : in.y;
+    con_island[ci] = island[parent[x]];
+}
+
 }  // namespace
+
+// Islands of the constraint table built by launch_compact: con_island[ci]
+// for ci < totals[0], and totals[2] = number of islands (every dynamic body
+// belongs to one).  parent and island hold nb entries, tile ceil(nb / 1024)
+// (and at least 1).  Constraints are bounded by cap.
+void launch_islands(hipStream_t s, const ContactBuf& c, uint32_t cap, const BodyBuf& b, uint32_t nb,
+                    uint32_t* parent, uint32_t* island, uint2* tile, uint32_t* totals, uint32_t* con_island) {
+    const uint32_t tiles = (nb + kTile - 1) / kTile;
+    const uint32_t cb = (cap + 255) / 256;
+    if (nb) k_uf_init<<<(nb + 255) / 256, 256, 0, s>>>(parent, nb);
+    if (cb) k_uf_union<<<cb, 256, 0, s>>>(c, totals, b, parent);
+    if (tiles) k_uf_roots<<<tiles, kTile, 0, s>>>(parent, nb, b, tile);
+    k_compact_scan<<<1, kTile, 0, s>>>(tile, tiles, totals + 2);
+    if (tiles) k_island_ids<<<tiles, kTile, 0, s>>>(parent, nb, b, tile, island);
+    if (cb) k_con_island<<<cb, 256, 0, s>>>(c, totals, b, parent, island, con_island);
+}
 
 // Constraint table of n pairs into c.info / c.pair; totals[0..1] = constraints,
 // points.  tile holds ceil(n / 1024) entries.

@@ -120,6 +120,9 @@ void kzg_destroy(kzg_ctx* c) {
     c->pin_info.release();
     c->compact_tile.release();
     c->compact_tot.release();
+    c->uf_parent.release();
+    c->uf_island.release();
+    c->con_island.release();
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -256,24 +259,32 @@ int kzg_narrowphase(kzg_ctx* c, const uint32_t* pairs, uint32_t n) {
     KZG_HIP(c, c->gjk.ensure(size_t(std::max<uint32_t>(n, 1)) * kGjkRecBytes));
     KZG_HIP(c, c->c_info.ensure(std::max<uint32_t>(n, 1)));
     KZG_HIP(c, c->c_pair.ensure(std::max<uint32_t>(n, 1)));
-    KZG_HIP(c, c->compact_tile.ensure((std::max<uint32_t>(n, 1) + 1023) / 1024));
-    KZG_HIP(c, c->compact_tot.ensure(2));
-    KZG_HIP(c, c->pin_totals.ensure(4 * sizeof(uint32_t)));
+    KZG_HIP(c, c->compact_tile.ensure((std::max(std::max<uint32_t>(n, 1), c->nbodies) + 1023) / 1024));
+    KZG_HIP(c, c->compact_tot.ensure(4));
+    KZG_HIP(c, c->pin_totals.ensure(8 * sizeof(uint32_t)));
+    KZG_HIP(c, c->uf_parent.ensure(std::max<uint32_t>(c->nbodies, 1)));
+    KZG_HIP(c, c->uf_island.ensure(std::max<uint32_t>(c->nbodies, 1)));
+    KZG_HIP(c, c->con_island.ensure(std::max<uint32_t>(n, 1)));
     int rc = 0;
     c->recorded = 0;
     if ((rc = upload(c, c->d_pairs, reinterpret_cast<const uint2*>(pairs), n))) return rc;
     KZG_HIP(c, record(c, 0));
     launch_narrowphase(c->stream, c->d_pairs.p, n, c->bodies(), c->shapes(), c->manifolds(), c->work.p,
                        c->work_count.p, c->gjk.p, 0.02f, 1, std::min<uint32_t>(n, c->cu_count * 16u));
-    // The constraint table (BuildContactConstraints' compaction, :454-471)
-    // is built on the device right behind the narrowphase.
-    launch_compact(c->stream, c->m_count.p, n, c->d_pairs.p, c->bodies(), c->contacts(), c->compact_tile.p,
-                   c->compact_tot.p);
     KZG_HIP(c, hipGetLastError());
     KZG_HIP(c, record(c, 1));
+    // The constraint table (BuildContactConstraints' compaction, :454-471)
+    // and its islands (BuildIslands, :587-664) are built on the device
+    // right behind the narrowphase.
+    launch_compact(c->stream, c->m_count.p, n, c->d_pairs.p, c->bodies(), c->contacts(), c->compact_tile.p,
+                   c->compact_tot.p);
+    launch_islands(c->stream, c->contacts(), n, c->bodies(), c->nbodies, c->uf_parent.p, c->uf_island.p,
+                   c->compact_tile.p, c->compact_tot.p, c->con_island.p);
+    KZG_HIP(c, hipGetLastError());
+    KZG_HIP(c, record(c, 13));
     uint32_t* tot = c->pin_totals.as<uint32_t>();
-    KZG_HIP(c, hipMemcpyAsync(tot, c->compact_tot.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    KZG_HIP(c, hipMemcpyAsync(tot + 2, c->cache_used.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    KZG_HIP(c, hipMemcpyAsync(tot, c->compact_tot.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    KZG_HIP(c, hipMemcpyAsync(tot + 4, c->cache_used.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     c->phase = 1;
     return KZG_OK;
 }
@@ -295,7 +306,8 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     const uint32_t* tot = c->pin_totals.as<uint32_t>();
     c->ncons = tot[0];
     c->npoints = tot[1];
-    const uint32_t cache_used = tot[2];
+    const uint32_t islands = tot[2];
+    const uint32_t cache_used = tot[4];
     const uint32_t np = c->npoints;
     int rc;
     // The store at the end of this step adds at most ncons keys.
@@ -312,10 +324,14 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     KZG_HIP(c, c->p_nvel.ensure(np1));
     // The constraint table comes down while the build kernel runs; the host
     // derives the solve schedule from it.
-    KZG_HIP(c, c->pin_info.ensure(sizeof(U4) * std::max<uint32_t>(c->ncons, 1)));
+    KZG_HIP(c, c->pin_info.ensure((sizeof(U4) + sizeof(uint32_t)) * std::max<uint32_t>(c->ncons, 1)));
     c->h_info = c->pin_info.as<U4>();
-    if (c->ncons)
+    c->h_con_island = reinterpret_cast<const uint32_t*>(c->h_info + c->ncons);
+    if (c->ncons) {
         KZG_HIP(c, hipMemcpyAsync(c->pin_info.p, c->c_info.p, sizeof(U4) * c->ncons, hipMemcpyDeviceToHost, c->stream));
+        KZG_HIP(c, hipMemcpyAsync(const_cast<uint32_t*>(c->h_con_island), c->con_island.p,
+                                  sizeof(uint32_t) * c->ncons, hipMemcpyDeviceToHost, c->stream));
+    }
     KZG_HIP(c, hipEventRecord(c->ev[12], c->stream));
     KZG_HIP(c, record(c, 2));
     launch_build(c->stream, c->contacts(), c->manifolds(), c->bodies(), c->cache(), c->ncons,
@@ -330,6 +346,8 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     si.dynamic = c->h_dynamic.data();
     si.num_constraints = c->ncons;
     si.info = c->h_info;
+    si.island_of_con = c->h_con_island;
+    si.num_islands = islands;
     si.use_simd = c->world.use_simd_solver != 0;
     si.max_groups = c->cu_count * 4u;
     const auto ts = std::chrono::steady_clock::now();
@@ -565,6 +583,7 @@ int kzg_get_timings(kzg_ctx* c, kzg_timings* out) {
             out = ms;
     };
     span(0, 1, t.narrowphase_ms);
+    span(1, 13, t.build_device_ms);
     span(2, 3, t.build_kernel_ms);
     t.build_ms = float(c->build_host_ms);
     t.schedule_ms = float(c->schedule_host_ms);

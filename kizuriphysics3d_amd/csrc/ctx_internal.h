@@ -42,6 +42,8 @@ hipError_t launch_solve_split(hipStream_t s, const VelRecBuf& vr, const WsRecBuf
                               hipEvent_t coop_begin, hipEvent_t coop_end);
 void launch_compact(hipStream_t s, const uint32_t* m_count, uint32_t n, const uint2* pairs, const BodyBuf& b,
                     const ContactBuf& c, uint2* tile, uint32_t* totals);
+void launch_islands(hipStream_t s, const ContactBuf& c, uint32_t cap, const BodyBuf& b, uint32_t nb,
+                    uint32_t* parent, uint32_t* island, uint2* tile, uint32_t* totals, uint32_t* con_island);
 void launch_cache_store(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const CacheBuf& cache, uint32_t nc);
 void launch_cache_rehash(hipStream_t s, const CacheBuf& src, const CacheBuf& dst);
 void launch_finish(hipStream_t s, const BodyBuf& b, uint32_t n, float dt, int split);
@@ -142,7 +144,8 @@ struct kzg_ctx {
 
     // constraints and points
     uint32_t ncons = 0, npoints = 0;
-    const U4* h_info = nullptr;  // pin_info, valid after kzg_build_constraints
+    const U4* h_info = nullptr;           // in pin_info, valid after kzg_build_constraints
+    const uint32_t* h_con_island = nullptr;  // in pin_info, after the table
     DevVec<uint4> c_info;
     DevVec<uint32_t> c_pair;
     DevVec<uint4> p_body;
@@ -177,8 +180,9 @@ struct kzg_ctx {
     Pinned pin_totals;  // constraints, points and cache occupancy of the step
     Pinned pin_info;    // the step's constraint table (the schedule's input)
     DevVec<uint2> compact_tile;
-    DevVec<uint32_t> compact_tot;
-    hipEvent_t ev[13] = {};  // 12: the constraint table has reached the host
+    DevVec<uint32_t> compact_tot;  // constraints, points, islands, 0
+    DevVec<uint32_t> uf_parent, uf_island, con_island;
+    hipEvent_t ev[14] = {};  // 12: the constraint table has reached the host; 13: device build end
     uint32_t recorded = 0;  // bit i: ev[i] recorded since the last narrowphase
     kzg_timings timings{};
     double build_host_ms = 0.0, schedule_host_ms = 0.0;

@@ -119,6 +119,55 @@ void ScheduleBuilder::flatten(Seq& s, uint32_t groups, uint32_t num_points, Leve
 }
 
 void ScheduleBuilder::islands(const ScheduleInput& in, ScheduleOutput& out) {
+    const uint32_t nc = in.num_constraints;
+    uint32_t ni = 0;
+    if (in.island_of_con) {
+        ni = in.num_islands;
+        island_of_con_.assign(in.island_of_con, in.island_of_con + nc);
+    } else {
+        ni = union_find(in);
+    }
+    // The constraints of each island in constraint order (counting sort).
+    island_points_.assign(ni, 0);
+    island_off_.assign(ni + 1, 0);
+    for (uint32_t c = 0; c < nc; ++c) {
+        const uint32_t is = island_of_con_[c];
+        if (is != kNone) {
+            island_off_[is + 1] += 1;
+            island_points_[is] += in.info[c].w & 0xFFu;
+        }
+    }
+    for (uint32_t i = 0; i < ni; ++i) island_off_[i + 1] += island_off_[i];
+    island_cons_.resize(island_off_[ni]);
+    order_.assign(island_off_.begin(), island_off_.end() - 1);
+    for (uint32_t c = 0; c < nc; ++c)
+        if (island_of_con_[c] != kNone) island_cons_[order_[island_of_con_[c]]++] = c;
+    out.islands = ni;
+    // Groups: largest islands first, each to the least-loaded group.
+    const uint32_t groups = std::max<uint32_t>(1, std::min<uint32_t>(ni, in.max_groups));
+    out.groups = groups;
+    groups_ = groups;
+    group_of_island_.assign(ni, 0);
+    order_.resize(ni);
+    for (uint32_t i = 0; i < ni; ++i) order_[i] = i;
+    std::stable_sort(order_.begin(), order_.end(),
+                     [&](uint32_t a, uint32_t b) { return island_points_[a] > island_points_[b]; });
+    using Load = std::pair<uint64_t, uint32_t>;
+    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+    for (uint32_t g = 0; g < groups; ++g) heap.push({0, g});
+    for (uint32_t is : order_) {
+        Load l = heap.top();
+        heap.pop();
+        group_of_island_[is] = l.second;
+        l.first += island_points_[is];
+        heap.push(l);
+    }
+}
+
+// Union by rank with path halving over the dynamic-dynamic contacts; islands
+// numbered by the first appearance of their root in slot order.  Fills
+// island_of_con_ and returns the island count.
+uint32_t ScheduleBuilder::union_find(const ScheduleInput& in) {
     const uint32_t nb = in.num_bodies, nc = in.num_constraints;
     const uint8_t* dyn = in.dynamic;
     dyn_index_.assign(nb, kNone);
@@ -150,48 +199,16 @@ void ScheduleBuilder::islands(const ScheduleInput& in, ScheduleOutput& out) {
         const uint32_t r = find(i);
         if (root_island_[r] == kNone) root_island_[r] = ni++;
     }
-    // A body's island, then the constraints of each island in constraint
-    // order (counting sort).
     for (uint32_t i = 0; i < nd; ++i) root_island_[i] = root_island_[find(i)];
     island_of_con_.resize(nc);
-    island_points_.assign(ni, 0);
-    island_off_.assign(ni + 1, 0);
     for (uint32_t c = 0; c < nc; ++c) {
         const U4& ci = in.info[c];
         uint32_t is = kNone;
         if (dyn[ci.x]) is = root_island_[dyn_index_[ci.x]];
         else if (dyn[ci.y]) is = root_island_[dyn_index_[ci.y]];
         island_of_con_[c] = is;
-        if (is != kNone) {
-            island_off_[is + 1] += 1;
-            island_points_[is] += ci.w & 0xFFu;
-        }
     }
-    for (uint32_t i = 0; i < ni; ++i) island_off_[i + 1] += island_off_[i];
-    island_cons_.resize(island_off_[ni]);
-    order_.assign(island_off_.begin(), island_off_.end() - 1);
-    for (uint32_t c = 0; c < nc; ++c)
-        if (island_of_con_[c] != kNone) island_cons_[order_[island_of_con_[c]]++] = c;
-    out.islands = ni;
-    // Groups: largest islands first, each to the least-loaded group.
-    const uint32_t groups = std::max<uint32_t>(1, std::min<uint32_t>(ni, in.max_groups));
-    out.groups = groups;
-    groups_ = groups;
-    group_of_island_.assign(ni, 0);
-    order_.resize(ni);
-    for (uint32_t i = 0; i < ni; ++i) order_[i] = i;
-    std::stable_sort(order_.begin(), order_.end(),
-                     [&](uint32_t a, uint32_t b) { return island_points_[a] > island_points_[b]; });
-    using Load = std::pair<uint64_t, uint32_t>;
-    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
-    for (uint32_t g = 0; g < groups; ++g) heap.push({0, g});
-    for (uint32_t is : order_) {
-        Load l = heap.top();
-        heap.pop();
-        group_of_island_[is] = l.second;
-        l.first += island_points_[is];
-        heap.push(l);
-    }
+    return ni;
 }
 
 // Levels of the warm-start/bias sequence (constraint order, every

@@ -40,6 +40,11 @@ struct ScheduleInput {
     const U4* info = nullptr;          // body A, body B, first point, num points | sensor << 8
     bool use_simd = true;
     uint32_t max_groups = 1024;
+    // Optional: the island of each constraint, numbered as BuildIslands
+    // numbers them, and the island count (every dynamic body is in one).
+    // When given, the builder takes them instead of running union-find.
+    const uint32_t* island_of_con = nullptr;
+    uint32_t num_islands = 0;
 };
 
 struct ScheduleOutput {
@@ -72,6 +77,7 @@ private:
         uint32_t n = 0;
     };
     void islands(const ScheduleInput& in, ScheduleOutput& out);
+    uint32_t union_find(const ScheduleInput& in);
     void ws_levels(const ScheduleInput& in);
     void vel_sequence(const ScheduleInput& in);
     void flatten(Seq& s, uint32_t groups, uint32_t num_points, LevelSchedule& out);

@@ -29,7 +29,7 @@ TIMINGS_DTYPE = np.dtype([
     ("integrate_positions_ms", "<f4"), ("solve_kernel_ms", "<f4"), ("velocity_levels", "<u4"),
     ("bias_levels", "<u4"), ("groups", "<u4"), ("islands", "<u4"), ("build_kernel_ms", "<f4"),
     ("cooperative_blocks", "<u4"), ("cooperative_ws_points", "<u4"), ("cooperative_vel_points", "<u4"),
-    ("cooperative_kernel_ms", "<f4"), ("schedule_ms", "<f4"),
+    ("cooperative_kernel_ms", "<f4"), ("schedule_ms", "<f4"), ("build_device_ms", "<f4"),
 ])
 assert KZG_BODY_DTYPE.itemsize == 136
 

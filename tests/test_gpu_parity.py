@@ -73,6 +73,29 @@ def test_pile_trajectory_bit_exact():
     _trajectory(S.pile(512, seed=0x4B5A0102), 240)
 
 
+def test_device_islands_match_host_union_find():
+    """The constraint table and islands built on the device behind the
+    narrowphase: the island count equals the host union-find's over the same
+    table (BuildIslands numbering), along a 512-body pile."""
+    scene = S.pile(512, seed=0x4B5A0102)
+    R.init_jobs(0)
+    rw = R.RefWorld(scene)
+    ctx = _ctx_for(scene, rw)
+    dyn = (scene.body_array()["motion_type"] == S.MOTION_DYNAMIC).astype(np.uint8)
+    checked = 0
+    for i in range(200):
+        rw.step_phases()
+        ctx.step(rw.pairs())
+        if i % 25 == 24:
+            hdr, _ = ctx.constraints()
+            first = np.concatenate([[0], np.cumsum(hdr[:, 2])[:-1]]).astype(np.uint32)
+            info = np.stack([hdr[:, 0], hdr[:, 1], first, hdr[:, 2] | ((hdr[:, 3] & 1) << 8)], 1).astype(np.uint32)
+            _, _, _, st = D.schedule(dyn, info, True, 1 << 20)
+            assert ctx.timings()["islands"] == st[0], (i, ctx.timings()["islands"], st)
+            checked += 1
+    assert checked == 8
+
+
 @pytest.mark.parametrize("blocks", [2, 7, 64])
 def test_pile_trajectory_cooperative_bit_exact(blocks):
     """The largest group's levels split over `blocks` workgroups with a
