@@ -103,13 +103,29 @@ __global__ void __launch_bounds__(1024) k_compact_emit(const uint32_t* m_count, 
 // islands by the first appearance of their root in slot order, i.e. by that
 // smallest slot; island ids are therefore the exclusive count of roots below.
 __device__ __forceinline__ bool dyn_body(const BodyBuf& b, uint32_t x) { return meta_motion(b.meta[x]) == 2u; }
+__device__ __forceinline__ uint32_t uf_load(const uint32_t* parent, uint32_t x) {
+    return __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ __forceinline__ uint32_t uf_find(const uint32_t* parent, uint32_t x) {
-    uint32_t p = __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t p = uf_load(parent, x);
     while (p != x) {
         x = p;
-        p = __hip_atomic_load(parent + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        p = uf_load(parent, x);
     }
     return x;
+}
+// Find with path halving: a non-root is pointed at its grandparent (an
+// ancestor, so parent[x] <= x still holds; roots are never written here, so
+// the linking CAS is unaffected).
+__device__ __forceinline__ uint32_t uf_find_halve(uint32_t* parent, uint32_t x) {
+    for (;;) {
+        const uint32_t p = uf_load(parent, x);
+        if (p == x) return x;
+        const uint32_t gp = uf_load(parent, p);
+        if (gp == p) return p;
+        __hip_atomic_store(parent + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = gp;
+    }
 }
 
 __global__ void k_uf_init(uint32_t* parent, uint32_t nb) {
@@ -124,8 +140,8 @@ __global__ void k_uf_union(ContactBuf c, const uint32_t* totals, BodyBuf b, uint
     if (!dyn_body(b, in.x) || !dyn_body(b, in.y)) return;
     uint32_t x = in.x, y = in.y;
     for (;;) {
-        x = uf_find(parent, x);
-        y = uf_find(parent, y);
+        x = uf_find_halve(parent, x);
+        y = uf_find_halve(parent, y);
         if (x == y) return;
         const uint32_t hi = x > y ? x : y, lo = x > y ? y : x;
         if (atomicCAS(parent + hi, hi, lo) == hi) return;

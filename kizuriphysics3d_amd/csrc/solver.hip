@@ -486,16 +486,18 @@ __device__ __forceinline__ void sweep(const LevelOffsets& off, uint2 range, cons
     sweep(off, range, bd, 1u, 0u, f, [] { __syncthreads(); });
 }
 
-// Device-wide barrier of the cooperating workgroups: monotonic counter,
-// agent-scope fences publish the body updates of a level before the next.
+// Device-wide barrier of the cooperating workgroups: monotonic counter.  An
+// agent-scope release fence before the arrival publishes the workgroup's
+// body updates of the level; an acquire fence after the wait makes the
+// others' visible.
 __device__ __forceinline__ void grid_sync(uint32_t* counter, uint32_t target) {
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         atomicAdd(counter, 1u);
         while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
             __builtin_amdgcn_s_sleep(1);
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 }
