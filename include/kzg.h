@@ -132,6 +132,17 @@ int kzg_download_constraints(kzg_ctx* ctx, uint32_t* hdr, float* pts);
 int kzg_get_residuals(kzg_ctx* ctx, double* out, uint32_t cap);
 int kzg_get_timings(kzg_ctx* ctx, kzg_timings* out);
 
+/* The warm-start cache (the contact cache of PhysicsWorld.cpp:405-452),
+ * for snapshots: capacity in slots (a power of two) and occupied slots.
+ * Its image is KZG_CACHE_SLOT_BYTES bytes per slot in four arrays: keys
+ * (u64 MakePairKey, 0 = empty), point counts (u32), points (4 float4 per
+ * slot: pointOnA and normal impulse), tangent impulses (4 float2 per slot).
+ * kzg_upload_cache replaces the cache with such an image. */
+#define KZG_CACHE_SLOT_BYTES 108
+int kzg_cache_info(kzg_ctx* ctx, uint32_t* capacity, uint32_t* used);
+int kzg_download_cache(kzg_ctx* ctx, void* out, uint64_t bytes);
+int kzg_upload_cache(kzg_ctx* ctx, const void* in, uint64_t bytes, uint32_t capacity, uint32_t used);
+
 /* Standalone narrowphase over (shape, transform) pairs with the uploaded
  * shape table: xf = quaternion xyzw + translation, 7 floats.  Output:
  * counts[n], points[n * 4].  ms (optional) = device time of the kernels. */

@@ -68,6 +68,21 @@ int kzw_get_hull(kzw_world* w, uint32_t shape, float* verts, uint32_t* tris, uin
 /* The device context of the world (null for a host-only world). */
 kzg_ctx* kzw_device(kzw_world* w);
 
+/* Snapshots (SerializeWorld / DeserializeWorld, src/Serialization/Serializer.cpp:182-234).
+ * kzw_save writes the reference's KZP1 world record (bodies with their
+ * shapes, settings and current state; readable by DeserializeWorld) followed
+ * by an extension that makes the snapshot resumable bit for bit: world
+ * settings, the shape table with its built hulls, body descriptions and
+ * states, the broadphase tree, and the warm-start cache.  With out null it
+ * only reports the size.
+ * kzw_load creates a world from such a snapshot, or from a plain KZP1 record
+ * written by the reference; a plain record carries no world settings, so
+ * `settings` is required then (and ignored otherwise).  A plain record
+ * rebuilds hulls, broadphase and an empty cache exactly as DeserializeWorld
+ * does. */
+int kzw_save(kzw_world* w, uint8_t* out, uint64_t cap, uint64_t* size);
+int kzw_load(kzw_world** out, const uint8_t* data, uint64_t size, const kz_world_desc* settings, int device);
+
 #ifdef __cplusplus
 }
 #endif

@@ -478,6 +478,50 @@ int kzg_query_sizes(kzg_ctx* c, uint32_t* np, uint32_t* nc, uint32_t* npts) {
     return KZG_OK;
 }
 
+int kzg_cache_info(kzg_ctx* c, uint32_t* cap, uint32_t* used) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    hipSetDevice(c->device);
+    uint32_t u = 0;
+    if (c->cache_cap) {
+        int rc = download(c, &u, c->cache_used, 1);
+        if (rc) return rc;
+    }
+    if (cap) *cap = c->cache_cap;
+    if (used) *used = u;
+    return KZG_OK;
+}
+
+int kzg_download_cache(kzg_ctx* c, void* out, uint64_t bytes) {
+    if (!c || !out) return KZG_ERR_ARGUMENT;
+    const size_t cap = c->cache_cap;
+    if (bytes != cap * KZG_CACHE_SLOT_BYTES) return fail(c, KZG_ERR_ARGUMENT, "cache image size mismatch");
+    hipSetDevice(c->device);
+    char* o = static_cast<char*>(out);
+    int rc;
+    if ((rc = download(c, reinterpret_cast<unsigned long long*>(o), c->cache_key, cap)) ||
+        (rc = download(c, reinterpret_cast<uint32_t*>(o + 8 * cap), c->cache_count, cap)) ||
+        (rc = download(c, reinterpret_cast<float4*>(o + 12 * cap), c->cache_p, cap * kPairPoints)) ||
+        (rc = download(c, reinterpret_cast<float2*>(o + 76 * cap), c->cache_t, cap * kPairPoints)))
+        return rc;
+    return KZG_OK;
+}
+
+int kzg_upload_cache(kzg_ctx* c, const void* in, uint64_t bytes, uint32_t cap, uint32_t used) {
+    if (!c || !in || cap < 2 || (cap & (cap - 1)) || used > cap / 2 || bytes != uint64_t(cap) * KZG_CACHE_SLOT_BYTES)
+        return fail(c, KZG_ERR_ARGUMENT, "bad cache image (capacity a power of two, load at most 1/2)");
+    hipSetDevice(c->device);
+    const char* p = static_cast<const char*>(in);
+    int rc;
+    if ((rc = upload(c, c->cache_key, reinterpret_cast<const unsigned long long*>(p), cap)) ||
+        (rc = upload(c, c->cache_count, reinterpret_cast<const uint32_t*>(p + 8 * size_t(cap)), cap)) ||
+        (rc = upload(c, c->cache_p, reinterpret_cast<const float4*>(p + 12 * size_t(cap)), size_t(cap) * kPairPoints)) ||
+        (rc = upload(c, c->cache_t, reinterpret_cast<const float2*>(p + 76 * size_t(cap)), size_t(cap) * kPairPoints)) ||
+        (rc = upload(c, c->cache_used, &used, 1)))
+        return rc;
+    c->cache_cap = cap;
+    return KZG_OK;
+}
+
 int kzg_download_states(kzg_ctx* c, kz_body_state* out, uint32_t n) {
     if (!c || !out || n != c->nbodies) return fail(c, KZG_ERR_ARGUMENT, "state buffer does not match the body table");
     hipSetDevice(c->device);

@@ -3,6 +3,8 @@
 // beside it, operation for operation (compiled with -ffp-contract=off).
 #include "host_geometry.h"
 
+#include <cstring>
+
 #include <algorithm>
 #include <cmath>
 
@@ -339,6 +341,70 @@ void BroadPhase::free_node(uint32_t id) {
     nodes_[id].next = free_;
     nodes_[id].height = -1;
     free_ = id;
+}
+
+namespace {
+constexpr size_t kNodeWords = 18;  // fat (6), tight (6), parent, child1, child2, user, next, height
+inline uint32_t fbits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+inline float bitsf(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+}  // namespace
+
+void BroadPhase::save(std::vector<uint32_t>& out) const {
+    out.clear();
+    out.push_back(uint32_t(nodes_.size()));
+    out.push_back(root_);
+    out.push_back(free_);
+    out.push_back(uint32_t(layers_.size()));
+    for (const Node& n : nodes_) {
+        for (const Aabb* b : {&n.fat, &n.tight}) {
+            for (float f : {b->mn.x, b->mn.y, b->mn.z, b->mx.x, b->mx.y, b->mx.z}) out.push_back(fbits(f));
+        }
+        out.push_back(n.parent);
+        out.push_back(n.child1);
+        out.push_back(n.child2);
+        out.push_back(n.user);
+        out.push_back(n.next);
+        out.push_back(uint32_t(n.height));
+    }
+    out.insert(out.end(), layers_.begin(), layers_.end());
+    out.insert(out.end(), masks_.begin(), masks_.end());
+}
+
+bool BroadPhase::restore(const uint32_t* in, size_t n) {
+    if (n < 4) return false;
+    const size_t nn = in[0], nl = in[3];
+    if (n != 4 + nn * kNodeWords + 2 * nl) return false;
+    std::vector<Node> nodes(nn);
+    const uint32_t* p = in + 4;
+    for (size_t i = 0; i < nn; ++i, p += kNodeWords) {
+        Node& d = nodes[i];
+        d.fat = Aabb{f3{bitsf(p[0]), bitsf(p[1]), bitsf(p[2])}, f3{bitsf(p[3]), bitsf(p[4]), bitsf(p[5])}};
+        d.tight = Aabb{f3{bitsf(p[6]), bitsf(p[7]), bitsf(p[8])}, f3{bitsf(p[9]), bitsf(p[10]), bitsf(p[11])}};
+        d.parent = p[12];
+        d.child1 = p[13];
+        d.child2 = p[14];
+        d.user = p[15];
+        d.next = p[16];
+        d.height = int32_t(p[17]);
+        for (uint32_t link : {d.parent, d.child1, d.child2, d.next})
+            if (link != kNull && link >= nn) return false;
+    }
+    if ((in[1] != kNull && in[1] >= nn) || (in[2] != kNull && in[2] >= nn)) return false;
+    nodes_ = std::move(nodes);
+    root_ = in[1];
+    free_ = in[2];
+    layers_.assign(p, p + nl);
+    masks_.assign(p + nl, p + 2 * nl);
+    pairs_.clear();
+    return true;
 }
 
 uint32_t BroadPhase::create_proxy(const Aabb& tight, uint32_t user, uint32_t layer, uint32_t mask) {

@@ -59,6 +59,11 @@ public:
     void update_pairs();
     const std::vector<uint32_t>& pairs() const { return pairs_; }  // proxy A, proxy B, ...
     uint32_t user(uint32_t id) const { return nodes_[id].user; }
+    // The whole tree as plain words (snapshots): node records, root, free
+    // list, proxy layers and masks.  restore() returns false on a malformed
+    // image and leaves the tree unchanged.
+    void save(std::vector<uint32_t>& out) const;
+    bool restore(const uint32_t* in, size_t n);
 
 private:
     static constexpr uint32_t kNull = 0xFFFFFFFFu;

@@ -290,3 +290,333 @@ int kzw_get_hull(kzw_world* w, uint32_t shape, float* verts, uint32_t* tris, uin
 kzg_ctx* kzw_device(kzw_world* w) { return w ? w->ctx : nullptr; }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Snapshots.  The file starts with the reference's KZP1 world record
+// (src/Serialization/Serializer.cpp:182-205: magic, version, body count, then
+// per body its shape, its BodySettings with the current state, and the
+// active flag), so the reference's DeserializeWorld reads it as a plain
+// world.  A KZP1 reader stops after the last body; what follows is this
+// library's extension, which makes a resumed world step on bit-identically:
+// the world settings, the shape table with the built hulls, the body
+// descriptions and states, the broadphase tree, and the warm-start cache.
+// Extension: "KZX1", version, then blocks (u32 tag, u64 length, payload)
+// ending with "END ".
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr uint32_t kKzp1Magic = 0x31505A4Bu;  // "KZP1"
+constexpr uint32_t kKzx1Magic = 0x31585A4Bu;  // "KZX1"
+constexpr uint32_t tag4(const char (&s)[5]) {
+    return uint32_t(uint8_t(s[0])) | uint32_t(uint8_t(s[1])) << 8 | uint32_t(uint8_t(s[2])) << 16 |
+           uint32_t(uint8_t(s[3])) << 24;
+}
+
+struct Writer {
+    std::vector<uint8_t> b;
+    template <class T> void put(const T& v) {
+        const auto* p = reinterpret_cast<const uint8_t*>(&v);
+        b.insert(b.end(), p, p + sizeof(T));
+    }
+    void bytes(const void* p, size_t n) {
+        const auto* q = static_cast<const uint8_t*>(p);
+        b.insert(b.end(), q, q + n);
+    }
+    size_t begin_block(uint32_t tag) {
+        put(tag);
+        put(uint64_t(0));
+        return b.size();
+    }
+    void end_block(size_t start) {
+        const uint64_t len = b.size() - start;
+        std::memcpy(b.data() + start - sizeof(uint64_t), &len, sizeof(len));
+    }
+};
+
+struct Reader {
+    const uint8_t* p;
+    size_t n, off = 0;
+    template <class T> bool get(T& v) {
+        if (off + sizeof(T) > n) return false;
+        std::memcpy(&v, p + off, sizeof(T));
+        off += sizeof(T);
+        return true;
+    }
+    bool bytes(void* dst, size_t k) {
+        if (off + k > n) return false;
+        std::memcpy(dst, p + off, k);
+        off += k;
+        return true;
+    }
+};
+
+// One KZP1 body: the shape (ShapeType code, parameters, hull points) and the
+// BodySettings fields in the reference's order.
+void write_kzp1_body(Writer& w, const kzw_world& W, uint32_t i) {
+    const kz_body_desc& d = W.bodies[i];
+    const kz_body_state& s = W.states[i];
+    const auto& sh = W.shapes[d.shape];
+    w.put(uint8_t(sh.desc.type));
+    switch (sh.desc.type) {
+        case KZ_SHAPE_SPHERE: w.put(sh.desc.params[0]); break;
+        case KZ_SHAPE_BOX: w.bytes(sh.desc.params, 3 * sizeof(float)); break;
+        case KZ_SHAPE_CAPSULE: w.bytes(sh.desc.params, 2 * sizeof(float)); break;
+        case KZ_SHAPE_PLANE: w.bytes(sh.desc.params, 4 * sizeof(float)); break;
+        default: {  // convex hull: the built hull's points (ConvexHullShape::GetPoints)
+            w.put(uint32_t(sh.hull.verts.size()));
+            for (const f3& v : sh.hull.verts) {
+                w.put(v.x);
+                w.put(v.y);
+                w.put(v.z);
+            }
+        }
+    }
+    w.bytes(s.position, sizeof(s.position));
+    w.bytes(s.rotation, sizeof(s.rotation));
+    w.bytes(s.linear_velocity, sizeof(s.linear_velocity));
+    w.bytes(s.angular_velocity, sizeof(s.angular_velocity));
+    w.put(uint8_t(d.motion_type));
+    w.put(uint8_t(0));  // motionQuality: Discrete
+    w.put(0.0f);        // ccdMotionThreshold
+    for (float f : {d.friction, d.restitution, d.linear_damping, d.angular_damping, d.gravity_factor,
+                    d.max_linear_velocity, d.max_angular_velocity})
+        w.put(f);
+    w.put(uint32_t(1));            // objectLayer
+    w.put(uint32_t(0xFFFFFFFFu));  // collisionMask
+    w.put(uint8_t(d.allow_sleeping ? 1 : 0));
+    w.put(uint8_t(d.is_sensor ? 1 : 0));
+    w.put(uint64_t(0));  // userData
+    w.put(d.mass_override);
+    w.put(d.density);
+    w.put(uint8_t(1));  // active
+}
+
+struct Kzp1Body {
+    kz_shape_desc shape{};
+    std::vector<float> points;
+    kz_body_desc body{};
+};
+
+bool read_kzp1_body(Reader& r, Kzp1Body& o) {
+    uint8_t type = 0;
+    if (!r.get(type)) return false;
+    o.shape.type = type;
+    switch (type) {
+        case KZ_SHAPE_SPHERE:
+            if (!r.bytes(o.shape.params, sizeof(float))) return false;
+            break;
+        case KZ_SHAPE_BOX:
+            if (!r.bytes(o.shape.params, 3 * sizeof(float))) return false;
+            break;
+        case KZ_SHAPE_CAPSULE:
+            if (!r.bytes(o.shape.params, 2 * sizeof(float))) return false;
+            break;
+        case KZ_SHAPE_PLANE:
+            if (!r.bytes(o.shape.params, 4 * sizeof(float))) return false;
+            break;
+        case KZ_SHAPE_CONVEX_HULL: {
+            uint32_t n = 0;
+            if (!r.get(n) || n > 100000u) return false;
+            o.points.resize(3 * size_t(n));
+            if (!r.bytes(o.points.data(), sizeof(float) * o.points.size())) return false;
+            o.shape.hull_count = n;
+            break;
+        }
+        default:
+            return false;  // cylinder and other shapes are outside the device path
+    }
+    kz_body_desc& d = o.body;
+    uint8_t motion = 0, quality = 0, sleeping = 0, sensor = 0, active = 1;
+    float ccd = 0.0f;
+    uint32_t layer = 0, mask = 0;
+    uint64_t user = 0;
+    if (!r.bytes(d.position, sizeof(d.position)) || !r.bytes(d.rotation, sizeof(d.rotation)) ||
+        !r.bytes(d.linear_velocity, sizeof(d.linear_velocity)) ||
+        !r.bytes(d.angular_velocity, sizeof(d.angular_velocity)) || !r.get(motion) || !r.get(quality) ||
+        !r.get(ccd) || !r.get(d.friction) || !r.get(d.restitution) || !r.get(d.linear_damping) ||
+        !r.get(d.angular_damping) || !r.get(d.gravity_factor) || !r.get(d.max_linear_velocity) ||
+        !r.get(d.max_angular_velocity) || !r.get(layer) || !r.get(mask) || !r.get(sleeping) || !r.get(sensor) ||
+        !r.get(user) || !r.get(d.mass_override) || !r.get(d.density) || !r.get(active))
+        return false;
+    d.motion_type = motion;
+    d.allow_sleeping = sleeping;
+    d.is_sensor = sensor;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kzw_save(kzw_world* w, uint8_t* out, uint64_t cap, uint64_t* size) {
+    if (!w || !size) return KZG_ERR_ARGUMENT;
+    Writer wr;
+    const uint32_t nb = uint32_t(w->bodies.size());
+    wr.put(kKzp1Magic);
+    wr.put(uint32_t(1));
+    wr.put(nb);
+    for (uint32_t i = 0; i < nb; ++i) write_kzp1_body(wr, *w, i);
+    wr.put(kKzx1Magic);
+    wr.put(uint32_t(1));
+    size_t blk = wr.begin_block(tag4("WSET"));
+    wr.put(w->settings);
+    wr.end_block(blk);
+    blk = wr.begin_block(tag4("SHAP"));
+    wr.put(uint32_t(w->shapes.size()));
+    for (const auto& s : w->shapes) {
+        wr.put(s.desc);
+        wr.put(uint32_t(s.hull.verts.size()));
+        wr.put(uint32_t(s.hull.tris.size()));
+        wr.bytes(s.hull.verts.data(), sizeof(f3) * s.hull.verts.size());
+        wr.bytes(s.hull.tris.data(), sizeof(uint32_t) * s.hull.tris.size());
+        wr.put(s.hull.bounds);
+        wr.put(s.hull.bounding_radius);
+        wr.put(s.local);
+    }
+    wr.end_block(blk);
+    blk = wr.begin_block(tag4("BODY"));
+    wr.put(nb);
+    wr.bytes(w->bodies.data(), sizeof(kz_body_desc) * nb);
+    wr.bytes(w->states.data(), sizeof(kz_body_state) * nb);
+    wr.end_block(blk);
+    blk = wr.begin_block(tag4("BPHT"));
+    std::vector<uint32_t> tree;
+    w->bp.save(tree);
+    wr.put(uint64_t(tree.size()));
+    wr.bytes(tree.data(), sizeof(uint32_t) * tree.size());
+    wr.bytes(w->proxy.data(), sizeof(uint32_t) * nb);
+    wr.end_block(blk);
+    if (w->ctx) {
+        uint32_t ccap = 0, used = 0;
+        int rc = kzg_cache_info(w->ctx, &ccap, &used);
+        if (rc) return wfail(w, rc, kzg_last_error(w->ctx));
+        if (ccap) {
+            blk = wr.begin_block(tag4("CACH"));
+            wr.put(ccap);
+            wr.put(used);
+            const size_t at = wr.b.size();
+            wr.b.resize(at + size_t(ccap) * KZG_CACHE_SLOT_BYTES);
+            rc = kzg_download_cache(w->ctx, wr.b.data() + at, uint64_t(ccap) * KZG_CACHE_SLOT_BYTES);
+            if (rc) return wfail(w, rc, kzg_last_error(w->ctx));
+            wr.end_block(blk);
+        }
+    }
+    wr.put(tag4("END "));
+    wr.put(uint64_t(0));
+    *size = wr.b.size();
+    if (out) {
+        if (cap < wr.b.size()) return wfail(w, KZG_ERR_ARGUMENT, "snapshot buffer too small");
+        std::memcpy(out, wr.b.data(), wr.b.size());
+    }
+    return KZG_OK;
+}
+
+int kzw_load(kzw_world** out, const uint8_t* data, uint64_t size, const kz_world_desc* settings, int device) {
+    if (!out || !data) return KZG_ERR_ARGUMENT;
+    *out = nullptr;
+    Reader r{data, size_t(size)};
+    uint32_t magic = 0, version = 0, nb = 0;
+    if (!r.get(magic) || magic != kKzp1Magic || !r.get(version) || version != 1 || !r.get(nb))
+        return KZG_ERR_ARGUMENT;
+    std::vector<Kzp1Body> kb(nb);
+    for (auto& b : kb)
+        if (!read_kzp1_body(r, b)) return KZG_ERR_UNSUPPORTED;
+    // Extension blocks, if any.
+    const uint8_t *wset = nullptr, *shap = nullptr, *body = nullptr, *bpht = nullptr, *cach = nullptr;
+    size_t lshap = 0, lbody = 0, lbpht = 0, lcach = 0;
+    uint32_t xm = 0, xv = 0;
+    if (r.get(xm) && xm == kKzx1Magic && r.get(xv) && xv == 1) {
+        for (;;) {
+            uint32_t tag = 0;
+            uint64_t len = 0;
+            if (!r.get(tag) || !r.get(len) || r.off + len > r.n) return KZG_ERR_ARGUMENT;
+            const uint8_t* p = r.p + r.off;
+            if (tag == tag4("END ")) break;
+            if (tag == tag4("WSET") && len == sizeof(kz_world_desc)) wset = p;
+            else if (tag == tag4("SHAP")) { shap = p; lshap = len; }
+            else if (tag == tag4("BODY")) { body = p; lbody = len; }
+            else if (tag == tag4("BPHT")) { bpht = p; lbpht = len; }
+            else if (tag == tag4("CACH")) { cach = p; lcach = len; }
+            r.off += size_t(len);
+        }
+    }
+    kz_world_desc ws{};
+    if (wset) std::memcpy(&ws, wset, sizeof(ws));
+    else if (settings) ws = *settings;
+    else return KZG_ERR_ARGUMENT;  // a plain KZP1 record carries no world settings
+    kzw_world* w = nullptr;
+    int rc = kzw_create(&w, &ws, device);
+    if (rc) return rc;
+    auto bad = [&](int code) {
+        kzw_destroy(w);
+        return code;
+    };
+    std::vector<kz_body_desc> descs(nb);
+    std::vector<kz_body_state> states(nb);
+    if (shap && body) {
+        // Exact tables: shapes with their built hulls, body descriptions and states.
+        Reader s{shap, lshap};
+        uint32_t ns = 0;
+        if (!s.get(ns)) return bad(KZG_ERR_ARGUMENT);
+        for (uint32_t i = 0; i < ns; ++i) {
+            kzw_world::ShapeRec rec;
+            uint32_t nv = 0, nt = 0;
+            if (!s.get(rec.desc) || !s.get(nv) || !s.get(nt) || nv > 256 || nt > 3 * 512 || nt % 3)
+                return bad(KZG_ERR_ARGUMENT);
+            rec.hull.verts.resize(nv);
+            rec.hull.tris.resize(nt);
+            if (!s.bytes(rec.hull.verts.data(), sizeof(f3) * nv) || !s.bytes(rec.hull.tris.data(), sizeof(uint32_t) * nt) ||
+                !s.get(rec.hull.bounds) || !s.get(rec.hull.bounding_radius) || !s.get(rec.local))
+                return bad(KZG_ERR_ARGUMENT);
+            for (uint32_t t : rec.hull.tris)
+                if (t >= nv) return bad(KZG_ERR_ARGUMENT);
+            w->shapes.push_back(std::move(rec));
+        }
+        w->shapes_dirty = true;
+        Reader bb{body, lbody};
+        uint32_t nb2 = 0;
+        if (!bb.get(nb2) || nb2 != nb || !bb.bytes(descs.data(), sizeof(kz_body_desc) * nb) ||
+            !bb.bytes(states.data(), sizeof(kz_body_state) * nb))
+            return bad(KZG_ERR_ARGUMENT);
+    } else {
+        // Plain KZP1: one shape per body, hulls rebuilt from their points as
+        // ConvexHullShape::Create builds them.
+        for (uint32_t i = 0; i < nb; ++i) {
+            kz_shape_desc sd = kb[i].shape;
+            sd.hull_offset = 0;
+            rc = kzw_add_shapes(w, &sd, 1, kb[i].points.data(), sd.hull_count);
+            if (rc) return bad(rc);
+            descs[i] = kb[i].body;
+            descs[i].shape = i;
+            std::memcpy(states[i].position, descs[i].position, sizeof(states[i].position));
+            std::memcpy(states[i].rotation, descs[i].rotation, sizeof(states[i].rotation));
+            std::memcpy(states[i].linear_velocity, descs[i].linear_velocity, sizeof(states[i].linear_velocity));
+            std::memcpy(states[i].angular_velocity, descs[i].angular_velocity, sizeof(states[i].angular_velocity));
+        }
+    }
+    if ((rc = kzw_add_bodies(w, descs.data(), nb))) return bad(rc);
+    std::memcpy(w->states.data(), states.data(), sizeof(kz_body_state) * nb);
+    if (bpht) {
+        Reader t{bpht, lbpht};
+        uint64_t words = 0;
+        if (!t.get(words) || t.off + words * 4 + size_t(nb) * 4 != t.n) return bad(KZG_ERR_ARGUMENT);
+        if (!w->bp.restore(reinterpret_cast<const uint32_t*>(bpht + t.off), size_t(words)))
+            return bad(KZG_ERR_ARGUMENT);
+        std::memcpy(w->proxy.data(), bpht + t.off + words * 4, sizeof(uint32_t) * nb);
+    }
+    if (w->ctx) {
+        if ((rc = sync_device(w))) return bad(rc);
+        if (cach) {
+            Reader c{cach, lcach};
+            uint32_t ccap = 0, used = 0;
+            if (!c.get(ccap) || !c.get(used) || c.off + uint64_t(ccap) * KZG_CACHE_SLOT_BYTES != c.n)
+                return bad(KZG_ERR_ARGUMENT);
+            rc = kzg_upload_cache(w->ctx, cach + c.off, uint64_t(ccap) * KZG_CACHE_SLOT_BYTES, ccap, used);
+            if (rc) return bad(rc);
+        }
+    }
+    *out = w;
+    return KZG_OK;
+}
+
+}  // extern "C"

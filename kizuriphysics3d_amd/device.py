@@ -38,7 +38,8 @@ EXPORTS = [
     "kzg_narrowphase", "kzg_build_constraints", "kzg_integrate_velocities", "kzg_solve", "kzg_integrate_positions",
     "kzg_step", "kzg_query_sizes", "kzg_download_states", "kzg_download_manifolds", "kzg_download_constraints",
     "kzg_get_residuals", "kzg_get_timings", "kzg_collide_batch", "kzg_schedule", "kzg_set_warm_start",
-    "kzg_set_iterations", "kzg_set_cooperative", "kzg_mlp_init", "kzg_mlp_set_params", "kzg_mlp_get_params", "kzg_mlp_set_normalization",
+    "kzg_set_iterations", "kzg_set_cooperative", "kzg_cache_info", "kzg_download_cache", "kzg_upload_cache",
+    "kzg_mlp_init", "kzg_mlp_set_params", "kzg_mlp_get_params", "kzg_mlp_set_normalization",
     "kzg_mlp_get_normalization", "kzg_mlp_collect", "kzg_mlp_num_samples", "kzg_mlp_get_samples",
     "kzg_mlp_fit_normalization", "kzg_mlp_train", "kzg_mlp_forward", "kzg_mlp_gradients",
 ]
@@ -78,6 +79,8 @@ def library():
             "kzg_schedule": ([u32, vp, u32, vp, i32, u32, vp, vp, vp, vp], i32),
             "kzg_set_warm_start": ([vp, i32], i32), "kzg_set_iterations": ([vp, u32, u32], i32),
             "kzg_set_cooperative": ([vp, u32], i32),
+            "kzg_cache_info": ([vp, vp, vp], i32), "kzg_download_cache": ([vp, vp, C.c_uint64], i32),
+            "kzg_upload_cache": ([vp, vp, C.c_uint64, u32, u32], i32),
             "kzg_mlp_init": ([vp, C.c_uint64], i32), "kzg_mlp_set_params": ([vp, vp, u32], i32),
             "kzg_mlp_get_params": ([vp, vp, u32], i32), "kzg_mlp_set_normalization": ([vp, vp, vp], i32),
             "kzg_mlp_get_normalization": ([vp, vp, vp], i32), "kzg_mlp_collect": ([vp, u32], i32),

@@ -38,6 +38,7 @@ def _lib():
             "kzw_get_pairs": ([vp, vp, u32], i32), "kzw_get_stats": ([vp, vp], i32),
             "kzw_get_mass": ([vp, vp, u32], i32), "kzw_get_hull": ([vp, u32, vp, vp, vp, vp], i32),
             "kzw_device": ([vp], vp),
+            "kzw_save": ([vp, vp, C.c_uint64, vp], i32), "kzw_load": ([vp, vp, C.c_uint64, vp, i32], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -49,7 +50,7 @@ def _lib():
 
 KZW_EXPORTS = ["kzw_create", "kzw_destroy", "kzw_last_error", "kzw_add_shapes", "kzw_add_bodies", "kzw_step",
                "kzw_update_broadphase", "kzw_get_states", "kzw_set_states", "kzw_get_pairs", "kzw_get_stats",
-               "kzw_get_mass", "kzw_get_hull", "kzw_device"]
+               "kzw_get_mass", "kzw_get_hull", "kzw_device", "kzw_save", "kzw_load"]
 
 
 def _p(a):
@@ -59,8 +60,12 @@ def _p(a):
 class PhysicsWorld:
     """A world built from a scenes.Scene; the contact path runs on the GPU."""
 
-    def __init__(self, scene: S.Scene, device: int = 0):
+    def __init__(self, scene: S.Scene | None, device: int = 0):
         L = _lib()
+        if scene is None:  # filled in by load()
+            self.h = None
+            self.device = device
+            return
         self._world = np.array(scene.world, dtype=S.WORLD_DTYPE)
         h = C.c_void_p()
         rc = L.kzw_create(C.byref(h), _p(self._world), device)
@@ -75,6 +80,33 @@ class PhysicsWorld:
         self._check(L.kzw_add_bodies(self.h, _p(bodies), len(bodies)), "kzw_add_bodies")
         self.num_shapes = len(shapes)
         self.num_bodies = len(bodies)
+
+    def save(self) -> bytes:
+        """Snapshot (kzw_save): KZP1 world record + resumable extension."""
+        size = C.c_uint64()
+        self._check(_lib().kzw_save(self.h, None, 0, C.byref(size)), "kzw_save")
+        buf = np.zeros(size.value, np.uint8)
+        self._check(_lib().kzw_save(self.h, _p(buf), size.value, C.byref(size)), "kzw_save")
+        return buf.tobytes()
+
+    @classmethod
+    def load(cls, data: bytes, device: int = 0, world: np.ndarray | None = None) -> "PhysicsWorld":
+        """World from a snapshot or a plain KZP1 record (then `world`, a
+        WORLD_DTYPE record, gives the settings)."""
+        L = _lib()
+        buf = np.frombuffer(data, np.uint8)
+        ws = None if world is None else np.array(world, dtype=S.WORLD_DTYPE)
+        h = C.c_void_p()
+        rc = L.kzw_load(C.byref(h), _p(buf), len(buf), None if ws is None else _p(ws), device)
+        if rc != 0:
+            raise D.KzgError(f"kzw_load failed ({rc})")
+        w = cls(None, device)
+        w.h = h
+        # body count from the record header (KZP1: magic, version, count)
+        w.num_bodies = int(np.frombuffer(data[8:12], np.uint32)[0])
+        w.num_shapes = None
+        w._world = ws
+        return w
 
     def close(self):
         if getattr(self, "h", None):

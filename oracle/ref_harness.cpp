@@ -180,6 +180,45 @@ void* kzr_world_create(const kz_world_desc* wd, const kz_shape_desc* shapes, uin
 
 void kzr_world_destroy(void* w) { delete static_cast<RefWorld*>(w); }
 
+// SerializeWorld (src/Serialization/Serializer.cpp:182-205) into out (cap
+// bytes); returns the record size, 0 on failure (out may be null to query).
+uint64_t kzr_save_kzp1(void* wp, uint8_t* out, uint64_t cap) {
+    Vector<u8> buf;
+    if (!SerializeWorld(static_cast<RefWorld*>(wp)->world, buf)) return 0;
+    if (out && cap >= buf.Size()) std::memcpy(out, buf.Data(), buf.Size());
+    return uint64_t(buf.Size());
+}
+
+// A fresh world with settings wd filled by DeserializeWorld
+// (src/Serialization/Serializer.cpp:207-234).
+void* kzr_load_kzp1(const kz_world_desc* wd, const uint8_t* data, uint64_t size) {
+    auto* rw = new RefWorld();
+    WorldSettings& ws = rw->world.GetSettings();
+    ws.gravity = Vec3(wd->gravity[0], wd->gravity[1], wd->gravity[2]);
+    ws.solver.velocityIterations = wd->velocity_iterations;
+    ws.solver.positionIterations = wd->position_iterations;
+    ws.solver.baumgarte = wd->baumgarte;
+    ws.solver.penetrationSlop = wd->penetration_slop;
+    ws.solver.restitutionThreshold = wd->restitution_threshold;
+    ws.solver.maxBiasVelocity = wd->max_bias_velocity;
+    ws.solver.warmStartFactor = wd->warm_start_factor;
+    ws.solver.useSIMDSolver = wd->use_simd_solver != 0;
+    ws.solver.useVelocityBias = wd->use_velocity_bias != 0;
+    ws.allowSleeping = wd->allow_sleeping != 0;
+    ws.useMultithreading = wd->use_multithreading != 0;
+    ws.fixedTimestep = wd->fixed_timestep;
+    if (!DeserializeWorld(rw->world, data, size_t(size))) {
+        delete rw;
+        return nullptr;
+    }
+    const BodyManager& bm = rw->world.GetBodyManager();
+    for (u32 i = 0; i < bm.GetBodySlotCount(); ++i) {
+        const Body* b = bm.GetBodyBySlot(i);
+        if (b) rw->ids.push_back(b->GetID());
+    }
+    return rw;
+}
+
 // One PhysicsWorld::StepFixed.
 void kzr_step(void* w, float dt) { static_cast<RefWorld*>(w)->world.StepFixed(dt); }
 

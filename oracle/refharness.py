@@ -50,6 +50,8 @@ def lib():
         L.kzr_hull_create.argtypes = [vp, u32, vp, vp, vp, vp]; L.kzr_hull_create.restype = i32
         L.kzr_collide_batch.argtypes = [vp, u32, vp, vp, vp, vp, vp, u32, f32, vp, vp, i32]
         L.kzr_collide_batch.restype = f64
+        L.kzr_save_kzp1.argtypes = [vp, vp, C.c_uint64]; L.kzr_save_kzp1.restype = C.c_uint64
+        L.kzr_load_kzp1.argtypes = [vp, vp, C.c_uint64]; L.kzr_load_kzp1.restype = vp
         _lib = L
     return _lib
 
@@ -65,8 +67,11 @@ def init_jobs(workers: int = 0) -> int:
 class RefWorld:
     """A reference PhysicsWorld built from a scenes.Scene."""
 
-    def __init__(self, scene: S.Scene, world: np.ndarray | None = None):
+    def __init__(self, scene: S.Scene | None, world: np.ndarray | None = None):
         L = lib()
+        if scene is None:  # filled in by load_kzp1()
+            self.h = None
+            return
         self._world = np.array(scene.world if world is None else world, dtype=S.WORLD_DTYPE)
         self._shapes = scene.shape_array()
         self._hull = np.ascontiguousarray(scene.hull_array(), dtype=np.float32)
@@ -76,6 +81,29 @@ class RefWorld:
         if not self.h:
             raise ValueError("reference world creation failed")
         self.n = len(self._bodies)
+
+    def save_kzp1(self) -> bytes:
+        """The reference's SerializeWorld record of this world."""
+        n = lib().kzr_save_kzp1(self.h, None, 0)
+        if n == 0:
+            raise ValueError("SerializeWorld failed")
+        buf = np.zeros(n, np.uint8)
+        lib().kzr_save_kzp1(self.h, _p(buf), n)
+        return buf.tobytes()
+
+    @classmethod
+    def load_kzp1(cls, data: bytes, world: np.ndarray) -> "RefWorld":
+        """A fresh reference world with settings `world` filled by DeserializeWorld."""
+        ws = np.array(world, dtype=S.WORLD_DTYPE)
+        buf = np.frombuffer(data, np.uint8).copy()
+        h = lib().kzr_load_kzp1(_p(ws), _p(buf), len(buf))
+        if not h:
+            raise ValueError("DeserializeWorld failed")
+        rw = cls(None)
+        rw.h = h
+        rw._world = ws
+        rw.n = int(lib().kzr_num_bodies(h))
+        return rw
 
     def close(self):
         if self.h:
