@@ -6,6 +6,8 @@
 #include <cstring>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <cmath>
 
 namespace kz {
@@ -591,42 +593,115 @@ uint32_t BroadPhase::balance(uint32_t ia) {
 }
 
 // QueryPairs (BroadPhase.h:61-104) + UpdatePairs (BroadPhase.cpp:368-381).
+// One step of the reference's self-query (DynamicAABBTree::QueryPairs,
+// include/Kizuri/BroadPhase/BroadPhase.h:62-104, with UpdatePairs' filters,
+// src/BroadPhase/BroadPhase.cpp:368-381): pop the top entry, emit a leaf
+// pair or push the entries to visit.
+void BroadPhase::query_step(std::vector<Entry>& stack, std::vector<uint32_t>& out) const {
+    const Entry e = stack.back();
+    stack.pop_back();
+    const uint32_t n1 = e.n1, n2 = e.n2;
+    const Node& a = nodes_[n1];
+    if (n1 == n2) {
+        if (a.leaf()) return;
+        stack.push_back(Entry{a.child1, a.child1});
+        stack.push_back(Entry{a.child2, a.child2});
+        stack.push_back(Entry{a.child1, a.child2});
+        return;
+    }
+    const Node& b = nodes_[n2];
+    if (!overlaps(a.fat, b.fat)) return;
+    if (a.leaf() && b.leaf()) {
+        if (n1 < n2) {
+            if ((layers_[n1] & masks_[n2]) == 0 || (layers_[n2] & masks_[n1]) == 0) return;
+            if (!overlaps(nodes_[n1].tight, nodes_[n2].tight)) return;
+            out.push_back(n1);
+            out.push_back(n2);
+        }
+        return;
+    }
+    if (a.leaf() || (!b.leaf() && a.height < b.height)) {
+        stack.push_back(Entry{n1, b.child1});
+        stack.push_back(Entry{n1, b.child2});
+    } else {
+        stack.push_back(Entry{a.child1, n2});
+        stack.push_back(Entry{a.child2, n2});
+    }
+}
+
+// The pairs in the reference's order.  The traversal is a depth-first walk
+// with a LIFO stack, so the output is that of a pre-order walk of the
+// traversal tree whose children are visited in reverse push order.  Large
+// trees are split breadth-first while keeping that order: a list of items
+// (an emitted pair, or an entry still to walk) is refined by replacing each
+// entry with what popping it yields, in visiting order, until the list holds
+// enough entries; the entries' subtrees are then walked on host threads and
+// everything is concatenated in list order.
 void BroadPhase::update_pairs() {
     pairs_.clear();
     if (root_ == kNull) return;
     stack_.clear();
     stack_.push_back(Entry{root_, root_});
-    while (!stack_.empty()) {
-        const Entry e = stack_.back();
-        stack_.pop_back();
-        const uint32_t n1 = e.n1, n2 = e.n2;
-        const Node& a = nodes_[n1];
-        if (n1 == n2) {
-            if (a.leaf()) continue;
-            stack_.push_back(Entry{a.child1, a.child1});
-            stack_.push_back(Entry{a.child2, a.child2});
-            stack_.push_back(Entry{a.child1, a.child2});
-            continue;
-        }
-        const Node& b = nodes_[n2];
-        if (!overlaps(a.fat, b.fat)) continue;
-        if (a.leaf() && b.leaf()) {
-            if (n1 < n2) {
-                if ((layers_[n1] & masks_[n2]) == 0 || (layers_[n2] & masks_[n1]) == 0) continue;
-                if (!overlaps(nodes_[n1].tight, nodes_[n2].tight)) continue;
-                pairs_.push_back(n1);
-                pairs_.push_back(n2);
+    const size_t leaves = (nodes_.size() + 1) / 2;
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned threads = std::min(hw ? hw : 1u, 16u);
+    if (leaves < kParallelLeaves || threads < 2) {
+        while (!stack_.empty()) query_step(stack_, pairs_);
+        return;
+    }
+    // Items: an entry to walk (n1, n2), or an emitted pair (pair flag set).
+    struct Item {
+        Entry e;
+        bool pair;
+    };
+    std::vector<Item> list{{Entry{root_, root_}, false}}, next;
+    std::vector<Entry> st;
+    std::vector<uint32_t> out;
+    size_t pending = 1;
+    for (int round = 0; round < 64 && pending > 0 && pending < kSplitEntries; ++round) {
+        next.clear();
+        pending = 0;
+        for (const Item& it : list) {
+            if (it.pair) {
+                next.push_back(it);
+                continue;
             }
-            continue;
+            // One pop of this entry: its output, then the pushed entries in
+            // the order the stack will pop them.
+            st.assign(1, it.e);
+            out.clear();
+            query_step(st, out);
+            for (size_t k = 0; k < out.size(); k += 2) next.push_back(Item{Entry{out[k], out[k + 1]}, true});
+            for (size_t k = st.size(); k-- > 0;) next.push_back(Item{st[k], false});
+            pending += st.size();
         }
-        if (a.leaf() || (!b.leaf() && a.height < b.height)) {
-            stack_.push_back(Entry{n1, b.child1});
-            stack_.push_back(Entry{n1, b.child2});
+        list.swap(next);
+    }
+    std::vector<size_t> todo;
+    for (size_t k = 0; k < list.size(); ++k)
+        if (!list[k].pair) todo.push_back(k);
+    std::vector<std::vector<uint32_t>> outs(list.size());
+    std::atomic<size_t> cursor{0};
+    auto worker = [&] {
+        std::vector<Entry> ws;
+        for (size_t i; (i = cursor.fetch_add(1)) < todo.size();) {
+            ws.assign(1, list[todo[i]].e);
+            while (!ws.empty()) query_step(ws, outs[todo[i]]);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < threads; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+    for (size_t k = 0; k < list.size(); ++k) {
+        if (list[k].pair) {
+            pairs_.push_back(list[k].e.n1);
+            pairs_.push_back(list[k].e.n2);
         } else {
-            stack_.push_back(Entry{a.child1, n2});
-            stack_.push_back(Entry{a.child2, n2});
+            pairs_.insert(pairs_.end(), outs[k].begin(), outs[k].end());
         }
     }
+    stack_.clear();
 }
 
 }  // namespace kz

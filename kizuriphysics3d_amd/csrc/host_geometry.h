@@ -86,6 +86,9 @@ private:
         uint32_t n1, n2;
     };
     std::vector<Entry> stack_;
+    static constexpr size_t kParallelLeaves = 8192;  // smaller trees are walked on the calling thread
+    static constexpr size_t kSplitEntries = 4096;    // entries handed to the threads
+    void query_step(std::vector<Entry>& stack, std::vector<uint32_t>& out) const;
 };
 
 }  // namespace kz

@@ -93,3 +93,21 @@ def test_broadphase_pair_order_matches_reference_lockstep(scene_fn, steps):
         rw.step()
         w.update_broadphase()
         assert np.array_equal(w.pairs(), rw.pairs()), f"pair list differs at step {i}"
+
+
+@pytest.mark.reference
+def test_large_broadphase_pair_order_matches_the_reference():
+    """A tree beyond the threaded-walk threshold (16,384 bodies): the pairs,
+    walked in parallel subtrees and concatenated, come out in the
+    reference's order."""
+    from oracle import refharness as R
+    from kizuriphysics3d_amd.world import PhysicsWorld
+    scene = S.pile(16384, seed=0x4B5A0701, width=40.0, y_range=(1.0, 20.0))
+    R.init_jobs(0)
+    rw = R.RefWorld(scene)
+    ours = PhysicsWorld(scene, device=-1)
+    rw.step_phases(stop_after=0)
+    ours.update_broadphase()
+    rp, op = rw.pairs(), ours.pairs()
+    assert len(rp) > 10000
+    assert np.array_equal(rp, op)
