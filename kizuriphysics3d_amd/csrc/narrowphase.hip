@@ -1080,101 +1080,72 @@ __device__ __forceinline__ int collide_polyhedral(const WShape& A, const WShape&
 
 __device__ __forceinline__ bool polyhedral(const WShape& w) { return w.s.type == kBox || w.s.type == kHull; }
 
-// CollideConvexGeneric (CollideShape.cpp:233-299).  Results in L.out.
-__device__ __forceinline__ int collide_generic(const WShape& A, const WShape& B, const ShapeBuf& sb, float max_sep,
-                                               WaveLds& L, bool have_pre, const Gjk& pre) {
-    const float tol = 1.0e-4f;
-    Gjk g;
-    if (have_pre) g = pre;
-    else gjk(A, B, tol, g);
-    if (!g.intersect) {
-        if (g.distance > max_sep) return 0;
-        if (polyhedral(A) && polyhedral(B) && len2(g.normal) > 0.0f) {
-            const int k = collide_polyhedral(A, B, sb, g.normal, max_sep, L);
-            if (k > 0) return k;
-        }
-        put_contact(L, 0, g.pa, g.pb, g.normal, g.distance, 0);
-        return 1;
-    }
-    if (polyhedral(A) && polyhedral(B)) {
-        Epa e;
-        epa(A, B, g, tol, L, e);
-        if (e.valid) {
-            const int k = collide_polyhedral(A, B, sb, e.normal, max_sep, L);
-            if (k > 0) return k;
-            put_contact(L, 0, e.pa, e.pb, e.normal, -e.depth, 0);
-            return 1;
-        }
-    }
-    Epa e;
-    epa(A, B, g, tol, L, e);
-    if (e.valid) {
-        put_contact(L, 0, e.pa, e.pb, e.normal, -e.depth, 0);
-        return 1;
-    }
-    put_contact(L, 0, g.pa, g.pb, len2(g.normal) > 0.0f ? g.normal : f3{0.0f, 1.0f, 0.0f}, 0.0f, 0);
-    return 1;
-}
-
-// CollideSphereConvex (CollideShape.cpp:304-344): GJK against a 1e-4 proxy
-// sphere at the sphere centre.
-__device__ __forceinline__ int collide_sphere_convex(const WShape& Sph, const WShape& B, float max_sep, WaveLds& L,
-                                                     bool have_pre, const Gjk& pre) {
-    const float tol = 1.0e-4f;
-    WShape P = Sph;
-    P.s.type = kSphere;
-    P.s.prm.x = 1.0e-4f;
-    Gjk g;
-    if (have_pre) g = pre;
-    else gjk(P, B, tol, g);
-    const float r = Sph.s.prm.x;
-    const f3 c = Sph.t;
-    if (g.intersect) {
-        Epa e;
-        epa(P, B, g, tol, L, e);
-        if (e.valid) {
-            const f3 n = neg(e.normal);
-            put_contact(L, 0, add(c, mul(n, r)), e.pb, n, -(e.depth + r), 0);
-            return 1;
-        }
-        return 0;
-    }
-    const float dist = g.distance;
-    const float sep = dist - r;
-    if (sep > max_sep) return 0;
-    const f3 n = dist > kEps ? divs(sub(c, g.pb), dist) : f3{0.0f, 1.0f, 0.0f};
-    put_contact(L, 0, sub(c, mul(n, r)), g.pb, n, sep, 0);
-    return 1;
-}
-
-// `pre` (optional): the GJK result of each work item, computed by
-// k_collide_lanes; the wave kernel then continues from it.
+// The pairs k_collide_lanes hands on, each with its GJK result:
+//   * CollideSphereConvex with an intersecting proxy sphere: EPA between the
+//     proxy and the other shape; no point when EPA fails (:317-330);
+//   * CollideConvexGeneric, separated within maxSeparation, both shapes
+//     polyhedral: clipping with the GJK normal, else the GJK point
+//     (:240-249);
+//   * CollideConvexGeneric, intersecting: EPA, then clipping with the EPA
+//     normal for polyhedral pairs (else, or without clipped points, the
+//     EPA point); when EPA fails, the GJK witness with separation 0
+//     (:261-298).  The reference runs EPA a second time for a polyhedral
+//     pair whose first EPA failed; EPA is a function of the shapes and the
+//     GJK simplex, so that run fails again and is not repeated here.
+// One EPA and one clipping call site keep the kernel's code small.
 __global__ void __launch_bounds__(64) k_collide_wave(const uint2* pairs, const uint32_t* work,
                                                      const uint32_t* work_count, BodyBuf b, ShapeBuf sb,
                                                      ManifoldBuf m, float max_sep, const GjkRec* pre) {
     __shared__ WaveLds L;
     const uint32_t total = *work_count;
+    const float tol = 1.0e-4f;
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
-        Gjk pg{};
-        if (pre) pg = load_gjk(pre[item]);
+        const Gjk g = load_gjk(pre[item]);
         const uint32_t wv = work[item];
         const uint32_t pi = wv & 0x7FFFFFFFu;
         const bool swapped = (wv >> 31) != 0u;
         uint32_t ia = pairs[pi].x, ib = pairs[pi].y;
         if (swapped) { uint32_t t = ia; ia = ib; ib = t; }
-        WShape A, B;
-        load_wshape(A, sb, meta_shape(b.meta[ia]), b.x[ia], b.q[ia]);
-        load_wshape(B, sb, meta_shape(b.meta[ib]), b.x[ib], b.q[ib]);
-        int k;
+        const uint32_t sa = meta_shape(b.meta[ia]), sbi = meta_shape(b.meta[ib]);
+        const uint32_t ta = sb.type[sa], tb = sb.type[sbi];
+        const bool sph_a = ta == kSphere, sph_b = !sph_a && tb == kSphere, sphere = sph_a || sph_b;
+        const bool both_poly = (ta == kBox || ta == kHull) && (tb == kBox || tb == kHull);
+        // GJK operands: the sphere side (as the 1e-4 proxy) first, else A, B.
+        const uint32_t ix = sph_b ? ib : ia, iy = sph_b ? ia : ib;
+        WShape X, Y;
+        load_wshape(X, sb, sph_b ? sbi : sa, b.x[ix], b.q[ix]);
+        load_wshape(Y, sb, sph_b ? sa : sbi, b.x[iy], b.q[iy]);
+        const float r = X.s.prm.x;
+        if (sphere) {
+            X.s.type = kSphere;
+            X.s.prm.x = 1.0e-4f;
+        }
+        int k = 0;
         bool swap_result = swapped;
-        const bool hp = pre != nullptr;
-        if (A.s.type == kSphere) {
-            k = collide_sphere_convex(A, B, max_sep, L, hp, pg);
-        } else if (B.s.type == kSphere) {
-            k = collide_sphere_convex(B, A, max_sep, L, hp, pg);
-            if (k > 0) swap_result = !swap_result;
-        } else {
-            k = collide_generic(A, B, sb, max_sep, L, hp, pg);
+        Epa e;
+        e.valid = false;
+        const bool run_epa = g.intersect;
+        if (run_epa) epa(X, Y, g, tol, L, e);
+        const bool clip = !sphere && both_poly && ((!g.intersect && len2(g.normal) > 0.0f) || (g.intersect && e.valid));
+        if (clip) k = collide_polyhedral(X, Y, sb, g.intersect ? e.normal : g.normal, max_sep, L);
+        if (k == 0) {
+            if (sphere) {
+                if (e.valid) {
+                    const f3 n = neg(e.normal);
+                    put_contact(L, 0, add(X.t, mul(n, r)), e.pb, n, -(e.depth + r), 0);
+                    k = 1;
+                    if (sph_b) swap_result = !swap_result;
+                }
+            } else if (!g.intersect) {
+                put_contact(L, 0, g.pa, g.pb, g.normal, g.distance, 0);
+                k = 1;
+            } else if (e.valid) {
+                put_contact(L, 0, e.pa, e.pb, e.normal, -e.depth, 0);
+                k = 1;
+            } else {
+                put_contact(L, 0, g.pa, g.pb, len2(g.normal) > 0.0f ? g.normal : f3{0.0f, 1.0f, 0.0f}, 0.0f, 0);
+                k = 1;
+            }
         }
         __syncthreads();
         // Lane j writes result point j.
