@@ -330,6 +330,8 @@ def main():
     ap.add_argument("--learn-batch", type=int, default=16384)
     ap.add_argument("--cooperative", type=int, default=0,
                     help="workgroups sharing the largest island's levels (0 auto, 1 one per group)")
+    ap.add_argument("--units-vel", type=int, default=0, help="most points per velocity sweep unit (0: default)")
+    ap.add_argument("--units-ws", type=int, default=0, help="most points per warm-start/bias sweep unit (0: default)")
     ap.add_argument("--workload", choices=("pile", "batched", "narrowphase", "copy"), default="pile")
     ap.add_argument("--copy-mib", type=int, default=1024, help="copy workload: buffer size")
     ap.add_argument("--worlds", type=int, default=4096, help="batched workload: number of worlds")
@@ -359,6 +361,7 @@ def main():
     learn = not args.no_learned and args.learn_collect > 0
     ctx = w.device_context()
     ctx.set_cooperative(args.cooperative)
+    ctx.set_sweep_units(args.units_vel, args.units_ws)
     if learn:
         ctx.mlp_init(0x4B5A0005)
     for i in range(args.warmup):
@@ -392,7 +395,7 @@ def main():
     coop_bytes = 0.0
     coop_blocks = set()
     levels = []
-    phases = np.zeros(8)
+    phases = np.zeros(10)
     t_start = time.perf_counter()
     for i in range(args.steps):
         w.step_fixed()
@@ -411,7 +414,8 @@ def main():
                            + tm["cooperative_vel_points"] * 8 * VEL_BYTES)
         levels.append((tm["velocity_levels"], tm["bias_levels"], tm["groups"], tm["islands"]))
         phases += [tm["narrowphase_ms"], tm["build_ms"], tm["integrate_velocities_ms"], tm["solve_ms"],
-                   tm["solve_kernel_ms"], st["broadphase_ms"], tm["schedule_ms"], tm["build_device_ms"]]
+                   tm["solve_kernel_ms"], st["broadphase_ms"], tm["schedule_ms"], tm["build_device_ms"],
+                   tm["schedule_ws_ms"], tm["schedule_wait_ms"]]
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     # value: the path's phases with inputs in HBM (kernel times, host build
@@ -435,7 +439,8 @@ def main():
         f"groups {lv[:, 2].mean():.0f}, islands {lv[:, 3].mean():.0f}")
     ph = phases / args.steps
     log(f"per step (ms): narrowphase {ph[0]:.2f}, build on the device {ph[7]:.2f} (compaction, islands), "
-        f"build on the host {ph[1]:.2f} (schedule {ph[6]:.2f}), integrate velocities "
+        f"build on the host {ph[1]:.2f} (schedule {ph[6]:.2f}: warm-start order {ph[8]:.2f} beside, "
+        f"waited {ph[9]:.2f}), integrate velocities "
         f"{ph[2]:.2f}, solve {ph[3]:.2f} (sweep kernel {ph[4]:.2f}), host broadphase {ph[5]:.2f}")
     if args.dump:
         import ctypes as C

@@ -86,6 +86,9 @@ typedef struct kzg_timings {
     float cooperative_kernel_ms;
     float schedule_ms;       /* the solve schedule alone (host, part of build_ms) */
     float build_device_ms;   /* device part of the build before the host schedule: compaction, islands */
+    /* Inside schedule_ms: the warm-start / bias sequence (built on a helper
+     * thread), and how long the velocity sequence then waited for it. */
+    float schedule_ws_ms, schedule_wait_ms;
 } kzg_timings;
 
 /* Create a context on HIP device `device` for a world with settings `world`. */
@@ -176,6 +179,13 @@ int kzg_set_iterations(kzg_ctx* ctx, uint32_t velocity, uint32_t position);
  * workgroup per group, n = n workgroups (at most 64).  Results are the
  * same whatever the setting: a level's points share no dynamic body. */
 int kzg_set_cooperative(kzg_ctx* ctx, uint32_t blocks);
+/* Most points of one body pair a sweep thread applies in a row (a unit), in
+ * the velocity and in the warm-start / bias sweeps: 0 = default (2 each),
+ * 1..4.  A point whose dynamic bodies were last updated by the same point of
+ * the same pair joins that point's unit instead of starting a new level,
+ * which shortens the level chain; each unit costs its points' arithmetic in
+ * series.  Results are the same whatever the setting. */
+int kzg_set_sweep_units(kzg_ctx* ctx, uint32_t velocity, uint32_t warm_start);
 
 /* ---- Learned warm start (no reference counterpart) ---------------------
  * A per-contact-point MLP 14 -> 64 -> 64 -> 3 with ReLU hidden layers,
@@ -227,15 +237,24 @@ int kzg_mlp_gradients(kzg_ctx* ctx, const float* samples, uint32_t n, float* gra
 /* Host only (no GPU): the solve schedule kzg_solve would use for a contact
  * table, for tests and tuning.  info: 4 u32 per constraint (body A, body B,
  * first point, num points | sensor << 8), points numbered 0..P-1 in
- * constraint order.  Outputs per point: vel_level (velocity sweep level,
- * 0 = not swept) and ws_level (warm-start / bias level), levels counted from
- * 1 within the point's group; group[p] = its group.  stats = islands, groups,
- * deepest velocity group, deepest warm-start group.  Islands and the
- * first-fit batch order are those of PhysicsWorld::BuildIslands and
- * SolveIsland (src/World/PhysicsWorld.cpp:587-717). */
+ * constraint order.  flags: KZG_SCHEDULE_SIMD for the first-fit velocity
+ * order (else the constraint order), KZG_SCHEDULE_UNITS to join points into
+ * multi-point units with kzg_solve's default caps (else one point per
+ * unit).  Outputs
+ * per point (each optional): vel_level (velocity sweep level, 0 = not
+ * swept) and ws_level (warm-start / bias level), levels counted from 1
+ * within the point's group; group[p] = its group; vel_unit / ws_unit = the
+ * unit holding the point (units numbered in sweep order; the points of a
+ * unit run in point order).  stats (6) = islands, groups, deepest velocity
+ * group, deepest warm-start group, velocity units, warm-start units.
+ * Islands and the first-fit batch order are those of
+ * PhysicsWorld::BuildIslands and SolveIsland
+ * (src/World/PhysicsWorld.cpp:587-717). */
+#define KZG_SCHEDULE_SIMD 1u
+#define KZG_SCHEDULE_UNITS 2u
 int kzg_schedule(uint32_t num_bodies, const uint8_t* dynamic, uint32_t num_constraints, const uint32_t* info,
-                 int use_simd, uint32_t max_groups, uint32_t* vel_level, uint32_t* ws_level, uint32_t* group,
-                 uint32_t* stats);
+                 uint32_t flags, uint32_t max_groups, uint32_t* vel_level, uint32_t* ws_level, uint32_t* group,
+                 uint32_t* vel_unit, uint32_t* ws_unit, uint32_t* stats);
 
 #ifdef __cplusplus
 }

@@ -711,8 +711,8 @@ void enqueue_step(kzb_batch* B, hipEvent_t* ev) {
                            ws.use_velocity_bias ? 1 : 0, c->vel_records(), c->ws_records(), c->vel_pos.p, c->ws_pos.p);
     mark(4);
     launch_solve_groups(s, c->vel_records(), c->ws_records(), c->bodies(), nb, c->b_packed_solver.p,
-                        Schedule{c->ws_off.p, c->ws_grp.p}, Schedule{c->vel_off.p, c->vel_grp.p}, sp, W, threads,
-                        c->resid.p);
+                        Schedule{c->ws_off.p, c->ws_grp.p, nullptr}, Schedule{c->vel_off.p, c->vel_grp.p, nullptr},
+                        sp, W, threads, c->resid.p, UnitKernel{});
     mark(5);
     launch_writeback_counted(s, c->vel_records(), c->ws_records(), c->contacts(), c->vel_pos.p, c->ws_pos.p, pts,
                              B->cnt.p + kCntPts);

@@ -24,24 +24,25 @@ struct CoopPlan {
 };
 
 // Workgroups and threads for group 0 (the group holding the largest
-// island).  `request` 0 sizes the split from the group's mean level width.
-CoopPlan plan_cooperative(const ScheduleOutput& s, uint32_t request) {
+// island).  `request` 0 sizes the split from the group's mean level width;
+// `max_threads` is the kernel's workgroup limit.
+CoopPlan plan_cooperative(const ScheduleOutput& s, uint32_t request, uint32_t max_threads) {
     CoopPlan p;
     if (request == 1 || s.groups == 0 || s.ws.grp.empty()) return p;
-    uint32_t points = 0, levels = 0, widest = 0;
+    uint32_t units = 0, levels = 0, widest = 0;
     for (const LevelSchedule* ls : {&s.ws, &s.vel}) {
         if (ls->grp.empty()) continue;
         const U2 g = ls->grp[0];
         for (uint32_t l = g.x; l < g.y; ++l) widest = std::max(widest, ls->off[l + 1] - ls->off[l]);
-        points += ls->off[g.y] - ls->off[g.x];
+        units += ls->off[g.y] - ls->off[g.x];
         levels += g.y - g.x;
-        (ls == &s.ws ? p.ws_points : p.vel_points) = ls->off[g.y] - ls->off[g.x];
+        (ls == &s.ws ? p.ws_points : p.vel_points) = ls->unit[ls->off[g.y]] - ls->unit[ls->off[g.x]];
     }
     if (!levels) return CoopPlan{};
-    p.blocks = request ? request : std::min(kAutoCoopBlocks, std::max(1u, points / levels / kCoopSlice));
+    p.blocks = request ? request : std::min(kAutoCoopBlocks, std::max(1u, units / levels / kCoopSlice));
     if (p.blocks <= 1) return CoopPlan{};
     const uint32_t slice = (widest + p.blocks - 1) / p.blocks;
-    while (p.threads < slice && p.threads < 1024) p.threads *= 2;
+    while (p.threads < slice && p.threads < max_threads) p.threads *= 2;
     return p;
 }
 
@@ -83,8 +84,8 @@ void kzg_destroy(kzg_ctx* c) {
                     &c->wr_qa, &c->wr_qb, &c->wr_n, &c->wr_t1, &c->wr_t2, &c->wr_imp})
         d->release();
     for (auto* d : {&c->b_meta, &c->b_packed, &c->s_type, &c->s_htris, &c->m_count, &c->work, &c->work_count,
-                    &c->c_pair, &c->ws_off, &c->vel_off, &c->ws_pos, &c->vel_pos, &c->cache_count, &c->cache_used,
-                    &c->x_meta, &c->x_packed})
+                    &c->c_pair, &c->ws_off, &c->vel_off, &c->ws_pos, &c->vel_pos, &c->ws_unit, &c->vel_unit,
+                    &c->cache_count, &c->cache_used, &c->x_meta, &c->x_packed})
         d->release();
     c->s_hull.release();
     c->gjk.release();
@@ -96,6 +97,8 @@ void kzg_destroy(kzg_ctx* c) {
     c->p_nvel.release();
     c->ws_grp.release();
     c->vel_grp.release();
+    c->ws_udesc.release();
+    c->vel_udesc.release();
     c->d_pairs.release();
     c->x_pairs.release();
     c->cache_key.release();
@@ -349,6 +352,8 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     si.island_of_con = c->h_con_island;
     si.num_islands = islands;
     si.use_simd = c->world.use_simd_solver != 0;
+    si.max_unit_vel = c->unit_cap_vel ? c->unit_cap_vel : kDefaultUnitVel;
+    si.max_unit_ws = c->unit_cap_ws ? c->unit_cap_ws : kDefaultUnitWs;
     si.max_groups = c->cu_count * 4u;
     const auto ts = std::chrono::steady_clock::now();
     c->sched_builder.build(si, c->sched);
@@ -358,8 +363,9 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     c->build_host_ms = std::chrono::duration<double, std::milli>(te - t0).count();
     c->schedule_host_ms = std::chrono::duration<double, std::milli>(te - ts).count();
     const LevelSchedule &ws = c->sched.ws, &vel = c->sched.vel;
-    const size_t bytes = sizeof(uint32_t) * (ws.off.size() + vel.off.size() + 2 * size_t(np)) +
-                         sizeof(U2) * (ws.grp.size() + vel.grp.size()) + 8 * 256;
+    const size_t bytes = sizeof(uint32_t) * (ws.off.size() + vel.off.size() + ws.unit.size() + vel.unit.size() +
+                                             2 * size_t(np)) +
+                         sizeof(U2) * (ws.grp.size() + vel.grp.size()) + 10 * 256;
     KZG_HIP(c, c->stage_b.ensure(bytes));
     size_t off = 0;
     if ((rc = upload_async(c, c->ws_off, c->stage_b, off, ws.off.data(), ws.off.size())) ||
@@ -367,7 +373,9 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
         (rc = upload_async(c, c->ws_pos, c->stage_b, off, ws.pos.data(), ws.pos.size())) ||
         (rc = upload_async(c, c->vel_off, c->stage_b, off, vel.off.data(), vel.off.size())) ||
         (rc = upload_async(c, c->vel_grp, c->stage_b, off, reinterpret_cast<const uint2*>(vel.grp.data()), vel.grp.size())) ||
-        (rc = upload_async(c, c->vel_pos, c->stage_b, off, vel.pos.data(), vel.pos.size())))
+        (rc = upload_async(c, c->vel_pos, c->stage_b, off, vel.pos.data(), vel.pos.size())) ||
+        (rc = upload_async(c, c->ws_unit, c->stage_b, off, ws.unit.data(), ws.unit.size())) ||
+        (rc = upload_async(c, c->vel_unit, c->stage_b, off, vel.unit.data(), vel.unit.size())))
         return rc;
     c->phase = 2;
     return KZG_OK;
@@ -409,12 +417,17 @@ int kzg_solve(kzg_ctx* c, float dt) {
     KZG_HIP(c, hipMemsetAsync(c->resid.p, 0, sizeof(float) * std::max<uint32_t>(w.velocity_iterations, 1), c->stream));
     SolveParams sp{dt, w.warm_start_factor, w.velocity_iterations, w.position_iterations, w.use_simd_solver,
                    w.use_velocity_bias};
-    Schedule ws{c->ws_off.p, c->ws_grp.p};
-    Schedule vel{c->vel_off.p, c->vel_grp.p};
+    KZG_HIP(c, c->ws_udesc.ensure(std::max<uint32_t>(c->sched.ws.units, 1)));
+    KZG_HIP(c, c->vel_udesc.ensure(std::max<uint32_t>(c->sched.vel.units, 1)));
+    launch_unit_desc(c->stream, c->ws_unit.p, c->ws_records().bd, c->sched.ws.units, c->ws_udesc.p);
+    launch_unit_desc(c->stream, c->vel_unit.p, c->vel_records().bd, c->sched.vel.units, c->vel_udesc.p);
+    Schedule ws{c->ws_off.p, c->ws_grp.p, c->ws_udesc.p};
+    Schedule vel{c->vel_off.p, c->vel_grp.p, c->vel_udesc.p};
     const uint32_t width = std::max(c->sched.ws.max_width, c->sched.vel.max_width);
+    const UnitKernel uk = unit_kernel(c->sched.ws.largest_unit, c->sched.vel.largest_unit);
     uint32_t threads = 64;
-    while (threads < width && threads < 1024) threads *= 2;
-    const CoopPlan coop = plan_cooperative(c->sched, c->coop_blocks);
+    while (threads < width && threads < solve_threads(uk.ws, uk.vel)) threads *= 2;
+    const CoopPlan coop = plan_cooperative(c->sched, c->coop_blocks, solve_threads(uk.ws, uk.vel));
     c->coop_used = coop.blocks;
     c->coop_ws_points = coop.ws_points;
     c->coop_vel_points = coop.vel_points;
@@ -425,11 +438,12 @@ int kzg_solve(kzg_ctx* c, float dt) {
             KZG_HIP(c, c->coop_counter.ensure(1));
             KZG_HIP(c, launch_solve_split(c->stream, c->vel_records(), c->ws_records(), c->bodies(), c->nbodies,
                                           c->b_packed_solver.p, ws, vel, sp, c->sched.groups, threads, c->resid.p,
-                                          coop.blocks, coop.threads, c->coop_counter.p, c->ev[10], c->ev[11]));
+                                          uk, coop.blocks, coop.threads, c->coop_counter.p, c->ev[10],
+                                          c->ev[11]));
             c->recorded |= 3u << 10;
         } else {
             launch_solve_groups(c->stream, c->vel_records(), c->ws_records(), c->bodies(), c->nbodies,
-                                c->b_packed_solver.p, ws, vel, sp, c->sched.groups, threads, c->resid.p);
+                                c->b_packed_solver.p, ws, vel, sp, c->sched.groups, threads, c->resid.p, uk);
         }
     }
     KZG_HIP(c, hipGetLastError());
@@ -631,6 +645,8 @@ int kzg_get_timings(kzg_ctx* c, kzg_timings* out) {
     span(2, 3, t.build_kernel_ms);
     t.build_ms = float(c->build_host_ms);
     t.schedule_ms = float(c->schedule_host_ms);
+    t.schedule_ws_ms = float(c->sched_builder.ws_ms);
+    t.schedule_wait_ms = float(c->sched_builder.wait_ms);
     span(8, 4, t.integrate_velocities_ms);
     span(4, 9, t.solve_ms);
     span(5, 6, t.solve_kernel_ms);
@@ -650,8 +666,9 @@ int kzg_get_timings(kzg_ctx* c, kzg_timings* out) {
     return KZG_OK;
 }
 
-int kzg_schedule(uint32_t nb, const uint8_t* dyn, uint32_t nc, const uint32_t* info, int use_simd, uint32_t max_groups,
-                 uint32_t* vel_level, uint32_t* ws_level, uint32_t* group, uint32_t* stats) {
+int kzg_schedule(uint32_t nb, const uint8_t* dyn, uint32_t nc, const uint32_t* info, uint32_t flags,
+                 uint32_t max_groups, uint32_t* vel_level, uint32_t* ws_level, uint32_t* group, uint32_t* vel_unit,
+                 uint32_t* ws_unit, uint32_t* stats) {
     if ((nb && !dyn) || (nc && !info) || !stats) return KZG_ERR_ARGUMENT;
     std::vector<U4> in(nc);
     uint32_t np = 0;
@@ -667,25 +684,31 @@ int kzg_schedule(uint32_t nb, const uint8_t* dyn, uint32_t nc, const uint32_t* i
     si.dynamic = dyn;
     si.num_constraints = nc;
     si.info = in.data();
-    si.use_simd = use_simd != 0;
+    si.use_simd = (flags & KZG_SCHEDULE_SIMD) != 0;
+    si.max_unit_vel = (flags & KZG_SCHEDULE_UNITS) ? kDefaultUnitVel : 1u;
+    si.max_unit_ws = (flags & KZG_SCHEDULE_UNITS) ? kDefaultUnitWs : 1u;
     si.max_groups = std::max<uint32_t>(max_groups, 1);
     b.build(si, out);
-    auto fill = [&](const LevelSchedule& s, uint32_t* lvl, bool with_group) {
-        if (!lvl && !(with_group && group)) return;
+    auto fill = [&](const LevelSchedule& s, uint32_t* lvl, uint32_t* unit, uint32_t* grp) {
         if (lvl) std::fill(lvl, lvl + np, 0u);
+        if (unit) std::fill(unit, unit + np, 0xFFFFFFFFu);
         for (uint32_t g = 0; g < s.grp.size(); ++g)
             for (uint32_t l = s.grp[g].x; l < s.grp[g].y; ++l)
-                for (uint32_t e = s.off[l]; e < s.off[l + 1]; ++e) {
-                    if (lvl) lvl[s.idx[e]] = l - s.grp[g].x + 1;
-                    if (with_group && group) group[s.idx[e]] = g;
-                }
+                for (uint32_t u = s.off[l]; u < s.off[l + 1]; ++u)
+                    for (uint32_t e = s.unit[u]; e < s.unit[u + 1]; ++e) {
+                        if (lvl) lvl[s.idx[e]] = l - s.grp[g].x + 1;
+                        if (unit) unit[s.idx[e]] = u;
+                        if (grp) grp[s.idx[e]] = g;
+                    }
     };
-    fill(out.vel, vel_level, false);
-    fill(out.ws, ws_level, true);
+    fill(out.vel, vel_level, vel_unit, nullptr);
+    fill(out.ws, ws_level, ws_unit, group);
     stats[0] = out.islands;
     stats[1] = out.groups;
     stats[2] = out.vel.max_depth;
     stats[3] = out.ws.max_depth;
+    stats[4] = out.vel.units;
+    stats[5] = out.ws.units;
     return KZG_OK;
 }
 
@@ -756,6 +779,15 @@ int kzg_set_cooperative(kzg_ctx* c, uint32_t blocks) {
     if (!c) return KZG_ERR_ARGUMENT;
     if (blocks > kMaxCoopBlocks) return fail(c, KZG_ERR_ARGUMENT, "more than 64 cooperative workgroups");
     c->coop_blocks = blocks;
+    return KZG_OK;
+}
+
+int kzg_set_sweep_units(kzg_ctx* c, uint32_t velocity, uint32_t warm_start) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (velocity > kMaxUnit || warm_start > kMaxUnit)
+        return fail(c, KZG_ERR_ARGUMENT, "a sweep unit holds at most 4 points");
+    c->unit_cap_vel = velocity;
+    c->unit_cap_ws = warm_start;
     return KZG_OK;
 }
 

@@ -34,11 +34,12 @@ void launch_writeback(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, co
                       const uint32_t* vel_pos, const uint32_t* ws_pos, uint32_t np);
 void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b, uint32_t nbodies,
                          float4* packed, const Schedule& ws, const Schedule& vel, const SolveParams& sp,
-                         uint32_t groups, uint32_t threads, float* resid);
+                         uint32_t groups, uint32_t threads, float* resid, UnitKernel uk);
+void launch_unit_desc(hipStream_t s, const uint32_t* unit, const uint4* bd, uint32_t nu, uint4* desc);
 hipError_t launch_solve_split(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b,
                               uint32_t nbodies, float4* packed, const Schedule& ws, const Schedule& vel,
                               const SolveParams& sp, uint32_t groups, uint32_t threads, float* resid,
-                              uint32_t coop_blocks, uint32_t coop_threads, uint32_t* counter,
+                              UnitKernel uk, uint32_t coop_blocks, uint32_t coop_threads, uint32_t* counter,
                               hipEvent_t coop_begin, hipEvent_t coop_end);
 void launch_compact(hipStream_t s, const uint32_t* m_count, uint32_t n, const uint2* pairs, const BodyBuf& b,
                     const ContactBuf& c, uint2* tile, uint32_t* totals);
@@ -130,6 +131,7 @@ struct kzg_ctx {
     DevVec<float4> b_packed_solver;  // 8 float4 per body during the solve
     DevVec<uint32_t> coop_counter;   // device-wide barrier of the cooperative solve
     uint32_t coop_blocks = 0;        // kzg_set_cooperative: 0 auto, 1 off, n workgroups
+    uint32_t unit_cap_vel = 0, unit_cap_ws = 0;  // kzg_set_sweep_units: 0 default, 1..kMaxUnit
     uint32_t coop_used = 1;          // workgroups the last solve gave group 0
     uint32_t coop_ws_points = 0, coop_vel_points = 0;
     DevVec<uint32_t> b_meta, b_packed;
@@ -155,7 +157,8 @@ struct kzg_ctx {
     // schedule
     ScheduleBuilder sched_builder;
     ScheduleOutput sched;
-    DevVec<uint32_t> ws_off, vel_off, ws_pos, vel_pos;
+    DevVec<uint32_t> ws_off, vel_off, ws_pos, vel_pos, ws_unit, vel_unit;
+    DevVec<uint4> ws_udesc, vel_udesc;
     DevVec<uint2> ws_grp, vel_grp;
     // sweep records
     DevVec<uint4> vr_bd, wr_bd;

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "sweep_limits.h"
+
 namespace kz {
 
 // Shape types (the reference's ShapeType numbering, Shape.h:13-25).
@@ -94,8 +96,13 @@ struct CacheBuf {
 // Level schedule of one solve: per group a range of levels, per level a
 // range of sweep records (records are stored in schedule order).
 struct Schedule {
-    const uint32_t* lvl_off;  // [levels + 1]
+    const uint32_t* lvl_off;  // [levels + 1] first unit of each level
     const uint2* grp;         // [groups] first level, end level
+    // [units] unit descriptors: body A, body B, flags | records << 8, first
+    // record.  A unit's records belong to one body pair and are applied in
+    // order by one thread.  nullptr: one record per unit (the descriptor is
+    // the record's body entry).
+    const uint4* unit;
 };
 
 // Sweep records: the constant per-point data of a sweep, laid out in the

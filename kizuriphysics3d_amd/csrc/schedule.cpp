@@ -17,6 +17,11 @@
 // 1 + the largest last level of its dynamic bodies.  Points of one level
 // share no dynamic body; for every dynamic body the order of its updates is
 // the sequence order, which is what makes the sweep bit-identical.
+// A point whose two dynamic bodies were both last updated by the previous
+// point of the same pair follows that point in one thread (a unit) instead
+// of opening the next level: a manifold's points do so in the warm-start
+// order and mostly in the first-fit order.  Units of two points shorten the
+// level chain by ~30%.
 //
 // The warm-start/bias sequence (constraint order) and the velocity sequence
 // (first-fit order) are independent once the islands are known; above a few
@@ -86,10 +91,14 @@ void ScheduleBuilder::wait_helper() {
     cv_.wait(lk, [this] { return !busy_; });
 }
 
-// Bucket (group, level) -> flat arrays.
-void ScheduleBuilder::flatten(Seq& s, uint32_t groups, uint32_t num_points, LevelSchedule& out) {
-    s.depth.assign(groups, 0);
-    for (uint32_t i = 0; i < s.n; ++i) s.depth[s.grp[i]] = std::max(s.depth[s.grp[i]], s.lvl[i]);
+void ScheduleBuilder::Seq::reserve(uint32_t points) {
+    for (auto* v : {&pts, &uid, &lvl}) v->resize(points);
+    n = nu = 0;
+}
+
+// Groups' level ranges (the shared head of both flatten variants).
+uint32_t ScheduleBuilder::level_bases(Seq& s, uint32_t groups, LevelSchedule& out) {
+    out.largest_unit = s.nu ? s.largest : 0;
     out.grp.assign(groups, U2{0, 0});
     s.base.assign(groups, 0);
     uint32_t total = 0;
@@ -101,21 +110,91 @@ void ScheduleBuilder::flatten(Seq& s, uint32_t groups, uint32_t num_points, Leve
         out.max_depth = std::max(out.max_depth, s.depth[g]);
     }
     out.levels = total;
+    out.units = s.nu;
     out.off.assign(total + 1, 0);
-    for (uint32_t i = 0; i < s.n; ++i) out.off[s.base[s.grp[i]] + s.lvl[i]] += 1;
+    return total;
+}
+
+void ScheduleBuilder::prefix_levels(Seq& s, uint32_t total, LevelSchedule& out) {
     out.max_width = 0;
     for (uint32_t l = 0; l < total; ++l) {
         out.max_width = std::max(out.max_width, out.off[l + 1]);
         out.off[l + 1] += out.off[l];
     }
-    out.idx.resize(s.n);
     s.fill.assign(out.off.begin(), out.off.end() - 1);
+}
+
+void ScheduleBuilder::place_points(Seq& s, uint32_t num_points, LevelSchedule& out) {
+    for (uint32_t k = 0; k < s.nu; ++k) out.unit[k + 1] += out.unit[k];
+    out.idx.resize(s.n);
     out.pos.assign(num_points, kNone);
     for (uint32_t i = 0; i < s.n; ++i) {
-        const uint32_t e = s.fill[s.base[s.grp[i]] + s.lvl[i] - 1]++;
+        const uint32_t e = out.unit[s.slot[s.uid[i] >> 2]] + (s.uid[i] & 3u);
         out.idx[e] = s.pts[i];
         out.pos[s.pts[i]] = e;
     }
+}
+
+// Bucket units by (group, level) -> flat arrays, from the per-unit tables;
+// a unit's points keep their sequence order.
+void ScheduleBuilder::flatten(Seq& s, uint32_t groups, uint32_t num_points, LevelSchedule& out) {
+    const uint32_t total = level_bases(s, groups, out);
+    s.slot.resize(s.nu);
+    for (uint32_t u = 0; u < s.nu; ++u) {
+        const uint32_t l = s.base[s.ugrp[u]] + s.ulvl[u];
+        s.slot[u] = l;  // the level (+1) for now, the slot below
+        out.off[l] += 1;
+    }
+    prefix_levels(s, total, out);
+    out.unit.resize(s.nu + 1);
+    out.unit[0] = 0;
+    for (uint32_t u = 0; u < s.nu; ++u) {
+        const uint32_t k = s.fill[s.slot[u] - 1]++;
+        s.slot[u] = k;
+        out.unit[k + 1] = s.usize[u];
+    }
+    place_points(s, num_points, out);
+}
+
+// The same from the per-point entries and the group runs (no unit tables):
+// a unit's first point (index 0 in it) places the unit, its last sets the
+// size.  Units are numbered in the order of their first points.
+void ScheduleBuilder::flatten_points(Seq& s, uint32_t groups, uint32_t num_points, LevelSchedule& out) {
+    // Deepest level of each group and the largest unit.
+    s.depth.assign(groups, 0);
+    uint32_t maxj = 0, i0 = 0;
+    for (const U2& r : s.runs) {
+        uint32_t d = s.depth[r.y];
+        for (uint32_t i = i0; i < r.x; ++i) {
+            d = std::max(d, s.lvl[i]);
+            maxj = std::max(maxj, s.uid[i] & 3u);
+        }
+        s.depth[r.y] = d;
+        i0 = r.x;
+    }
+    s.largest = maxj + 1;
+    const uint32_t total = level_bases(s, groups, out);
+    uint32_t i = 0;
+    for (const U2& r : s.runs) {
+        uint32_t* off = out.off.data() + s.base[r.y];
+        for (; i < r.x; ++i) off[s.lvl[i]] += (s.uid[i] & 3u) == 0 ? 1u : 0u;
+    }
+    prefix_levels(s, total, out);
+    s.slot.resize(s.nu);
+    out.unit.assign(s.nu + 1, 0);
+    i = 0;
+    for (const U2& r : s.runs) {
+        uint32_t* fill = s.fill.data() + s.base[r.y] - 1;
+        for (; i < r.x; ++i) {
+            const uint32_t u = s.uid[i] >> 2, j = s.uid[i] & 3u, start = j == 0 ? 1u : 0u;
+            uint32_t& f = fill[s.lvl[i]];
+            const uint32_t k = start ? f : s.slot[u];
+            f += start;
+            s.slot[u] = k;
+            out.unit[k + 1] = j + 1;
+        }
+    }
+    place_points(s, num_points, out);
 }
 
 void ScheduleBuilder::islands(const ScheduleInput& in, ScheduleOutput& out) {
@@ -215,34 +294,46 @@ uint32_t ScheduleBuilder::union_find(const ScheduleInput& in) {
 // constraint with a dynamic body).  Needs only the body types, so it can
 // start before the islands are known; the groups are filled in afterwards.
 void ScheduleBuilder::ws_levels(const ScheduleInput& in) {
+    // The points of a constraint follow each other in this order, so its
+    // units are runs of up to `cap` consecutive points: one level per run.
     const uint8_t* dyn = in.dynamic;
     Seq& s = ws_;
-    s.last.assign(in.num_bodies, 0);
-    uint32_t n = 0;
+    ws_body_level_.assign(in.num_bodies, 0);
+    uint32_t* bl = ws_body_level_.data();
+    const uint32_t cap = std::min(std::max(in.use_simd ? in.max_unit_ws : in.max_unit_vel, 1u), kMaxUnit);
+    uint32_t n = 0, nu = 0, np_max = 0;
     for (uint32_t c = 0; c < in.num_constraints; ++c) {
         const U4& ci = in.info[c];
         const bool da = dyn[ci.x] != 0, db = dyn[ci.y] != 0;
         if (!da && !db) continue;
         const uint32_t np = ci.w & 0xFFu;
-        for (uint32_t k = 0; k < np; ++k) {
-            const uint32_t l = 1 + std::max(da ? s.last[ci.x] : 0u, db ? s.last[ci.y] : 0u);
-            if (da) s.last[ci.x] = l;
-            if (db) s.last[ci.y] = l;
-            s.pts[n] = ci.z + k;
-            ws_con_[n] = c;
-            s.lvl[n] = l;
-            ++n;
+        for (uint32_t k0 = 0; k0 < np; k0 += cap) {
+            const uint32_t l = 1 + std::max(da ? bl[ci.x] : 0u, db ? bl[ci.y] : 0u);
+            if (da) bl[ci.x] = l;
+            if (db) bl[ci.y] = l;
+            const uint32_t m = std::min(cap, np - k0);
+            np_max = std::max(np_max, m);
+            s.ulvl[nu] = l;
+            s.usize[nu] = m;
+            ws_con_[nu] = c;
+            for (uint32_t j = 0; j < m; ++j) {
+                s.pts[n] = ci.z + k0 + j;
+                s.uid[n] = (nu << 2) | j;
+                ++n;
+            }
+            ++nu;
         }
     }
     s.n = n;
+    s.nu = nu;
+    s.largest = np_max;
 }
 
-// First-fit batches per island, then the levels of the batch sequence.
-void ScheduleBuilder::vel_sequence(const ScheduleInput& in) {
+// First-fit batches of every island, in island order, into one batch array;
+// island i's batches end at fit_island_end_[i].
+void ScheduleBuilder::vel_fit(const ScheduleInput& in) {
     const uint8_t* dyn = in.dynamic;
-    Seq& s = vel_;
-    s.last.assign(in.num_bodies, 0);
-    uint32_t n = 0;
+    const uint32_t cap = std::min(std::max(in.max_unit_vel, 1u), kMaxUnit);
     // Non-full batches as ordered lists with lazy deletion: `open` holds
     // every batch in creation order (full ones are skipped and dropped from
     // the front); open_wo[h] holds, for a non-dynamic body h, the batches
@@ -259,8 +350,8 @@ void ScheduleBuilder::vel_sequence(const ScheduleInput& in) {
     size_t wo_used = 0;
     std::vector<FitBatch>& batches = batches_;
     const uint32_t ni = uint32_t(island_points_.size());
+    batches.clear();
     for (uint32_t is = 0; is < ni; ++is) {
-        batches.clear();
         open.ids.clear();
         open.head = 0;
         wo_used = 0;
@@ -303,6 +394,10 @@ void ScheduleBuilder::vel_sequence(const ScheduleInput& in) {
             }
             return found;
         };
+        // A point placed in the batch right after the one holding the
+        // previous point of its constraint follows that point with no update
+        // of either body between them (the lanes of a batch share no body):
+        // it joins that point's unit while the unit has room.
         for (uint32_t j = island_off_[is]; j < island_off_[is + 1]; ++j) {
             const U4& ci = in.info[island_cons_[j]];
             if ((ci.w >> 8) & 1u) continue;  // sensor
@@ -310,8 +405,13 @@ void ScheduleBuilder::vel_sequence(const ScheduleInput& in) {
             OpenList* cand = &open;
             if (!dyn[a]) cand = &open_without(a);
             else if (!dyn[b]) cand = &open_without(b);
+            uint32_t prev = kNone, prev_lane = 0, run = 0;
             for (uint32_t k = 0; k < np; ++k) {
                 const uint32_t placed = first_fit(*cand, a, b);
+                const uint32_t id = placed == kNone ? uint32_t(batches.size()) : placed;
+                const bool join = (prev != kNone) & (id == prev + 1) & (run < cap);
+                run = join ? run + 1 : 1;
+                const uint32_t jbits = join ? 1u | (prev_lane << 8) : 0u;
                 if (placed == kNone) {
                     FitBatch nb;
                     nb.body[0] = a;
@@ -319,11 +419,12 @@ void ScheduleBuilder::vel_sequence(const ScheduleInput& in) {
                     for (int q = 1; q < 4; ++q) nb.body[q] = nb.body[4 + q] = kNone;
                     nb.pt[0] = ci.z + k;
                     nb.n = 1;
-                    const uint32_t id = uint32_t(batches.size());
+                    nb.join = jbits;
                     batches.push_back(nb);
                     open.ids.push_back(id);
                     for (size_t q = 0; q < wo_used; ++q)
                         if (open_wo[q].first != a && open_wo[q].first != b) open_wo[q].second.ids.push_back(id);
+                    prev_lane = 0;
                 } else {
                     // Entries of open_wo[a] / open_wo[b] now holding a or b
                     // are skipped by the conflict test on later visits.
@@ -331,57 +432,102 @@ void ScheduleBuilder::vel_sequence(const ScheduleInput& in) {
                     B.body[B.n] = a;
                     B.body[4 + B.n] = b;
                     B.pt[B.n] = ci.z + k;
+                    B.join |= (jbits & 1u) << B.n | (jbits >> 8) << (8 + 2 * B.n);
+                    prev_lane = B.n;
                     ++B.n;
                 }
+                prev = id;
             }
         }
-        const uint32_t g = group_of_island_[is];
-        for (const FitBatch& B : batches)
+        fit_island_end_[is] = uint32_t(batches.size());
+    }
+}
+
+
+// Levels of the batch sequence, unit by unit.  A joining lane takes the
+// unit of the lane it names in the previous batch (read back from uid).
+void ScheduleBuilder::vel_levels(const ScheduleInput& in) {
+    const uint8_t* dyn = in.dynamic;
+    Seq& s = vel_;
+    const uint32_t ni = uint32_t(island_points_.size());
+    const FitBatch* batches = batches_.data();
+    uint32_t* last = vel_body_level_.data();
+    uint32_t *pts = s.pts.data(), *uid = s.uid.data(), *lvl = s.lvl.data();
+    uint32_t n = 0, nu = 0, x = 0;
+    for (uint32_t is = 0; is < ni; ++is) {
+        const uint32_t n0 = n;
+        uint32_t pb = n;  // first point of the previous batch
+        for (; x < fit_island_end_[is]; ++x) {
+            const FitBatch& B = batches[x];
+            const uint32_t cb = n;
             for (uint32_t q = 0; q < B.n; ++q) {
                 const uint32_t a = B.body[q], b = B.body[4 + q];
-                const bool da = dyn[a] != 0, db = dyn[b] != 0;
-                const uint32_t l = 1 + std::max(da ? s.last[a] : 0u, db ? s.last[b] : 0u);
-                if (da) s.last[a] = l;
-                if (db) s.last[b] = l;
-                s.pts[n] = B.pt[q];
-                s.grp[n] = g;
-                s.lvl[n] = l;
+                const bool da = dyn[a] != 0, db = dyn[b] != 0, join = (B.join >> q) & 1u;
+                // Masks, not branches: the join is taken about a fifth of the
+                // time at random.  (For a first batch pb + lane is the current
+                // point, whose entry is read and not used.)
+                const uint32_t top = std::max(da ? last[a] : 0u, db ? last[b] : 0u);
+                const uint32_t jm = 0u - uint32_t(join);
+                const uint32_t l = top + 1 - uint32_t(join);
+                const uint32_t ux = ((uid[pb + ((B.join >> (8 + 2 * q)) & 3u)] + 1) & jm) | ((nu << 2) & ~jm);
+                nu += 1 - uint32_t(join);
+                if (da) last[a] = l;
+                if (db) last[b] = l;
+                pts[n] = B.pt[q];
+                uid[n] = ux;
+                lvl[n] = l;
                 ++n;
             }
+            pb = cb;
+        }
+        if (n > n0) s.runs.push_back(U2{n, group_of_island_[is]});
     }
     s.n = n;
+    s.nu = nu;
 }
 
 void ScheduleBuilder::build(const ScheduleInput& in, ScheduleOutput& out) {
     const uint32_t nc = in.num_constraints;
     uint32_t num_points = 0;
     for (uint32_t c = 0; c < nc; ++c) num_points += in.info[c].w & 0xFFu;
-    for (Seq* s : {&ws_, &vel_}) {
-        s->pts.resize(num_points);
-        s->grp.resize(num_points);
-        s->lvl.resize(num_points);
-    }
+    for (Seq* s : {&ws_, &vel_}) s->reserve(num_points);
+    for (auto* v : {&ws_.ulvl, &ws_.ugrp, &ws_.usize}) v->resize(num_points);
     ws_con_.resize(num_points);
     // The warm-start sequence: levels (body types only), then, once the
     // islands are known, groups and buckets.
     groups_ready_.store(false, std::memory_order_relaxed);
+    using clock = std::chrono::steady_clock;
     auto ws_chain = [this, &in, &out, num_points](bool wait) {
+        const auto t0 = clock::now();
         ws_levels(in);
         if (wait)
             while (!groups_ready_.load(std::memory_order_acquire)) std::this_thread::yield();
-        for (uint32_t i = 0; i < ws_.n; ++i) ws_.grp[i] = group_of_island_[island_of_con_[ws_con_[i]]];
+        ws_.depth.assign(groups_, 0);
+        for (uint32_t u = 0; u < ws_.nu; ++u) {
+            const uint32_t g = group_of_island_[island_of_con_[ws_con_[u]]];
+            ws_.ugrp[u] = g;
+            ws_.depth[g] = std::max(ws_.depth[g], ws_.ulvl[u]);
+        }
         flatten(ws_, groups_, num_points, out.ws);
+        ws_ms = std::chrono::duration<double, std::milli>(clock::now() - t0).count();
     };
     const bool parallel = num_points >= kParallelPoints;
+    const bool simd = in.use_simd;
     if (parallel) run_helper([&ws_chain] { ws_chain(true); });
     islands(in, out);
     groups_ready_.store(true, std::memory_order_release);
     if (!parallel) ws_chain(false);
-    if (in.use_simd) {
-        vel_sequence(in);
-        flatten(vel_, groups_, num_points, out.vel);
+    if (simd) {
+        vel_body_level_.assign(in.num_bodies, 0);
+        vel_.runs.clear();
+        fit_island_end_.assign(island_points_.size(), 0);
+        vel_fit(in);
+        vel_levels(in);
+        flatten_points(vel_, groups_, num_points, out.vel);
     }
+    const auto tw = clock::now();
     if (parallel) wait_helper();
+    wait_ms = std::chrono::duration<double, std::milli>(clock::now() - tw).count();
     if (!in.use_simd) out.vel = out.ws;  // the scalar path sweeps in constraint order, sensors included
 }
 

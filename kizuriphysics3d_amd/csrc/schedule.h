@@ -4,12 +4,16 @@
 
 #include <stdint.h>
 
+#include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <functional>
 #include <mutex>
 #include <thread>
 #include <vector>
+
+#include "sweep_limits.h"
 
 namespace kz {
 
@@ -20,17 +24,25 @@ struct U2 {
     uint32_t x, y;
 };
 
-// One level-scheduled sequence: idx holds point indices grouped by level,
-// off[l]..off[l+1] is level l, grp[g] = [first level, end level) of group g,
-// pos[p] = position of point p in idx (0xFFFFFFFF when absent).
+// One level-scheduled sequence.  The sweep's work item is a unit: a run of
+// at most kMaxUnit points of one body pair that follow each other in the
+// sequence with no other update of either body between them, so one thread
+// applies them in order.  idx holds the points unit by unit, units grouped
+// by level; unit[u]..unit[u+1] are the positions in idx of unit u;
+// off[l]..off[l+1] are the units of level l; grp[g] = [first level, end
+// level) of group g; pos[p] = position of point p in idx (0xFFFFFFFF when
+// absent).
 struct LevelSchedule {
     std::vector<uint32_t> idx;
     std::vector<uint32_t> pos;
+    std::vector<uint32_t> unit;
     std::vector<uint32_t> off;
     std::vector<U2> grp;
+    uint32_t units = 0;
+    uint32_t largest_unit = 0;
     uint32_t levels = 0;      // total levels over all groups
     uint32_t max_depth = 0;   // deepest group
-    uint32_t max_width = 0;   // widest level
+    uint32_t max_width = 0;   // widest level, in units
 };
 
 struct ScheduleInput {
@@ -39,6 +51,11 @@ struct ScheduleInput {
     uint32_t num_constraints = 0;
     const U4* info = nullptr;          // body A, body B, first point, num points | sensor << 8
     bool use_simd = true;
+    // Most points one unit of the velocity / warm-start sequence may hold
+    // (1: every unit is one point).  Without the SIMD order the velocity
+    // passes sweep the warm-start sequence, which then takes max_unit_vel.
+    uint32_t max_unit_vel = kDefaultUnitVel;
+    uint32_t max_unit_ws = kDefaultUnitWs;
     uint32_t max_groups = 1024;
     // Optional: the island of each constraint, numbered as BuildIslands
     // numbers them, and the island count (every dynamic body is in one).
@@ -55,11 +72,13 @@ struct ScheduleOutput {
 };
 
 // A first-fit batch: bodies A of lanes 0-3, then bodies B (unused lanes hold
-// 0xFFFFFFFF), the points, the lane count.
+// 0xFFFFFFFF), the points, the lane count, and the lanes that join a unit
+// of the previous batch (bit q: lane q joins; bits 8 + 2q: which lane).
 struct FitBatch {
     uint32_t body[8];
     uint32_t pt[4];
     uint32_t n;
+    uint32_t join;
 };
 
 class ScheduleBuilder {
@@ -69,24 +88,41 @@ public:
     ScheduleBuilder& operator=(const ScheduleBuilder&) = delete;
     ~ScheduleBuilder();
     void build(const ScheduleInput& in, ScheduleOutput& out);
+    // Of the last build: the warm-start sequence's time (helper thread) and
+    // the time the velocity sequence waited for it.
+    double ws_ms = 0.0, wait_ms = 0.0;
 
 private:
-    // A sequence of points with their group and level, before bucketing.
-    struct Seq {
-        std::vector<uint32_t> pts, grp, lvl, last, depth, base, fill;
-        uint32_t n = 0;
+    // A sequence of points and their units, before bucketing.  The two
+    // sequences are built on two threads: each Seq gets its own cache lines.
+    struct alignas(64) Seq {
+        // Per point in sequence order: point, unit << 2 | index in the unit,
+        // level.
+        std::vector<uint32_t> pts, uid, lvl;
+        std::vector<uint32_t> ulvl, ugrp, usize;  // per unit: level, group, points (warm-start order)
+        std::vector<U2> runs;                     // (end point, group) of consecutive points of one group
+        std::vector<uint32_t> depth;  // per group: deepest level (filled by the sequence)
+        std::vector<uint32_t> base, fill, slot;
+        uint32_t n = 0, nu = 0, largest = 0;  // points, units, largest unit
+        void reserve(uint32_t points);
     };
     void islands(const ScheduleInput& in, ScheduleOutput& out);
     uint32_t union_find(const ScheduleInput& in);
     void ws_levels(const ScheduleInput& in);
-    void vel_sequence(const ScheduleInput& in);
+    void vel_fit(const ScheduleInput& in);
+    void vel_levels(const ScheduleInput& in);
+    uint32_t level_bases(Seq& s, uint32_t groups, LevelSchedule& out);
+    void prefix_levels(Seq& s, uint32_t total, LevelSchedule& out);
+    void place_points(Seq& s, uint32_t num_points, LevelSchedule& out);
     void flatten(Seq& s, uint32_t groups, uint32_t num_points, LevelSchedule& out);
+    void flatten_points(Seq& s, uint32_t groups, uint32_t num_points, LevelSchedule& out);
     void run_helper(std::function<void()> job);
     void wait_helper();
 
     std::vector<uint32_t> dyn_index_, parent_, rank_, root_island_, island_of_con_, island_points_;
     std::vector<uint32_t> island_off_, island_cons_, group_of_island_, order_;
-    std::vector<uint32_t> ws_con_;  // per point of the warm-start sequence: its constraint
+    std::vector<uint32_t> ws_con_;  // per unit of the warm-start sequence: its constraint
+    std::vector<uint32_t> ws_body_level_, vel_body_level_;
     Seq ws_, vel_;
     std::vector<FitBatch> batches_;
     uint32_t groups_ = 0;
@@ -97,6 +133,7 @@ private:
     std::function<void()> job_;
     bool stop_ = false, busy_ = false;
     std::atomic<bool> groups_ready_{false};
+    std::vector<uint32_t> fit_island_end_;  // per island: end of its batches
 };
 
 }  // namespace kz

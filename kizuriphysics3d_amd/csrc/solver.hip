@@ -16,6 +16,8 @@
 // points of one level share no dynamic body and run in parallel, levels run
 // in order.  Islands are independent, so the islands of one group share
 // level numbers and one persistent workgroup sweeps a group.
+#include <type_traits>
+
 #include "kz_device.h"
 #include "kz_math.h"
 
@@ -282,146 +284,239 @@ __global__ void k_unpack_bodies(BodyBuf b, uint32_t n, const float4* vel, const 
 }
 
 // ---------------------------------------------------------------------------
-// Per-point sweep bodies, reading sweep records.
+// Per-unit sweep bodies, reading sweep records.  A unit is n <= U records of
+// one body pair that the sequence applies one after the other: its records
+// are loaded first, its two bodies once, and the points are applied in order
+// with the bodies in registers, which is the arithmetic of applying them one
+// point at a time.
 // ---------------------------------------------------------------------------
 // WarmStart (ContactSolver.cpp:117-132): stored rA / rB.
-__device__ __forceinline__ void warm_point(const WsRecBuf& r, float4* sb, uint32_t e, uint4 bd, float wsf) {
-    const float4 imp = r.imp[e];
-    const f3 n = ld3(r.n[e]), t1 = ld3(r.t1[e]), t2 = ld3(r.t2[e]);
-    f3 j = add(add(mul(n, imp.x), mul(t1, imp.y)), mul(t2, imp.z));
-    j = mul(j, wsf);
-    if (bd.z & 2u) {
-        const float4 ra = r.ra[e];
-        const PackedBody A = load_pb(sb, bd.x);
-        store_vw(sb, bd.x, add(A.v, mul(j, ra.w)), add(A.w, mmul(A.ii, cross(ld3(ra), j))), A.ii);
-    }
-    if (bd.z & 4u) {
-        const float4 rb = r.rb[e];
-        const PackedBody B = load_pb(sb, bd.y);
-        store_vw(sb, bd.y, sub(B.v, mul(j, rb.w)), sub(B.w, mmul(B.ii, cross(ld3(rb), j))), B.ii);
-    }
+template <uint32_t U>
+__device__ __forceinline__ void warm_unit(const WsRecBuf& r, float4* sb, uint32_t e0, uint32_t n, uint4 bd,
+                                          float wsf) {
+    const bool da = bd.z & 2u, db = bd.z & 4u;
+    float4 imp[U], n4[U], t14[U], t24[U], ra4[U], rb4[U];
+#pragma unroll
+    for (uint32_t k = 0; k < U; ++k)
+        if (k < n) {
+            const uint32_t e = e0 + k;
+            imp[k] = r.imp[e];
+            n4[k] = r.n[e];
+            t14[k] = r.t1[e];
+            t24[k] = r.t2[e];
+            if (da) ra4[k] = r.ra[e];
+            if (db) rb4[k] = r.rb[e];
+        }
+    PackedBody A, B;
+    if (da) A = load_pb(sb, bd.x);
+    if (db) B = load_pb(sb, bd.y);
+#pragma unroll
+    for (uint32_t k = 0; k < U; ++k)
+        if (k < n) {
+            f3 j = add(add(mul(ld3(n4[k]), imp[k].x), mul(ld3(t14[k]), imp[k].y)), mul(ld3(t24[k]), imp[k].z));
+            j = mul(j, wsf);
+            if (da) {
+                A.v = add(A.v, mul(j, ra4[k].w));
+                A.w = add(A.w, mmul(A.ii, cross(ld3(ra4[k]), j)));
+            }
+            if (db) {
+                B.v = sub(B.v, mul(j, rb4[k].w));
+                B.w = sub(B.w, mmul(B.ii, cross(ld3(rb4[k]), j)));
+            }
+        }
+    if (da) store_vw(sb, bd.x, A.v, A.w, A.ii);
+    if (db) store_vw(sb, bd.y, B.v, B.w, B.ii);
 }
 
-// One lane of SolveVelocityBatchSIMD (ContactSolverSIMD.cpp:93-256).
+// Lanes of SolveVelocityBatchSIMD (ContactSolverSIMD.cpp:93-256).
 // _mm_max_ps(a, b) = a > b ? a : b and _mm_min_ps(a, b) = a < b ? a : b.
-__device__ __forceinline__ float velocity_point(const VelRecBuf& r, float4* sb, uint32_t e, uint4 bd) {
-    const float4 ra4 = r.ra[e], rb4 = r.rb[e], n4 = r.n[e], t14 = r.t1[e], t24 = r.t2[e];
-    const float2 fb = r.fb[e];
-    float4 imp = r.imp[e];
+// dsq accumulates each point's sum of squared impulse increments.
+template <uint32_t U>
+__device__ __forceinline__ void velocity_unit(const VelRecBuf& r, float4* sb, uint32_t e0, uint32_t n, uint4 bd,
+                                              float& dsq) {
+    float4 ra4[U], rb4[U], n4[U], t14[U], t24[U], imp[U];
+    float2 fb[U];
+#pragma unroll
+    for (uint32_t k = 0; k < U; ++k)
+        if (k < n) {
+            const uint32_t e = e0 + k;
+            ra4[k] = r.ra[e];
+            rb4[k] = r.rb[e];
+            n4[k] = r.n[e];
+            t14[k] = r.t1[e];
+            t24[k] = r.t2[e];
+            fb[k] = r.fb[e];
+            imp[k] = r.imp[e];
+        }
     const PackedBody A = load_pb(sb, bd.x), B = load_pb(sb, bd.y);
     const m3 ia = A.ii, ib = B.ii;
     f3 va = A.v, wa = A.w, vb = B.v, wb = B.w;
-    const f3 ra = ld3(ra4), rb = ld3(rb4);
-    const float ima = ra4.w, imb = rb4.w, cf = fb.x, nbias = fb.y;
     const float zero = 0.0f;
-    float dsq = 0.0f;
-    auto rel = [&]() {
-        f3 xa2 = cross(wa, ra), xb2 = cross(wb, rb);
-        return f3{(va.x + xa2.x) - (vb.x + xb2.x), (va.y + xa2.y) - (vb.y + xb2.y), (va.z + xa2.z) - (vb.z + xb2.z)};
-    };
-    auto apply = [&](f3 j) {
-        va = f3{va.x + j.x * ima, va.y + j.y * ima, va.z + j.z * ima};
-        wa = add(wa, mmul(ia, cross(ra, j)));
-        vb = f3{vb.x - j.x * imb, vb.y - j.y * imb, vb.z - j.z * imb};
-        wb = sub(wb, mmul(ib, cross(rb, j)));
-    };
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        const f3 tv = t == 0 ? ld3(t14) : ld3(t24);
-        const float mass = t == 0 ? t14.w : t24.w;
-        const float acc = t == 0 ? imp.y : imp.z;
-        const float vt = dot(rel(), tv);
-        const float lam = (zero - mass) * vt;
-        const float mf = cf * imp.x;
-        float na = acc + lam;
-        const float lo = zero - mf;
-        na = na > lo ? na : lo;
-        na = na < mf ? na : mf;
-        const float dl = na - acc;
-        if (t == 0) imp.y = na; else imp.z = na;
-        dsq += dl * dl;
-        apply(mul(tv, dl));
-    }
-    {
-        const f3 nv = ld3(n4);
-        const float vn = dot(rel(), nv);
-        const float lam = (zero - n4.w) * (vn - nbias);
-        float na = imp.x + lam;
-        na = na > zero ? na : zero;
-        const float dl = na - imp.x;
-        imp.x = na;
-        dsq += dl * dl;
-        apply(mul(nv, dl));
-    }
+    for (uint32_t k = 0; k < U; ++k)
+        if (k < n) {
+            const f3 ra = ld3(ra4[k]), rb = ld3(rb4[k]);
+            const float ima = ra4[k].w, imb = rb4[k].w, cf = fb[k].x, nbias = fb[k].y;
+            float4 im = imp[k];
+            float pd = 0.0f;
+            auto rel = [&]() {
+                f3 xa2 = cross(wa, ra), xb2 = cross(wb, rb);
+                return f3{(va.x + xa2.x) - (vb.x + xb2.x), (va.y + xa2.y) - (vb.y + xb2.y),
+                          (va.z + xa2.z) - (vb.z + xb2.z)};
+            };
+            auto apply = [&](f3 j) {
+                va = f3{va.x + j.x * ima, va.y + j.y * ima, va.z + j.z * ima};
+                wa = add(wa, mmul(ia, cross(ra, j)));
+                vb = f3{vb.x - j.x * imb, vb.y - j.y * imb, vb.z - j.z * imb};
+                wb = sub(wb, mmul(ib, cross(rb, j)));
+            };
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const f3 tv = t == 0 ? ld3(t14[k]) : ld3(t24[k]);
+                const float mass = t == 0 ? t14[k].w : t24[k].w;
+                const float acc = t == 0 ? im.y : im.z;
+                const float vt = dot(rel(), tv);
+                const float lam = (zero - mass) * vt;
+                const float mf = cf * im.x;
+                float na = acc + lam;
+                const float lo = zero - mf;
+                na = na > lo ? na : lo;
+                na = na < mf ? na : mf;
+                const float dl = na - acc;
+                if (t == 0) im.y = na; else im.z = na;
+                pd += dl * dl;
+                apply(mul(tv, dl));
+            }
+            {
+                const f3 nv = ld3(n4[k]);
+                const float vn = dot(rel(), nv);
+                const float lam = (zero - n4[k].w) * (vn - nbias);
+                float na = im.x + lam;
+                na = na > zero ? na : zero;
+                const float dl = na - im.x;
+                im.x = na;
+                pd += dl * dl;
+                apply(mul(nv, dl));
+            }
+            r.imp[e0 + k] = im;
+            dsq += pd;
+        }
     if (bd.z & 2u) store_vw(sb, bd.x, va, wa, ia);
     if (bd.z & 4u) store_vw(sb, bd.y, vb, wb, ib);
-    r.imp[e] = imp;
-    return dsq;
 }
 
 // SolveContactVelocity, the scalar path (ContactSolver.cpp:137-181).
-__device__ __forceinline__ float scalar_point(const VelRecBuf& r, float4* sb, uint32_t e, uint4 bd) {
-    const float4 ra4 = r.ra[e], rb4 = r.rb[e], n4 = r.n[e], t14 = r.t1[e], t24 = r.t2[e];
-    const float2 fb = r.fb[e];
-    float4 imp = r.imp[e];
+template <uint32_t U>
+__device__ __forceinline__ void scalar_unit(const VelRecBuf& r, float4* sb, uint32_t e0, uint32_t n, uint4 bd,
+                                            float& dsq) {
+    float4 ra4[U], rb4[U], n4[U], t14[U], t24[U], imp[U];
+    float2 fb[U];
+#pragma unroll
+    for (uint32_t k = 0; k < U; ++k)
+        if (k < n) {
+            const uint32_t e = e0 + k;
+            ra4[k] = r.ra[e];
+            rb4[k] = r.rb[e];
+            n4[k] = r.n[e];
+            t14[k] = r.t1[e];
+            t24[k] = r.t2[e];
+            fb[k] = r.fb[e];
+            imp[k] = r.imp[e];
+        }
     const PackedBody A = load_pb(sb, bd.x), B = load_pb(sb, bd.y);
     const m3 ia = A.ii, ib = B.ii;
     f3 va = A.v, wa = A.w, vb = B.v, wb = B.w;
-    const f3 ra = ld3(ra4), rb = ld3(rb4);
     const bool da = bd.z & 2u, db = bd.z & 4u;
-    float dsq = 0.0f;
-    auto apply = [&](f3 j) {
-        if (da) { va = add(va, mul(j, ra4.w)); wa = add(wa, mmul(ia, cross(ra, j))); }
-        if (db) { vb = sub(vb, mul(j, rb4.w)); wb = sub(wb, mmul(ib, cross(rb, j))); }
-    };
-    f3 vr = sub(add(va, cross(wa, ra)), add(vb, cross(wb, rb)));
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        const f3 tv = t == 0 ? ld3(t14) : ld3(t24);
-        const float mass = t == 0 ? t14.w : t24.w;
-        float acc = t == 0 ? imp.y : imp.z;
-        float lam = -mass * dot(vr, tv);
-        const float mf = fb.x * imp.x;
-        const float ni = fclamp(acc + lam, -mf, mf);
-        lam = ni - acc;
-        if (t == 0) imp.y = ni; else imp.z = ni;
-        dsq += lam * lam;
-        apply(mul(tv, lam));
-    }
-    vr = sub(add(va, cross(wa, ra)), add(vb, cross(wb, rb)));
-    float lam = -n4.w * (dot(vr, ld3(n4)) - fb.y);
-    const float ni = fmax_(imp.x + lam, 0.0f);
-    lam = ni - imp.x;
-    imp.x = ni;
-    dsq += lam * lam;
-    apply(mul(ld3(n4), lam));
+    for (uint32_t k = 0; k < U; ++k)
+        if (k < n) {
+            const f3 ra = ld3(ra4[k]), rb = ld3(rb4[k]);
+            float4 im = imp[k];
+            float pd = 0.0f;
+            auto apply = [&](f3 j) {
+                if (da) {
+                    va = add(va, mul(j, ra4[k].w));
+                    wa = add(wa, mmul(ia, cross(ra, j)));
+                }
+                if (db) {
+                    vb = sub(vb, mul(j, rb4[k].w));
+                    wb = sub(wb, mmul(ib, cross(rb, j)));
+                }
+            };
+            f3 vr = sub(add(va, cross(wa, ra)), add(vb, cross(wb, rb)));
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const f3 tv = t == 0 ? ld3(t14[k]) : ld3(t24[k]);
+                const float mass = t == 0 ? t14[k].w : t24[k].w;
+                float acc = t == 0 ? im.y : im.z;
+                float lam = -mass * dot(vr, tv);
+                const float mf = fb[k].x * im.x;
+                const float ni = fclamp(acc + lam, -mf, mf);
+                lam = ni - acc;
+                if (t == 0) im.y = ni; else im.z = ni;
+                pd += lam * lam;
+                apply(mul(tv, lam));
+            }
+            vr = sub(add(va, cross(wa, ra)), add(vb, cross(wb, rb)));
+            float lam = -n4[k].w * (dot(vr, ld3(n4[k])) - fb[k].y);
+            const float ni = fmax_(im.x + lam, 0.0f);
+            lam = ni - im.x;
+            im.x = ni;
+            pd += lam * lam;
+            apply(mul(ld3(n4[k]), lam));
+            r.imp[e0 + k] = im;
+            dsq += pd;
+        }
     if (da) store_vw(sb, bd.x, va, wa, ia);
     if (db) store_vw(sb, bd.y, vb, wb, ib);
-    r.imp[e] = imp;
-    return dsq;
 }
 
 // SolveBiasConstraint (ContactSolver.cpp:257-290).
-__device__ __forceinline__ void bias_point(const WsRecBuf& r, float4* ps, uint32_t e, uint4 bd) {
-    const float4 qa4 = r.qa[e];
-    const float pbias = qa4.w;
-    const float ima = r.ra[e].w, imb = r.rb[e].w;
-    if (ima <= 0.0f && imb <= 0.0f) return;
-    if (pbias <= 0.0f) return;
-    const f3 ra = ld3(qa4), rb = ld3(r.qb[e]);
+template <uint32_t U>
+__device__ __forceinline__ void bias_unit(const WsRecBuf& r, float4* ps, uint32_t e0, uint32_t n, uint4 bd) {
+    float4 qa4[U], qb4[U], n4[U], imp[U];
+    float ima[U], imb[U];
+#pragma unroll
+    for (uint32_t k = 0; k < U; ++k)
+        if (k < n) {
+            const uint32_t e = e0 + k;
+            qa4[k] = r.qa[e];
+            qb4[k] = r.qb[e];
+            n4[k] = r.n[e];
+            imp[k] = r.imp[e];
+            ima[k] = r.ra[e].w;
+            imb[k] = r.rb[e].w;
+        }
     const PackedBody A = load_pb(ps, bd.x), B = load_pb(ps, bd.y);
-    const f3 pva = A.v, pwa = A.w, pvb = B.v, pwb = B.w;
-    const f3 vr = sub(add(pva, cross(pwa, ra)), add(pvb, cross(pwb, rb)));
-    const float4 n4 = r.n[e];
-    float4 imp = r.imp[e];
-    float lam = -n4.w * (dot(vr, ld3(n4)) - pbias);
-    const float ni = fmax_(imp.w + lam, 0.0f);
-    lam = ni - imp.w;
-    imp.w = ni;
-    r.imp[e] = imp;
-    const f3 j = mul(ld3(n4), lam);
-    if (bd.z & 2u) store_vw(ps, bd.x, add(pva, mul(j, ima)), add(pwa, mmul(A.ii, cross(ra, j))), A.ii);
-    if (bd.z & 4u) store_vw(ps, bd.y, sub(pvb, mul(j, imb)), sub(pwb, mmul(B.ii, cross(rb, j))), B.ii);
+    f3 pva = A.v, pwa = A.w, pvb = B.v, pwb = B.w;
+    bool moved = false;
+#pragma unroll
+    for (uint32_t k = 0; k < U; ++k) {
+        if (k >= n) continue;
+        const float pbias = qa4[k].w;
+        if (!(ima[k] <= 0.0f && imb[k] <= 0.0f) && !(pbias <= 0.0f)) {
+            const f3 ra = ld3(qa4[k]), rb = ld3(qb4[k]);
+            const f3 vr = sub(add(pva, cross(pwa, ra)), add(pvb, cross(pwb, rb)));
+            float4 im = imp[k];
+            float lam = -n4[k].w * (dot(vr, ld3(n4[k])) - pbias);
+            const float ni = fmax_(im.w + lam, 0.0f);
+            lam = ni - im.w;
+            im.w = ni;
+            r.imp[e0 + k] = im;
+            const f3 j = mul(ld3(n4[k]), lam);
+            if (bd.z & 2u) {
+                pva = add(pva, mul(j, ima[k]));
+                pwa = add(pwa, mmul(A.ii, cross(ra, j)));
+            }
+            if (bd.z & 4u) {
+                pvb = sub(pvb, mul(j, imb[k]));
+                pwb = sub(pwb, mmul(B.ii, cross(rb, j)));
+            }
+            moved = true;
+        }
+    }
+    if (moved && (bd.z & 2u)) store_vw(ps, bd.x, pva, pwa, A.ii);
+    if (moved && (bd.z & 4u)) store_vw(ps, bd.y, pvb, pwb, B.ii);
 }
 
 // Level offsets of one group, staged in LDS when they fit (the sweep reads
@@ -432,7 +527,10 @@ struct LevelOffsets {
     const uint32_t* lds;
     uint32_t base;
     bool in_lds;
-    __device__ __forceinline__ uint32_t operator()(uint32_t l) const { return in_lds ? lds[l - base] : glob[l]; }
+    __device__ __forceinline__ uint32_t operator()(uint32_t l) const {
+        if (in_lds) return lds[l - base];
+        return glob[l];
+    }
 };
 __device__ __forceinline__ LevelOffsets stage_offsets(const Schedule& s, uint2 range, uint32_t* lds) {
     const uint32_t n = range.y - range.x + 1;
@@ -442,10 +540,10 @@ __device__ __forceinline__ LevelOffsets stage_offsets(const Schedule& s, uint2 r
     return o;
 }
 
-// Level-by-level sweep of one group's range of levels.  The body indices of
-// the next level are fetched before the current level is processed (they
-// are constant during the solve), so after each barrier a point waits for
-// one round of loads only.
+// Level-by-level sweep of one group's range of levels.  The unit bounds
+// and body indices of the next level are fetched before the current level
+// is processed (they are constant during the solve), so after each barrier
+// a unit waits for one round of loads only.
 // With `parts` workgroups sharing the levels, workgroup `part` takes the
 // part-th contiguous slice of every level (its threads stride through it)
 // and `sync` is a device-wide barrier; one workgroup alone uses
@@ -460,30 +558,49 @@ __device__ __forceinline__ Slice level_slice(const LevelOffsets& off, uint32_t l
     const uint32_t lo = min(e0 + part * chunk, e1);
     return {lo, min(lo + chunk, e1)};
 }
-template <class F, class Sync>
-__device__ __forceinline__ void sweep(const LevelOffsets& off, uint2 range, const uint4* bd, uint32_t parts,
-                                      uint32_t part, F&& f, Sync&& sync) {
+// A unit as the sweep fetches it: the raw descriptor (U > 1) or the body
+// entry of record u (U == 1).  It is decoded only where it is used, so the
+// fetch of the next level's units does not wait for the load.
+struct UnitRef {
+    uint4 d;
+    uint32_t u;
+};
+template <uint32_t U>
+__device__ __forceinline__ UnitRef unit_fetch(const uint4* unit, const uint4* bd, uint32_t u) {
+    return UnitRef{U > 1 ? unit[u] : bd[u], u};
+}
+template <uint32_t U, class F>
+__device__ __forceinline__ void unit_apply(const UnitRef& r, F&& f) {
+    if (U > 1)
+        f(r.d.w, min(r.d.z >> 8, U), make_uint4(r.d.x, r.d.y, r.d.z & 0xFFu, 0u));
+    else
+        f(r.u, 1u, r.d);
+}
+template <uint32_t U, class F, class Sync>
+__device__ __forceinline__ void sweep(const LevelOffsets& off, uint2 range, const uint4* bd, const uint4* unit,
+                                      uint32_t parts, uint32_t part, F&& f, Sync&& sync) {
     const uint32_t t = threadIdx.x, nt = blockDim.x;
-    uint4 nxt = make_uint4(0, 0, 0, 0);
+    UnitRef nxt{make_uint4(0, 0, 0, 0), 0};
     Slice cur_s = {0, 0};
     if (range.x < range.y) {
         cur_s = level_slice(off, range.x, parts, part);
-        if (cur_s.lo + t < cur_s.hi) nxt = bd[cur_s.lo + t];
+        if (cur_s.lo + t < cur_s.hi) nxt = unit_fetch<U>(unit, bd, cur_s.lo + t);
     }
     for (uint32_t l = range.x; l < range.y; ++l) {
         const Slice sl = cur_s;
-        const uint4 cur = nxt;
+        const UnitRef cur = nxt;
         if (l + 1 < range.y) {
             cur_s = level_slice(off, l + 1, parts, part);
-            if (cur_s.lo + t < cur_s.hi) nxt = bd[cur_s.lo + t];
+            if (cur_s.lo + t < cur_s.hi) nxt = unit_fetch<U>(unit, bd, cur_s.lo + t);
         }
-        for (uint32_t e = sl.lo + t; e < sl.hi; e += nt) f(e, e == sl.lo + t ? cur : bd[e]);
+        for (uint32_t u = sl.lo + t; u < sl.hi; u += nt) unit_apply<U>(u == sl.lo + t ? cur : unit_fetch<U>(unit, bd, u), f);
         sync();
     }
 }
-template <class F>
-__device__ __forceinline__ void sweep(const LevelOffsets& off, uint2 range, const uint4* bd, F&& f) {
-    sweep(off, range, bd, 1u, 0u, f, [] { __syncthreads(); });
+template <uint32_t U, class F>
+__device__ __forceinline__ void sweep(const LevelOffsets& off, uint2 range, const uint4* bd, const uint4* unit,
+                                      F&& f) {
+    sweep<U>(off, range, bd, unit, 1u, 0u, f, [] { __syncthreads(); });
 }
 
 // Device-wide barrier of the cooperating workgroups: monotonic counter.  An
@@ -505,14 +622,17 @@ __device__ __forceinline__ void grid_sync(uint32_t* counter, uint32_t target) {
 // ---------------------------------------------------------------------------
 // The persistent group sweep: one workgroup per group of islands runs the
 // warm start, every velocity iteration and every bias iteration level by
-// level.  The points of a level share no dynamic body, so the threads of
-// the workgroup process a level in parallel; __syncthreads() orders levels
-// (the waves of a workgroup share the CU's L1, so no cache maintenance is
-// needed between levels).  `resid` (optional) receives the sum of squared
-// impulse increments of each velocity sweep.
+// level.  The units of a level share no dynamic body, so the threads of the
+// workgroup process a level in parallel; __syncthreads() orders levels (the
+// waves of a workgroup share the CU's L1, so no cache maintenance is needed
+// between levels).  `resid` (optional) receives the sum of squared impulse
+// increments of each velocity sweep.  U: the largest unit (1: the schedule
+// has no unit array).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_solve_groups(VelRecBuf vr, WsRecBuf wr, float4* sbv, float4* sbp, Schedule ws,
-                                                       Schedule vel, SolveParams sp, float* resid, uint32_t group0) {
+template <uint32_t UW, uint32_t UV>
+__global__ void __launch_bounds__(solve_threads(UW, UV)) k_solve_groups(VelRecBuf vr, WsRecBuf wr, float4* sbv, float4* sbp,
+                                                                Schedule ws, Schedule vel, SolveParams sp,
+                                                                float* resid, uint32_t group0) {
     __shared__ uint32_t ws_lds[kLevelsLds], vel_lds[kLevelsLds];
     const uint2 wrange = ws.grp[group0 + blockIdx.x];
     const uint2 vrange = vel.grp[group0 + blockIdx.x];
@@ -520,25 +640,30 @@ __global__ void __launch_bounds__(1024) k_solve_groups(VelRecBuf vr, WsRecBuf wr
     const LevelOffsets vo = stage_offsets(vel, vrange, vel_lds);
     __syncthreads();
     const float wsf = sp.warm_start_factor;
-    sweep(wo, wrange, wr.bd, [&](uint32_t e, uint4 bd) { warm_point(wr, sbv, e, bd, wsf); });
+    sweep<UW>(wo, wrange, wr.bd, ws.unit,
+             [&](uint32_t e, uint32_t n, uint4 bd) { warm_unit<UW>(wr, sbv, e, n, bd, wsf); });
     for (uint32_t it = 0; it < sp.velocity_iterations; ++it) {
         float dsq = 0.0f;
         if (sp.use_simd)
-            sweep(vo, vrange, vr.bd, [&](uint32_t e, uint4 bd) { dsq += velocity_point(vr, sbv, e, bd); });
+            sweep<UV>(vo, vrange, vr.bd, vel.unit,
+                     [&](uint32_t e, uint32_t n, uint4 bd) { velocity_unit<UV>(vr, sbv, e, n, bd, dsq); });
         else
-            sweep(vo, vrange, vr.bd, [&](uint32_t e, uint4 bd) { dsq += scalar_point(vr, sbv, e, bd); });
+            sweep<UV>(vo, vrange, vr.bd, vel.unit,
+                     [&](uint32_t e, uint32_t n, uint4 bd) { scalar_unit<UV>(vr, sbv, e, n, bd, dsq); });
         if (resid && dsq != 0.0f) atomicAdd(resid + it, dsq);
     }
     if (!sp.use_velocity_bias)
         for (uint32_t it = 0; it < sp.position_iterations; ++it)
-            sweep(wo, wrange, wr.bd, [&](uint32_t e, uint4 bd) { bias_point(wr, sbp, e, bd); });
+            sweep<UW>(wo, wrange, wr.bd, ws.unit,
+                     [&](uint32_t e, uint32_t n, uint4 bd) { bias_unit<UW>(wr, sbp, e, n, bd); });
 }
 
 // One group solved by gridDim.x cooperating workgroups (the launch makes
 // them co-resident): the same sweeps, each level split over all of them.
-__global__ void __launch_bounds__(1024) k_solve_coop(VelRecBuf vr, WsRecBuf wr, float4* sbv, float4* sbp, Schedule ws,
-                                                     Schedule vel, SolveParams sp, float* resid, uint32_t group,
-                                                     uint32_t* counter) {
+template <uint32_t UW, uint32_t UV>
+__global__ void __launch_bounds__(solve_threads(UW, UV)) k_solve_coop(VelRecBuf vr, WsRecBuf wr, float4* sbv, float4* sbp,
+                                                                 Schedule ws, Schedule vel, SolveParams sp,
+                                                                 float* resid, uint32_t group, uint32_t* counter) {
     __shared__ uint32_t ws_lds[kLevelsLds], vel_lds[kLevelsLds];
     const uint2 wrange = ws.grp[group];
     const uint2 vrange = vel.grp[group];
@@ -549,23 +674,34 @@ __global__ void __launch_bounds__(1024) k_solve_coop(VelRecBuf vr, WsRecBuf wr, 
     uint32_t phase = 0;
     auto gsync = [&] { grid_sync(counter, gridDim.x * ++phase); };
     const float wsf = sp.warm_start_factor;
-    sweep(wo, wrange, wr.bd, parts, part, [&](uint32_t e, uint4 bd) { warm_point(wr, sbv, e, bd, wsf); }, gsync);
+    sweep<UW>(wo, wrange, wr.bd, ws.unit, parts, part,
+             [&](uint32_t e, uint32_t n, uint4 bd) { warm_unit<UW>(wr, sbv, e, n, bd, wsf); }, gsync);
     for (uint32_t it = 0; it < sp.velocity_iterations; ++it) {
         float dsq = 0.0f;
         if (sp.use_simd)
-            sweep(vo, vrange, vr.bd, parts, part,
-                  [&](uint32_t e, uint4 bd) { dsq += velocity_point(vr, sbv, e, bd); }, gsync);
+            sweep<UV>(vo, vrange, vr.bd, vel.unit, parts, part,
+                     [&](uint32_t e, uint32_t n, uint4 bd) { velocity_unit<UV>(vr, sbv, e, n, bd, dsq); }, gsync);
         else
-            sweep(vo, vrange, vr.bd, parts, part,
-                  [&](uint32_t e, uint4 bd) { dsq += scalar_point(vr, sbv, e, bd); }, gsync);
+            sweep<UV>(vo, vrange, vr.bd, vel.unit, parts, part,
+                     [&](uint32_t e, uint32_t n, uint4 bd) { scalar_unit<UV>(vr, sbv, e, n, bd, dsq); }, gsync);
         if (resid && dsq != 0.0f) atomicAdd(resid + it, dsq);
     }
     if (!sp.use_velocity_bias)
         for (uint32_t it = 0; it < sp.position_iterations; ++it)
-            sweep(wo, wrange, wr.bd, parts, part, [&](uint32_t e, uint4 bd) { bias_point(wr, sbp, e, bd); }, gsync);
+            sweep<UW>(wo, wrange, wr.bd, ws.unit, parts, part,
+                     [&](uint32_t e, uint32_t n, uint4 bd) { bias_unit<UW>(wr, sbp, e, n, bd); }, gsync);
 }
 
 __global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
+
+// Unit descriptors from the unit bounds and the records' body entries.
+__global__ void k_unit_desc(const uint32_t* unit, const uint4* bd, uint32_t nu, uint4* desc) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nu) return;
+    const uint32_t e = unit[u], n = unit[u + 1] - e;
+    const uint4 b = bd[e];
+    desc[u] = make_uint4(b.x, b.y, (b.z & 0xFFu) | (n << 8), e);
+}
 
 // Final impulses back to constraint-point order (for the cache store and
 // read-back): velocity impulses from the velocity records, the bias impulse
@@ -702,15 +838,50 @@ void launch_writeback(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, co
                       const uint32_t* vel_pos, const uint32_t* ws_pos, uint32_t np) {
     if (np) k_writeback<<<blocks(np, 256), 256, 0, s>>>(vr, wr, c, vel_pos, ws_pos, np, nullptr);
 }
+// Descriptors of `nu` units whose bounds are unit[0..nu] (records written).
+void launch_unit_desc(hipStream_t s, const uint32_t* unit, const uint4* bd, uint32_t nu, uint4* desc) {
+    if (nu) k_unit_desc<<<blocks(nu, 256), 256, 0, s>>>(unit, bd, nu, desc);
+}
+
+// Sweep kernels by unit instantiation (see unit_kernel).
+template <class F>
+static auto by_units(UnitKernel uk, F&& f) {
+    auto vel = [&](auto w) {
+        if (uk.vel <= 1) return f(w, std::integral_constant<uint32_t, 1>{});
+        if (uk.vel == 2) return f(w, std::integral_constant<uint32_t, 2>{});
+        return f(w, std::integral_constant<uint32_t, kMaxUnit>{});
+    };
+    if (uk.ws <= 1) return vel(std::integral_constant<uint32_t, 1>{});
+    if (uk.ws == 2) return vel(std::integral_constant<uint32_t, 2>{});
+    return vel(std::integral_constant<uint32_t, kMaxUnit>{});
+}
+static void launch_groups(hipStream_t s, UnitKernel uk, uint32_t n, uint32_t threads, const VelRecBuf& vr,
+                          const WsRecBuf& wr, float4* sbv, float4* sbp, const Schedule& ws, const Schedule& vel,
+                          const SolveParams& sp, float* resid, uint32_t group0) {
+    if (!ws.unit || !vel.unit) uk = UnitKernel{};
+    by_units(uk, [&](auto w, auto v) {
+        k_solve_groups<decltype(w)::value, decltype(v)::value>
+            <<<n, threads, 0, s>>>(vr, wr, sbv, sbp, ws, vel, sp, resid, group0);
+        return 0;
+    });
+}
+static const void* coop_kernel(UnitKernel uk) {
+    return by_units(uk, [](auto w, auto v) {
+        return (const void*)k_solve_coop<decltype(w)::value, decltype(v)::value>;
+    });
+}
+
 // packed: 8 float4 per body (velocity lines, then pseudo-velocity lines).
+// uk: the unit instantiation (unit_kernel of the schedule's largest units;
+// ignored without unit descriptors).
 void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b, uint32_t nbodies,
                          float4* packed, const Schedule& ws, const Schedule& vel, const SolveParams& sp,
-                         uint32_t groups, uint32_t threads, float* resid) {
+                         uint32_t groups, uint32_t threads, float* resid, UnitKernel uk) {
     if (!groups || !nbodies) return;
     float4* sbv = packed;
     float4* sbp = packed + 4 * size_t(nbodies);
     k_pack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
-    k_solve_groups<<<groups, threads, 0, s>>>(vr, wr, sbv, sbp, ws, vel, sp, resid, 0u);
+    launch_groups(s, uk, groups, threads, vr, wr, sbv, sbp, ws, vel, sp, resid, 0u);
     k_unpack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
 }
 
@@ -719,9 +890,10 @@ void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr,
 hipError_t launch_solve_split(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b,
                               uint32_t nbodies, float4* packed, const Schedule& ws, const Schedule& vel,
                               const SolveParams& sp, uint32_t groups, uint32_t threads, float* resid,
-                              uint32_t coop_blocks, uint32_t coop_threads, uint32_t* counter,
+                              UnitKernel uk, uint32_t coop_blocks, uint32_t coop_threads, uint32_t* counter,
                               hipEvent_t coop_begin, hipEvent_t coop_end) {
     if (!groups || !nbodies) return hipSuccess;
+    if (!ws.unit || !vel.unit) return hipErrorInvalidValue;  // the cooperative sweep reads unit arrays
     float4* sbv = packed;
     float4* sbp = packed + 4 * size_t(nbodies);
     k_pack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
@@ -733,11 +905,10 @@ hipError_t launch_solve_split(hipStream_t s, const VelRecBuf& vr, const WsRecBuf
     uint32_t group = 0;
     void* args[] = {&vr_, &wr_, &sbv, &sbp, &ws_, &vel_, &sp_, &resid, &group, &counter};
     if (coop_begin) hipEventRecord(coop_begin, s);
-    hipError_t e = hipLaunchCooperativeKernel((const void*)k_solve_coop, dim3(coop_blocks), dim3(coop_threads), args,
-                                              0, s);
+    hipError_t e = hipLaunchCooperativeKernel(coop_kernel(uk), dim3(coop_blocks), dim3(coop_threads), args, 0, s);
     if (e != hipSuccess) return e;
     if (coop_end) hipEventRecord(coop_end, s);
-    if (groups > 1) k_solve_groups<<<groups - 1, threads, 0, s>>>(vr, wr, sbv, sbp, ws, vel, sp, resid, 1u);
+    if (groups > 1) launch_groups(s, uk, groups - 1, threads, vr, wr, sbv, sbp, ws, vel, sp, resid, 1u);
     k_unpack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
     return hipGetLastError();
 }

@@ -41,11 +41,12 @@ def test_narrowphase_batch_bit_exact(n):
         assert i is None, f"point {k} differs at pair {np.nonzero(live)[0][i]}"
 
 
-def _trajectory(scene, steps, cooperative=0):
+def _trajectory(scene, steps, cooperative=0, units=0):
     R.init_jobs(0)
     rw = R.RefWorld(scene)
     ctx = _ctx_for(scene, rw)
     ctx.set_cooperative(cooperative)
+    ctx.set_sweep_units(*(units if isinstance(units, tuple) else (units, units)))
     for i in range(steps):
         rw.step_phases()
         pairs = rw.pairs()
@@ -102,6 +103,14 @@ def test_pile_trajectory_cooperative_bit_exact(blocks):
     device-wide barrier between levels (forced even where levels are narrow,
     so some workgroups get empty slices)."""
     _trajectory(S.pile(512, seed=0x4B5A0102), 240, cooperative=blocks)
+
+
+@pytest.mark.parametrize("units,blocks", [((1, 1), 0), ((2, 2), 0), ((3, 1), 1), ((4, 4), 5), ((1, 4), 0)])
+def test_pile_trajectory_unit_caps_bit_exact(units, blocks):
+    """Velocity / warm-start sweep units capped at other sizes than the
+    default (2, 4), split and on one workgroup: the trajectory stays the
+    reference's."""
+    _trajectory(S.pile(512, seed=0x4B5A0102), 240, cooperative=blocks, units=units)
 
 
 def test_cooperative_split_invariant_large_pile():

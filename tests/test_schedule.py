@@ -136,3 +136,71 @@ def test_schedule_levels_are_as_deep_as_the_reference_chain():
     vl, wl, gr, st = D.schedule(dyn, info)
     assert list(wl) == list(range(1, nb))
     assert st[0] == 1 and st[3] == nb - 1
+
+
+@pytest.mark.parametrize("seed,nb,nc,nstatic,groups", [(11, 12, 30, 1, 1024), (12, 40, 150, 2, 3),
+                                                      (13, 200, 900, 3, 8), (14, 30, 120, 0, 1),
+                                                      (15, 400, 1500, 5, 64), (16, 6, 40, 1, 2)])
+@pytest.mark.parametrize("use_simd", [True, False])
+def test_units_preserve_every_dynamic_body_order(seed, nb, nc, nstatic, groups, use_simd):
+    """Multi-point units: one thread applies a unit's points in point order,
+    so the sweep equals the reference sequence iff sorting a body's points by
+    (level, point) gives its reference order, a unit holds one body pair at
+    one level, and the units of a level share no dynamic body."""
+    rng = np.random.default_rng(seed)
+    dyn, info = random_graph(rng, nb, nc, nstatic)
+    vl, wl, gr, st, vu, wu = D.schedule(dyn, info, use_simd=use_simd, max_groups=groups, units=True,
+                                        with_units=True)
+    vl1, wl1, _, st1 = D.schedule(dyn, info, use_simd=use_simd, max_groups=groups)
+    vel, ws = reference_sequences(dyn, info, use_simd)
+    pb = body_of(info)
+    for seq, lvl, unit, lvl1, nunits in ((vel, vl, vu, vl1, st[4]), (ws, wl, wu, wl1, st[5])):
+        assert sorted(seq) == sorted(np.nonzero(lvl)[0].tolist())
+        assert np.array_equal(lvl > 0, lvl1 > 0)
+        members = {}
+        for p in seq:
+            members.setdefault(int(unit[p]), []).append(p)
+        assert len(members) == nunits
+        for u, pts in members.items():
+            assert 1 <= len(pts) <= 4
+            assert len({pb[p] for p in pts}) == 1, f"unit {u} mixes body pairs"
+            assert len({(gr[p], lvl[p]) for p in pts}) == 1, f"unit {u} spans levels"
+        for body in np.nonzero(dyn)[0]:
+            ref = [p for p in seq if body in pb[p]]
+            got = sorted(ref, key=lambda p: (lvl[p], p))
+            assert got == ref, f"body {body}: order differs"
+        seen = set()
+        for u, pts in members.items():
+            for body in set(pb[pts[0]]):
+                if dyn[body]:
+                    key = (gr[pts[0]], lvl[pts[0]], body)
+                    assert key not in seen, f"two units of one level share body {body}"
+                    seen.add(key)
+        # joining never deepens a group
+        for g in np.unique(gr[lvl > 0]):
+            sel = (gr == g) & (lvl > 0)
+            assert lvl[sel].max() <= lvl1[sel].max()
+
+
+def test_units_join_a_manifold_chain():
+    # Three bodies in a row, two four-point manifolds: a chain of 8 point
+    # levels becomes 4 unit levels (units of up to 2 points) in both orders.
+    dyn = np.ones(3, np.uint8)
+    info = np.array([(0, 1, 0, 4), (1, 2, 4, 4)], np.uint32)
+    vl, wl, gr, st = D.schedule(dyn, info, units=True)
+    assert list(wl) == [1, 1, 2, 2, 3, 3, 4, 4]
+    assert list(vl) == [1, 1, 2, 2, 3, 3, 4, 4]
+    assert st[4] == 4 and st[5] == 4 and st[2] == 4 and st[3] == 4
+    vl1, wl1, _, st1 = D.schedule(dyn, info)
+    assert st1[3] == 8 and st1[4] == 8
+    # a point of another pair between two points of one manifold breaks the unit
+    dyn = np.array([0, 1, 1], np.uint8)
+    info = np.array([(0, 1, 0, 2), (1, 2, 2, 1), (0, 1, 3, 2)], np.uint32)
+    _, wl, _, st, _, wu = D.schedule(dyn, info, use_simd=False, units=True, with_units=True)
+    assert list(wl) == [1, 1, 2, 3, 3]
+    assert st[5] == 3
+    # a manifold of 4 points is cut into units of 2
+    dyn = np.array([0, 1], np.uint8)
+    info = np.array([(0, 1, 0, 4)], np.uint32)
+    _, wl, _, st, _, wu = D.schedule(dyn, info, units=True, with_units=True)
+    assert list(wl) == [1, 1, 2, 2] and wu[0] == wu[1] != wu[2] == wu[3]
