@@ -25,15 +25,17 @@ struct CoopPlan {
 
 // Workgroups and threads for group 0 (the group holding the largest
 // island).  `request` 0 sizes the split from the group's mean level width;
-// `max_threads` is the kernel's workgroup limit.
-CoopPlan plan_cooperative(const ScheduleOutput& s, uint32_t request, uint32_t max_threads) {
+// `max_threads` is the kernel's workgroup limit; a velocity unit takes
+// `vel_lanes` threads.
+CoopPlan plan_cooperative(const ScheduleOutput& s, uint32_t request, uint32_t max_threads, uint32_t vel_lanes) {
     CoopPlan p;
     if (request == 1 || s.groups == 0 || s.ws.grp.empty()) return p;
     uint32_t units = 0, levels = 0, widest = 0;
     for (const LevelSchedule* ls : {&s.ws, &s.vel}) {
         if (ls->grp.empty()) continue;
         const U2 g = ls->grp[0];
-        for (uint32_t l = g.x; l < g.y; ++l) widest = std::max(widest, ls->off[l + 1] - ls->off[l]);
+        const uint32_t lanes = ls == &s.vel ? vel_lanes : 1u;
+        for (uint32_t l = g.x; l < g.y; ++l) widest = std::max(widest, lanes * (ls->off[l + 1] - ls->off[l]));
         units += ls->off[g.y] - ls->off[g.x];
         levels += g.y - g.x;
         (ls == &s.ws ? p.ws_points : p.vel_points) = ls->unit[ls->off[g.y]] - ls->unit[ls->off[g.x]];
@@ -423,11 +425,13 @@ int kzg_solve(kzg_ctx* c, float dt) {
     launch_unit_desc(c->stream, c->vel_unit.p, c->vel_records().bd, c->sched.vel.units, c->vel_udesc.p);
     Schedule ws{c->ws_off.p, c->ws_grp.p, c->ws_udesc.p};
     Schedule vel{c->vel_off.p, c->vel_grp.p, c->vel_udesc.p};
-    const uint32_t width = std::max(c->sched.ws.max_width, c->sched.vel.max_width);
+    // The SIMD velocity sweeps take two threads per unit (a lane pair).
+    const uint32_t vel_lanes = w.use_simd_solver ? 2u : 1u;
+    const uint32_t width = std::max(c->sched.ws.max_width, vel_lanes * c->sched.vel.max_width);
     const UnitKernel uk = unit_kernel(c->sched.ws.largest_unit, c->sched.vel.largest_unit);
     uint32_t threads = 64;
     while (threads < width && threads < solve_threads(uk.ws, uk.vel)) threads *= 2;
-    const CoopPlan coop = plan_cooperative(c->sched, c->coop_blocks, solve_threads(uk.ws, uk.vel));
+    const CoopPlan coop = plan_cooperative(c->sched, c->coop_blocks, solve_threads(uk.ws, uk.vel), vel_lanes);
     c->coop_used = coop.blocks;
     c->coop_ws_points = coop.ws_points;
     c->coop_vel_points = coop.vel_points;
