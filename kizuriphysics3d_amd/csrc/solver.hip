@@ -630,13 +630,12 @@ __device__ __forceinline__ LevelOffsets stage_offsets(const Schedule& s, uint2 r
     return o;
 }
 
-// Level-by-level sweep of one group's range of levels.  The unit bounds
-// and body indices of the next level are fetched before the current level
-// is processed (they are constant during the solve), so after each barrier
-// a unit waits for one round of loads only.
-// With `parts` workgroups sharing the levels, workgroup `part` takes the
-// part-th contiguous slice of every level (its threads stride through it)
-// and `sync` is a device-wide barrier; one workgroup alone uses
+// Levels are swept one at a time.  The unit descriptors of the next level
+// are fetched before the current level is processed (they are constant
+// during the solve), so after each barrier a unit waits for one round of
+// loads only.  With `parts` workgroups sharing the levels, workgroup `part`
+// takes the part-th contiguous slice of every level (its threads stride
+// through it) and the barrier is device-wide; one workgroup alone uses
 // __syncthreads().
 struct Slice {
     uint32_t lo, hi;
@@ -666,36 +665,6 @@ __device__ __forceinline__ void unit_apply(const UnitRef& r, uint32_t side, F&& 
     else
         f(r.u, 1u, r.d, side);
 }
-// L threads per unit (1, or 2 for a lane pair: the unit's side is the
-// thread's low bit).
-template <uint32_t U, uint32_t L, class F, class Sync>
-__device__ __forceinline__ void sweep(const LevelOffsets& off, uint2 range, const uint4* bd, const uint4* unit,
-                                      uint32_t parts, uint32_t part, F&& f, Sync&& sync) {
-    const uint32_t t = threadIdx.x / L, nt = blockDim.x / L, side = threadIdx.x % L;
-    UnitRef nxt{make_uint4(0, 0, 0, 0), 0};
-    Slice cur_s = {0, 0};
-    if (range.x < range.y) {
-        cur_s = level_slice(off, range.x, parts, part);
-        if (cur_s.lo + t < cur_s.hi) nxt = unit_fetch<U>(unit, bd, cur_s.lo + t);
-    }
-    for (uint32_t l = range.x; l < range.y; ++l) {
-        const Slice sl = cur_s;
-        const UnitRef cur = nxt;
-        if (l + 1 < range.y) {
-            cur_s = level_slice(off, l + 1, parts, part);
-            if (cur_s.lo + t < cur_s.hi) nxt = unit_fetch<U>(unit, bd, cur_s.lo + t);
-        }
-        for (uint32_t u = sl.lo + t; u < sl.hi; u += nt)
-            unit_apply<U>(u == sl.lo + t ? cur : unit_fetch<U>(unit, bd, u), side, f);
-        sync();
-    }
-}
-template <uint32_t U, uint32_t L, class F>
-__device__ __forceinline__ void sweep(const LevelOffsets& off, uint2 range, const uint4* bd, const uint4* unit,
-                                      F&& f) {
-    sweep<U, L>(off, range, bd, unit, 1u, 0u, f, [] { __syncthreads(); });
-}
-
 // Device-wide barrier of the cooperating workgroups: monotonic counter.  An
 // agent-scope release fence before the arrival publishes the workgroup's
 // body updates of the level; an acquire fence after the wait makes the
@@ -713,43 +682,119 @@ __device__ __forceinline__ void grid_sync(uint32_t* counter, uint32_t target) {
 }
 
 // ---------------------------------------------------------------------------
-// The persistent group sweep: one workgroup per group of islands runs the
-// warm start, every velocity iteration and every bias iteration level by
-// level.  The units of a level share no dynamic body, so the threads of the
-// workgroup process a level in parallel; __syncthreads() orders levels (the
-// waves of a workgroup share the CU's L1, so no cache maintenance is needed
-// between levels).  `resid` (optional) receives the sum of squared impulse
-// increments of each velocity sweep.  U: the largest unit (1: the schedule
-// has no unit array).
+// A sweep through one group's levels, one level per step, with the next
+// level's units prefetched; after the group's last level it starts over
+// (the next iteration).  L threads per unit.
 // ---------------------------------------------------------------------------
+template <uint32_t U, uint32_t L>
+struct LevelStream {
+    LevelOffsets off;
+    uint2 range;
+    const uint4 *bd, *unit;
+    uint32_t parts, part, t, nt, side, l;
+    Slice s;
+    UnitRef nxt;
+    __device__ __forceinline__ LevelStream(const LevelOffsets& o, uint2 r, const uint4* b, const uint4* un,
+                                           uint32_t parts_, uint32_t part_)
+        : off(o), range(r), bd(b), unit(un), parts(parts_), part(part_), t(threadIdx.x / L), nt(blockDim.x / L),
+          side(threadIdx.x % L), l(r.x), s{0, 0}, nxt{make_uint4(0, 0, 0, 0), 0} {
+        if (r.x < r.y) fetch(r.x);
+    }
+    __device__ __forceinline__ void fetch(uint32_t lv) {
+        s = level_slice(off, lv, parts, part);
+        if (s.lo + t < s.hi) nxt = unit_fetch<U>(unit, bd, s.lo + t);
+    }
+    // Applies f to this thread's units of the current level and moves on;
+    // true when that was the group's last level.
+    template <class F>
+    __device__ __forceinline__ bool step(F&& f) {
+        const Slice sl = s;
+        const UnitRef cur = nxt;
+        const bool last = l + 1 == range.y;
+        l = last ? range.x : l + 1;
+        fetch(l);
+        for (uint32_t u = sl.lo + t; u < sl.hi; u += nt)
+            unit_apply<U>(u == sl.lo + t ? cur : unit_fetch<U>(unit, bd, u), side, f);
+        return last;
+    }
+};
+
+// The solve of one group of islands (WarmStart, then SolveVelocity x
+// velocityIterations, then SolveBias x positionIterations; PhysicsWorld.cpp
+// SolveIsland), level by level with `sync` between levels.  The bias
+// iterations change only pseudo velocities and the bias impulses, which no
+// velocity iteration reads or writes, so bias level k runs in the same
+// interval as velocity step k: the same per-body order of updates with
+// fewer barriers.  `resid` (optional) receives the sum of squared impulse
+// increments of each velocity sweep.
+template <uint32_t UW, uint32_t UV, uint32_t L, class VelF, class Sync>
+__device__ __forceinline__ void velocity_and_bias(const WsRecBuf& wr, float4* sbp, const LevelOffsets& wo, uint2 wrange,
+                                                  const LevelOffsets& vo, uint2 vrange, const uint4* vbd,
+                                                  const Schedule& ws, const Schedule& vel, const SolveParams& sp,
+                                                  float* resid, uint32_t parts, uint32_t part, VelF&& vf,
+                                                  Sync&& sync) {
+    const uint32_t vsteps = (vrange.y - vrange.x) * sp.velocity_iterations;
+    const uint32_t bsteps = sp.use_velocity_bias ? 0u : (wrange.y - wrange.x) * sp.position_iterations;
+    LevelStream<UV, L> v(vo, vrange, vbd, vel.unit, parts, part);
+    LevelStream<UW, 1> b(wo, wrange, wr.bd, ws.unit, parts, part);
+    float dsq = 0.0f;
+    uint32_t it = 0;
+    const uint32_t steps = vsteps > bsteps ? vsteps : bsteps;
+    for (uint32_t k = 0; k < steps; ++k) {
+        if (k < vsteps && v.step([&](uint32_t e, uint32_t n, uint4 bd, uint32_t side) { vf(e, n, bd, side, dsq); })) {
+            if (resid && dsq != 0.0f) atomicAdd(resid + it, dsq);
+            dsq = 0.0f;
+            ++it;
+        }
+        if (k < bsteps) b.step([&](uint32_t e, uint32_t n, uint4 bd, uint32_t) { bias_unit<UW>(wr, sbp, e, n, bd); });
+        sync();
+    }
+}
+template <uint32_t UW, uint32_t UV, class Sync>
+__device__ __forceinline__ void solve_group(const VelRecBuf& vr, const WsRecBuf& wr, float4* sbv, float4* sbp,
+                                            const Schedule& ws, const Schedule& vel, const SolveParams& sp,
+                                            float* resid, uint32_t group, uint32_t parts, uint32_t part,
+                                            uint32_t* ws_lds, uint32_t* vel_lds, Sync&& sync) {
+    const uint2 wrange = ws.grp[group];
+    const uint2 vrange = vel.grp[group];
+    const LevelOffsets wo = stage_offsets(ws, wrange, ws_lds);
+    const LevelOffsets vo = stage_offsets(vel, vrange, vel_lds);
+    __syncthreads();
+    const float wsf = sp.warm_start_factor;
+    {
+        LevelStream<UW, 1> w(wo, wrange, wr.bd, ws.unit, parts, part);
+        for (uint32_t k = wrange.x; k < wrange.y; ++k) {
+            w.step([&](uint32_t e, uint32_t n, uint4 bd, uint32_t) { warm_unit<UW>(wr, sbv, e, n, bd, wsf); });
+            sync();
+        }
+    }
+    if (sp.use_simd)
+        velocity_and_bias<UW, UV, 2>(
+            wr, sbp, wo, wrange, vo, vrange, vr.bd, ws, vel, sp, resid, parts, part,
+            [&](uint32_t e, uint32_t n, uint4 bd, uint32_t side, float& dsq) {
+                velocity_pair<UV>(vr, sbv, e, n, bd, side, dsq);
+            },
+            sync);
+    else
+        velocity_and_bias<UW, UV, 1>(
+            wr, sbp, wo, wrange, vo, vrange, vr.bd, ws, vel, sp, resid, parts, part,
+            [&](uint32_t e, uint32_t n, uint4 bd, uint32_t, float& dsq) { scalar_unit<UV>(vr, sbv, e, n, bd, dsq); },
+            sync);
+}
+
+// The persistent group sweep: one workgroup per group of islands.  The units
+// of a level share no dynamic body, so the threads of the workgroup process
+// a level in parallel; __syncthreads() orders levels (the waves of a
+// workgroup share the CU's L1, so no cache maintenance is needed between
+// levels).  UW / UV: the largest warm-start / bias and velocity unit (1 also
+// serves schedules without unit descriptors).
 template <uint32_t UW, uint32_t UV>
 __global__ void __launch_bounds__(solve_threads(UW, UV)) k_solve_groups(VelRecBuf vr, WsRecBuf wr, float4* sbv, float4* sbp,
                                                                 Schedule ws, Schedule vel, SolveParams sp,
                                                                 float* resid, uint32_t group0) {
     __shared__ uint32_t ws_lds[kLevelsLds], vel_lds[kLevelsLds];
-    const uint2 wrange = ws.grp[group0 + blockIdx.x];
-    const uint2 vrange = vel.grp[group0 + blockIdx.x];
-    const LevelOffsets wo = stage_offsets(ws, wrange, ws_lds);
-    const LevelOffsets vo = stage_offsets(vel, vrange, vel_lds);
-    __syncthreads();
-    const float wsf = sp.warm_start_factor;
-    sweep<UW, 1>(wo, wrange, wr.bd, ws.unit,
-                 [&](uint32_t e, uint32_t n, uint4 bd, uint32_t) { warm_unit<UW>(wr, sbv, e, n, bd, wsf); });
-    for (uint32_t it = 0; it < sp.velocity_iterations; ++it) {
-        float dsq = 0.0f;
-        if (sp.use_simd)
-            sweep<UV, 2>(vo, vrange, vr.bd, vel.unit, [&](uint32_t e, uint32_t n, uint4 bd, uint32_t side) {
-                velocity_pair<UV>(vr, sbv, e, n, bd, side, dsq);
-            });
-        else
-            sweep<UV, 1>(vo, vrange, vr.bd, vel.unit,
-                         [&](uint32_t e, uint32_t n, uint4 bd, uint32_t) { scalar_unit<UV>(vr, sbv, e, n, bd, dsq); });
-        if (resid && dsq != 0.0f) atomicAdd(resid + it, dsq);
-    }
-    if (!sp.use_velocity_bias)
-        for (uint32_t it = 0; it < sp.position_iterations; ++it)
-            sweep<UW, 1>(wo, wrange, wr.bd, ws.unit,
-                         [&](uint32_t e, uint32_t n, uint4 bd, uint32_t) { bias_unit<UW>(wr, sbp, e, n, bd); });
+    solve_group<UW, UV>(vr, wr, sbv, sbp, ws, vel, sp, resid, group0 + blockIdx.x, 1u, 0u, ws_lds, vel_lds,
+                        [] { __syncthreads(); });
 }
 
 // One group solved by gridDim.x cooperating workgroups (the launch makes
@@ -759,36 +804,9 @@ __global__ void __launch_bounds__(solve_threads(UW, UV)) k_solve_coop(VelRecBuf 
                                                                  Schedule ws, Schedule vel, SolveParams sp,
                                                                  float* resid, uint32_t group, uint32_t* counter) {
     __shared__ uint32_t ws_lds[kLevelsLds], vel_lds[kLevelsLds];
-    const uint2 wrange = ws.grp[group];
-    const uint2 vrange = vel.grp[group];
-    const LevelOffsets wo = stage_offsets(ws, wrange, ws_lds);
-    const LevelOffsets vo = stage_offsets(vel, vrange, vel_lds);
-    __syncthreads();
-    const uint32_t parts = gridDim.x, part = blockIdx.x;
     uint32_t phase = 0;
-    auto gsync = [&] { grid_sync(counter, gridDim.x * ++phase); };
-    const float wsf = sp.warm_start_factor;
-    sweep<UW, 1>(
-        wo, wrange, wr.bd, ws.unit, parts, part,
-        [&](uint32_t e, uint32_t n, uint4 bd, uint32_t) { warm_unit<UW>(wr, sbv, e, n, bd, wsf); }, gsync);
-    for (uint32_t it = 0; it < sp.velocity_iterations; ++it) {
-        float dsq = 0.0f;
-        if (sp.use_simd)
-            sweep<UV, 2>(
-                vo, vrange, vr.bd, vel.unit, parts, part,
-                [&](uint32_t e, uint32_t n, uint4 bd, uint32_t side) { velocity_pair<UV>(vr, sbv, e, n, bd, side, dsq); },
-                gsync);
-        else
-            sweep<UV, 1>(
-                vo, vrange, vr.bd, vel.unit, parts, part,
-                [&](uint32_t e, uint32_t n, uint4 bd, uint32_t) { scalar_unit<UV>(vr, sbv, e, n, bd, dsq); }, gsync);
-        if (resid && dsq != 0.0f) atomicAdd(resid + it, dsq);
-    }
-    if (!sp.use_velocity_bias)
-        for (uint32_t it = 0; it < sp.position_iterations; ++it)
-            sweep<UW, 1>(
-                wo, wrange, wr.bd, ws.unit, parts, part,
-                [&](uint32_t e, uint32_t n, uint4 bd, uint32_t) { bias_unit<UW>(wr, sbp, e, n, bd); }, gsync);
+    solve_group<UW, UV>(vr, wr, sbv, sbp, ws, vel, sp, resid, group, gridDim.x, blockIdx.x, ws_lds, vel_lds,
+                        [&] { grid_sync(counter, gridDim.x * ++phase); });
 }
 
 __global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
