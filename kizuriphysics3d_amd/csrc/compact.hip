@@ -148,6 +148,64 @@ __global__ void k_uf_union(ContactBuf c, const uint32_t* totals, BodyBuf b, uint
     }
 }
 
+// The same union-find for a world whose parent array fits in one
+// workgroup's LDS: 4 bytes per body plus a bit per body for the dynamic
+// flags.  LDS atomics and finds cost tens of cycles instead of a device-scope
+// round trip, and the result is the same (each component's root is its
+// smallest slot).  Writes parent[i] = root of i.
+constexpr uint32_t kUfLdsBodies = 36864;
+constexpr uint32_t uf_lds_bytes(uint32_t nb) { return 4u * nb + 4u * ((nb + 31u) / 32u); }
+__device__ __forceinline__ uint32_t lds_find(uint32_t* par, uint32_t x) {
+    for (;;) {
+        const uint32_t p = __hip_atomic_load(par + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (p == x) return x;
+        const uint32_t gp = __hip_atomic_load(par + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (gp == p) return p;
+        __hip_atomic_store(par + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        x = gp;
+    }
+}
+__global__ void __launch_bounds__(1024) k_uf_lds(ContactBuf c, const uint32_t* totals, BodyBuf b, uint32_t nb,
+                                                 uint32_t* parent) {
+    extern __shared__ uint32_t lds[];
+    uint32_t* par = lds;
+    uint32_t* dynbits = lds + nb;
+    const uint32_t nw = (nb + 31u) / 32u;
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) par[i] = i;
+    for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) {
+        uint32_t bits = 0;
+        for (uint32_t k = 0; k < 32u && 32u * w + k < nb; ++k) bits |= (dyn_body(b, 32u * w + k) ? 1u : 0u) << k;
+        dynbits[w] = bits;
+    }
+    __syncthreads();
+    auto dyn = [&](uint32_t x) { return (dynbits[x >> 5] >> (x & 31u)) & 1u; };
+    const uint32_t nc = totals[0];
+    // Four constraints per thread and round, loaded together.
+    for (uint32_t base = 0; base < nc; base += 4u * blockDim.x) {
+        uint4 in[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            const uint32_t ci = base + k * blockDim.x + threadIdx.x;
+            in[k] = ci < nc ? c.info[ci] : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            const uint32_t ci = base + k * blockDim.x + threadIdx.x;
+            if (ci >= nc || !dyn(in[k].x) || !dyn(in[k].y)) continue;
+            uint32_t x = in[k].x, y = in[k].y;
+            for (;;) {
+                x = lds_find(par, x);
+                y = lds_find(par, y);
+                if (x == y) break;
+                const uint32_t hi = x > y ? x : y, lo = x > y ? y : x;
+                if (atomicCAS(par + hi, hi, lo) == hi) break;
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) parent[i] = lds_find(par, i);
+}
+
 // parent[i] = root of i; flag[i] = i is a dynamic root.
 __global__ void __launch_bounds__(1024) k_uf_roots(uint32_t* parent, uint32_t nb, BodyBuf b, uint2* tile) {
     __shared__ uint32_t sh[32];
@@ -194,8 +252,20 @@ void launch_islands(hipStream_t s, const ContactBuf& c, uint32_t cap, const Body
                     uint32_t* parent, uint32_t* island, uint2* tile, uint32_t* totals, uint32_t* con_island) {
     const uint32_t tiles = (nb + kTile - 1) / kTile;
     const uint32_t cb = (cap + 255) / 256;
-    if (nb) k_uf_init<<<(nb + 255) / 256, 256, 0, s>>>(parent, nb);
-    if (cb) k_uf_union<<<cb, 256, 0, s>>>(c, totals, b, parent);
+    // The LDS variant where the device lets one workgroup hold the array.
+    int dev = 0, optin = 0;
+    bool lds = nb && nb <= kUfLdsBodies && hipGetDevice(&dev) == hipSuccess &&
+               hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess &&
+               uint32_t(optin) >= uf_lds_bytes(nb);
+    if (lds)
+        lds = hipFuncSetAttribute((const void*)k_uf_lds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  int(uf_lds_bytes(nb))) == hipSuccess;
+    if (lds) {
+        k_uf_lds<<<1, 1024, uf_lds_bytes(nb), s>>>(c, totals, b, nb, parent);
+    } else {
+        if (nb) k_uf_init<<<(nb + 255) / 256, 256, 0, s>>>(parent, nb);
+        if (cb) k_uf_union<<<cb, 256, 0, s>>>(c, totals, b, parent);
+    }
     if (tiles) k_uf_roots<<<tiles, kTile, 0, s>>>(parent, nb, b, tile);
     k_compact_scan<<<1, kTile, 0, s>>>(tile, tiles, totals + 2);
     if (tiles) k_island_ids<<<tiles, kTile, 0, s>>>(parent, nb, b, tile, island);
