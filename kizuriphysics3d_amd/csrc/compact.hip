@@ -172,27 +172,39 @@ __global__ void __launch_bounds__(1024) k_uf_lds(ContactBuf c, const uint32_t* t
     uint32_t* dynbits = lds + nb;
     const uint32_t nw = (nb + 31u) / 32u;
     for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) par[i] = i;
-    for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) {
-        uint32_t bits = 0;
-        for (uint32_t k = 0; k < 32u && 32u * w + k < nb; ++k) bits |= (dyn_body(b, 32u * w + k) ? 1u : 0u) << k;
-        dynbits[w] = bits;
+    // Dynamic flags: one coalesced load per body, packed by a wave ballot
+    // (the loop bound is a multiple of 64, so whole waves take part).
+    const uint32_t nb64 = (nb + 63u) & ~63u;
+    for (uint32_t i = threadIdx.x; i < nb64; i += blockDim.x) {
+        const uint64_t m = __ballot(i < nb && dyn_body(b, i));
+        if ((threadIdx.x & 63u) == 0) {
+            dynbits[i >> 5] = uint32_t(m);
+            if ((i >> 5) + 1 < nw) dynbits[(i >> 5) + 1] = uint32_t(m >> 32);
+        }
     }
     __syncthreads();
     auto dyn = [&](uint32_t x) { return (dynbits[x >> 5] >> (x & 31u)) & 1u; };
     const uint32_t nc = totals[0];
-    // Four constraints per thread and round, loaded together.
-    for (uint32_t base = 0; base < nc; base += 4u * blockDim.x) {
-        uint4 in[4];
+    // Eight constraints per thread and round; the next round's are loaded
+    // while this round's are linked.
+    constexpr uint32_t R = 8;
+    const uint32_t step = R * blockDim.x;
+    auto load = [&](uint32_t base, uint4* in) {
 #pragma unroll
-        for (uint32_t k = 0; k < 4; ++k) {
+        for (uint32_t k = 0; k < R; ++k) {
             const uint32_t ci = base + k * blockDim.x + threadIdx.x;
             in[k] = ci < nc ? c.info[ci] : make_uint4(0, 0, 0, 0);
         }
+    };
+    uint4 cur[R], nxt[R];
+    load(0, cur);
+    for (uint32_t base = 0; base < nc; base += step) {
+        if (base + step < nc) load(base + step, nxt);
 #pragma unroll
-        for (uint32_t k = 0; k < 4; ++k) {
+        for (uint32_t k = 0; k < R; ++k) {
             const uint32_t ci = base + k * blockDim.x + threadIdx.x;
-            if (ci >= nc || !dyn(in[k].x) || !dyn(in[k].y)) continue;
-            uint32_t x = in[k].x, y = in[k].y;
+            if (ci >= nc || !dyn(cur[k].x) || !dyn(cur[k].y)) continue;
+            uint32_t x = cur[k].x, y = cur[k].y;
             for (;;) {
                 x = lds_find(par, x);
                 y = lds_find(par, y);
@@ -201,6 +213,8 @@ __global__ void __launch_bounds__(1024) k_uf_lds(ContactBuf c, const uint32_t* t
                 if (atomicCAS(par + hi, hi, lo) == hi) break;
             }
         }
+#pragma unroll
+        for (uint32_t k = 0; k < R; ++k) cur[k] = nxt[k];
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) parent[i] = lds_find(par, i);
