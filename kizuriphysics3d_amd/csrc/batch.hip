@@ -639,13 +639,14 @@ int ensure_capacity(kzb_batch* B, uint32_t pairs) {
     if (pairs <= B->cap_pairs && B->cap_pairs) return KZG_OK;
     kzg_ctx* c = B->ctx;
     const uint32_t cap = std::max<uint32_t>(1024, pairs + pairs / 4);
+    if (cap > kWorkPairMask) return bfail(B, KZG_ERR_UNSUPPORTED, "more than 2^27 - 1 broadphase pairs");
     const size_t pts = 4 * size_t(cap), W = B->worlds.size();
     KZB_HIP(B, c->d_pairs.ensure(cap));
     KZB_HIP(B, c->m_count.ensure(cap));
     KZB_HIP(B, c->m_pa.ensure(pts));
     KZB_HIP(B, c->m_pb.ensure(pts));
     KZB_HIP(B, c->m_n.ensure(pts));
-    KZB_HIP(B, c->work.ensure(2 * size_t(cap)));
+    KZB_HIP(B, c->work.ensure(work_entries(cap)));
     KZB_HIP(B, c->gjk.ensure(size_t(cap) * kGjkRecBytes));
     KZB_HIP(B, c->c_info.ensure(cap));
     KZB_HIP(B, c->c_pair.ensure(cap));

@@ -252,6 +252,7 @@ int kzg_upload_states(kzg_ctx* c, const kz_body_state* s, uint32_t n) {
 
 int kzg_narrowphase(kzg_ctx* c, const uint32_t* pairs, uint32_t n) {
     if (!c || (n && !pairs)) return fail(c, KZG_ERR_ARGUMENT, "null pair array");
+    if (n > kWorkPairMask) return fail(c, KZG_ERR_UNSUPPORTED, "more than 2^27 - 1 broadphase pairs");
     hipSetDevice(c->device);
     for (uint32_t i = 0; i < 2 * n; ++i)
         if (pairs[i] >= c->nbodies) return fail(c, KZG_ERR_ARGUMENT, "pair " + std::to_string(i / 2) + ": bad body index");
@@ -260,7 +261,7 @@ int kzg_narrowphase(kzg_ctx* c, const uint32_t* pairs, uint32_t n) {
     KZG_HIP(c, c->m_pa.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
     KZG_HIP(c, c->m_pb.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
     KZG_HIP(c, c->m_n.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
-    KZG_HIP(c, c->work.ensure(2 * size_t(std::max<uint32_t>(n, 1))));
+    KZG_HIP(c, c->work.ensure(work_entries(std::max<uint32_t>(n, 1))));
     KZG_HIP(c, c->gjk.ensure(size_t(std::max<uint32_t>(n, 1)) * kGjkRecBytes));
     KZG_HIP(c, c->c_info.ensure(std::max<uint32_t>(n, 1)));
     KZG_HIP(c, c->c_pair.ensure(std::max<uint32_t>(n, 1)));
@@ -719,6 +720,7 @@ int kzg_schedule(uint32_t nb, const uint8_t* dyn, uint32_t nc, const uint32_t* i
 int kzg_collide_batch(kzg_ctx* c, const uint32_t* sa, const uint32_t* sb, const float* xa, const float* xb,
                       uint32_t n, float max_sep, uint32_t* counts, kz_contact_point* pts, float* ms) {
     if (!c || (n && (!sa || !sb || !xa || !xb || !counts || !pts))) return fail(c, KZG_ERR_ARGUMENT, "null argument");
+    if (n > kWorkPairMask) return fail(c, KZG_ERR_UNSUPPORTED, "more than 2^27 - 1 pairs");
     hipSetDevice(c->device);
     std::vector<float4> x(2 * size_t(n)), q(2 * size_t(n));
     std::vector<uint32_t> meta(2 * size_t(n));
@@ -743,7 +745,7 @@ int kzg_collide_batch(kzg_ctx* c, const uint32_t* sa, const uint32_t* sb, const 
     KZG_HIP(c, c->m_pa.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
     KZG_HIP(c, c->m_pb.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
     KZG_HIP(c, c->m_n.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
-    KZG_HIP(c, c->work.ensure(2 * size_t(std::max<uint32_t>(n, 1))));
+    KZG_HIP(c, c->work.ensure(work_entries(std::max<uint32_t>(n, 1))));
     KZG_HIP(c, c->gjk.ensure(size_t(std::max<uint32_t>(n, 1)) * kGjkRecBytes));
     BodyBuf bb{c->x_x.p, c->x_q.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, c->x_meta.p, nullptr};
     c->recorded = 0;

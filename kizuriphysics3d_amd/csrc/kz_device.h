@@ -13,6 +13,23 @@ namespace kz {
 
 // Shape types (the reference's ShapeType numbering, Shape.h:13-25).
 enum : uint32_t { kSphere = 0, kBox = 1, kCapsule = 2, kHull = 4, kPlane = 8 };
+// Narrowphase work lists.  An item is the pair index (bits 0-26), its
+// (type A, type B) class (bits 27-30; sphere, box, capsule, hull -> 0..3)
+// and the swap flag (bit 31).  The GJK kernel's list is sorted by class in
+// chunks of kWorkChunk items, so a wave of that kernel mostly holds one kind
+// of pair.
+constexpr uint32_t kWorkClasses = 16;
+constexpr uint32_t kWorkChunk = 1024;
+constexpr uint32_t kWorkPairBits = 27;
+constexpr uint32_t kWorkPairMask = (1u << kWorkPairBits) - 1u;
+// Entries of a work buffer for `cap` pairs: the list, the class-major list,
+// the list handed to the wave kernel, and the per-chunk class counts.
+inline size_t work_entries(uint32_t cap) {
+    return 3 * size_t(cap) + size_t(kWorkClasses) * ((cap + kWorkChunk - 1) / kWorkChunk);
+}
+__host__ __device__ inline uint32_t work_class(uint32_t ta, uint32_t tb) {
+    return 4u * (ta == 4u ? 3u : ta) + (tb == 4u ? 3u : tb);
+}
 
 // Body metadata word: shape index (bits 0-23), motion type (24-25), sensor (26).
 __host__ __device__ inline uint32_t meta_shape(uint32_t m) { return m & 0x00FFFFFFu; }

@@ -137,11 +137,12 @@ __device__ __forceinline__ ShapeView load_shape(const ShapeBuf& s, uint32_t i) {
 
 // Pair inputs: body indices.  For the world path body = slot; for the
 // standalone batch every pair side is its own "body" carrying a transform.
-__global__ void k_collide_simple(const uint2* pairs, uint32_t n, const uint32_t* n_dev, BodyBuf b, ShapeBuf s,
-                                 ManifoldBuf m, uint32_t* work, uint32_t* work_count, float max_sep, int world_mode) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n_dev) n = *n_dev;  // count kept on the device (graph-captured steps)
-    if (i >= n) return;
+// The pair i: the closed-form classes (PhysicsWorld::Collide's skip and
+// order, then NarrowPhase::Collide's plane and sphere-sphere paths) write
+// their manifold; an iterative pair returns its work item.
+__device__ __forceinline__ void collide_simple_pair(const uint2* pairs, uint32_t i, const BodyBuf& b, const ShapeBuf& s,
+                                                    const ManifoldBuf& m, float max_sep, int world_mode,
+                                                    uint32_t& item) {
     const uint2 pr = pairs[i];
     uint32_t ia = pr.x, ib = pr.y;
     m.count[i] = 0;
@@ -169,9 +170,8 @@ __global__ void k_collide_simple(const uint2* pairs, uint32_t n, const uint32_t*
     } else if (A.type == kSphere && B.type == kSphere) {
         k = sphere_sphere(A.prm.x, l3(xa), B.prm.x, l3(xb), max_sep, c);
     } else {
-        // Iterative class: hand the (ordered) pair to the wave kernel.
-        const uint32_t slot = atomicAdd(work_count, 1u);
-        work[slot] = i | (swapped ? 0x80000000u : 0u);
+        // Iterative class: hand the (ordered) pair to the GJK kernel.
+        item = i | (work_class(A.type, B.type) << kWorkPairBits) | (swapped ? 0x80000000u : 0u);
         return;
     }
     for (int j = 0; j < k; ++j) {
@@ -179,6 +179,76 @@ __global__ void k_collide_simple(const uint2* pairs, uint32_t n, const uint32_t*
         write_contact(m, i, j, c[j]);
     }
     m.count[i] = uint32_t(k);
+}
+
+__global__ void k_collide_simple(const uint2* pairs, uint32_t n, const uint32_t* n_dev, BodyBuf b, ShapeBuf s,
+                                 ManifoldBuf m, uint32_t* work, uint32_t* work_count, float max_sep, int world_mode) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n_dev) n = *n_dev;  // count kept on the device (graph-captured steps)
+    if (i >= n) return;
+    uint32_t item = 0xFFFFFFFFu;
+    collide_simple_pair(pairs, i, b, s, m, max_sep, world_mode, item);
+    if (item != 0xFFFFFFFFu) work[atomicAdd(work_count, 1u)] = item;
+}
+
+// The work list in class-major order (the order inside a class is
+// arbitrary: the pairs are independent), in three passes over chunks of
+// kWorkChunk items: class counts per chunk (LDS), an exclusive scan over
+// (class, chunk), and the scatter.  No global atomics.
+__device__ __forceinline__ uint32_t item_class(uint32_t item) {
+    return (item >> kWorkPairBits) & (kWorkClasses - 1u);
+}
+__global__ void __launch_bounds__(kWorkChunk) k_bin_count(const uint32_t* work, const uint32_t* work_count,
+                                                          uint32_t* hist) {
+    __shared__ uint32_t cnt[kWorkClasses];
+    const uint32_t total = *work_count, lo = blockIdx.x * kWorkChunk;
+    if (lo >= total) return;
+    if (threadIdx.x < kWorkClasses) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t i = lo + threadIdx.x;
+    if (i < total) atomicAdd(cnt + item_class(work[i]), 1u);
+    __syncthreads();
+    if (threadIdx.x < kWorkClasses) hist[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = cnt[threadIdx.x];
+}
+// One workgroup: hist (classes x chunks, class-major) -> exclusive offsets.
+__global__ void __launch_bounds__(1024) k_bin_scan(uint32_t* hist, const uint32_t* work_count, uint32_t chunks_cap) {
+    __shared__ uint32_t part[1024];
+    const uint32_t chunks = (*work_count + kWorkChunk - 1) / kWorkChunk;
+    const uint32_t n = kWorkClasses * chunks_cap;
+    // Each thread scans a contiguous run; runs are then offset by a scan of their sums.
+    const uint32_t per = (n + blockDim.x - 1) / blockDim.x, lo = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t k = lo; k < lo + per && k < n; ++k)
+        if (k % chunks_cap < chunks) sum += hist[k];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
+        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t acc = part[threadIdx.x] - sum;
+    for (uint32_t k = lo; k < lo + per && k < n; ++k)
+        if (k % chunks_cap < chunks) {
+            const uint32_t v = hist[k];
+            hist[k] = acc;
+            acc += v;
+        }
+}
+__global__ void __launch_bounds__(kWorkChunk) k_bin_scatter(const uint32_t* work, const uint32_t* work_count,
+                                                            const uint32_t* hist, uint32_t* out) {
+    __shared__ uint32_t cnt[kWorkClasses];
+    const uint32_t total = *work_count, lo = blockIdx.x * kWorkChunk;
+    if (lo >= total) return;
+    if (threadIdx.x < kWorkClasses) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t i = lo + threadIdx.x;
+    if (i < total) {
+        const uint32_t item = work[i], c = item_class(item);
+        const uint32_t rank = atomicAdd(cnt + c, 1u);
+        out[hist[size_t(c) * gridDim.x + blockIdx.x] + rank] = item;
+    }
 }
 
 // ===========================================================================
@@ -1102,7 +1172,7 @@ __global__ void __launch_bounds__(64) k_collide_wave(const uint2* pairs, const u
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
         const Gjk g = load_gjk(pre[item]);
         const uint32_t wv = work[item];
-        const uint32_t pi = wv & 0x7FFFFFFFu;
+        const uint32_t pi = wv & kWorkPairMask;
         const bool swapped = (wv >> 31) != 0u;
         uint32_t ia = pairs[pi].x, ib = pairs[pi].y;
         if (swapped) { uint32_t t = ia; ia = ib; ib = t; }
@@ -1208,6 +1278,8 @@ __device__ __forceinline__ void store_gjk(GjkRec& o, const Gjk& g) {
     }
 }
 
+// The list is sorted by class in chunks, so neighbouring lanes mostly run
+// the same supports for a similar number of GJK iterations.
 __global__ void __launch_bounds__(256) k_collide_lanes(const uint2* pairs, const uint32_t* work,
                                                        const uint32_t* work_count, uint32_t* fwd,
                                                        uint32_t* fwd_count, GjkRec* fwd_gjk, BodyBuf b, ShapeBuf sb,
@@ -1215,7 +1287,7 @@ __global__ void __launch_bounds__(256) k_collide_lanes(const uint2* pairs, const
     const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= *work_count) return;
     const uint32_t wv = work[item];
-    const uint32_t pi = wv & 0x7FFFFFFFu;
+    const uint32_t pi = wv & kWorkPairMask;
     const bool swapped = (wv >> 31) != 0u;
     uint32_t ia = pairs[pi].x, ib = pairs[pi].y;
     if (swapped) { uint32_t t = ia; ia = ib; ib = t; }
@@ -1282,7 +1354,18 @@ __global__ void __launch_bounds__(256) k_collide_lanes(const uint2* pairs, const
 
 static inline uint32_t nblocks(uint32_t n, uint32_t t) { return (n + t - 1) / t; }
 
-// `work` holds 2 n entries, `work_count` two counters, `gjk` n records.
+// work[0, cap) -> class-major work[cap, 2 cap); chunk counts after 3 cap.
+static void launch_bins(hipStream_t s, uint32_t* work, const uint32_t* work_count, uint32_t cap) {
+    const uint32_t chunks = nblocks(cap, kWorkChunk);
+    uint32_t* hist = work + 3 * size_t(cap);
+    k_bin_count<<<chunks, kWorkChunk, 0, s>>>(work, work_count, hist);
+    k_bin_scan<<<1, 1024, 0, s>>>(hist, work_count, chunks);
+    k_bin_scatter<<<chunks, kWorkChunk, 0, s>>>(work, work_count, hist, work + cap);
+}
+
+// `work` holds 3 n + 16 ceil(n / 1024) entries (the list, the class-major
+// list, the list handed to the wave kernel, the chunk counts), `work_count`
+// two counters, `gjk` n records; n < 2^27 (the pair index bits of an item).
 void launch_narrowphase(hipStream_t s, const uint2* pairs, uint32_t n, const BodyBuf& b, const ShapeBuf& sb,
                         const ManifoldBuf& m, uint32_t* work, uint32_t* work_count, void* gjk, float max_sep,
                         int world_mode, uint32_t wave_blocks) {
@@ -1290,26 +1373,29 @@ void launch_narrowphase(hipStream_t s, const uint2* pairs, uint32_t n, const Bod
     hipMemsetAsync(work_count, 0, 2 * sizeof(uint32_t), s);
     k_collide_simple<<<nblocks(n, 256), 256, 0, s>>>(pairs, n, nullptr, b, sb, m, work, work_count, max_sep,
                                                       world_mode);
+    launch_bins(s, work, work_count, n);
     GjkRec* rec = static_cast<GjkRec*>(gjk);
-    k_collide_lanes<<<nblocks(n, 256), 256, 0, s>>>(pairs, work, work_count, work + n, work_count + 1, rec, b, sb, m,
-                                                     max_sep);
+    k_collide_lanes<<<nblocks(n, 256), 256, 0, s>>>(pairs, work + n, work_count, work + 2 * size_t(n), work_count + 1,
+                                                     rec, b, sb, m, max_sep);
     uint32_t grid = wave_blocks ? wave_blocks : (n < 65536u ? n : 65536u);
-    k_collide_wave<<<grid, 64, 0, s>>>(pairs, work + n, work_count + 1, b, sb, m, max_sep, rec);
+    k_collide_wave<<<grid, 64, 0, s>>>(pairs, work + 2 * size_t(n), work_count + 1, b, sb, m, max_sep, rec);
 }
 
 // Capacity-sized launch with the pair count read on the device; both work
 // counters must already be zero (no memset: the launch may be
-// graph-captured).  work: 2 cap entries; gjk: cap records.
+// graph-captured).  work: 3 cap + 16 ceil(cap / 1024) entries; gjk: cap
+// records; cap < 2^27.
 void launch_narrowphase_counted(hipStream_t s, const uint2* pairs, uint32_t cap, const uint32_t* n_dev,
                                 const BodyBuf& b, const ShapeBuf& sb, const ManifoldBuf& m, uint32_t* work,
                                 uint32_t* work_count, void* gjk, float max_sep, uint32_t wave_blocks) {
     if (!cap) return;
     k_collide_simple<<<nblocks(cap, 256), 256, 0, s>>>(pairs, cap, n_dev, b, sb, m, work, work_count, max_sep, 1);
+    launch_bins(s, work, work_count, cap);
     GjkRec* rec = static_cast<GjkRec*>(gjk);
-    k_collide_lanes<<<nblocks(cap, 256), 256, 0, s>>>(pairs, work, work_count, work + cap, work_count + 1, rec, b, sb,
-                                                       m, max_sep);
-    k_collide_wave<<<wave_blocks ? wave_blocks : 1024u, 64, 0, s>>>(pairs, work + cap, work_count + 1, b, sb, m,
-                                                                    max_sep, rec);
+    k_collide_lanes<<<nblocks(cap, 256), 256, 0, s>>>(pairs, work + cap, work_count, work + 2 * size_t(cap),
+                                                       work_count + 1, rec, b, sb, m, max_sep);
+    k_collide_wave<<<wave_blocks ? wave_blocks : 1024u, 64, 0, s>>>(pairs, work + 2 * size_t(cap), work_count + 1, b,
+                                                                    sb, m, max_sep, rec);
 }
 
 }  // namespace kz
