@@ -395,7 +395,7 @@ def main():
     coop_bytes = 0.0
     coop_blocks = set()
     levels = []
-    phases = np.zeros(10)
+    phases = np.zeros(13)
     t_start = time.perf_counter()
     for i in range(args.steps):
         w.step_fixed()
@@ -403,8 +403,10 @@ def main():
         tm = device_timings(w)
         points += st["num_contact_points"]
         path_ms += st["contact_path_ms"]
+        # Device work that ran while the host finished the velocity schedule
+        # (the early warm start) counts once.
         resident_ms += (tm["narrowphase_ms"] + tm["build_device_ms"] + tm["build_ms"] + tm["integrate_velocities_ms"] +
-                        tm["solve_ms"])
+                        tm["solve_ms"] - tm["overlap_ms"])
         solve_kernel_ms += tm["solve_kernel_ms"]
         solve_bytes += st["num_contact_points"] * (WARM_BYTES + 8 * VEL_BYTES + 2 * BIAS_BYTES)
         coop_blocks.add(tm["cooperative_blocks"])
@@ -415,7 +417,8 @@ def main():
         levels.append((tm["velocity_levels"], tm["bias_levels"], tm["groups"], tm["islands"]))
         phases += [tm["narrowphase_ms"], tm["build_ms"], tm["integrate_velocities_ms"], tm["solve_ms"],
                    tm["solve_kernel_ms"], st["broadphase_ms"], tm["schedule_ms"], tm["build_device_ms"],
-                   tm["schedule_ws_ms"], tm["schedule_wait_ms"]]
+                   tm["schedule_ws_ms"], tm["schedule_wait_ms"], tm["early_warm_start_ms"],
+                   tm["schedule_velocity_after_ms"], tm["overlap_ms"]]
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     # value: the path's phases with inputs in HBM (kernel times, host build
@@ -439,9 +442,10 @@ def main():
         f"groups {lv[:, 2].mean():.0f}, islands {lv[:, 3].mean():.0f}")
     ph = phases / args.steps
     log(f"per step (ms): narrowphase {ph[0]:.2f}, build on the device {ph[7]:.2f} (compaction, islands), "
-        f"build on the host {ph[1]:.2f} (schedule {ph[6]:.2f}: warm-start order {ph[8]:.2f} beside, "
-        f"waited {ph[9]:.2f}), integrate velocities "
-        f"{ph[2]:.2f}, solve {ph[3]:.2f} (sweep kernel {ph[4]:.2f}), host broadphase {ph[5]:.2f}")
+        f"build on the host {ph[1]:.2f} (schedule {ph[6]:.2f}: warm-start order {ph[8]:.2f}, "
+        f"velocity order {ph[11]:.2f} after it, waited {ph[9]:.2f}), integrate velocities "
+        f"{ph[2]:.2f}, solve {ph[3]:.2f} (sweep kernel {ph[4]:.2f}; early warm start {ph[10]:.2f}), "
+        f"overlapped {ph[12]:.2f}, host broadphase {ph[5]:.2f}")
     if args.dump:
         import ctypes as C
         from kizuriphysics3d_amd import device as D

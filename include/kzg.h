@@ -89,6 +89,17 @@ typedef struct kzg_timings {
     /* Inside schedule_ms: the warm-start / bias sequence (built on a helper
      * thread), and how long the velocity sequence then waited for it. */
     float schedule_ws_ms, schedule_wait_ms;
+    /* Early warm start (SIMD solver, cache warm start, large worlds):
+     * kzg_build_constraints returns once the warm-start schedule is ready and
+     * kzg_solve sweeps the warm start while the host finishes the velocity
+     * schedule.  early_warm_start_ms: the device time of that part of the
+     * solve (inside solve_ms); schedule_velocity_after_ms: the host time the
+     * velocity schedule took after the warm-start one was ready (inside
+     * schedule_ms); overlap_ms: the device time (integrate velocities and the
+     * early warm start) hidden under it, min of the two.  The step's device
+     * and host phases then add up to narrowphase + build_device + build +
+     * integrate_velocities + solve - overlap.  All zero otherwise. */
+    float early_warm_start_ms, schedule_velocity_after_ms, overlap_ms;
 } kzg_timings;
 
 /* Create a context on HIP device `device` for a world with settings `world`. */
@@ -186,6 +197,14 @@ int kzg_set_cooperative(kzg_ctx* ctx, uint32_t blocks);
  * which shortens the level chain; each unit costs its points' arithmetic in
  * series.  Results are the same whatever the setting. */
 int kzg_set_sweep_units(kzg_ctx* ctx, uint32_t velocity, uint32_t warm_start);
+/* Early warm start (no reference counterpart: an overlap of host and device
+ * work inside PhysicsWorld::StepFixed's order).  With the SIMD solver and the
+ * cache warm start, kzg_build_constraints may return once the warm-start
+ * schedule is built, and kzg_solve then sweeps the warm start on the device
+ * while the host finishes the velocity (first-fit) schedule.  mode: 0 = auto
+ * (worlds of 16,384 or more contact points), 1 = off, 2 = always.  Results
+ * are the same either way. */
+int kzg_set_early_warm_start(kzg_ctx* ctx, uint32_t mode);
 
 /* ---- Learned warm start (no reference counterpart) ---------------------
  * A per-contact-point MLP 14 -> 64 -> 64 -> 3 with ReLU hidden layers,
@@ -240,7 +259,9 @@ int kzg_mlp_gradients(kzg_ctx* ctx, const float* samples, uint32_t n, float* gra
  * constraint order.  flags: KZG_SCHEDULE_SIMD for the first-fit velocity
  * order (else the constraint order), KZG_SCHEDULE_UNITS to join points into
  * multi-point units with kzg_solve's default caps (else one point per
- * unit).  Outputs
+ * unit), KZG_SCHEDULE_DEFER to build it as kzg_build_constraints does for
+ * an early warm start (warm-start order first, velocity order finished on
+ * the helper thread; the result is the same).  Outputs
  * per point (each optional): vel_level (velocity sweep level, 0 = not
  * swept) and ws_level (warm-start / bias level), levels counted from 1
  * within the point's group; group[p] = its group; vel_unit / ws_unit = the
@@ -252,6 +273,7 @@ int kzg_mlp_gradients(kzg_ctx* ctx, const float* samples, uint32_t n, float* gra
  * (src/World/PhysicsWorld.cpp:587-717). */
 #define KZG_SCHEDULE_SIMD 1u
 #define KZG_SCHEDULE_UNITS 2u
+#define KZG_SCHEDULE_DEFER 4u
 int kzg_schedule(uint32_t num_bodies, const uint8_t* dynamic, uint32_t num_constraints, const uint32_t* info,
                  uint32_t flags, uint32_t max_groups, uint32_t* vel_level, uint32_t* ws_level, uint32_t* group,
                  uint32_t* vel_unit, uint32_t* ws_unit, uint32_t* stats);

@@ -27,12 +27,13 @@ struct CoopPlan {
 // island).  `request` 0 sizes the split from the group's mean level width;
 // `max_threads` is the kernel's workgroup limit; a velocity unit takes
 // `vel_lanes` threads.
-CoopPlan plan_cooperative(const ScheduleOutput& s, uint32_t request, uint32_t max_threads, uint32_t vel_lanes) {
+CoopPlan plan_cooperative(const ScheduleOutput& s, uint32_t request, uint32_t max_threads, uint32_t vel_lanes,
+                          bool with_vel = true) {
     CoopPlan p;
     if (request == 1 || s.groups == 0 || s.ws.grp.empty()) return p;
     uint32_t units = 0, levels = 0, widest = 0;
     for (const LevelSchedule* ls : {&s.ws, &s.vel}) {
-        if (ls->grp.empty()) continue;
+        if (ls->grp.empty() || (ls == &s.vel && !with_vel)) continue;
         const U2 g = ls->grp[0];
         const uint32_t lanes = ls == &s.vel ? vel_lanes : 1u;
         for (uint32_t l = g.x; l < g.y; ++l) widest = std::max(widest, lanes * (ls->off[l + 1] - ls->off[l]));
@@ -46,6 +47,12 @@ CoopPlan plan_cooperative(const ScheduleOutput& s, uint32_t request, uint32_t ma
     const uint32_t slice = (widest + p.blocks - 1) / p.blocks;
     while (p.threads < slice && p.threads < max_threads) p.threads *= 2;
     return p;
+}
+
+// A deferred velocity schedule must be complete before anything reads the
+// host arrays it is built from or the schedule itself.
+void join_schedule(kzg_ctx* c) {
+    if (c->sched_builder.pending()) c->sched_builder.finish();
 }
 
 }  // namespace
@@ -77,6 +84,7 @@ int kzg_create(kzg_ctx** out, const kz_world_desc* world, int device) {
 
 void kzg_destroy(kzg_ctx* c) {
     if (!c) return;
+    join_schedule(c);
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto* d : {&c->b_x, &c->b_q, &c->b_v, &c->b_w, &c->b_pv, &c->b_pw, &c->b_iw, &c->b_il, &c->b_mat, &c->b_lim,
@@ -121,6 +129,7 @@ void kzg_destroy(kzg_ctx* c) {
     c->stage2.release();
     c->stage_a.release();
     c->stage_b.release();
+    c->stage_v.release();
     c->pin_totals.release();
     c->pin_info.release();
     c->compact_tile.release();
@@ -184,6 +193,7 @@ int kzg_upload_shapes(kzg_ctx* c, const kz_shape_desc* shapes, uint32_t n, const
 int kzg_upload_bodies(kzg_ctx* c, const kzg_body* b, uint32_t n) {
     if (!c || (n && !b)) return fail(c, KZG_ERR_ARGUMENT, "null body array");
     if (n >= (1u << 24)) return fail(c, KZG_ERR_ARGUMENT, "too many bodies");
+    join_schedule(c);
     hipSetDevice(c->device);
     std::vector<float4> x(n), q(n), v(n), w(n), z(n, f4(0, 0, 0, 0)), il(3 * size_t(n)), mat(n), lim(n);
     std::vector<uint32_t> meta(n), packed(n);
@@ -253,6 +263,7 @@ int kzg_upload_states(kzg_ctx* c, const kz_body_state* s, uint32_t n) {
 int kzg_narrowphase(kzg_ctx* c, const uint32_t* pairs, uint32_t n) {
     if (!c || (n && !pairs)) return fail(c, KZG_ERR_ARGUMENT, "null pair array");
     if (n > kWorkPairMask) return fail(c, KZG_ERR_UNSUPPORTED, "more than 2^27 - 1 broadphase pairs");
+    join_schedule(c);
     hipSetDevice(c->device);
     for (uint32_t i = 0; i < 2 * n; ++i)
         if (pairs[i] >= c->nbodies) return fail(c, KZG_ERR_ARGUMENT, "pair " + std::to_string(i / 2) + ": bad body index");
@@ -305,6 +316,7 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
         return fail(c, KZG_ERR_STATE, "learned warm start needs network weights (kzg_mlp_init or kzg_mlp_set_params)");
     c->cur_mode = mode;
     c->no_store = (warm & KZG_WARM_NO_STORE) != 0;
+    join_schedule(c);
     hipSetDevice(c->device);
     // Constraint and point totals and the cache occupancy (the wait is the
     // narrowphase and the compaction still running).
@@ -358,14 +370,40 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     si.max_unit_vel = c->unit_cap_vel ? c->unit_cap_vel : kDefaultUnitVel;
     si.max_unit_ws = c->unit_cap_ws ? c->unit_cap_ws : kDefaultUnitWs;
     si.max_groups = c->cu_count * 4u;
+    // With the SIMD solver and the cache warm start the velocity schedule may
+    // finish on the builder's helper thread: kzg_solve sweeps the warm start
+    // meanwhile.  (The learned warm start writes velocity records before the
+    // sweeps, so it waits for the whole schedule.)
+    const bool may_defer = c->world.use_simd_solver && mode != KZG_WARM_LEARNED && mode != KZG_WARM_LEARNED_MISS &&
+                           c->early_ws_mode != 1;
+    const uint32_t defer = !may_defer               ? ScheduleBuilder::kDeferNever
+                           : c->early_ws_mode == 2 ? ScheduleBuilder::kDeferAlways
+                                                   : ScheduleBuilder::kDeferLarge;
     const auto ts = std::chrono::steady_clock::now();
-    c->sched_builder.build(si, c->sched);
+    c->early_ws = c->sched_builder.start(si, c->sched, defer);
+    c->vel_after_ms = 0.0;
     // Host compute of the build ends here; what follows stages the
     // schedule for the device.
     const auto te = std::chrono::steady_clock::now();
     c->build_host_ms = std::chrono::duration<double, std::milli>(te - t0).count();
     c->schedule_host_ms = std::chrono::duration<double, std::milli>(te - ts).count();
-    const LevelSchedule &ws = c->sched.ws, &vel = c->sched.vel;
+    const LevelSchedule& ws = c->sched.ws;
+    if (c->early_ws) {
+        // The velocity schedule is still being built: only the warm-start one goes up.
+        const size_t bytes = sizeof(uint32_t) * (ws.off.size() + ws.unit.size() + size_t(np)) +
+                             sizeof(U2) * ws.grp.size() + 5 * 256;
+        KZG_HIP(c, c->stage_b.ensure(bytes));
+        size_t off = 0;
+        if ((rc = upload_async(c, c->ws_off, c->stage_b, off, ws.off.data(), ws.off.size())) ||
+            (rc = upload_async(c, c->ws_grp, c->stage_b, off, reinterpret_cast<const uint2*>(ws.grp.data()),
+                               ws.grp.size())) ||
+            (rc = upload_async(c, c->ws_pos, c->stage_b, off, ws.pos.data(), ws.pos.size())) ||
+            (rc = upload_async(c, c->ws_unit, c->stage_b, off, ws.unit.data(), ws.unit.size())))
+            return rc;
+        c->phase = 2;
+        return KZG_OK;
+    }
+    const LevelSchedule& vel = c->sched.vel;
     const size_t bytes = sizeof(uint32_t) * (ws.off.size() + vel.off.size() + ws.unit.size() + vel.unit.size() +
                                              2 * size_t(np)) +
                          sizeof(U2) * (ws.grp.size() + vel.grp.size()) + 10 * 256;
@@ -408,10 +446,12 @@ int kzg_solve(kzg_ctx* c, float dt) {
         base = c->base.p;
     }
     const float g = std::sqrt(w.gravity[0] * w.gravity[0] + w.gravity[1] * w.gravity[1] + w.gravity[2] * w.gravity[2]);
+    const bool early = c->early_ws;
+    // Early warm start: the velocity records wait for their schedule.
     launch_prepare(c->stream, c->contacts(), c->bodies(), c->npoints, dt, w.restitution_threshold, w.baumgarte,
                    w.penetration_slop, w.max_bias_velocity, w.use_simd_solver ? 1 : 0, w.use_velocity_bias ? 1 : 0,
-                   c->vel_records(), c->ws_records(), c->vel_pos.p, c->ws_pos.p, (g > 0.0f ? g : 1.0f) * dt, feat,
-                   base);
+                   c->vel_records(), c->ws_records(), early ? nullptr : c->vel_pos.p, c->ws_pos.p,
+                   (g > 0.0f ? g : 1.0f) * dt, feat, base);
     KZG_HIP(c, hipGetLastError());
     if (learned)
         launch_mlp_warm(c->stream, c->mlp(), feat, base, c->npoints, c->cur_mode == KZG_WARM_LEARNED_MISS ? 1 : 0,
@@ -421,13 +461,62 @@ int kzg_solve(kzg_ctx* c, float dt) {
     SolveParams sp{dt, w.warm_start_factor, w.velocity_iterations, w.position_iterations, w.use_simd_solver,
                    w.use_velocity_bias};
     KZG_HIP(c, c->ws_udesc.ensure(std::max<uint32_t>(c->sched.ws.units, 1)));
-    KZG_HIP(c, c->vel_udesc.ensure(std::max<uint32_t>(c->sched.vel.units, 1)));
     launch_unit_desc(c->stream, c->ws_unit.p, c->ws_records().bd, c->sched.ws.units, c->ws_udesc.p);
-    launch_unit_desc(c->stream, c->vel_unit.p, c->vel_records().bd, c->sched.vel.units, c->vel_udesc.p);
     Schedule ws{c->ws_off.p, c->ws_grp.p, c->ws_udesc.p};
-    Schedule vel{c->vel_off.p, c->vel_grp.p, c->vel_udesc.p};
     // The SIMD velocity sweeps take two threads per unit (a lane pair).
     const uint32_t vel_lanes = w.use_simd_solver ? 2u : 1u;
+    if (early && c->ncons) {
+        // The warm-start sweep, while the host finishes the velocity schedule.
+        KZG_HIP(c, c->b_packed_solver.ensure(8 * size_t(std::max<uint32_t>(c->nbodies, 1))));
+        const UnitKernel uk = unit_kernel(c->sched.ws.largest_unit, 1u);
+        uint32_t threads = 64;
+        while (threads < c->sched.ws.max_width && threads < solve_threads(uk.ws, uk.vel)) threads *= 2;
+        const CoopPlan coop = plan_cooperative(c->sched, c->coop_blocks, solve_threads(uk.ws, uk.vel), 1u, false);
+        SolveParams spw = sp;
+        spw.phases = 1u;
+        const Schedule none{nullptr, nullptr, nullptr};
+        KZG_HIP(c, record(c, 14));
+        if (coop.blocks > 1) {
+            KZG_HIP(c, c->coop_counter.ensure(1));
+            KZG_HIP(c, launch_solve_split(c->stream, c->vel_records(), c->ws_records(), c->bodies(), c->nbodies,
+                                          c->b_packed_solver.p, ws, none, spw, c->sched.groups, threads, c->resid.p,
+                                          uk, coop.blocks, coop.threads, c->coop_counter.p, c->ev[17], c->ev[18],
+                                          kPackBodies));
+            c->recorded |= 3u << 17;
+        } else {
+            launch_solve_groups(c->stream, c->vel_records(), c->ws_records(), c->bodies(), c->nbodies,
+                                c->b_packed_solver.p, ws, none, spw, c->sched.groups, threads, c->resid.p, uk,
+                                kPackBodies);
+        }
+        KZG_HIP(c, hipGetLastError());
+        KZG_HIP(c, record(c, 15));
+        sp.phases = 2u;
+    }
+    if (early) {
+        // The velocity schedule: wait for it, send it up, then its records.
+        c->sched_builder.finish();
+        c->vel_after_ms = c->sched_builder.vel_after_ms;
+        c->build_host_ms += c->vel_after_ms;
+        c->schedule_host_ms += c->vel_after_ms;
+        const LevelSchedule& vs = c->sched.vel;
+        const size_t bytes = sizeof(uint32_t) * (vs.off.size() + vs.unit.size() + size_t(c->npoints)) +
+                             sizeof(U2) * vs.grp.size() + 5 * 256;
+        KZG_HIP(c, c->stage_v.ensure(bytes));
+        size_t off = 0;
+        int rc;
+        if ((rc = upload_async(c, c->vel_off, c->stage_v, off, vs.off.data(), vs.off.size())) ||
+            (rc = upload_async(c, c->vel_grp, c->stage_v, off, reinterpret_cast<const uint2*>(vs.grp.data()),
+                               vs.grp.size())) ||
+            (rc = upload_async(c, c->vel_pos, c->stage_v, off, vs.pos.data(), vs.pos.size())) ||
+            (rc = upload_async(c, c->vel_unit, c->stage_v, off, vs.unit.data(), vs.unit.size())))
+            return rc;
+        KZG_HIP(c, record(c, 16));
+        launch_velocity_records(c->stream, c->contacts(), c->ws_records(), c->vel_records(), c->vel_pos.p,
+                                c->ws_pos.p, c->npoints, w.use_simd_solver ? 1 : 0, w.use_velocity_bias ? 1 : 0);
+    }
+    KZG_HIP(c, c->vel_udesc.ensure(std::max<uint32_t>(c->sched.vel.units, 1)));
+    launch_unit_desc(c->stream, c->vel_unit.p, c->vel_records().bd, c->sched.vel.units, c->vel_udesc.p);
+    Schedule vel{c->vel_off.p, c->vel_grp.p, c->vel_udesc.p};
     const uint32_t width = std::max(c->sched.ws.max_width, vel_lanes * c->sched.vel.max_width);
     const UnitKernel uk = unit_kernel(c->sched.ws.largest_unit, c->sched.vel.largest_unit);
     uint32_t threads = 64;
@@ -444,11 +533,12 @@ int kzg_solve(kzg_ctx* c, float dt) {
             KZG_HIP(c, launch_solve_split(c->stream, c->vel_records(), c->ws_records(), c->bodies(), c->nbodies,
                                           c->b_packed_solver.p, ws, vel, sp, c->sched.groups, threads, c->resid.p,
                                           uk, coop.blocks, coop.threads, c->coop_counter.p, c->ev[10],
-                                          c->ev[11]));
+                                          c->ev[11], early ? kUnpackBodies : kPackBodies | kUnpackBodies));
             c->recorded |= 3u << 10;
         } else {
             launch_solve_groups(c->stream, c->vel_records(), c->ws_records(), c->bodies(), c->nbodies,
-                                c->b_packed_solver.p, ws, vel, sp, c->sched.groups, threads, c->resid.p, uk);
+                                c->b_packed_solver.p, ws, vel, sp, c->sched.groups, threads, c->resid.p, uk,
+                                early ? kUnpackBodies : kPackBodies | kUnpackBodies);
         }
     }
     KZG_HIP(c, hipGetLastError());
@@ -635,6 +725,7 @@ int kzg_get_residuals(kzg_ctx* c, double* out, uint32_t cap) {
 
 int kzg_get_timings(kzg_ctx* c, kzg_timings* out) {
     if (!c || !out) return KZG_ERR_ARGUMENT;
+    join_schedule(c);
     hipSetDevice(c->device);
     KZG_HIP(c, hipStreamSynchronize(c->stream));
     kzg_timings t{};
@@ -653,8 +744,22 @@ int kzg_get_timings(kzg_ctx* c, kzg_timings* out) {
     t.schedule_ws_ms = float(c->sched_builder.ws_ms);
     t.schedule_wait_ms = float(c->sched_builder.wait_ms);
     span(8, 4, t.integrate_velocities_ms);
-    span(4, 9, t.solve_ms);
-    span(5, 6, t.solve_kernel_ms);
+    if (c->early_ws) {
+        // The solve's device work without the wait for the velocity schedule
+        // and its upload: the warm-start part, then the velocity part.
+        float part = 0.0f, sweep = 0.0f;
+        span(4, 15, t.early_warm_start_ms);
+        span(16, 9, part);
+        t.solve_ms = t.early_warm_start_ms + part;
+        span(14, 15, sweep);
+        span(5, 6, t.solve_kernel_ms);
+        t.solve_kernel_ms += sweep;
+        t.schedule_velocity_after_ms = float(c->vel_after_ms);
+        t.overlap_ms = std::min(t.schedule_velocity_after_ms, t.integrate_velocities_ms + t.early_warm_start_ms);
+    } else {
+        span(4, 9, t.solve_ms);
+        span(5, 6, t.solve_kernel_ms);
+    }
     span(9, 7, t.integrate_positions_ms);
     (void)hipGetLastError();
     t.velocity_levels = c->sched.vel.max_depth;
@@ -665,6 +770,9 @@ int kzg_get_timings(kzg_ctx* c, kzg_timings* out) {
         t.cooperative_ws_points = c->coop_ws_points;
         t.cooperative_vel_points = c->coop_vel_points;
         span(10, 11, t.cooperative_kernel_ms);
+        float early = 0.0f;
+        span(17, 18, early);
+        t.cooperative_kernel_ms += early;
     }
     t.islands = c->sched.islands;
     *out = t;
@@ -693,7 +801,12 @@ int kzg_schedule(uint32_t nb, const uint8_t* dyn, uint32_t nc, const uint32_t* i
     si.max_unit_vel = (flags & KZG_SCHEDULE_UNITS) ? kDefaultUnitVel : 1u;
     si.max_unit_ws = (flags & KZG_SCHEDULE_UNITS) ? kDefaultUnitWs : 1u;
     si.max_groups = std::max<uint32_t>(max_groups, 1);
-    b.build(si, out);
+    if (flags & KZG_SCHEDULE_DEFER) {
+        b.start(si, out, ScheduleBuilder::kDeferAlways);
+        b.finish();
+    } else {
+        b.build(si, out);
+    }
     auto fill = [&](const LevelSchedule& s, uint32_t* lvl, uint32_t* unit, uint32_t* grp) {
         if (lvl) std::fill(lvl, lvl + np, 0u);
         if (unit) std::fill(unit, unit + np, 0xFFFFFFFFu);
@@ -785,6 +898,13 @@ int kzg_set_cooperative(kzg_ctx* c, uint32_t blocks) {
     if (!c) return KZG_ERR_ARGUMENT;
     if (blocks > kMaxCoopBlocks) return fail(c, KZG_ERR_ARGUMENT, "more than 64 cooperative workgroups");
     c->coop_blocks = blocks;
+    return KZG_OK;
+}
+
+int kzg_set_early_warm_start(kzg_ctx* c, uint32_t mode) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (mode > 2) return fail(c, KZG_ERR_ARGUMENT, "early warm start mode is 0 (auto), 1 (off) or 2 (always)");
+    c->early_ws_mode = mode;
     return KZG_OK;
 }
 

@@ -32,15 +32,21 @@ void launch_prepare(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32
                     float4* base);
 void launch_writeback(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const ContactBuf& c,
                       const uint32_t* vel_pos, const uint32_t* ws_pos, uint32_t np);
+// `bodies`: kPackBodies / kUnpackBodies (kz_device.h).
 void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b, uint32_t nbodies,
                          float4* packed, const Schedule& ws, const Schedule& vel, const SolveParams& sp,
-                         uint32_t groups, uint32_t threads, float* resid, UnitKernel uk);
+                         uint32_t groups, uint32_t threads, float* resid, UnitKernel uk,
+                         uint32_t bodies = kPackBodies | kUnpackBodies);
+void launch_velocity_records(hipStream_t s, const ContactBuf& c, const WsRecBuf& wr, const VelRecBuf& vr,
+                             const uint32_t* vel_pos, const uint32_t* ws_pos, uint32_t np, int use_simd,
+                             int use_vbias);
 void launch_unit_desc(hipStream_t s, const uint32_t* unit, const uint4* bd, uint32_t nu, uint4* desc);
 hipError_t launch_solve_split(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b,
                               uint32_t nbodies, float4* packed, const Schedule& ws, const Schedule& vel,
                               const SolveParams& sp, uint32_t groups, uint32_t threads, float* resid,
                               UnitKernel uk, uint32_t coop_blocks, uint32_t coop_threads, uint32_t* counter,
-                              hipEvent_t coop_begin, hipEvent_t coop_end);
+                              hipEvent_t coop_begin, hipEvent_t coop_end,
+                              uint32_t bodies = kPackBodies | kUnpackBodies);
 void launch_compact(hipStream_t s, const uint32_t* m_count, uint32_t n, const uint2* pairs, const BodyBuf& b,
                     const ContactBuf& c, uint2* tile, uint32_t* totals);
 void launch_islands(hipStream_t s, const ContactBuf& c, uint32_t cap, const BodyBuf& b, uint32_t nb,
@@ -179,16 +185,23 @@ struct kzg_ctx {
     uint32_t resid_points = 0;
 
     // staging and timing
-    Pinned stage, stage2, stage_a, stage_b;
+    Pinned stage, stage2, stage_a, stage_b, stage_v;  // stage_v: a deferred velocity schedule
     Pinned pin_totals;  // constraints, points and cache occupancy of the step
     Pinned pin_info;    // the step's constraint table (the schedule's input)
     DevVec<uint2> compact_tile;
     DevVec<uint32_t> compact_tot;  // constraints, points, islands, 0
     DevVec<uint32_t> uf_parent, uf_island, con_island;
-    hipEvent_t ev[14] = {};  // 12: the constraint table has reached the host; 13: device build end
+    hipEvent_t ev[19] = {};  // 12: the constraint table has reached the host; 13: device build end;
+    // 14/15: early warm-start sweep begin/end, 16: velocity part begin, 17/18: early cooperative kernel
     uint32_t recorded = 0;  // bit i: ev[i] recorded since the last narrowphase
     kzg_timings timings{};
     double build_host_ms = 0.0, schedule_host_ms = 0.0;
+    // This step's warm-start sweep was launched before the velocity schedule
+    // was ready (the builder finishes it in kzg_solve); the host time the
+    // velocity schedule took after the warm-start one was ready.
+    bool early_ws = false;
+    uint32_t early_ws_mode = 0;  // kzg_set_early_warm_start: 0 auto, 1 off, 2 always
+    double vel_after_ms = 0.0;
     int phase = 0;  // 0 idle, 1 narrowphase done, 2 built, 3 solved
 
     // learned warm start

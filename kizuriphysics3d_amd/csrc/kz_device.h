@@ -155,7 +155,14 @@ struct SolveParams {
     uint32_t position_iterations;
     uint32_t use_simd;
     uint32_t use_velocity_bias;
+    // Which sweeps a launch runs: 1 the warm start, 2 the velocity
+    // iterations (with the bias sweeps alongside), 3 both in order.
+    uint32_t phases = 3;
 };
+// The sweeps run on packed copies of the bodies: whether a sweep launch
+// packs them first and unpacks them after (a warm-start-only launch packs,
+// the velocity launch that follows it unpacks).
+constexpr uint32_t kPackBodies = 1, kUnpackBodies = 2;
 
 __host__ __device__ inline uint64_t pair_key(uint32_t a, uint32_t b) {
     return a < b ? ((uint64_t(a) << 32) | b) : ((uint64_t(b) << 32) | a);

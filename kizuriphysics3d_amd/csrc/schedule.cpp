@@ -25,7 +25,10 @@
 //
 // The warm-start/bias sequence (constraint order) and the velocity sequence
 // (first-fit order) are independent once the islands are known; above a few
-// thousand points the first runs on a helper thread beside the second.
+// thousand points the first runs on a helper thread beside the second.  A
+// deferred build swaps them: the caller builds the warm-start sequence and
+// gets it back while the velocity sequence, the longer one, is still being
+// built on the helper, so the device can sweep the warm start meanwhile.
 #include "schedule.h"
 
 #include <algorithm>
@@ -487,6 +490,14 @@ void ScheduleBuilder::vel_levels(const ScheduleInput& in) {
 }
 
 void ScheduleBuilder::build(const ScheduleInput& in, ScheduleOutput& out) {
+    start(in, out, kDeferNever);
+    finish();
+}
+
+bool ScheduleBuilder::start(const ScheduleInput& in_arg, ScheduleOutput& out, uint32_t defer) {
+    finish();
+    in_ = in_arg;
+    const ScheduleInput& in = in_;
     const uint32_t nc = in.num_constraints;
     uint32_t num_points = 0;
     for (uint32_t c = 0; c < nc; ++c) num_points += in.info[c].w & 0xFFu;
@@ -511,24 +522,49 @@ void ScheduleBuilder::build(const ScheduleInput& in, ScheduleOutput& out) {
         flatten(ws_, groups_, num_points, out.ws);
         ws_ms = std::chrono::duration<double, std::milli>(clock::now() - t0).count();
     };
-    const bool parallel = num_points >= kParallelPoints;
-    const bool simd = in.use_simd;
-    if (parallel) run_helper([&ws_chain] { ws_chain(true); });
-    islands(in, out);
-    groups_ready_.store(true, std::memory_order_release);
-    if (!parallel) ws_chain(false);
-    if (simd) {
+    auto vel_chain = [this, &in, &out, num_points] {
         vel_body_level_.assign(in.num_bodies, 0);
         vel_.runs.clear();
         fit_island_end_.assign(island_points_.size(), 0);
         vel_fit(in);
         vel_levels(in);
         flatten_points(vel_, groups_, num_points, out.vel);
+    };
+    const bool parallel = num_points >= kParallelPoints;
+    const bool simd = in.use_simd;
+    vel_after_ms = wait_ms = 0.0;
+    if (simd && num_points && (defer == kDeferAlways || (defer == kDeferLarge && parallel))) {
+        // The islands first (both sequences need the groups), then the
+        // velocity sequence on the helper and the warm-start one here.
+        islands(in, out);
+        run_helper([this, vel_chain] {
+            vel_chain();
+            vel_done_ = clock::now();
+        });
+        ws_chain(false);
+        ws_ready_ = clock::now();
+        pending_ = true;
+        return true;
     }
+    if (parallel) run_helper([&ws_chain] { ws_chain(true); });
+    islands(in, out);
+    groups_ready_.store(true, std::memory_order_release);
+    if (!parallel) ws_chain(false);
+    if (simd) vel_chain();
     const auto tw = clock::now();
     if (parallel) wait_helper();
     wait_ms = std::chrono::duration<double, std::milli>(clock::now() - tw).count();
     if (!in.use_simd) out.vel = out.ws;  // the scalar path sweeps in constraint order, sensors included
+    return false;
+}
+
+void ScheduleBuilder::finish() {
+    if (!pending_) return;
+    const auto tw = std::chrono::steady_clock::now();
+    wait_helper();
+    pending_ = false;
+    wait_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw).count();
+    vel_after_ms = std::max(0.0, std::chrono::duration<double, std::milli>(vel_done_ - ws_ready_).count());
 }
 
 }  // namespace kz

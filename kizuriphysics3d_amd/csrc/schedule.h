@@ -88,9 +88,21 @@ public:
     ScheduleBuilder& operator=(const ScheduleBuilder&) = delete;
     ~ScheduleBuilder();
     void build(const ScheduleInput& in, ScheduleOutput& out);
-    // Of the last build: the warm-start sequence's time (helper thread) and
-    // the time the velocity sequence waited for it.
-    double ws_ms = 0.0, wait_ms = 0.0;
+    // A build in two halves.  start() returns once the islands, the groups
+    // and the warm-start sequence (out.ws) are complete.  With the SIMD
+    // solver and `defer` = kDeferLarge (above the two-thread size) or
+    // kDeferAlways, the velocity sequence is still being built on the helper
+    // thread then; it reads the arrays `in` points to and writes out.vel
+    // until finish() returns, so neither may be touched before.  start()
+    // returns whether the velocity sequence is pending.
+    static constexpr uint32_t kDeferNever = 0, kDeferLarge = 1, kDeferAlways = 2;
+    bool start(const ScheduleInput& in, ScheduleOutput& out, uint32_t defer);
+    void finish();
+    bool pending() const { return pending_; }
+    // Of the last build: the warm-start sequence's time, the time spent
+    // waiting for the other thread's sequence, and (deferred builds) the
+    // velocity sequence's time after start() returned.
+    double ws_ms = 0.0, wait_ms = 0.0, vel_after_ms = 0.0;
 
 private:
     // A sequence of points and their units, before bucketing.  The two
@@ -133,6 +145,9 @@ private:
     std::function<void()> job_;
     bool stop_ = false, busy_ = false;
     std::atomic<bool> groups_ready_{false};
+    bool pending_ = false;
+    ScheduleInput in_{};
+    std::chrono::steady_clock::time_point ws_ready_, vel_done_;
     std::vector<uint32_t> fit_island_end_;  // per island: end of its batches
 };
 

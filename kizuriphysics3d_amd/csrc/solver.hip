@@ -207,7 +207,7 @@ __global__ void k_prepare(ContactBuf c, BodyBuf b, uint32_t np, const uint32_t* 
     // anchors the reference rotates on every visit (ContactSolverSIMD.cpp:
     // 106-107, ContactSolver.cpp:268-269) are rotated once here: same values.
     const f3 qra = rotate(qa, la), qrb = rotate(qb, lb);
-    const uint32_t e = vel_pos[p];
+    const uint32_t e = vel_pos ? vel_pos[p] : 0xFFFFFFFFu;  // none: k_velocity_records writes them later
     if (e != 0xFFFFFFFFu) {
         // SIMD lanes add (positionBias or 0) to the restitution bias; the
         // scalar path adds positionBias only when enabled (ContactSolverSIMD.cpp:128,
@@ -233,6 +233,31 @@ __global__ void k_prepare(ContactBuf c, BodyBuf b, uint32_t np, const uint32_t* 
     wr.t1[w] = st3(t1, 0.0f);
     wr.t2[w] = st3(t2, 0.0f);
     wr.imp[w] = imp;
+}
+
+// The velocity records of a prepare launch that skipped them, from what it
+// wrote: the constraint table (normal and tangents with their masses, the
+// clamped impulses, friction, restitution and position biases) and the
+// warm-start records (rotated anchors, inverse masses).  Every field is the
+// same value k_prepare stores, so the records are identical.
+__global__ void k_velocity_records(ContactBuf c, WsRecBuf wr, VelRecBuf vr, const uint32_t* vel_pos,
+                                   const uint32_t* ws_pos, uint32_t np, int use_simd, int use_vbias) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const uint32_t e = vel_pos[p];
+    if (e == 0xFFFFFFFFu) return;
+    const uint32_t w = ws_pos[p];
+    const float4 ra = wr.ra[w], rb = wr.rb[w], qa = wr.qa[w], qb = wr.qb[w], imp = c.imp[p];
+    const float rbias = c.ra[p].w, pbias = c.rb[p].w;
+    const float nbias = use_simd ? rbias + (use_vbias ? pbias : 0.0f) : (use_vbias ? rbias + pbias : rbias);
+    vr.bd[e] = c.body[p];
+    vr.ra[e] = make_float4(qa.x, qa.y, qa.z, ra.w);
+    vr.rb[e] = make_float4(qb.x, qb.y, qb.z, rb.w);
+    vr.n[e] = c.n[p];
+    vr.t1[e] = c.t1[p];
+    vr.t2[e] = c.t2[p];
+    vr.fb[e] = make_float2(c.la[p].w, nbias);
+    vr.imp[e] = make_float4(imp.x, imp.y, imp.z, 0.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -755,19 +780,23 @@ __device__ __forceinline__ void solve_group(const VelRecBuf& vr, const WsRecBuf&
                                             const Schedule& ws, const Schedule& vel, const SolveParams& sp,
                                             float* resid, uint32_t group, uint32_t parts, uint32_t part,
                                             uint32_t* ws_lds, uint32_t* vel_lds, Sync&& sync) {
+    // A warm-start-only launch may run before the velocity schedule exists:
+    // it reads nothing of it.
+    const bool velocity = (sp.phases & 2u) != 0;
     const uint2 wrange = ws.grp[group];
-    const uint2 vrange = vel.grp[group];
+    const uint2 vrange = velocity ? vel.grp[group] : make_uint2(0u, 0u);
     const LevelOffsets wo = stage_offsets(ws, wrange, ws_lds);
-    const LevelOffsets vo = stage_offsets(vel, vrange, vel_lds);
+    const LevelOffsets vo = velocity ? stage_offsets(vel, vrange, vel_lds) : wo;
     __syncthreads();
     const float wsf = sp.warm_start_factor;
-    {
+    if (sp.phases & 1u) {
         LevelStream<UW, 1> w(wo, wrange, wr.bd, ws.unit, parts, part);
         for (uint32_t k = wrange.x; k < wrange.y; ++k) {
             w.step([&](uint32_t e, uint32_t n, uint4 bd, uint32_t) { warm_unit<UW>(wr, sbv, e, n, bd, wsf); });
             sync();
         }
     }
+    if (!velocity) return;
     if (sp.use_simd)
         velocity_and_bias<UW, UV, 2>(
             wr, sbp, wo, wrange, vo, vrange, vr.bd, ws, vel, sp, resid, parts, part,
@@ -951,6 +980,11 @@ void launch_prepare(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32
     PrepParams pp{dt, rthr, baum, slop, maxb, use_simd, use_vbias, gdt, feat, base};
     if (np) k_prepare<<<blocks(np, 256), 256, 0, s>>>(c, b, np, nullptr, pp, vr, wr, vel_pos, ws_pos);
 }
+void launch_velocity_records(hipStream_t s, const ContactBuf& c, const WsRecBuf& wr, const VelRecBuf& vr,
+                             const uint32_t* vel_pos, const uint32_t* ws_pos, uint32_t np, int use_simd,
+                             int use_vbias) {
+    if (np) k_velocity_records<<<blocks(np, 256), 256, 0, s>>>(c, wr, vr, vel_pos, ws_pos, np, use_simd, use_vbias);
+}
 void launch_writeback(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const ContactBuf& c,
                       const uint32_t* vel_pos, const uint32_t* ws_pos, uint32_t np) {
     if (np) k_writeback<<<blocks(np, 256), 256, 0, s>>>(vr, wr, c, vel_pos, ws_pos, np, nullptr);
@@ -975,7 +1009,9 @@ static auto by_units(UnitKernel uk, F&& f) {
 static void launch_groups(hipStream_t s, UnitKernel uk, uint32_t n, uint32_t threads, const VelRecBuf& vr,
                           const WsRecBuf& wr, float4* sbv, float4* sbp, const Schedule& ws, const Schedule& vel,
                           const SolveParams& sp, float* resid, uint32_t group0) {
-    if (!ws.unit || !vel.unit) uk = UnitKernel{};
+    // Schedules without unit descriptors (the batched worlds) take one-point
+    // units; a warm-start-only launch has no velocity schedule and needs none.
+    if (!ws.unit || (!vel.unit && (sp.phases & 2u))) uk = UnitKernel{};
     by_units(uk, [&](auto w, auto v) {
         k_solve_groups<decltype(w)::value, decltype(v)::value>
             <<<n, threads, 0, s>>>(vr, wr, sbv, sbp, ws, vel, sp, resid, group0);
@@ -993,13 +1029,13 @@ static const void* coop_kernel(UnitKernel uk) {
 // ignored without unit descriptors).
 void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b, uint32_t nbodies,
                          float4* packed, const Schedule& ws, const Schedule& vel, const SolveParams& sp,
-                         uint32_t groups, uint32_t threads, float* resid, UnitKernel uk) {
+                         uint32_t groups, uint32_t threads, float* resid, UnitKernel uk, uint32_t bodies) {
     if (!groups || !nbodies) return;
     float4* sbv = packed;
     float4* sbp = packed + 4 * size_t(nbodies);
-    k_pack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
+    if (bodies & kPackBodies) k_pack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
     launch_groups(s, uk, groups, threads, vr, wr, sbv, sbp, ws, vel, sp, resid, 0u);
-    k_unpack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
+    if (bodies & kUnpackBodies) k_unpack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
 }
 
 // Group 0 by `coop_blocks` cooperating workgroups of `coop_threads`, the
@@ -1008,12 +1044,14 @@ hipError_t launch_solve_split(hipStream_t s, const VelRecBuf& vr, const WsRecBuf
                               uint32_t nbodies, float4* packed, const Schedule& ws, const Schedule& vel,
                               const SolveParams& sp, uint32_t groups, uint32_t threads, float* resid,
                               UnitKernel uk, uint32_t coop_blocks, uint32_t coop_threads, uint32_t* counter,
-                              hipEvent_t coop_begin, hipEvent_t coop_end) {
+                              hipEvent_t coop_begin, hipEvent_t coop_end, uint32_t bodies) {
     if (!groups || !nbodies) return hipSuccess;
-    if (!ws.unit || !vel.unit) return hipErrorInvalidValue;  // the cooperative sweep reads unit arrays
+    // The cooperative sweep reads unit arrays (the velocity one only when it
+    // runs the velocity iterations).
+    if (!ws.unit || (!vel.unit && (sp.phases & 2u))) return hipErrorInvalidValue;
     float4* sbv = packed;
     float4* sbp = packed + 4 * size_t(nbodies);
-    k_pack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
+    if (bodies & kPackBodies) k_pack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
     k_zero_u32<<<1, 1, 0, s>>>(counter);
     VelRecBuf vr_ = vr;
     WsRecBuf wr_ = wr;
@@ -1026,7 +1064,7 @@ hipError_t launch_solve_split(hipStream_t s, const VelRecBuf& vr, const WsRecBuf
     if (e != hipSuccess) return e;
     if (coop_end) hipEventRecord(coop_end, s);
     if (groups > 1) launch_groups(s, uk, groups - 1, threads, vr, wr, sbv, sbp, ws, vel, sp, resid, 1u);
-    k_unpack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
+    if (bodies & kUnpackBodies) k_unpack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
     return hipGetLastError();
 }
 void launch_cache_store(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const CacheBuf& cache, uint32_t nc) {

@@ -30,7 +30,8 @@ TIMINGS_DTYPE = np.dtype([
     ("bias_levels", "<u4"), ("groups", "<u4"), ("islands", "<u4"), ("build_kernel_ms", "<f4"),
     ("cooperative_blocks", "<u4"), ("cooperative_ws_points", "<u4"), ("cooperative_vel_points", "<u4"),
     ("cooperative_kernel_ms", "<f4"), ("schedule_ms", "<f4"), ("build_device_ms", "<f4"),
-    ("schedule_ws_ms", "<f4"), ("schedule_wait_ms", "<f4"),
+    ("schedule_ws_ms", "<f4"), ("schedule_wait_ms", "<f4"), ("early_warm_start_ms", "<f4"),
+    ("schedule_velocity_after_ms", "<f4"), ("overlap_ms", "<f4"),
 ])
 assert KZG_BODY_DTYPE.itemsize == 136
 
@@ -39,7 +40,7 @@ EXPORTS = [
     "kzg_narrowphase", "kzg_build_constraints", "kzg_integrate_velocities", "kzg_solve", "kzg_integrate_positions",
     "kzg_step", "kzg_query_sizes", "kzg_download_states", "kzg_download_manifolds", "kzg_download_constraints",
     "kzg_get_residuals", "kzg_get_timings", "kzg_collide_batch", "kzg_schedule", "kzg_set_warm_start",
-    "kzg_set_iterations", "kzg_set_cooperative", "kzg_set_sweep_units", "kzg_cache_info", "kzg_download_cache", "kzg_upload_cache",
+    "kzg_set_iterations", "kzg_set_cooperative", "kzg_set_sweep_units", "kzg_set_early_warm_start", "kzg_cache_info", "kzg_download_cache", "kzg_upload_cache",
     "kzg_mlp_init", "kzg_mlp_set_params", "kzg_mlp_get_params", "kzg_mlp_set_normalization",
     "kzg_mlp_get_normalization", "kzg_mlp_collect", "kzg_mlp_num_samples", "kzg_mlp_get_samples",
     "kzg_mlp_fit_normalization", "kzg_mlp_train", "kzg_mlp_forward", "kzg_mlp_gradients",
@@ -81,6 +82,7 @@ def library():
             "kzg_set_warm_start": ([vp, i32], i32), "kzg_set_iterations": ([vp, u32, u32], i32),
             "kzg_set_cooperative": ([vp, u32], i32),
             "kzg_set_sweep_units": ([vp, u32, u32], i32),
+            "kzg_set_early_warm_start": ([vp, u32], i32),
             "kzg_cache_info": ([vp, vp, vp], i32), "kzg_download_cache": ([vp, vp, C.c_uint64], i32),
             "kzg_upload_cache": ([vp, vp, C.c_uint64, u32, u32], i32),
             "kzg_mlp_init": ([vp, C.c_uint64], i32), "kzg_mlp_set_params": ([vp, vp, u32], i32),
@@ -132,18 +134,19 @@ def device_bodies(bodies: np.ndarray, mass: np.ndarray) -> np.ndarray:
 
 
 def schedule(dynamic: np.ndarray, info: np.ndarray, use_simd: bool = True, max_groups: int = 1024,
-             units: bool = False, with_units: bool = False):
+             units: bool = False, with_units: bool = False, defer: bool = False):
     """Host-only solve schedule (kzg_schedule): per-point velocity level,
     warm-start level, group, and (islands, groups, vel depth, ws depth, vel
     units, ws units).  units=True joins points into multi-point units as the
     device solve does; with_units=True also returns the per-point velocity and
-    warm-start unit numbers."""
+    warm-start unit numbers.  defer=True builds it the way an early warm
+    start does (the velocity order finished on the helper thread)."""
     dyn = np.ascontiguousarray(dynamic, np.uint8)
     info = np.ascontiguousarray(info, np.uint32).reshape(-1, 4)
     npts = int((info[:, 3] & 0xFF).sum()) if len(info) else 0
     vl, wl, gr, vu, wu = (np.zeros(max(npts, 1), np.uint32) for _ in range(5))
     st = np.zeros(6, np.uint32)
-    flags = (1 if use_simd else 0) | (2 if units else 0)
+    flags = (1 if use_simd else 0) | (2 if units else 0) | (4 if defer else 0)
     rc = library().kzg_schedule(len(dyn), _p(dyn), len(info), _p(info), flags, max_groups, _p(vl), _p(wl), _p(gr),
                                 _p(vu), _p(wu), _p(st))
     if rc != 0:
@@ -290,6 +293,11 @@ class DeviceContext:
         velocity and the warm-start / bias sweeps: 0 default, 1..4."""
         ws = velocity if warm_start is None else warm_start
         self._check(library().kzg_set_sweep_units(self.h, int(velocity), int(ws)), "kzg_set_sweep_units")
+
+    def set_early_warm_start(self, mode: int):
+        """Sweep the warm start while the host finishes the velocity schedule:
+        0 auto (16,384 or more points), 1 off, 2 always."""
+        self._check(library().kzg_set_early_warm_start(self.h, int(mode)), "kzg_set_early_warm_start")
 
     def mlp_init(self, seed: int = 0x4B5A0005):
         self._check(library().kzg_mlp_init(self.h, C.c_uint64(seed)), "kzg_mlp_init")

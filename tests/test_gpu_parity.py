@@ -41,11 +41,12 @@ def test_narrowphase_batch_bit_exact(n):
         assert i is None, f"point {k} differs at pair {np.nonzero(live)[0][i]}"
 
 
-def _trajectory(scene, steps, cooperative=0, units=0):
+def _trajectory(scene, steps, cooperative=0, units=0, early=0):
     R.init_jobs(0)
     rw = R.RefWorld(scene)
     ctx = _ctx_for(scene, rw)
     ctx.set_cooperative(cooperative)
+    ctx.set_early_warm_start(early)
     ctx.set_sweep_units(*(units if isinstance(units, tuple) else (units, units)))
     for i in range(steps):
         rw.step_phases()
@@ -111,6 +112,36 @@ def test_pile_trajectory_unit_caps_bit_exact(units, blocks):
     default (2, 4), split and on one workgroup: the trajectory stays the
     reference's."""
     _trajectory(S.pile(512, seed=0x4B5A0102), 240, cooperative=blocks, units=units)
+
+
+@pytest.mark.parametrize("blocks", [0, 1, 7])
+def test_pile_trajectory_early_warm_start_bit_exact(blocks):
+    """The warm-start sweep launched before the velocity schedule is ready
+    (forced at this size), with the velocity records written afterwards from
+    the prepared table: the trajectory stays the reference's."""
+    _trajectory(S.pile(512, seed=0x4B5A0102), 240, cooperative=blocks, early=2)
+
+
+def test_early_warm_start_off_equals_on_large_pile():
+    """On a 4096-body pile, with the largest group split over cooperating
+    workgroups, the state after every step is the same with the early warm
+    start off and forced on."""
+    from kizuriphysics3d_amd.world import PhysicsWorld
+    scene = S.pile(4096, seed=0x4B5A0104)
+    worlds = []
+    for mode in (1, 2):
+        w = PhysicsWorld(scene)
+        w.device_context().set_early_warm_start(mode)
+        worlds.append(w)
+    early = 0
+    for i in range(300):
+        for w in worlds:
+            w.step_fixed()
+        j = first_mismatch(worlds[0].states(), worlds[1].states())
+        assert j is None, f"step {i}: body {j} differs with the early warm start"
+        early += worlds[1].device_context().timings()["early_warm_start_ms"] > 0
+    assert early > 0, "the warm start never started early"
+    print(f"early warm start on {early} of 300 steps", flush=True)
 
 
 def test_cooperative_split_invariant_large_pile():

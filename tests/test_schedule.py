@@ -204,3 +204,19 @@ def test_units_join_a_manifold_chain():
     info = np.array([(0, 1, 0, 4)], np.uint32)
     _, wl, _, st, _, wu = D.schedule(dyn, info, units=True, with_units=True)
     assert list(wl) == [1, 1, 2, 2] and wu[0] == wu[1] != wu[2] == wu[3]
+
+
+@pytest.mark.parametrize("use_simd", [True, False])
+@pytest.mark.parametrize("groups", [1, 64])
+@pytest.mark.parametrize("nb,nc", [(3000, 12000), (40, 150)])
+def test_deferred_build_equals_plain(use_simd, groups, nb, nc):
+    """The early-warm-start build (warm-start order first, velocity order
+    finished on the helper thread) gives the same schedule as the plain one,
+    above the size where the plain build uses two threads (~30k points) and
+    below it."""
+    rng = np.random.default_rng(21)
+    dyn, info = random_graph(rng, nb, nc, 20 if nb > 100 else 2)
+    plain = D.schedule(dyn, info, use_simd=use_simd, max_groups=groups, units=True, with_units=True)
+    deferred = D.schedule(dyn, info, use_simd=use_simd, max_groups=groups, units=True, with_units=True, defer=True)
+    for a, b in zip(plain, deferred):
+        assert np.array_equal(a, b)
