@@ -15,8 +15,8 @@ float4 hull_face_normal(const float* v, uint32_t a, uint32_t b, uint32_t cc) {
 }
 
 constexpr uint32_t kMaxCoopBlocks = 64;
-constexpr uint32_t kAutoCoopBlocks = 8;   // auto mode's ceiling (fastest measured on the 32k pile)
-constexpr uint32_t kCoopSlice = 64;       // auto mode: points per workgroup and level, on average
+constexpr uint32_t kAutoCoopBlocks = 16;  // auto mode's ceiling (fastest measured on the 32k pile: 16 > 12 > 8 > 24)
+constexpr uint32_t kCoopSlice = 48;       // auto mode: units per workgroup and level, on average
 
 struct CoopPlan {
     uint32_t blocks = 1, threads = 64;
