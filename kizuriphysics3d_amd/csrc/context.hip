@@ -74,7 +74,7 @@ int kzg_create(kzg_ctx** out, const kz_world_desc* world, int device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = uint32_t(prop.multiProcessorCount);
     for (auto& e : c->ev) hipEventCreate(&e);
-    if (c->work_count.ensure(2) != hipSuccess || ensure_cache(c, 512) != KZG_OK) {
+    if (c->work_count.ensure(3) != hipSuccess || ensure_cache(c, 512) != KZG_OK) {
         kzg_destroy(c);
         return KZG_ERR_DEVICE;
     }

@@ -1163,13 +1163,27 @@ __device__ __forceinline__ bool polyhedral(const WShape& w) { return w.s.type ==
 //     pair whose first EPA failed; EPA is a function of the shapes and the
 //     GJK simplex, so that run fails again and is not repeated here.
 // One EPA and one clipping call site keep the kernel's code small.
+// The next item of a wave after its first (its block index).  With `next`,
+// the items come from one counter shared by the grid, so the waves that drew
+// cheap pairs take more of them: on mixed hulls (8-64 vertices) and piles,
+// where a few EPA pairs cost many times the rest, the slowest wave otherwise
+// decides the kernel's time.  Without it (the batched worlds, boxes mostly,
+// of even cost, where one atomic per item on one address costs more than the
+// balance gains) the items go by grid stride.  Called with the whole wave
+// active (the item is wave-uniform).
+__device__ __forceinline__ uint32_t next_wave_item(uint32_t item, uint32_t* next) {
+    if (!next) return item + gridDim.x;
+    uint32_t k = 0;
+    if (threadIdx.x == 0) k = atomicAdd(next, 1u);
+    return gridDim.x + __builtin_amdgcn_readfirstlane(k);
+}
 __global__ void __launch_bounds__(64) k_collide_wave(const uint2* pairs, const uint32_t* work,
-                                                     const uint32_t* work_count, BodyBuf b, ShapeBuf sb,
-                                                     ManifoldBuf m, float max_sep, const GjkRec* pre) {
+                                                     const uint32_t* work_count, uint32_t* next, BodyBuf b,
+                                                     ShapeBuf sb, ManifoldBuf m, float max_sep, const GjkRec* pre) {
     __shared__ WaveLds L;
     const uint32_t total = *work_count;
     const float tol = 1.0e-4f;
-    for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
+    for (uint32_t item = blockIdx.x; item < total; item = next_wave_item(item, next)) {
         const Gjk g = load_gjk(pre[item]);
         const uint32_t wv = work[item];
         const uint32_t pi = wv & kWorkPairMask;
@@ -1365,12 +1379,13 @@ static void launch_bins(hipStream_t s, uint32_t* work, const uint32_t* work_coun
 
 // `work` holds 3 n + 16 ceil(n / 1024) entries (the list, the class-major
 // list, the list handed to the wave kernel, the chunk counts), `work_count`
-// two counters, `gjk` n records; n < 2^27 (the pair index bits of an item).
+// three counters (the two lists, the wave kernel's next item), `gjk` n
+// records; n < 2^27 (the pair index bits of an item).
 void launch_narrowphase(hipStream_t s, const uint2* pairs, uint32_t n, const BodyBuf& b, const ShapeBuf& sb,
                         const ManifoldBuf& m, uint32_t* work, uint32_t* work_count, void* gjk, float max_sep,
                         int world_mode, uint32_t wave_blocks) {
     if (!n) return;
-    hipMemsetAsync(work_count, 0, 2 * sizeof(uint32_t), s);
+    hipMemsetAsync(work_count, 0, 3 * sizeof(uint32_t), s);
     k_collide_simple<<<nblocks(n, 256), 256, 0, s>>>(pairs, n, nullptr, b, sb, m, work, work_count, max_sep,
                                                       world_mode);
     launch_bins(s, work, work_count, n);
@@ -1378,12 +1393,13 @@ void launch_narrowphase(hipStream_t s, const uint2* pairs, uint32_t n, const Bod
     k_collide_lanes<<<nblocks(n, 256), 256, 0, s>>>(pairs, work + n, work_count, work + 2 * size_t(n), work_count + 1,
                                                      rec, b, sb, m, max_sep);
     uint32_t grid = wave_blocks ? wave_blocks : (n < 65536u ? n : 65536u);
-    k_collide_wave<<<grid, 64, 0, s>>>(pairs, work + 2 * size_t(n), work_count + 1, b, sb, m, max_sep, rec);
+    k_collide_wave<<<grid, 64, 0, s>>>(pairs, work + 2 * size_t(n), work_count + 1, work_count + 2, b, sb, m,
+                                       max_sep, rec);
 }
 
-// Capacity-sized launch with the pair count read on the device; both work
-// counters must already be zero (no memset: the launch may be
-// graph-captured).  work: 3 cap + 16 ceil(cap / 1024) entries; gjk: cap
+// Capacity-sized launch with the pair count read on the device; the two
+// list counters must already be zero (no memset: the launch may be
+// graph-captured).  The wave kernel takes its items by grid stride.  work: 3 cap + 16 ceil(cap / 1024) entries; gjk: cap
 // records; cap < 2^27.
 void launch_narrowphase_counted(hipStream_t s, const uint2* pairs, uint32_t cap, const uint32_t* n_dev,
                                 const BodyBuf& b, const ShapeBuf& sb, const ManifoldBuf& m, uint32_t* work,
@@ -1394,8 +1410,8 @@ void launch_narrowphase_counted(hipStream_t s, const uint2* pairs, uint32_t cap,
     GjkRec* rec = static_cast<GjkRec*>(gjk);
     k_collide_lanes<<<nblocks(cap, 256), 256, 0, s>>>(pairs, work + cap, work_count, work + 2 * size_t(cap),
                                                        work_count + 1, rec, b, sb, m, max_sep);
-    k_collide_wave<<<wave_blocks ? wave_blocks : 1024u, 64, 0, s>>>(pairs, work + 2 * size_t(cap), work_count + 1, b,
-                                                                    sb, m, max_sep, rec);
+    k_collide_wave<<<wave_blocks ? wave_blocks : 1024u, 64, 0, s>>>(pairs, work + 2 * size_t(cap), work_count + 1,
+                                                                    nullptr, b, sb, m, max_sep, rec);
 }
 
 }  // namespace kz
