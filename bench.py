@@ -330,6 +330,7 @@ def main():
     ap.add_argument("--learn-batch", type=int, default=16384)
     ap.add_argument("--cooperative", type=int, default=0,
                     help="workgroups sharing the largest island's levels (0 auto, 1 one per group)")
+    ap.add_argument("--np-waves", type=int, default=0, help="narrowphase EPA / clipping waves (0: auto)")
     ap.add_argument("--units-vel", type=int, default=0, help="most points per velocity sweep unit (0: default)")
     ap.add_argument("--units-ws", type=int, default=0, help="most points per warm-start/bias sweep unit (0: default)")
     ap.add_argument("--workload", choices=("pile", "batched", "narrowphase", "copy"), default="pile")
@@ -362,6 +363,7 @@ def main():
     ctx = w.device_context()
     ctx.set_cooperative(args.cooperative)
     ctx.set_sweep_units(args.units_vel, args.units_ws)
+    ctx.set_narrowphase_waves(args.np_waves)
     if learn:
         ctx.mlp_init(0x4B5A0005)
     for i in range(args.warmup):

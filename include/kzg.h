@@ -197,6 +197,11 @@ int kzg_set_cooperative(kzg_ctx* ctx, uint32_t blocks);
  * which shortens the level chain; each unit costs its points' arithmetic in
  * series.  Results are the same whatever the setting. */
 int kzg_set_sweep_units(kzg_ctx* ctx, uint32_t velocity, uint32_t warm_start);
+/* Waves of the narrowphase's EPA / clipping kernel (one pair per wave at a
+ * time, the next taken from a shared counter): 0 = auto (16 per compute
+ * unit, at most one per pair).  Results are the same for every count. */
+int kzg_set_narrowphase_waves(kzg_ctx* ctx, uint32_t waves);
+
 /* Early warm start (no reference counterpart: an overlap of host and device
  * work inside PhysicsWorld::StepFixed's order).  With the SIMD solver and the
  * cache warm start, kzg_build_constraints may return once the warm-start

@@ -864,7 +864,8 @@ int kzg_collide_batch(kzg_ctx* c, const uint32_t* sa, const uint32_t* sb, const 
     c->recorded = 0;
     KZG_HIP(c, record(c, 0));
     launch_narrowphase(c->stream, c->x_pairs.p, n, bb, c->shapes(), c->manifolds(), c->work.p, c->work_count.p,
-                       c->gjk.p, max_sep, 0, std::min<uint32_t>(std::max<uint32_t>(n, 1), c->cu_count * 16u));
+                       c->gjk.p, max_sep, 0,
+                       c->np_waves ? c->np_waves : std::min<uint32_t>(std::max<uint32_t>(n, 1), c->cu_count * 16u));
     KZG_HIP(c, hipGetLastError());
     KZG_HIP(c, record(c, 1));
     KZG_HIP(c, hipStreamSynchronize(c->stream));
@@ -898,6 +899,13 @@ int kzg_set_cooperative(kzg_ctx* c, uint32_t blocks) {
     if (!c) return KZG_ERR_ARGUMENT;
     if (blocks > kMaxCoopBlocks) return fail(c, KZG_ERR_ARGUMENT, "more than 64 cooperative workgroups");
     c->coop_blocks = blocks;
+    return KZG_OK;
+}
+
+int kzg_set_narrowphase_waves(kzg_ctx* c, uint32_t waves) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (waves > (1u << 20)) return fail(c, KZG_ERR_ARGUMENT, "more than 2^20 narrowphase waves");
+    c->np_waves = waves;
     return KZG_OK;
 }
 

@@ -200,6 +200,7 @@ struct kzg_ctx {
     // was ready (the builder finishes it in kzg_solve); the host time the
     // velocity schedule took after the warm-start one was ready.
     bool early_ws = false;
+    uint32_t np_waves = 0;       // kzg_set_narrowphase_waves: 0 auto (16 per CU, at most one per pair)
     uint32_t early_ws_mode = 0;  // kzg_set_early_warm_start: 0 auto, 1 off, 2 always
     double vel_after_ms = 0.0;
     int phase = 0;  // 0 idle, 1 narrowphase done, 2 built, 3 solved
