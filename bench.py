@@ -5,14 +5,15 @@ Default workload (the headline metric): contacts solved per second on the
 random convex pile of 32,768 bodies (25 % spheres, boxes, capsules, convex
 hulls of 16-32 points; dropped from y in [2, 60] into a 64 x 64 m walled bin;
 density 1000; FP32; 8 velocity + 2 position iterations; dt = 1/60; sleeping
-off).  W warm-up StepFixed calls let the pile form, then K StepFixed calls
-are timed.  A step is one StepFixed: host broadphase, then narrowphase,
+off).  A fixed settle phase of S untimed StepFixed calls (default 300; the
+pile comes to rest around step 250) runs whatever --warmup says, then W
+warm-up calls, then K timed calls.  A step is one StepFixed: host broadphase, then narrowphase,
 constraint build, solve and integration on the GPU.  value = sum(contact
 points) / sum(narrowphase kernels + host build compute + integrate
 velocities + solve), with inputs in HBM; the rate including the PCIe
 transfers at the boundary is reported beside it.  The learned warm start is
-trained during the last warm-up steps and compared with the cache and cold
-warm starts on the final state (PGS residual after k sweeps).
+trained during the last settle / warm-up steps and compared with the cache
+and cold warm starts on the final state (PGS residual after k sweeps).
 
 Other workloads:
   --workload batched      4,096 independent worlds of 16-512 bodies (box
@@ -26,7 +27,8 @@ Other workloads:
 Every workload reports the reference CPU path (the reference engine built
 from its own sources, oracle/_ref) on a bounded sample of the same workload.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--bodies N] [--workload pile|batched|narrowphase]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--settle S] [--bodies N]
+                    [--workload pile|batched|narrowphase|copy]
 """
 from __future__ import annotations
 
@@ -82,12 +84,31 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(scene, states, budget_s: float, threads: int):
-    """The reference engine on the GPU run's pile: restart it from the same
-    body states and time its narrowphase + build + solve phases."""
+# A one-GPU box grants the process a share of 16 host cores, whatever the
+# machine has: the reference's JobSystem gets that many threads at most.
+HOST_CORE_SHARE = 16
+
+
+def host_threads():
+    """Threads for the reference CPU path and what the host offers:
+    (threads, {cpu_count, affinity, share, model})."""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    threads = max(1, min(allowed, HOST_CORE_SHARE))
+    return threads, {"cpu_count": os.cpu_count(), "affinity": allowed, "core_share": HOST_CORE_SHARE,
+                     "model": cpu_model()}
+
+
+def cpu_baseline(scene, states, budget_s: float, first_step: int):
+    """The reference engine on the GPU run's pile: restart it from the body
+    states at the start of the timed window and time its narrowphase + build
+    + solve phases."""
     from oracle import refharness as R
     if not R.available():
         return None
+    threads, host = host_threads()
     workers = R.init_jobs(max(threads - 1, 1))
     rw = R.RefWorld(scene)
     rw.set_states(states)
@@ -111,10 +132,12 @@ def cpu_baseline(scene, states, budget_s: float, threads: int):
     if ms <= 0:
         return None
     return {"value": pts / (ms / 1e3), "unit": "contacts/s", "cores": workers + 1, "kind": "reference",
-            "sample": f"{n} StepFixed calls of the reference on the same {len(states)}-body pile restarted from "
-                      f"the GPU run's body state (after one untimed warm-start step); phases Collide + "
-                      f"BuildContactConstraints + IntegrateVelocities + SolveContacts; JobSystem with {workers} "
-                      f"workers + caller on {cpu_model()}"}
+            "host": host, "points_per_step": pts / n,
+            "sample": f"{n} StepFixed calls of the reference (steps {first_step + 1}-{first_step + n}) on the same "
+                      f"{len(states)}-body pile, restarted from the GPU run's body state after step {first_step - 1} "
+                      f"(the restart step {first_step}, with an empty warm-start cache, is not timed); phases "
+                      f"Collide + BuildContactConstraints + IntegrateVelocities + SolveContacts; JobSystem with "
+                      f"{workers} workers + caller"}
 
 
 def bench_batched(args):
@@ -127,11 +150,12 @@ def bench_batched(args):
     nb = sum(len(sc.bodies) for sc in scenes)
     log(f"batch: {len(scenes)} worlds, {nb} bodies, built in {time.time() - t0:.1f} s")
     tuned_steps = 0
-    for i in range(args.warmup):
+    untimed = args.settle + args.warmup
+    for i in range(untimed):
         bw.step()
         if (i + 1) % 50 == 0:
             st = bw.stats()
-            log(f"warmup {i + 1}/{args.warmup}: {st['num_points']} points, device {st['device_ms']:.2f} ms, "
+            log(f"settle/warmup {i + 1}/{untimed}: {st['num_points']} points, device {st['device_ms']:.2f} ms, "
                 f"host broadphase {st['broadphase_ms']:.1f} ms, step {st['step_ms']:.1f} ms")
     cands = bw.candidates()
     best, times = bw.autotune(repeats=2)
@@ -158,20 +182,22 @@ def bench_batched(args):
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline_worlds(scenes, bw, args.cpu_budget, min(os.cpu_count() or 1, 16))
+            cpu = cpu_baseline_worlds(scenes, bw, args.cpu_budget)
         except Exception as e:
             log(f"cpu baseline unavailable: {e}")
-    total_steps = args.warmup + tuned_steps + args.steps
+    total_steps = untimed + tuned_steps + args.steps
     return {
         "metric": "contacts solved/s, 4096 batched worlds",
         "value": points / (dev_ms / 1e3),
         "unit": "contacts/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup + tuned_steps,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{len(scenes)} worlds of 16-512 bodies (log-uniform; box stacks and convex piles, "
                                f"each on its own ground plane), 8+2 iterations, dt 1/60, steps "
-                               f"{total_steps - args.steps + 1}-{total_steps}",
+                               f"{total_steps - args.steps + 1}-{total_steps} after {args.settle} settle, "
+                               f"{args.warmup} warm-up and {tuned_steps} autotuning steps",
+                   "settle_steps": args.settle, "autotune_steps": tuned_steps,
                    "worlds": len(scenes), "bodies": nb, "contact_points_per_step": points / args.steps,
                    "device_ms_per_step": dev_ms / args.steps,
                    "device_ms_per_step_with_state_copies": copies_ms / args.steps,
@@ -183,12 +209,13 @@ def bench_batched(args):
     }
 
 
-def cpu_baseline_worlds(scenes, bw, budget_s: float, threads: int):
+def cpu_baseline_worlds(scenes, bw, budget_s: float):
     """The reference engine stepping worlds of the batch one after another,
     restarted from the GPU run's states (one untimed warm-start step each)."""
     from oracle import refharness as R
     if not R.available():
         return None
+    threads, host = host_threads()
     workers = R.init_jobs(max(threads - 1, 1))
     t_begin = time.time()
     pts, ms, n, worlds = 0, 0.0, 0, 0
@@ -209,10 +236,11 @@ def cpu_baseline_worlds(scenes, bw, budget_s: float, threads: int):
     if ms <= 0:
         return None
     return {"value": pts / (ms / 1e3), "unit": "contacts/s", "cores": workers + 1, "kind": "reference",
+            "host": host,
             "sample": f"{worlds} of the batch's worlds, each restarted from the GPU run's state and stepped 3 times "
                       f"after one untimed warm-start step ({n} StepFixed calls); phases Collide + "
                       f"BuildContactConstraints + IntegrateVelocities + SolveContacts; JobSystem with {workers} "
-                      f"workers + caller on {cpu_model()}"}
+                      f"workers + caller"}
 
 
 def bench_narrowphase(args):
@@ -255,7 +283,8 @@ def bench_narrowphase(args):
         try:
             from oracle import refharness as R
             if R.available():
-                workers = R.init_jobs(max(min(os.cpu_count() or 1, 16) - 1, 1))
+                threads, host = host_threads()
+                workers = R.init_jobs(max(threads - 1, 1))
                 n = min(args.pairs, 1 << 16)
                 sample = S.PairBatch(pb.shapes, pb.hull_points, pb.shape_a[:n], pb.shape_b[:n], pb.xf_a[:n],
                                      pb.xf_b[:n])
@@ -267,8 +296,9 @@ def bench_narrowphase(args):
                     reps += 1
                 cpu_s = (time.perf_counter() - t1) / reps
                 cpu = {"value": n / cpu_s, "unit": "pairs/s", "cores": workers + 1, "kind": "reference",
+                       "host": host,
                        "sample": f"the first {n} pairs of the batch through NarrowPhase::Collide, {reps} passes, "
-                                 f"ParallelFor with {workers} workers + caller on {cpu_model()}"}
+                                 f"ParallelFor with {workers} workers + caller"}
         except Exception as e:
             log(f"cpu baseline unavailable: {e}")
     return {
@@ -318,8 +348,11 @@ def bench_copy(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--settle", type=int, default=300,
+                    help="pile and batched: untimed steps before the warm-up, whatever --warmup is (the 32k pile "
+                         "comes to rest around step 250)")
     ap.add_argument("--bodies", type=int, default=32768)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of reference CPU work")
     ap.add_argument("--no-cpu", action="store_true")
@@ -366,14 +399,18 @@ def main():
     ctx.set_narrowphase_waves(args.np_waves)
     if learn:
         ctx.mlp_init(0x4B5A0005)
-    for i in range(args.warmup):
-        if learn and i == max(args.warmup - args.learn_collect, 0):
+    # Settle, then warm up: neither is timed.  The learned warm start
+    # collects its rows over the last learn_collect of these steps.
+    untimed = args.settle + args.warmup
+    for i in range(untimed):
+        if learn and i == max(untimed - args.learn_collect, 0):
             ctx.mlp_collect(1 << 24)  # rows of (features, impulse corrections) from these steps
         w.step_fixed()
-        if (i + 1) % 50 == 0:
+        if (i + 1) % 50 == 0 or i + 1 == untimed:
             st = w.stats()
-            log(f"warmup {i + 1}/{args.warmup}: {st['num_contact_points']} points, step {st['step_ms']:.1f} ms "
-                f"(contact path {st['contact_path_ms']:.1f} ms, broadphase {st['broadphase_ms']:.1f} ms)")
+            log(f"{'settle' if i < args.settle else 'warmup'} {i + 1}/{untimed}: {st['num_contact_points']} points, "
+                f"step {st['step_ms']:.1f} ms (contact path {st['contact_path_ms']:.1f} ms, broadphase "
+                f"{st['broadphase_ms']:.1f} ms)")
     train = None
     if learn:
         t1 = time.perf_counter()
@@ -384,9 +421,10 @@ def main():
         ctx.mlp_collect(0)  # the timed steps run the plain path
         train = {"rows": int(held), "adam_steps": args.learn_steps, "batch": args.learn_batch,
                  "seconds": train_s, "loss_first": float(losses[:10].mean()), "loss_last": float(losses[-100:].mean()),
-                 "collected_over_steps": f"{args.warmup - args.learn_collect + 1}-{args.warmup}"}
+                 "collected_over_steps": f"{max(untimed - args.learn_collect, 0) + 1}-{untimed}"}
         log(f"learned warm start: {held} rows, {args.learn_steps} Adam steps of {args.learn_batch} in {train_s:.2f} s, "
             f"loss {train['loss_first']:.3f} -> {train['loss_last']:.3f}")
+    start_states = w.states()  # the CPU baseline restarts from here
     torch.cuda.synchronize()
     points = 0
     path_ms = 0.0
@@ -467,7 +505,7 @@ def main():
         res = Lw.compare_warm_starts(w, penetration_iterations=(4, 8))
         cache8 = float(res["cache"]["residual"][-1])
         residual = {"definition": "RMS over contact points of the impulse increments of sweep k (N s)",
-                    "state": f"after step {args.warmup + args.steps}", "points": res["cache"]["points"],
+                    "state": f"after step {untimed + args.steps}", "points": res["cache"]["points"],
                     "k": list(range(1, len(res["cache"]["residual"]) + 1)), "train": train}
         for name, r in res.items():
             residual[name] = [float(x) for x in r["residual"]]
@@ -475,10 +513,15 @@ def main():
             residual[name + "_sweeps_to_cache8"] = Lw.iterations_to_reach(r["residual"], cache8)
             log(f"residual {name:13s} k=1..8 {np.array2string(r['residual'], precision=3)} "
                 f"penetration k=4 {r['penetration'][4]}")
+    # The world is closed before the CPU baseline and before interpreter
+    # teardown: its helper thread is joined and its device memory, events and
+    # stream are released while the runtime is whole.
+    w.close()
+    torch.cuda.synchronize()
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(scene, w.states(), args.cpu_budget, min(os.cpu_count() or 1, 16))
+            cpu = cpu_baseline(scene, start_states, args.cpu_budget, untimed + 1)
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline unavailable: {e}")
     line = {
@@ -495,8 +538,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"random convex pile, {args.bodies} bodies, 8+2 iterations, dt 1/60, "
-                               f"steps {args.warmup + 1}-{args.warmup + args.steps}",
-                   "bodies": args.bodies, "contact_points_per_step": points / args.steps,
+                               f"settled: steps {untimed + 1}-{untimed + args.steps} after {args.settle} settle "
+                               f"and {args.warmup} warm-up steps",
+                   "bodies": args.bodies, "settle_steps": args.settle,
+                   "timed_steps": [untimed + 1, untimed + args.steps],
+                   "contact_points_per_step": points / args.steps,
                    "contact_path_ms_per_step": resident_ms / args.steps,
                    "contact_path_ms_per_step_with_pcie": path_ms / args.steps,
                    "contacts_per_s_with_pcie": pcie_value},

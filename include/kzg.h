@@ -202,6 +202,13 @@ int kzg_set_sweep_units(kzg_ctx* ctx, uint32_t velocity, uint32_t warm_start);
  * unit, at most one per pair).  Results are the same for every count. */
 int kzg_set_narrowphase_waves(kzg_ctx* ctx, uint32_t waves);
 
+/* Islands (BuildIslands, PhysicsWorld.cpp:587-664) are a union-find on the
+ * device: in one workgroup's LDS for worlds of at most `max_bodies` bodies
+ * (0 = default, 36,864, also the ceiling), else in global memory with
+ * atomics.  1 sends every world to the global-memory version.  Results are
+ * the same either way (roots are each component's smallest slot). */
+int kzg_set_island_lds_limit(kzg_ctx* ctx, uint32_t max_bodies);
+
 /* Early warm start (no reference counterpart: an overlap of host and device
  * work inside PhysicsWorld::StepFixed's order).  With the SIMD solver and the
  * cache warm start, kzg_build_constraints may return once the warm-start

@@ -263,12 +263,16 @@ __global__ void k_con_island(ContactBuf c, const uint32_t* totals, BodyBuf b, co
 // belongs to one).  parent and island hold nb entries, tile ceil(nb / 1024)
 // (and at least 1).  Constraints are bounded by cap.
 void launch_islands(hipStream_t s, const ContactBuf& c, uint32_t cap, const BodyBuf& b, uint32_t nb,
-                    uint32_t* parent, uint32_t* island, uint2* tile, uint32_t* totals, uint32_t* con_island) {
+                    uint32_t* parent, uint32_t* island, uint2* tile, uint32_t* totals, uint32_t* con_island,
+                    uint32_t lds_limit) {
     const uint32_t tiles = (nb + kTile - 1) / kTile;
     const uint32_t cb = (cap + 255) / 256;
-    // The LDS variant where the device lets one workgroup hold the array.
+    // The LDS variant where the device lets one workgroup hold the array
+    // (worlds of at most min(lds_limit, kUfLdsBodies) bodies; 0 = no limit
+    // of the caller's).
+    const uint32_t limit = lds_limit ? std::min(lds_limit, kUfLdsBodies) : kUfLdsBodies;
     int dev = 0, optin = 0;
-    bool lds = nb && nb <= kUfLdsBodies && hipGetDevice(&dev) == hipSuccess &&
+    bool lds = nb && nb <= limit && hipGetDevice(&dev) == hipSuccess &&
                hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess &&
                uint32_t(optin) >= uf_lds_bytes(nb);
     if (lds)

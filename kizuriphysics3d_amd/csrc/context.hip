@@ -26,14 +26,17 @@ struct CoopPlan {
 // Workgroups and threads for group 0 (the group holding the largest
 // island).  `request` 0 sizes the split from the group's mean level width;
 // `max_threads` is the kernel's workgroup limit; a velocity unit takes
-// `vel_lanes` threads.
+// `vel_lanes` threads.  With `with_vel` false the velocity schedule is not
+// read at all: the builder's helper thread may still be writing it.
 CoopPlan plan_cooperative(const ScheduleOutput& s, uint32_t request, uint32_t max_threads, uint32_t vel_lanes,
                           bool with_vel = true) {
     CoopPlan p;
     if (request == 1 || s.groups == 0 || s.ws.grp.empty()) return p;
     uint32_t units = 0, levels = 0, widest = 0;
-    for (const LevelSchedule* ls : {&s.ws, &s.vel}) {
-        if (ls->grp.empty() || (ls == &s.vel && !with_vel)) continue;
+    const LevelSchedule* both[2] = {&s.ws, &s.vel};
+    for (uint32_t k = 0; k < (with_vel ? 2u : 1u); ++k) {
+        const LevelSchedule* ls = both[k];
+        if (ls->grp.empty()) continue;
         const U2 g = ls->grp[0];
         const uint32_t lanes = ls == &s.vel ? vel_lanes : 1u;
         for (uint32_t l = g.x; l < g.y; ++l) widest = std::max(widest, lanes * (ls->off[l + 1] - ls->off[l]));
@@ -287,7 +290,8 @@ int kzg_narrowphase(kzg_ctx* c, const uint32_t* pairs, uint32_t n) {
     if ((rc = upload(c, c->d_pairs, reinterpret_cast<const uint2*>(pairs), n))) return rc;
     KZG_HIP(c, record(c, 0));
     launch_narrowphase(c->stream, c->d_pairs.p, n, c->bodies(), c->shapes(), c->manifolds(), c->work.p,
-                       c->work_count.p, c->gjk.p, 0.02f, 1, std::min<uint32_t>(n, c->cu_count * 16u));
+                       c->work_count.p, c->gjk.p, 0.02f, 1,
+                       c->np_waves ? c->np_waves : std::min<uint32_t>(std::max<uint32_t>(n, 1), c->cu_count * 16u));
     KZG_HIP(c, hipGetLastError());
     KZG_HIP(c, record(c, 1));
     // The constraint table (BuildContactConstraints' compaction, :454-471)
@@ -296,7 +300,7 @@ int kzg_narrowphase(kzg_ctx* c, const uint32_t* pairs, uint32_t n) {
     launch_compact(c->stream, c->m_count.p, n, c->d_pairs.p, c->bodies(), c->contacts(), c->compact_tile.p,
                    c->compact_tot.p);
     launch_islands(c->stream, c->contacts(), n, c->bodies(), c->nbodies, c->uf_parent.p, c->uf_island.p,
-                   c->compact_tile.p, c->compact_tot.p, c->con_island.p);
+                   c->compact_tile.p, c->compact_tot.p, c->con_island.p, c->island_lds_limit);
     KZG_HIP(c, hipGetLastError());
     KZG_HIP(c, record(c, 13));
     uint32_t* tot = c->pin_totals.as<uint32_t>();
@@ -906,6 +910,12 @@ int kzg_set_narrowphase_waves(kzg_ctx* c, uint32_t waves) {
     if (!c) return KZG_ERR_ARGUMENT;
     if (waves > (1u << 20)) return fail(c, KZG_ERR_ARGUMENT, "more than 2^20 narrowphase waves");
     c->np_waves = waves;
+    return KZG_OK;
+}
+
+int kzg_set_island_lds_limit(kzg_ctx* c, uint32_t max_bodies) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    c->island_lds_limit = max_bodies;
     return KZG_OK;
 }
 

@@ -49,8 +49,11 @@ hipError_t launch_solve_split(hipStream_t s, const VelRecBuf& vr, const WsRecBuf
                               uint32_t bodies = kPackBodies | kUnpackBodies);
 void launch_compact(hipStream_t s, const uint32_t* m_count, uint32_t n, const uint2* pairs, const BodyBuf& b,
                     const ContactBuf& c, uint2* tile, uint32_t* totals);
+// lds_limit: most bodies for the one-workgroup LDS union-find (0: the
+// kernel's own limit); larger worlds take the global-memory union-find.
 void launch_islands(hipStream_t s, const ContactBuf& c, uint32_t cap, const BodyBuf& b, uint32_t nb,
-                    uint32_t* parent, uint32_t* island, uint2* tile, uint32_t* totals, uint32_t* con_island);
+                    uint32_t* parent, uint32_t* island, uint2* tile, uint32_t* totals, uint32_t* con_island,
+                    uint32_t lds_limit = 0);
 void launch_cache_store(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const CacheBuf& cache, uint32_t nc);
 void launch_cache_rehash(hipStream_t s, const CacheBuf& src, const CacheBuf& dst);
 void launch_finish(hipStream_t s, const BodyBuf& b, uint32_t n, float dt, int split);
@@ -201,6 +204,7 @@ struct kzg_ctx {
     // velocity schedule took after the warm-start one was ready.
     bool early_ws = false;
     uint32_t np_waves = 0;       // kzg_set_narrowphase_waves: 0 auto (16 per CU, at most one per pair)
+    uint32_t island_lds_limit = 0;  // kzg_set_island_lds_limit: 0 default
     uint32_t early_ws_mode = 0;  // kzg_set_early_warm_start: 0 auto, 1 off, 2 always
     double vel_after_ms = 0.0;
     int phase = 0;  // 0 idle, 1 narrowphase done, 2 built, 3 solved

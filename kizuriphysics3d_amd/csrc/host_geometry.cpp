@@ -399,7 +399,28 @@ bool BroadPhase::restore(const uint32_t* in, size_t n) {
         for (uint32_t link : {d.parent, d.child1, d.child2, d.next})
             if (link != kNull && link >= nn) return false;
     }
-    if ((in[1] != kNull && in[1] >= nn) || (in[2] != kNull && in[2] >= nn)) return false;
+    if ((in[1] != kNull && in[1] >= nn) || (in[2] != kNull && in[2] >= nn) || nl > nn) return false;
+    // The tree under the root: every node reached once, internal nodes with
+    // two children whose parent link points back, leaves inside the proxy
+    // layer table (the pair query reads it by leaf id).
+    if (in[1] != kNull) {
+        std::vector<uint8_t> seen(nn, 0);
+        std::vector<uint32_t> todo{in[1]};
+        while (!todo.empty()) {
+            const uint32_t id = todo.back();
+            todo.pop_back();
+            if (seen[id]) return false;
+            seen[id] = 1;
+            const Node& d = nodes[id];
+            if (d.leaf()) {
+                if (d.child2 != kNull || id >= nl) return false;
+                continue;
+            }
+            if (d.child2 == kNull || nodes[d.child1].parent != id || nodes[d.child2].parent != id) return false;
+            todo.push_back(d.child1);
+            todo.push_back(d.child2);
+        }
+    }
     nodes_ = std::move(nodes);
     root_ = in[1];
     free_ = in[2];
@@ -407,6 +428,11 @@ bool BroadPhase::restore(const uint32_t* in, size_t n) {
     masks_.assign(p + nl, p + 2 * nl);
     pairs_.clear();
     return true;
+}
+
+bool BroadPhase::is_proxy(uint32_t id, uint32_t user) const {
+    return id < nodes_.size() && id < layers_.size() && nodes_[id].leaf() && nodes_[id].height == 0 &&
+           nodes_[id].user == user;
 }
 
 uint32_t BroadPhase::create_proxy(const Aabb& tight, uint32_t user, uint32_t layer, uint32_t mask) {

@@ -59,6 +59,8 @@ public:
     void update_pairs();
     const std::vector<uint32_t>& pairs() const { return pairs_; }  // proxy A, proxy B, ...
     uint32_t user(uint32_t id) const { return nodes_[id].user; }
+    // Whether `id` is a leaf of the tree carrying `user` (snapshot checks).
+    bool is_proxy(uint32_t id, uint32_t user) const;
     // The whole tree as plain words (snapshots): node records, root, free
     // list, proxy layers and masks.  restore() returns false on a malformed
     // image and leaves the tree unchanged.

@@ -306,6 +306,9 @@ kzg_ctx* kzw_device(kzw_world* w) { return w ? w->ctx : nullptr; }
 namespace {
 
 constexpr uint32_t kKzp1Magic = 0x31505A4Bu;  // "KZP1"
+// Smallest KZP1 body record (a sphere): shape type and radius, then the body
+// fields read_kzp1_body reads (118 bytes).
+constexpr size_t kMinKzp1Body = 1 + 4 + 12 + 16 + 12 + 12 + 1 + 1 + 4 + 7 * 4 + 4 + 4 + 1 + 1 + 8 + 4 + 4 + 1;
 constexpr uint32_t kKzx1Magic = 0x31585A4Bu;  // "KZX1"
 constexpr uint32_t tag4(const char (&s)[5]) {
     return uint32_t(uint8_t(s[0])) | uint32_t(uint8_t(s[1])) << 8 | uint32_t(uint8_t(s[2])) << 16 |
@@ -343,7 +346,7 @@ struct Reader {
         return true;
     }
     bool bytes(void* dst, size_t k) {
-        if (off + k > n) return false;
+        if (k > n - off) return false;
         std::memcpy(dst, p + off, k);
         off += k;
         return true;
@@ -416,7 +419,7 @@ bool read_kzp1_body(Reader& r, Kzp1Body& o) {
             break;
         case KZ_SHAPE_CONVEX_HULL: {
             uint32_t n = 0;
-            if (!r.get(n) || n > 100000u) return false;
+            if (!r.get(n) || n > 100000u || size_t(n) * 3 * sizeof(float) > r.n - r.off) return false;
             o.points.resize(3 * size_t(n));
             if (!r.bytes(o.points.data(), sizeof(float) * o.points.size())) return false;
             o.shape.hull_count = n;
@@ -518,6 +521,10 @@ int kzw_load(kzw_world** out, const uint8_t* data, uint64_t size, const kz_world
     uint32_t magic = 0, version = 0, nb = 0;
     if (!r.get(magic) || magic != kKzp1Magic || !r.get(version) || version != 1 || !r.get(nb))
         return KZG_ERR_ARGUMENT;
+    // Every KZP1 body record holds at least kMinKzp1Body bytes: a count
+    // beyond what the record can hold is malformed (and must not size an
+    // allocation).
+    if (nb > (r.n - r.off) / kMinKzp1Body || nb >= (1u << 24)) return KZG_ERR_ARGUMENT;
     std::vector<Kzp1Body> kb(nb);
     for (auto& b : kb)
         if (!read_kzp1_body(r, b)) return KZG_ERR_UNSUPPORTED;
@@ -529,7 +536,7 @@ int kzw_load(kzw_world** out, const uint8_t* data, uint64_t size, const kz_world
         for (;;) {
             uint32_t tag = 0;
             uint64_t len = 0;
-            if (!r.get(tag) || !r.get(len) || r.off + len > r.n) return KZG_ERR_ARGUMENT;
+            if (!r.get(tag) || !r.get(len) || len > r.n - r.off) return KZG_ERR_ARGUMENT;
             const uint8_t* p = r.p + r.off;
             if (tag == tag4("END ")) break;
             if (tag == tag4("WSET") && len == sizeof(kz_world_desc)) wset = p;
@@ -563,13 +570,21 @@ int kzw_load(kzw_world** out, const uint8_t* data, uint64_t size, const kz_world
             uint32_t nv = 0, nt = 0;
             if (!s.get(rec.desc) || !s.get(nv) || !s.get(nt) || nv > 256 || nt > 3 * 512 || nt % 3)
                 return bad(KZG_ERR_ARGUMENT);
+            // The checks kzw_add_shapes applies: a type of the device path,
+            // and a hull (only for hull shapes) of at least a tetrahedron.
+            const uint32_t t = rec.desc.type;
+            if (t != KZ_SHAPE_SPHERE && t != KZ_SHAPE_BOX && t != KZ_SHAPE_CAPSULE && t != KZ_SHAPE_CONVEX_HULL &&
+                t != KZ_SHAPE_PLANE)
+                return bad(KZG_ERR_UNSUPPORTED);
+            if ((t == KZ_SHAPE_CONVEX_HULL) != (nv > 0) || (nv && (nv < 4 || nt < 3 * 4)) || (!nv && nt))
+                return bad(KZG_ERR_ARGUMENT);
             rec.hull.verts.resize(nv);
             rec.hull.tris.resize(nt);
             if (!s.bytes(rec.hull.verts.data(), sizeof(f3) * nv) || !s.bytes(rec.hull.tris.data(), sizeof(uint32_t) * nt) ||
                 !s.get(rec.hull.bounds) || !s.get(rec.hull.bounding_radius) || !s.get(rec.local))
                 return bad(KZG_ERR_ARGUMENT);
-            for (uint32_t t : rec.hull.tris)
-                if (t >= nv) return bad(KZG_ERR_ARGUMENT);
+            for (uint32_t tri : rec.hull.tris)
+                if (tri >= nv) return bad(KZG_ERR_ARGUMENT);
             w->shapes.push_back(std::move(rec));
         }
         w->shapes_dirty = true;
@@ -599,17 +614,23 @@ int kzw_load(kzw_world** out, const uint8_t* data, uint64_t size, const kz_world
     if (bpht) {
         Reader t{bpht, lbpht};
         uint64_t words = 0;
-        if (!t.get(words) || t.off + words * 4 + size_t(nb) * 4 != t.n) return bad(KZG_ERR_ARGUMENT);
-        if (!w->bp.restore(reinterpret_cast<const uint32_t*>(bpht + t.off), size_t(words)))
+        if (!t.get(words) || words > (t.n - t.off) / 4 || t.n - t.off - words * 4 != size_t(nb) * 4)
             return bad(KZG_ERR_ARGUMENT);
-        std::memcpy(w->proxy.data(), bpht + t.off + words * 4, sizeof(uint32_t) * nb);
+        // The image is copied out: the record need not be 4-byte aligned.
+        std::vector<uint32_t> tree(static_cast<size_t>(words)), proxy(nb);
+        std::memcpy(tree.data(), bpht + t.off, size_t(words) * 4);
+        std::memcpy(proxy.data(), bpht + t.off + words * 4, sizeof(uint32_t) * nb);
+        if (!w->bp.restore(tree.data(), tree.size())) return bad(KZG_ERR_ARGUMENT);
+        for (uint32_t i = 0; i < nb; ++i)
+            if (!w->bp.is_proxy(proxy[i], i)) return bad(KZG_ERR_ARGUMENT);
+        w->proxy = proxy;
     }
     if (w->ctx) {
         if ((rc = sync_device(w))) return bad(rc);
         if (cach) {
             Reader c{cach, lcach};
             uint32_t ccap = 0, used = 0;
-            if (!c.get(ccap) || !c.get(used) || c.off + uint64_t(ccap) * KZG_CACHE_SLOT_BYTES != c.n)
+            if (!c.get(ccap) || !c.get(used) || uint64_t(ccap) * KZG_CACHE_SLOT_BYTES != c.n - c.off)
                 return bad(KZG_ERR_ARGUMENT);
             rc = kzg_upload_cache(w->ctx, cach + c.off, uint64_t(ccap) * KZG_CACHE_SLOT_BYTES, ccap, used);
             if (rc) return bad(rc);

@@ -40,7 +40,7 @@ EXPORTS = [
     "kzg_narrowphase", "kzg_build_constraints", "kzg_integrate_velocities", "kzg_solve", "kzg_integrate_positions",
     "kzg_step", "kzg_query_sizes", "kzg_download_states", "kzg_download_manifolds", "kzg_download_constraints",
     "kzg_get_residuals", "kzg_get_timings", "kzg_collide_batch", "kzg_schedule", "kzg_set_warm_start",
-    "kzg_set_iterations", "kzg_set_cooperative", "kzg_set_sweep_units", "kzg_set_early_warm_start", "kzg_set_narrowphase_waves", "kzg_cache_info", "kzg_download_cache", "kzg_upload_cache",
+    "kzg_set_iterations", "kzg_set_cooperative", "kzg_set_sweep_units", "kzg_set_early_warm_start", "kzg_set_narrowphase_waves", "kzg_set_island_lds_limit", "kzg_cache_info", "kzg_download_cache", "kzg_upload_cache",
     "kzg_mlp_init", "kzg_mlp_set_params", "kzg_mlp_get_params", "kzg_mlp_set_normalization",
     "kzg_mlp_get_normalization", "kzg_mlp_collect", "kzg_mlp_num_samples", "kzg_mlp_get_samples",
     "kzg_mlp_fit_normalization", "kzg_mlp_train", "kzg_mlp_forward", "kzg_mlp_gradients",
@@ -84,6 +84,7 @@ def library():
             "kzg_set_sweep_units": ([vp, u32, u32], i32),
             "kzg_set_early_warm_start": ([vp, u32], i32),
             "kzg_set_narrowphase_waves": ([vp, u32], i32),
+            "kzg_set_island_lds_limit": ([vp, u32], i32),
             "kzg_cache_info": ([vp, vp, vp], i32), "kzg_download_cache": ([vp, vp, C.c_uint64], i32),
             "kzg_upload_cache": ([vp, vp, C.c_uint64, u32, u32], i32),
             "kzg_mlp_init": ([vp, C.c_uint64], i32), "kzg_mlp_set_params": ([vp, vp, u32], i32),
@@ -298,6 +299,11 @@ class DeviceContext:
     def set_narrowphase_waves(self, waves: int):
         """Waves of the EPA / clipping kernel: 0 auto (16 per CU)."""
         self._check(library().kzg_set_narrowphase_waves(self.h, int(waves)), "kzg_set_narrowphase_waves")
+
+    def set_island_lds_limit(self, max_bodies: int):
+        """Most bodies for the one-workgroup LDS union-find (0 default, 1:
+        always the global-memory union-find)."""
+        self._check(library().kzg_set_island_lds_limit(self.h, int(max_bodies)), "kzg_set_island_lds_limit")
 
     def set_early_warm_start(self, mode: int):
         """Sweep the warm start while the host finishes the velocity schedule:

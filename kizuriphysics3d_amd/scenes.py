@@ -442,3 +442,122 @@ def pair_batch(num_pairs: int = 1 << 20, seed: int = 0x4B5A0003, num_shapes: int
     xf_a = np.concatenate([qa, pa], 1).astype(np.float32)
     xf_b = np.concatenate([qb, pb], 1).astype(np.float32)
     return PairBatch(shapes, hull, sa.astype(np.uint32), sb.astype(np.uint32), xf_a, xf_b)
+
+
+def _quat_from_to_y(n: np.ndarray) -> np.ndarray:
+    """Quaternions (xyzw) rotating +Y onto each unit row of n (float64)."""
+    y = np.array([0.0, 1.0, 0.0])
+    c = np.clip(n @ y, -1.0, 1.0)
+    axis = np.cross(np.broadcast_to(y, n.shape), n)
+    ln = np.linalg.norm(axis, axis=1, keepdims=True)
+    axis = np.where(ln > 1e-9, axis / np.maximum(ln, 1e-30), np.array([1.0, 0.0, 0.0]))
+    half = 0.5 * np.arccos(c)
+    return np.concatenate([axis * np.sin(half)[:, None], np.cos(half)[:, None]], 1)
+
+
+def _quat_mul(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Row-wise Hamilton product a * b (xyzw)."""
+    ax, ay, az, aw = a.T
+    bx, by, bz, bw = b.T
+    return np.stack([aw * bx + ax * bw + ay * bz - az * by, aw * by - ax * bz + ay * bw + az * bx,
+                     aw * bz + ax * by - ay * bx + az * bw, aw * bw - ax * bx - ay * by - az * bz], 1)
+
+
+def prism_hull_points(rng: SplitMix64, sides: int, radius: float, half_height: float, extra: int) -> list:
+    """A `sides`-gon prism (bottom ring at -half_height, then the top ring)
+    followed by `extra` points on the lateral surface: hulls with flat faces
+    that give plane pairs many touching vertices."""
+    pts = []
+    for y in (-half_height, half_height):
+        for k in range(sides):
+            a = 2.0 * np.pi * k / sides
+            pts.append((radius * float(np.cos(a)), y, radius * float(np.sin(a))))
+    for _ in range(extra):
+        a = rng.uniform(0.0, 2.0 * np.pi)
+        pts.append((radius * float(np.cos(a)), rng.uniform(-half_height, half_height), radius * float(np.sin(a))))
+    return pts
+
+
+def plane_pair_batch(num_pairs: int = 4096, seed: int = 0x4B5A0107, num_shapes: int = 256) -> PairBatch:
+    """Plane-convex pairs for CollidePlaneConvex (CollideShape.cpp:370-430)
+    and the plane dispatch of NarrowPhase::Collide (:511-520): planes with
+    random normals and offsets against spheres, boxes, capsules, ellipsoid
+    hulls of 8-64 vertices (more than the 16 the plane path reads) and prism
+    hulls of 10-30 vertices; the plane is shape A in half of the pairs and B
+    in the other half.  Half of the box and prism pairs rest a face on the
+    plane within a tilt of ~1e-3 rad, so that more than 4 vertices lie within
+    0.02 m and the reduction to 4 points runs; the rest take uniform random
+    rotations.  Penetration U[0, 0.05] m; 10 % separated by U[0, 0.02] m and
+    5 % by U[0.02, 0.1] m (no contact)."""
+    rng = SplitMix64(seed)
+    sc = Scene("plane-pairs", world_settings())
+    planes, spheres, boxes, caps, ells, prisms = [], [], [], [], [], []
+    for i in range(num_shapes):
+        k = i % 6
+        if k == 0:
+            u = rng.unit_vector()
+            planes.append(sc.add_shape(SHAPE_PLANE, (u[0], u[1], u[2], rng.uniform(-1.0, 1.0))))
+        elif k == 1:
+            spheres.append(sc.add_shape(SHAPE_SPHERE, (rng.uniform(0.25, 0.75),)))
+        elif k == 2:
+            boxes.append(sc.add_shape(SHAPE_BOX, (rng.uniform(0.2, 0.6), rng.uniform(0.2, 0.6), rng.uniform(0.2, 0.6))))
+        elif k == 3:
+            caps.append(sc.add_shape(SHAPE_CAPSULE, (rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.4))))
+        elif k == 4:
+            n = rng.integer(8, 64)
+            axes = (rng.uniform(0.3, 0.7), rng.uniform(0.3, 0.7), rng.uniform(0.3, 0.7))
+            ells.append(sc.add_shape(SHAPE_HULL, hull=ellipsoid_hull_points(rng, n, axes)))
+        else:
+            sides = rng.integer(5, 12)
+            prisms.append(sc.add_shape(SHAPE_HULL, hull=prism_hull_points(
+                rng, sides, rng.uniform(0.3, 0.6), rng.uniform(0.1, 0.4), rng.integer(0, 6))))
+    shapes = sc.shape_array()
+    hull = sc.hull_array()
+    g = np.random.Generator(np.random.PCG64(rng.next_u64()))
+    kinds = [np.array(spheres), np.array(boxes), np.array(caps), np.array(ells), np.array(prisms)]
+    cls = np.arange(num_pairs) % len(kinds)
+    conv = np.zeros(num_pairs, np.int64)
+    for k, arr in enumerate(kinds):
+        sel = cls == k
+        conv[sel] = g.choice(arr, int(sel.sum()))
+    pl = g.choice(np.array(planes), num_pairs)
+
+    def quats(n):
+        u = g.random((n, 3))
+        a, b = np.sqrt(1 - u[:, 0]), np.sqrt(u[:, 0])
+        t2, t3 = 2 * np.pi * u[:, 1], 2 * np.pi * u[:, 2]
+        return np.stack([a * np.sin(t2), a * np.cos(t2), b * np.sin(t3), b * np.cos(t3)], 1)
+
+    qp = quats(num_pairs)
+    tp = g.uniform(-20.0, 20.0, (num_pairs, 3))
+    prm = shapes["params"][pl].astype(np.float64)
+    nw = _quat_rotate(qp, prm[:, :3])
+    dw = prm[:, 3] + np.einsum("ij,ij->i", nw, tp)
+    qc = quats(num_pairs)
+    flat = ((cls == 1) | (cls == 4)) & (g.random(num_pairs) < 0.5)
+    if flat.any():
+        m = int(flat.sum())
+        spin = np.zeros((m, 4))
+        ang = g.uniform(0.0, 2.0 * np.pi, m)
+        spin[:, 1], spin[:, 3] = np.sin(0.5 * ang), np.cos(0.5 * ang)
+        tilt_axis = g.normal(size=(m, 3))
+        tilt_axis /= np.linalg.norm(tilt_axis, axis=1, keepdims=True)
+        tilt = g.uniform(0.0, 1e-3, m)
+        tq = np.concatenate([tilt_axis * np.sin(0.5 * tilt)[:, None], np.cos(0.5 * tilt)[:, None]], 1)
+        qc[flat] = _quat_mul(tq, _quat_mul(_quat_from_to_y(nw[flat]), spin))
+    pen = g.uniform(0.0, 0.05, num_pairs)
+    r = g.random(num_pairs)
+    offset = np.where(r < 0.10, g.uniform(0.0, 0.02, num_pairs), np.where(r < 0.15, g.uniform(0.02, 0.1, num_pairs),
+                                                                            -pen))
+    lateral = g.uniform(-5.0, 5.0, (num_pairs, 3))
+    sup = _support_points(shapes, hull, conv, qc, -nw)
+    pc = tp + lateral
+    pc += nw * (dw + offset - np.einsum("ij,ij->i", nw, pc + sup))[:, None]
+    xf_p = np.concatenate([qp, tp], 1).astype(np.float32)
+    xf_c = np.concatenate([qc, pc], 1).astype(np.float32)
+    first = (np.arange(num_pairs) // len(kinds)) % 2 == 0  # plane as shape A
+    sa = np.where(first, pl, conv).astype(np.uint32)
+    sb = np.where(first, conv, pl).astype(np.uint32)
+    xa = np.where(first[:, None], xf_p, xf_c)
+    xb = np.where(first[:, None], xf_c, xf_p)
+    return PairBatch(shapes, hull, sa, sb, np.ascontiguousarray(xa), np.ascontiguousarray(xb))

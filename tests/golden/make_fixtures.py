@@ -3,7 +3,8 @@
 Run in a container where the reference checkout exists (it is compiled from
 its own sources by ``make -C oracle ref``):
 
-    python tests/golden/make_fixtures.py
+    python tests/golden/make_fixtures.py          # every fixture
+    python tests/golden/make_fixtures.py planes   # planes2048.npz only
 
 The fixtures are data only: seeded scene ids, and the reference's outputs on
 them (broadphase pairs per step, body states, raw manifolds, contact
@@ -33,6 +34,9 @@ PILE_N = 256
 BATCH_SEED = 0x4B5A0303
 BATCH_N = 4096
 BATCH_SHAPES = 512
+PLANE_SEED = 0x4B5A0107
+PLANE_N = 2048
+PLANE_SHAPES = 256
 
 
 def state_digest(st: np.ndarray) -> np.ndarray:
@@ -96,8 +100,23 @@ def trajectory(scene: S.Scene, steps: int, path: str):
     print(path, os.path.getsize(path) // 1024, "KiB")
 
 
+def plane_pairs():
+    """CollidePlaneConvex over plane_pair_batch: every convex kind, both
+    orders, hulls of more than 16 vertices, faces resting on the plane."""
+    pb = S.plane_pair_batch(PLANE_N, seed=PLANE_SEED, num_shapes=PLANE_SHAPES)
+    counts, pts, _ = R.collide_batch(pb)
+    refs, verts, tris = pack_hulls(pb.shapes, reference_hulls(pb.shapes, pb.hull_points))
+    path = os.path.join(HERE, "planes2048.npz")
+    np.savez_compressed(path, counts=counts, points=pts[:, :4].copy(), hull_refs=refs, hull_verts=verts,
+                        hull_tris=tris)
+    print(path, os.path.getsize(path) // 1024, "KiB")
+
+
 def main():
     R.init_jobs(0)
+    if sys.argv[1:] == ["planes"]:
+        plane_pairs()
+        return
     trajectory(S.pyramid(), 300, os.path.join(HERE, "pyramid20.npz"))
     trajectory(S.pile(PILE_N, seed=PILE_SEED), 300, os.path.join(HERE, "pile256.npz"))
     pb = S.pair_batch(BATCH_N, seed=BATCH_SEED, num_shapes=BATCH_SHAPES)
@@ -118,6 +137,7 @@ def main():
     refs, verts, tris = pack_hulls(shapes, reference_hulls(shapes, sc.hull_array()))
     np.savez_compressed(os.path.join(HERE, "shapes64.npz"), mass=rw.mass(), hull_refs=refs, hull_verts=verts,
                         hull_tris=tris)
+    plane_pairs()
     print("done")
 
 

@@ -13,6 +13,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 # Scene parameters the fixtures were generated with (make_fixtures.py).
 PILE_SEED, PILE_N = 0x4B5A0202, 256
 BATCH_SEED, BATCH_N, BATCH_SHAPES = 0x4B5A0303, 4096, 512
+PLANE_SEED, PLANE_N, PLANE_SHAPES = 0x4B5A0107, 2048, 256
 
 _cache = {}
 

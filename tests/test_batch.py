@@ -64,6 +64,41 @@ def test_batched_worlds_match_single_worlds():
 
 
 @pytest.mark.gpu
+@pytest.mark.reference
+def test_batched_worlds_match_the_reference():
+    """Worlds of a batch against the reference stepping each of them alone,
+    bit for bit: box stacks and convex piles of 16-160 bodies, each on its
+    own ground plane (so the plane-convex path of CollideShape.cpp:370-430
+    carries every ground contact)."""
+    from oracle import refharness as R
+    from kizuriphysics3d_amd.batch import BatchedWorlds
+    from tests.helpers import first_mismatch
+    R.init_jobs(0)
+    scenes = _worlds(16, seed=0x4B5A0407)
+    batch = BatchedWorlds(scenes)
+    refs = [R.RefWorld(sc) for sc in scenes]
+    kinds = {sc.name.rstrip("0123456789") for sc in scenes}
+    assert kinds == {"stack", "pile"}
+    steps = 150
+    for i in range(steps):
+        batch.step()
+        for rw in refs:
+            rw.step()
+        if i % 10 == 9 or i == steps - 1:
+            for k, rw in enumerate(refs):
+                j = first_mismatch(rw.states(), batch.states(k))
+                assert j is None, f"step {i}: world {k} ({scenes[k].name}) body {j} differs from the reference"
+    st = batch.stats()
+    plane_points = 0
+    for rw, sc in zip(refs, scenes):
+        hdr, _ = rw.constraints()
+        plane_points += int(hdr[(hdr[:, 0] == 0) | (hdr[:, 1] == 0), 2].sum())  # body 0: the ground plane
+    print(f"{len(scenes)} worlds ({sorted(kinds)}), {st['num_bodies']} bodies, {steps} steps bit-identical to the "
+          f"reference; last step {st['num_points']} points, {plane_points} on the ground planes", flush=True)
+    assert plane_points > 100
+
+
+@pytest.mark.gpu
 def test_batched_schedule_matches_host_schedule():
     from kizuriphysics3d_amd.batch import BatchedWorlds
     from kizuriphysics3d_amd.world import PhysicsWorld

@@ -41,14 +41,42 @@ def test_narrowphase_batch_bit_exact(n):
         assert i is None, f"point {k} differs at pair {np.nonzero(live)[0][i]}"
 
 
-def _trajectory(scene, steps, cooperative=0, units=0, early=0):
+def test_plane_pairs_bit_exact():
+    """CollidePlaneConvex and the plane dispatch (CollideShape.cpp:370-430,
+    511-520) on the device: plane against sphere, box, capsule and hulls of
+    more than 16 vertices, both orders, faces resting on the plane (more
+    than 4 candidates reduced to 4), bit for bit against the reference run
+    here and against its stored outputs."""
+    from tests.golden_data import PLANE_N, PLANE_SEED, PLANE_SHAPES, load_fixture
+    R.init_jobs(0)
+    pb = S.plane_pair_batch(PLANE_N, seed=PLANE_SEED, num_shapes=PLANE_SHAPES)
+    rc, rp, _ = R.collide_batch(pb)
+    z = load_fixture("planes2048")
+    assert np.array_equal(rc, z["counts"]) and rp[:, :4].tobytes() == z["points"].tobytes()
+    ctx = D.DeviceContext(S.world_settings())
+    ctx.upload_shapes(pb.shapes, reference_hulls(pb.shapes, pb.hull_points))
+    gc, gp, ms = ctx.collide_batch(pb.shape_a, pb.shape_b, pb.xf_a, pb.xf_b)
+    print(f"plane pairs {PLANE_N}: {ms:.3f} ms, points/pair hist {np.bincount(rc, minlength=5)}", flush=True)
+    bad = np.nonzero(gc != rc)[0]
+    assert len(bad) == 0, f"point counts differ at pairs {bad[:10]} ref {rc[bad[:10]]} gpu {gc[bad[:10]]}"
+    for k in range(4):
+        live = rc > k
+        i = first_mismatch(rp[live, k], gp[live, k])
+        assert i is None, f"point {k} differs at pair {np.nonzero(live)[0][i]}"
+
+
+def _trajectory(scene, steps, cooperative=0, units=0, early=0, island_lds=0, np_waves=0, log_every=0):
     R.init_jobs(0)
     rw = R.RefWorld(scene)
     ctx = _ctx_for(scene, rw)
     ctx.set_cooperative(cooperative)
     ctx.set_early_warm_start(early)
+    ctx.set_island_lds_limit(island_lds)
+    ctx.set_narrowphase_waves(np_waves)
     ctx.set_sweep_units(*(units if isinstance(units, tuple) else (units, units)))
     for i in range(steps):
+        if log_every and i % log_every == 0:
+            print(f"{scene.name}: step {i}", flush=True)
         rw.step_phases()
         pairs = rw.pairs()
         ctx.step(pairs)
@@ -73,6 +101,44 @@ def test_pyramid_trajectory_bit_exact():
 
 def test_pile_trajectory_bit_exact():
     _trajectory(S.pile(512, seed=0x4B5A0102), 240)
+
+
+def test_pile_trajectory_global_union_find_bit_exact():
+    """Islands from the global-memory union-find (k_uf_init / k_uf_union, the
+    path of worlds above 36,864 bodies) forced on the 512-body pile."""
+    _trajectory(S.pile(512, seed=0x4B5A0102), 240, island_lds=1)
+
+
+def test_pile_trajectory_narrowphase_waves_bit_exact():
+    """The single-world narrowphase with a non-default wave count (64 and
+    4096 waves of the EPA / clipping kernel)."""
+    _trajectory(S.pile(512, seed=0x4B5A0102), 120, np_waves=64)
+    _trajectory(S.pile(512, seed=0x4B5A0102), 120, np_waves=4096)
+
+
+def test_pile65536_lockstep_bit_exact():
+    """The 65,536-body pile of the learned warm start's workload: 75 steps in
+    lockstep with the reference (the pile forming: up to ~45k contact points
+    a step), islands from the global-memory union-find (above the LDS
+    variant's 36,864 bodies) checked against the host union-find's count."""
+    scene = S.pile(65536)
+    R.init_jobs(0)
+    rw = R.RefWorld(scene)
+    ctx = _ctx_for(scene, rw)
+    dyn = (scene.body_array()["motion_type"] == S.MOTION_DYNAMIC).astype(np.uint8)
+    for i in range(75):
+        rw.step_phases()
+        ctx.step(rw.pairs())
+        j = first_mismatch(rw.states(), ctx.states())
+        assert j is None, f"step {i}: body {j} differs"
+        if i % 15 == 14:
+            hdr, _ = ctx.constraints()
+            first = np.concatenate([[0], np.cumsum(hdr[:, 2])[:-1]]).astype(np.uint32)
+            info = np.stack([hdr[:, 0], hdr[:, 1], first, hdr[:, 2] | ((hdr[:, 3] & 1) << 8)], 1).astype(np.uint32)
+            _, _, _, st = D.schedule(dyn, info, True, 1 << 20)
+            k, _ = rw.islands()
+            assert ctx.timings()["islands"] == st[0] == k, (i, ctx.timings()["islands"], st[0], k)
+            print(f"pile65536: step {i} bit-exact, {int(hdr[:, 2].sum())} points, {k} islands", flush=True)
 
 
 def test_device_islands_match_host_union_find():

@@ -57,6 +57,92 @@ def test_snapshot_rejects_malformed_input():
         PhysicsWorld.load(bytes(bad), device=-1)
 
 
+def _blocks(data: bytes):
+    """Offsets of the extension blocks: {tag: (payload offset, length)}."""
+    kzx = data.find(b"KZX1")
+    assert kzx > 0
+    off, out = kzx + 8, {}
+    while True:
+        tag = data[off:off + 4].decode()
+        n = int(np.frombuffer(data[off + 4:off + 12], np.uint64)[0])
+        out[tag] = (off + 12, n)
+        if tag == "END ":
+            return out
+        off += 12 + n
+
+
+def _must_fail(data):
+    with pytest.raises(Exception):
+        PhysicsWorld.load(bytes(data), device=-1)
+
+
+def test_snapshot_rejects_truncated_and_corrupted_records():
+    """kzw_load checks every length and index it reads: truncated records,
+    counts and lengths beyond the data, a shape the device path lacks, a
+    hull of fewer than 4 vertices and broadphase proxies that do not name
+    the body's leaf are errors, never reads out of bounds."""
+    scene = S.pile(48, seed=0x4B5A0603, width=6.0, y_range=(1.0, 4.0))
+    data = PhysicsWorld(scene, device=-1).save()
+    PhysicsWorld.load(data, device=-1).close()  # the intact record loads
+    for cut in list(range(0, len(data) - 16, max(len(data) // 97, 1))):
+        _must_fail(data[:cut])
+    blk = _blocks(data)
+    # a body count far beyond the record
+    bad = bytearray(data)
+    bad[8:12] = np.uint32(0x00FFFFFF).tobytes()
+    _must_fail(bad)
+    # a block length that wraps the bounds check
+    o, _ = blk["SHAP"]
+    bad = bytearray(data)
+    bad[o - 8:o] = np.uint64(0xFFFFFFFFFFFFFFF0).tobytes()
+    _must_fail(bad)
+    # SHAP: first record's type (kz_shape_desc starts with the type) -> cylinder-like unknown code
+    bad = bytearray(data)
+    bad[o + 4:o + 8] = np.uint32(7).tobytes()
+    _must_fail(bad)
+    # BPHT: proxy of body 0 pointing at another node
+    o, n = blk["BPHT"]
+    words = int(np.frombuffer(data[o:o + 8], np.uint64)[0])
+    proxies = o + 8 + 4 * words
+    bad = bytearray(data)
+    p0 = int(np.frombuffer(data[proxies:proxies + 4], np.uint32)[0])
+    bad[proxies:proxies + 4] = np.uint32(p0 ^ 1).tobytes()
+    _must_fail(bad)
+    bad[proxies:proxies + 4] = np.uint32(0xFFFFFFF0).tobytes()
+    _must_fail(bad)
+    # BPHT: a child link of the root pointing back at the root (a cycle)
+    root = int(np.frombuffer(data[o + 8 + 4:o + 8 + 8], np.uint32)[0])
+    node = o + 8 + 16 + 18 * 4 * root
+    bad = bytearray(data)
+    bad[node + 13 * 4:node + 14 * 4] = np.uint32(root).tobytes()
+    _must_fail(bad)
+
+
+def test_snapshot_rejects_degenerate_hull_record():
+    """A SHAP record that calls a shape a hull but carries no hull vertices
+    (ConvexHullShape::Create would have returned null) is rejected, as is a
+    hull record claiming 3 vertices."""
+    scene = S.pile(16, seed=0x4B5A0604, width=6.0, y_range=(1.0, 4.0))
+    data = PhysicsWorld(scene, device=-1).save()
+    o, n = _blocks(data)["SHAP"]
+    shapes = scene.shape_array()
+    assert shapes["type"][0] != S.SHAPE_HULL
+    desc = shapes.dtype.itemsize
+    bad = bytearray(data)
+    bad[o + 4:o + 8] = np.uint32(S.SHAPE_HULL).tobytes()  # shape 0 retyped, its record holds 0 vertices
+    _must_fail(bad)
+    off = o + 4
+    for i in range(len(shapes)):
+        nv, nt = (int(x) for x in np.frombuffer(data[off + desc:off + desc + 8], np.uint32))
+        if shapes["type"][i] == S.SHAPE_HULL:
+            bad = bytearray(data)
+            bad[off + desc:off + desc + 4] = np.uint32(3).tobytes()
+            _must_fail(bad)
+            return
+        off += desc + 8 + 12 * nv + 4 * nt + 24 + 4 + 24
+    pytest.fail("no hull shape in the scene")
+
+
 @pytest.mark.reference
 def test_kzp1_records_interoperate_with_the_reference():
     """The reference's SerializeWorld record loads here (hulls rebuilt as
