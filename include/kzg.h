@@ -164,6 +164,25 @@ int kzg_collide_batch(kzg_ctx* ctx, const uint32_t* shape_a, const uint32_t* sha
                       const float* xf_a, const float* xf_b, uint32_t n, float max_separation,
                       uint32_t* counts, kz_contact_point* points, float* ms);
 
+/* kzg_collide_batch with every pointer in device memory and no device
+ * allocation: no counterpart in the reference.  It is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  shape_a, shape_b: n u32 shape indices
+ * into the uploaded table; xf_a, xf_b: n x 7 floats (quaternion xyzw +
+ * translation); outputs counts[n] and points[n * 4] as kzg_collide_batch
+ * writes them.  The narrowphase's own buffers (work lists, GJK records,
+ * manifold slots) come from a scratch array the library holds on each GPU
+ * for the life of the process (about 9 MB), which bounds n to
+ * KZG_NO_ALLOC_MAX_PAIRS; a larger n is KZG_ERR_ARGUMENT.  Every call on
+ * one GPU shares that GPU's array, from any context: calls that overlap on
+ * one GPU race.  The call is synchronous on the context's stream.  A shape
+ * index outside the table gives that pair no contact and the call returns
+ * KZG_ERR_ARGUMENT. */
+#define KZG_NO_ALLOC_MAX_PAIRS 16384
+int kzg_collide_batch_device(kzg_ctx* ctx, const uint32_t* shape_a, const uint32_t* shape_b,
+                             const float* xf_a, const float* xf_b, uint32_t n, float max_separation,
+                             uint32_t* counts, kz_contact_point* points);
+
 /* Warm-start modes of kzg_build_constraints / kzg_set_warm_start.  The
  * reference initialises accumulated impulses from its manifold cache
  * (PhysicsWorld.cpp:492-530) or, under KZ_NOWS, with zeros (:493).  The

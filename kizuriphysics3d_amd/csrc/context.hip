@@ -1,8 +1,124 @@
 // context.hip - the kzg_* C-ABI: device buffers, uploads and the per-step
 // orchestration of the contact path on one HIP stream.
+#include <mutex>
+
 #include "ctx_internal.h"
 
 namespace {
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free standalone narrowphase
+// (kzg_collide_batch_device): every device buffer the narrowphase needs
+// beyond the caller's inputs and outputs, for at most kScratchPairs pairs,
+// in one file-scope device array.  Each GPU holds its own copy for the life
+// of the process; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device (kzg_create).
+constexpr uint32_t kScratchPairs = KZG_NO_ALLOC_MAX_PAIRS;
+constexpr size_t scratch_align(size_t x) { return (x + 255) & ~size_t(255); }
+struct ScratchLayout {
+    size_t x, q, meta, packed, pairs, work, counters, gjk, mcount, mpa, mpb, mn, total;
+};
+// counters: 3 work-list counters, the pair count, the error flag.
+constexpr ScratchLayout make_scratch_layout(uint32_t n) {
+    ScratchLayout L{};
+    size_t o = 0;
+    auto take = [&](size_t& field, size_t bytes) {
+        field = o;
+        o = scratch_align(o + bytes);
+    };
+    take(L.x, 2 * size_t(n) * sizeof(float4));
+    take(L.q, 2 * size_t(n) * sizeof(float4));
+    take(L.meta, 2 * size_t(n) * sizeof(uint32_t));
+    take(L.packed, 2 * size_t(n) * sizeof(uint32_t));
+    take(L.pairs, size_t(n) * sizeof(uint2));
+    take(L.work, (3 * size_t(n) + size_t(kWorkClasses) * ((n + kWorkChunk - 1) / kWorkChunk)) * sizeof(uint32_t));
+    take(L.counters, 8 * sizeof(uint32_t));
+    take(L.gjk, size_t(n) * kGjkRecBytes);
+    take(L.mcount, size_t(n) * sizeof(uint32_t));
+    take(L.mpa, size_t(n) * kPairPoints * sizeof(float4));
+    take(L.mpb, size_t(n) * kPairPoints * sizeof(float4));
+    take(L.mn, size_t(n) * kPairPoints * sizeof(float4));
+    L.total = o;
+    return L;
+}
+constexpr ScratchLayout kScratch = make_scratch_layout(kScratchPairs);
+static __device__ __align__(256) unsigned char scratch[kScratch.total];
+
+// Per device: the device address of `scratch`, resolved once with that
+// device current.
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[kMaxScratchDevices] = {};
+
+hipError_t resolve_scratch(int device, unsigned char** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    if (!g_scratch_dev[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_dev[device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_dev[device];
+    return hipSuccess;
+}
+
+// Clears the work-list counters and the error flag and stores the pair count
+// (a kernel, not a memset: a caller may capture the launches in a graph).
+__global__ void k_scratch_reset(uint32_t* counters, uint32_t n) {
+    if (threadIdx.x == 0) {
+        counters[0] = counters[1] = counters[2] = 0u;
+        counters[3] = n;
+        counters[4] = 0u;
+    }
+}
+
+// Pair i of the caller's arrays -> two scratch bodies (slots i and n + i)
+// and the pair (i, n + i).  Both bodies are "dynamic" with packed ids 0 < 1,
+// so the world-mode narrowphase collides A against B in the caller's order,
+// as NarrowPhase::Collide(shapeA, xfA, shapeB, xfB) does.  A shape index
+// outside the table raises the error flag and both bodies become static, so
+// the narrowphase skips the pair (count 0) and reads no shape record.
+__global__ void k_scratch_setup(const uint32_t* shape_a, const uint32_t* shape_b, const float* xf_a, const float* xf_b,
+                                uint32_t n, uint32_t num_shapes, BodyBuf b, uint2* pairs, uint32_t* counters) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t sa = shape_a[i], sb = shape_b[i];
+    const bool ok = sa < num_shapes && sb < num_shapes;
+    if (!ok) atomicOr(counters + 4, 1u);
+    const float* a = xf_a + 7 * size_t(i);
+    const float* c = xf_b + 7 * size_t(i);
+    b.q[i] = make_float4(a[0], a[1], a[2], a[3]);
+    b.x[i] = make_float4(a[4], a[5], a[6], 0.0f);
+    b.q[n + i] = make_float4(c[0], c[1], c[2], c[3]);
+    b.x[n + i] = make_float4(c[4], c[5], c[6], 0.0f);
+    const uint32_t motion = ok ? (2u << 24) : 0u;
+    b.meta[i] = (ok ? sa : 0u) | motion;
+    b.meta[n + i] = (ok ? sb : 0u) | motion;
+    b.packed[i] = 0u;
+    b.packed[n + i] = 1u;
+    pairs[i] = make_uint2(i, n + i);
+}
+
+// Manifold slots -> the caller's counts[n] and points[n * 4] in the layout
+// kzg_collide_batch returns (slots past a pair's count: zeros, normal +Y).
+__global__ void k_scratch_output(ManifoldBuf m, uint32_t n, uint32_t* counts, kz_contact_point* points) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * uint32_t(kPairPoints)) return;
+    const uint32_t i = t / kPairPoints, j = t % kPairPoints;
+    const uint32_t k = m.count[i];
+    if (j == 0) counts[i] = k;
+    const bool live = j < k;
+    const float4 pa = live ? m.pa[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 pb = live ? m.pb[t] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 nn = live ? m.n[t] : make_float4(0.0f, 1.0f, 0.0f, 0.0f);
+    float* o = reinterpret_cast<float*>(points + t);
+    o[0] = pa.x; o[1] = pa.y; o[2] = pa.z;
+    o[3] = pb.x; o[4] = pb.y; o[5] = pb.z;
+    o[6] = nn.x; o[7] = nn.y; o[8] = nn.z;
+    o[9] = pa.w;
+    o[10] = pb.w;  // feature id bits
+}
 
 
 // Face normal of a hull triangle in the hull's frame, as GetShapeFace
@@ -77,7 +193,8 @@ int kzg_create(kzg_ctx** out, const kz_world_desc* world, int device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = uint32_t(prop.multiProcessorCount);
     for (auto& e : c->ev) hipEventCreate(&e);
-    if (c->work_count.ensure(3) != hipSuccess || ensure_cache(c, 512) != KZG_OK) {
+    if (resolve_scratch(device, &c->scratch) != hipSuccess || c->pin_flag.ensure(4096) != hipSuccess ||
+        c->work_count.ensure(3) != hipSuccess || ensure_cache(c, 512) != KZG_OK) {
         kzg_destroy(c);
         return KZG_ERR_DEVICE;
     }
@@ -135,6 +252,7 @@ void kzg_destroy(kzg_ctx* c) {
     c->stage_v.release();
     c->pin_totals.release();
     c->pin_info.release();
+    c->pin_flag.release();
     c->compact_tile.release();
     c->compact_tot.release();
     c->uf_parent.release();
@@ -879,6 +997,46 @@ int kzg_collide_batch(kzg_ctx* c, const uint32_t* sa, const uint32_t* sb, const 
     return kzg_download_manifolds(c, counts, pts);
 }
 
+
+int kzg_collide_batch_device(kzg_ctx* c, const uint32_t* shape_a, const uint32_t* shape_b, const float* xf_a,
+                             const float* xf_b, uint32_t n, float max_sep, uint32_t* counts, kz_contact_point* points) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (n && (!shape_a || !shape_b || !xf_a || !xf_b || !counts || !points))
+        return fail(c, KZG_ERR_ARGUMENT, "kzg_collide_batch_device: null device pointer");
+    if (n > kScratchPairs)
+        return fail(c, KZG_ERR_ARGUMENT, "kzg_collide_batch_device: " + std::to_string(n) + " pairs exceed the "
+                                             "allocation-free limit of " + std::to_string(kScratchPairs));
+    if (!(max_sep >= 0.0f) || !std::isfinite(max_sep))
+        return fail(c, KZG_ERR_ARGUMENT, "kzg_collide_batch_device: bad max separation");
+    if (!c->scratch) return fail(c, KZG_ERR_STATE, "kzg_collide_batch_device: scratch not resolved");
+    if (!n) return KZG_OK;
+    if (!c->nshapes) return fail(c, KZG_ERR_STATE, "kzg_collide_batch_device: no shape table uploaded");
+    join_schedule(c);
+    hipSetDevice(c->device);
+    unsigned char* base = c->scratch;
+    auto at = [&](size_t off) { return static_cast<void*>(base + off); };
+    BodyBuf bb{static_cast<float4*>(at(kScratch.x)), static_cast<float4*>(at(kScratch.q)), nullptr, nullptr, nullptr,
+               nullptr, nullptr, nullptr, nullptr, nullptr, static_cast<uint32_t*>(at(kScratch.meta)),
+               static_cast<uint32_t*>(at(kScratch.packed))};
+    ManifoldBuf mb{static_cast<uint32_t*>(at(kScratch.mcount)), static_cast<float4*>(at(kScratch.mpa)),
+                   static_cast<float4*>(at(kScratch.mpb)), static_cast<float4*>(at(kScratch.mn))};
+    uint2* pairs = static_cast<uint2*>(at(kScratch.pairs));
+    uint32_t* counters = static_cast<uint32_t*>(at(kScratch.counters));
+    k_scratch_reset<<<1, 64, 0, c->stream>>>(counters, n);
+    k_scratch_setup<<<(n + 255) / 256, 256, 0, c->stream>>>(shape_a, shape_b, xf_a, xf_b, n, c->nshapes, bb, pairs,
+                                                             counters);
+    launch_narrowphase_counted(c->stream, pairs, n, counters + 3, bb, c->shapes(), mb,
+                               static_cast<uint32_t*>(at(kScratch.work)), counters, at(kScratch.gjk), max_sep,
+                               c->np_waves);
+    k_scratch_output<<<(n * kPairPoints + 255) / 256, 256, 0, c->stream>>>(mb, n, counts, points);
+    KZG_HIP(c, hipGetLastError());
+    uint32_t* flag = c->pin_flag.as<uint32_t>();
+    KZG_HIP(c, hipMemcpyAsync(flag, counters + 4, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    if (*flag) return fail(c, KZG_ERR_ARGUMENT, "kzg_collide_batch_device: shape index outside the shape table "
+                                                 "(those pairs report no contact)");
+    return KZG_OK;
+}
 
 // ---------------------------------------------------------------------------
 // Warm-start mode, iteration counts and the learned warm start.

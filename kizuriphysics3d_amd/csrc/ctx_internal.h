@@ -23,6 +23,12 @@ namespace kz {
 void launch_narrowphase(hipStream_t s, const uint2* pairs, uint32_t n, const BodyBuf& b, const ShapeBuf& sb,
                         const ManifoldBuf& m, uint32_t* work, uint32_t* work_count, void* gjk, float max_sep,
                         int world_mode, uint32_t wave_blocks);
+// The same kernels with the pair count read on the device and no memset:
+// the two list counters must already be zero.  world_mode 1 (skip pairs
+// with no dynamic body, packed-id order).  work: work_entries(cap).
+void launch_narrowphase_counted(hipStream_t s, const uint2* pairs, uint32_t cap, const uint32_t* n_dev,
+                                const BodyBuf& b, const ShapeBuf& sb, const ManifoldBuf& m, uint32_t* work,
+                                uint32_t* work_count, void* gjk, float max_sep, uint32_t wave_blocks);
 void launch_integrate_velocities(hipStream_t s, const BodyBuf& b, uint32_t n, float dt, const float* g);
 void launch_build(hipStream_t s, const ContactBuf& c, const ManifoldBuf& m, const BodyBuf& b, const CacheBuf& cache,
                   uint32_t nc, int warm);
@@ -224,6 +230,12 @@ struct kzg_ctx {
     int warm_mode = KZG_WARM_CACHE;  // kzg_step's mode
     int cur_mode = KZG_WARM_CACHE;   // mode of the step being built and solved
     bool no_store = false;
+
+    // allocation-free standalone narrowphase: this device's copy of the
+    // library's scratch array (resolved in kzg_create) and a pinned word for
+    // its error flag
+    unsigned char* scratch = nullptr;
+    Pinned pin_flag;
 
     // standalone batch scratch
     DevVec<float4> x_x, x_q;

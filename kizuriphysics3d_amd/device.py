@@ -39,7 +39,7 @@ EXPORTS = [
     "kzg_create", "kzg_destroy", "kzg_last_error", "kzg_upload_shapes", "kzg_upload_bodies", "kzg_upload_states",
     "kzg_narrowphase", "kzg_build_constraints", "kzg_integrate_velocities", "kzg_solve", "kzg_integrate_positions",
     "kzg_step", "kzg_query_sizes", "kzg_download_states", "kzg_download_manifolds", "kzg_download_constraints",
-    "kzg_get_residuals", "kzg_get_timings", "kzg_collide_batch", "kzg_schedule", "kzg_set_warm_start",
+    "kzg_get_residuals", "kzg_get_timings", "kzg_collide_batch", "kzg_collide_batch_device", "kzg_schedule", "kzg_set_warm_start",
     "kzg_set_iterations", "kzg_set_cooperative", "kzg_set_sweep_units", "kzg_set_early_warm_start", "kzg_set_narrowphase_waves", "kzg_set_island_lds_limit", "kzg_cache_info", "kzg_download_cache", "kzg_upload_cache",
     "kzg_mlp_init", "kzg_mlp_set_params", "kzg_mlp_get_params", "kzg_mlp_set_normalization",
     "kzg_mlp_get_normalization", "kzg_mlp_collect", "kzg_mlp_num_samples", "kzg_mlp_get_samples",
@@ -78,6 +78,7 @@ def library():
             "kzg_download_manifolds": ([vp, vp, vp], i32), "kzg_download_constraints": ([vp, vp, vp], i32),
             "kzg_get_residuals": ([vp, vp, u32], i32), "kzg_get_timings": ([vp, vp], i32),
             "kzg_collide_batch": ([vp, vp, vp, vp, vp, u32, f32, vp, vp, vp], i32),
+            "kzg_collide_batch_device": ([vp, vp, vp, vp, vp, u32, f32, vp, vp], i32),
             "kzg_schedule": ([u32, vp, u32, vp, u32, u32, vp, vp, vp, vp, vp, vp], i32),
             "kzg_set_warm_start": ([vp, i32], i32), "kzg_set_iterations": ([vp, u32, u32], i32),
             "kzg_set_cooperative": ([vp, u32], i32),
