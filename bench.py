@@ -363,6 +363,8 @@ def main():
     ap.add_argument("--learn-batch", type=int, default=16384)
     ap.add_argument("--cooperative", type=int, default=0,
                     help="workgroups sharing the largest island's levels (0 auto, 1 one per group)")
+    ap.add_argument("--placement", choices=("one-xcd", "spread"), default="one-xcd",
+                    help="where the workgroups sharing the largest island's levels run")
     ap.add_argument("--np-waves", type=int, default=0, help="narrowphase EPA / clipping waves (0: auto)")
     ap.add_argument("--units-vel", type=int, default=0, help="most points per velocity sweep unit (0: default)")
     ap.add_argument("--units-ws", type=int, default=0, help="most points per warm-start/bias sweep unit (0: default)")
@@ -395,6 +397,7 @@ def main():
     learn = not args.no_learned and args.learn_collect > 0
     ctx = w.device_context()
     ctx.set_cooperative(args.cooperative)
+    ctx.set_cooperative_placement(0 if args.placement == "one-xcd" else 1)
     ctx.set_sweep_units(args.units_vel, args.units_ws)
     ctx.set_narrowphase_waves(args.np_waves)
     if learn:

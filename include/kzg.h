@@ -209,6 +209,20 @@ int kzg_set_iterations(kzg_ctx* ctx, uint32_t velocity, uint32_t position);
  * workgroup per group, n = n workgroups (at most 64).  Results are the
  * same whatever the setting: a level's points share no dynamic body. */
 int kzg_set_cooperative(kzg_ctx* ctx, uint32_t blocks);
+/* Where the workgroups that share the largest group's levels run.
+ * KZG_COOP_ONE_XCD (default): all on one XCD, which gives the levels a
+ * barrier through that XCD's L2 with no cache write-back (the launch holds
+ * 8 workgroups per participant and one XCD's take part; at most 32).
+ * KZG_COOP_SPREAD: anywhere on the device, with agent-scope fences between
+ * levels.  Results are the same either way. */
+#define KZG_COOP_ONE_XCD 0u
+#define KZG_COOP_SPREAD 1u
+int kzg_set_cooperative_placement(kzg_ctx* ctx, uint32_t placement);
+/* Bound on the polls of one wait between levels of a cooperative sweep
+ * (0 = default, about a second).  A sweep whose wait runs out stops early
+ * on every workgroup, and the next call that waits for the device (build,
+ * state download, timings) returns KZG_ERR_DEVICE. */
+int kzg_set_spin_limit(kzg_ctx* ctx, uint32_t polls);
 /* Most points of one body pair a sweep thread applies in a row (a unit), in
  * the velocity and in the warm-start / bias sweeps: 0 = default (2 each),
  * 1..4.  A point whose dynamic bodies were last updated by the same point of

@@ -133,6 +133,8 @@ float4 hull_face_normal(const float* v, uint32_t a, uint32_t b, uint32_t cc) {
 constexpr uint32_t kMaxCoopBlocks = 64;
 constexpr uint32_t kAutoCoopBlocks = 16;  // auto mode's ceiling (fastest measured on the 32k pile: 16 > 12 > 8 > 24)
 constexpr uint32_t kCoopSlice = 48;       // auto mode: units per workgroup and level, on average
+constexpr uint32_t kXcdCoopBlocks = 32;   // one-XCD placement: at most one workgroup per CU of the XCD
+constexpr uint32_t kAutoXcdBlocks = 24;   // its auto ceiling (32k pile: 24 > 16 = 32 > 8, 3.29 / 3.43 / 4.18 ms)
 
 struct CoopPlan {
     uint32_t blocks = 1, threads = 64;
@@ -145,7 +147,7 @@ struct CoopPlan {
 // `vel_lanes` threads.  With `with_vel` false the velocity schedule is not
 // read at all: the builder's helper thread may still be writing it.
 CoopPlan plan_cooperative(const ScheduleOutput& s, uint32_t request, uint32_t max_threads, uint32_t vel_lanes,
-                          bool with_vel = true) {
+                          bool one_xcd, bool with_vel = true) {
     CoopPlan p;
     if (request == 1 || s.groups == 0 || s.ws.grp.empty()) return p;
     uint32_t units = 0, levels = 0, widest = 0;
@@ -161,11 +163,35 @@ CoopPlan plan_cooperative(const ScheduleOutput& s, uint32_t request, uint32_t ma
         (ls == &s.ws ? p.ws_points : p.vel_points) = ls->unit[ls->off[g.y]] - ls->unit[ls->off[g.x]];
     }
     if (!levels) return CoopPlan{};
-    p.blocks = request ? request : std::min(kAutoCoopBlocks, std::max(1u, units / levels / kCoopSlice));
+    p.blocks = request ? request
+                       : std::min(one_xcd ? kAutoXcdBlocks : kAutoCoopBlocks, std::max(1u, units / levels / kCoopSlice));
+    if (one_xcd) p.blocks = std::min(p.blocks, kXcdCoopBlocks);
     if (p.blocks <= 1) return CoopPlan{};
     const uint32_t slice = (widest + p.blocks - 1) / p.blocks;
     while (p.threads < slice && p.threads < max_threads) p.threads *= 2;
     return p;
+}
+
+// The cooperative sweep's launch shape and sync block: the block is
+// cleared once per solve (the give-up flag with it) before the first
+// cooperative launch.
+int begin_cooperative(kzg_ctx* c) {
+    KZG_HIP(c, c->coop_sync.ensure(kSyncWords));
+    if (!c->coop_launched) launch_sync_reset(c->stream, c->coop_sync.p);
+    c->coop_launched = true;
+    return KZG_OK;
+}
+// A cooperative sweep that gave up waiting (a workgroup never arrived within
+// the spin limit) left its flag in the step's pinned totals: report it from
+// the next call that waits for the stream.
+int check_sweep(kzg_ctx* c) {
+    uint32_t* tot = c->pin_totals.as<uint32_t>();
+    if (tot && tot[5]) {
+        tot[5] = 0;
+        return fail(c, KZG_ERR_DEVICE, "the cooperative sweep gave up waiting for a workgroup (spin limit); "
+                                       "that step's velocities are not the solve's");
+    }
+    return KZG_OK;
 }
 
 // A deferred velocity schedule must be complete before anything reads the
@@ -194,10 +220,12 @@ int kzg_create(kzg_ctx** out, const kz_world_desc* world, int device) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = uint32_t(prop.multiProcessorCount);
     for (auto& e : c->ev) hipEventCreate(&e);
     if (resolve_scratch(device, &c->scratch) != hipSuccess || c->pin_flag.ensure(4096) != hipSuccess ||
-        c->work_count.ensure(3) != hipSuccess || ensure_cache(c, 512) != KZG_OK) {
+        c->pin_totals.ensure(8 * sizeof(uint32_t)) != hipSuccess || c->work_count.ensure(3) != hipSuccess ||
+        ensure_cache(c, 512) != KZG_OK) {
         kzg_destroy(c);
         return KZG_ERR_DEVICE;
     }
+    std::memset(c->pin_totals.p, 0, 8 * sizeof(uint32_t));  // slot 5: a sweep's give-up flag
     *out = c;
     return KZG_OK;
 }
@@ -253,6 +281,7 @@ void kzg_destroy(kzg_ctx* c) {
     c->pin_totals.release();
     c->pin_info.release();
     c->pin_flag.release();
+    c->coop_sync.release();
     c->compact_tile.release();
     c->compact_tot.release();
     c->uf_parent.release();
@@ -443,6 +472,8 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     // Constraint and point totals and the cache occupancy (the wait is the
     // narrowphase and the compaction still running).
     KZG_HIP(c, hipStreamSynchronize(c->stream));
+    int rc0;
+    if ((rc0 = check_sweep(c))) return rc0;
     const uint32_t* tot = c->pin_totals.as<uint32_t>();
     c->ncons = tot[0];
     c->npoints = tot[1];
@@ -593,17 +624,19 @@ int kzg_solve(kzg_ctx* c, float dt) {
         const UnitKernel uk = unit_kernel(c->sched.ws.largest_unit, 1u);
         uint32_t threads = 64;
         while (threads < c->sched.ws.max_width && threads < solve_threads(uk.ws, uk.vel)) threads *= 2;
-        const CoopPlan coop = plan_cooperative(c->sched, c->coop_blocks, solve_threads(uk.ws, uk.vel), 1u, false);
+        const CoopPlan coop = plan_cooperative(c->sched, c->coop_blocks, solve_threads(uk.ws, uk.vel), 1u,
+                                               c->coop_placement == KZG_COOP_ONE_XCD, false);
         SolveParams spw = sp;
         spw.phases = 1u;
         const Schedule none{nullptr, nullptr, nullptr};
         KZG_HIP(c, record(c, 14));
         if (coop.blocks > 1) {
-            KZG_HIP(c, c->coop_counter.ensure(1));
+            int rc;
+            if ((rc = begin_cooperative(c))) return rc;
             KZG_HIP(c, launch_solve_split(c->stream, c->vel_records(), c->ws_records(), c->bodies(), c->nbodies,
                                           c->b_packed_solver.p, ws, none, spw, c->sched.groups, threads, c->resid.p,
-                                          uk, coop.blocks, coop.threads, c->coop_counter.p, c->ev[17], c->ev[18],
-                                          kPackBodies));
+                                          uk, coop.blocks, coop.threads, c->coop_sync.p, c->spin_limit_or_default(),
+                                          c->coop_placement == KZG_COOP_ONE_XCD, c->ev[17], c->ev[18], kPackBodies));
             c->recorded |= 3u << 17;
         } else {
             launch_solve_groups(c->stream, c->vel_records(), c->ws_records(), c->bodies(), c->nbodies,
@@ -643,7 +676,8 @@ int kzg_solve(kzg_ctx* c, float dt) {
     const UnitKernel uk = unit_kernel(c->sched.ws.largest_unit, c->sched.vel.largest_unit);
     uint32_t threads = 64;
     while (threads < width && threads < solve_threads(uk.ws, uk.vel)) threads *= 2;
-    const CoopPlan coop = plan_cooperative(c->sched, c->coop_blocks, solve_threads(uk.ws, uk.vel), vel_lanes);
+    const CoopPlan coop = plan_cooperative(c->sched, c->coop_blocks, solve_threads(uk.ws, uk.vel), vel_lanes,
+                                           c->coop_placement == KZG_COOP_ONE_XCD);
     c->coop_used = coop.blocks;
     c->coop_ws_points = coop.ws_points;
     c->coop_vel_points = coop.vel_points;
@@ -651,11 +685,13 @@ int kzg_solve(kzg_ctx* c, float dt) {
     if (c->ncons) {
         KZG_HIP(c, c->b_packed_solver.ensure(8 * size_t(std::max<uint32_t>(c->nbodies, 1))));
         if (coop.blocks > 1) {
-            KZG_HIP(c, c->coop_counter.ensure(1));
+            int rc;
+            if ((rc = begin_cooperative(c))) return rc;
             KZG_HIP(c, launch_solve_split(c->stream, c->vel_records(), c->ws_records(), c->bodies(), c->nbodies,
                                           c->b_packed_solver.p, ws, vel, sp, c->sched.groups, threads, c->resid.p,
-                                          uk, coop.blocks, coop.threads, c->coop_counter.p, c->ev[10],
-                                          c->ev[11], early ? kUnpackBodies : kPackBodies | kUnpackBodies));
+                                          uk, coop.blocks, coop.threads, c->coop_sync.p, c->spin_limit_or_default(),
+                                          c->coop_placement == KZG_COOP_ONE_XCD, c->ev[10], c->ev[11],
+                                          early ? kUnpackBodies : kPackBodies | kUnpackBodies));
             c->recorded |= 3u << 10;
         } else {
             launch_solve_groups(c->stream, c->vel_records(), c->ws_records(), c->bodies(), c->nbodies,
@@ -664,6 +700,11 @@ int kzg_solve(kzg_ctx* c, float dt) {
         }
     }
     KZG_HIP(c, hipGetLastError());
+    if (c->coop_launched) {
+        KZG_HIP(c, hipMemcpyAsync(c->pin_totals.as<uint32_t>() + 5, c->coop_sync.p + kSyncGiveUp, sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, c->stream));
+        c->coop_launched = false;
+    }
     KZG_HIP(c, record(c, 6));
     launch_writeback(c->stream, c->vel_records(), c->ws_records(), c->contacts(), c->vel_pos.p, c->ws_pos.p,
                      c->npoints);
@@ -761,6 +802,7 @@ int kzg_download_states(kzg_ctx* c, kz_body_state* out, uint32_t n) {
     if ((rc = download(c, x.data(), c->b_x, n)) || (rc = download(c, q.data(), c->b_q, n)) ||
         (rc = download(c, v.data(), c->b_v, n)) || (rc = download(c, w.data(), c->b_w, n)))
         return rc;
+    if ((rc = check_sweep(c))) return rc;
     for (uint32_t i = 0; i < n; ++i) {
         kz_body_state& s = out[i];
         s.position[0] = x[i].x; s.position[1] = x[i].y; s.position[2] = x[i].z;
@@ -850,6 +892,8 @@ int kzg_get_timings(kzg_ctx* c, kzg_timings* out) {
     join_schedule(c);
     hipSetDevice(c->device);
     KZG_HIP(c, hipStreamSynchronize(c->stream));
+    int rc0;
+    if ((rc0 = check_sweep(c))) return rc0;
     kzg_timings t{};
     // Only pairs of events recorded since the last narrowphase are read, so
     // no query fails and leaves a sticky runtime error behind.
@@ -1068,6 +1112,20 @@ int kzg_set_narrowphase_waves(kzg_ctx* c, uint32_t waves) {
     if (!c) return KZG_ERR_ARGUMENT;
     if (waves > (1u << 20)) return fail(c, KZG_ERR_ARGUMENT, "more than 2^20 narrowphase waves");
     c->np_waves = waves;
+    return KZG_OK;
+}
+
+int kzg_set_cooperative_placement(kzg_ctx* c, uint32_t placement) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (placement != KZG_COOP_ONE_XCD && placement != KZG_COOP_SPREAD)
+        return fail(c, KZG_ERR_ARGUMENT, "placement is KZG_COOP_ONE_XCD or KZG_COOP_SPREAD");
+    c->coop_placement = placement;
+    return KZG_OK;
+}
+
+int kzg_set_spin_limit(kzg_ctx* c, uint32_t polls) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    c->spin_limit = polls;
     return KZG_OK;
 }
 

@@ -50,9 +50,11 @@ void launch_unit_desc(hipStream_t s, const uint32_t* unit, const uint4* bd, uint
 hipError_t launch_solve_split(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b,
                               uint32_t nbodies, float4* packed, const Schedule& ws, const Schedule& vel,
                               const SolveParams& sp, uint32_t groups, uint32_t threads, float* resid,
-                              UnitKernel uk, uint32_t coop_blocks, uint32_t coop_threads, uint32_t* counter,
-                              hipEvent_t coop_begin, hipEvent_t coop_end,
+                              UnitKernel uk, uint32_t coop_blocks, uint32_t coop_threads, uint32_t* sync,
+                              uint32_t limit, bool one_xcd, hipEvent_t coop_begin, hipEvent_t coop_end,
                               uint32_t bodies = kPackBodies | kUnpackBodies);
+// Clears a sync block of kSyncWords words, its give-up flag included.
+void launch_sync_reset(hipStream_t s, uint32_t* sync);
 void launch_compact(hipStream_t s, const uint32_t* m_count, uint32_t n, const uint2* pairs, const BodyBuf& b,
                     const ContactBuf& c, uint2* tile, uint32_t* totals);
 // lds_limit: most bodies for the one-workgroup LDS union-find (0: the
@@ -144,8 +146,12 @@ struct kzg_ctx {
     std::vector<uint8_t> h_dynamic, h_sensor, h_motion;
     DevVec<float4> b_x, b_q, b_v, b_w, b_pv, b_pw, b_iw, b_il, b_mat, b_lim;
     DevVec<float4> b_packed_solver;  // 8 float4 per body during the solve
-    DevVec<uint32_t> coop_counter;   // device-wide barrier of the cooperative solve
+    DevVec<uint32_t> coop_sync;      // sync block of the cooperative solve (kSyncWords words)
     uint32_t coop_blocks = 0;        // kzg_set_cooperative: 0 auto, 1 off, n workgroups
+    uint32_t coop_placement = 0;     // kzg_set_cooperative_placement: KZG_COOP_ONE_XCD or KZG_COOP_SPREAD
+    uint32_t spin_limit = 0;         // kzg_set_spin_limit: 0 default
+    bool coop_launched = false;      // this step's solve made a cooperative launch
+    uint32_t spin_limit_or_default() const { return spin_limit ? spin_limit : kDefaultSpinLimit; }
     uint32_t unit_cap_vel = 0, unit_cap_ws = 0;  // kzg_set_sweep_units: 0 default, 1..kMaxUnit
     uint32_t coop_used = 1;          // workgroups the last solve gave group 0
     uint32_t coop_ws_points = 0, coop_vel_points = 0;

@@ -282,6 +282,43 @@ __device__ __forceinline__ void store_vw(float4* p, uint32_t i, f3 v, f3 w, cons
     p[4 * i] = make_float4(v.x, v.y, v.z, w.x);
     p[4 * i + 1] = make_float4(w.y, w.z, ii.c0.x, ii.c0.y);
 }
+__device__ __forceinline__ PackedBody unpack_pb(float4 a, float4 b, float4 c, float4 d) {
+    PackedBody r;
+    r.v = f3{a.x, a.y, a.z};
+    r.w = f3{a.w, b.x, b.y};
+    r.ii = m3{f3{b.z, b.w, c.x}, f3{c.y, c.z, c.w}, f3{d.x, d.y, d.z}};
+    return r;
+}
+
+// How a sweep reaches the body lines.  Lines: plain loads, for one
+// workgroup (its waves share the CU's L1) or workgroups that order levels
+// with agent-scope fences.
+struct Lines {
+    float4* p;
+    __device__ __forceinline__ PackedBody load(uint32_t i) const { return load_pb(p, i); }
+    __device__ __forceinline__ void store(uint32_t i, f3 v, f3 w, const m3& ii) const { store_vw(p, i, v, w, ii); }
+};
+// LinesL2: loads at device scope (the sc1 cache policy: they miss the CU's
+// L1 and are served by the L2), for workgroups of one XCD that order their
+// levels through that XCD's L2 without cache maintenance: every store has
+// reached the L2 before the barrier (xcd_sync), and no load can hit a line
+// its CU's L1 kept from an earlier level.  Stores are plain: the L1 writes
+// through.
+constexpr int kBufferDword3 = 0x00020000;  // raw buffer, gfx9 family
+constexpr int kLoadDeviceScope = 16;       // cache policy bit SC1
+struct LinesL2 {
+    float4* p;
+    __device__ __forceinline__ PackedBody load(uint32_t i) const {
+        const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(p, 0, 0x7FFFFFFF, kBufferDword3);
+        const uint32_t o = 64u * i;
+        const float4 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, o, 0, kLoadDeviceScope));
+        const float4 b = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, o + 16u, 0, kLoadDeviceScope));
+        const float4 c = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, o + 32u, 0, kLoadDeviceScope));
+        const float4 d = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, o + 48u, 0, kLoadDeviceScope));
+        return unpack_pb(a, b, c, d);
+    }
+    __device__ __forceinline__ void store(uint32_t i, f3 v, f3 w, const m3& ii) const { store_vw(p, i, v, w, ii); }
+};
 
 __global__ void k_pack_bodies(BodyBuf b, uint32_t n, float4* vel, float4* pse) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -316,8 +353,8 @@ __global__ void k_unpack_bodies(BodyBuf b, uint32_t n, const float4* vel, const 
 // point at a time.
 // ---------------------------------------------------------------------------
 // WarmStart (ContactSolver.cpp:117-132): stored rA / rB.
-template <uint32_t U>
-__device__ __forceinline__ void warm_unit(const WsRecBuf& r, float4* sb, uint32_t e0, uint32_t n, uint4 bd,
+template <uint32_t U, class SB>
+__device__ __forceinline__ void warm_unit(const WsRecBuf& r, const SB& sb, uint32_t e0, uint32_t n, uint4 bd,
                                           float wsf) {
     const bool da = bd.z & 2u, db = bd.z & 4u;
     float4 imp[U], n4[U], t14[U], t24[U], ra4[U], rb4[U];
@@ -333,8 +370,8 @@ __device__ __forceinline__ void warm_unit(const WsRecBuf& r, float4* sb, uint32_
             if (db) rb4[k] = r.rb[e];
         }
     PackedBody A, B;
-    if (da) A = load_pb(sb, bd.x);
-    if (db) B = load_pb(sb, bd.y);
+    if (da) A = sb.load(bd.x);
+    if (db) B = sb.load(bd.y);
 #pragma unroll
     for (uint32_t k = 0; k < U; ++k)
         if (k < n) {
@@ -349,15 +386,15 @@ __device__ __forceinline__ void warm_unit(const WsRecBuf& r, float4* sb, uint32_
                 B.w = sub(B.w, mmul(B.ii, cross(ld3(rb4[k]), j)));
             }
         }
-    if (da) store_vw(sb, bd.x, A.v, A.w, A.ii);
-    if (db) store_vw(sb, bd.y, B.v, B.w, B.ii);
+    if (da) sb.store(bd.x, A.v, A.w, A.ii);
+    if (db) sb.store(bd.y, B.v, B.w, B.ii);
 }
 
 // Lanes of SolveVelocityBatchSIMD (ContactSolverSIMD.cpp:93-256).
 // _mm_max_ps(a, b) = a > b ? a : b and _mm_min_ps(a, b) = a < b ? a : b.
 // dsq accumulates each point's sum of squared impulse increments.
-template <uint32_t U>
-__device__ __forceinline__ void velocity_unit(const VelRecBuf& r, float4* sb, uint32_t e0, uint32_t n, uint4 bd,
+template <uint32_t U, class SB>
+__device__ __forceinline__ void velocity_unit(const VelRecBuf& r, const SB& sb, uint32_t e0, uint32_t n, uint4 bd,
                                               float& dsq) {
     float4 ra4[U], rb4[U], n4[U], t14[U], t24[U], imp[U];
     float2 fb[U];
@@ -373,7 +410,7 @@ __device__ __forceinline__ void velocity_unit(const VelRecBuf& r, float4* sb, ui
             fb[k] = r.fb[e];
             imp[k] = r.imp[e];
         }
-    const PackedBody A = load_pb(sb, bd.x), B = load_pb(sb, bd.y);
+    const PackedBody A = sb.load(bd.x), B = sb.load(bd.y);
     const m3 ia = A.ii, ib = B.ii;
     f3 va = A.v, wa = A.w, vb = B.v, wb = B.w;
     const float zero = 0.0f;
@@ -426,8 +463,8 @@ __device__ __forceinline__ void velocity_unit(const VelRecBuf& r, float4* sb, ui
             r.imp[e0 + k] = im;
             dsq += pd;
         }
-    if (bd.z & 2u) store_vw(sb, bd.x, va, wa, ia);
-    if (bd.z & 4u) store_vw(sb, bd.y, vb, wb, ib);
+    if (bd.z & 2u) sb.store(bd.x, va, wa, ia);
+    if (bd.z & 4u) sb.store(bd.y, vb, wb, ib);
 }
 
 // The same lanes on two adjacent threads (a lane pair): side 0 holds body
@@ -442,8 +479,8 @@ __device__ __forceinline__ float pair_swap(float v) {
     // quad_perm [1, 0, 3, 2]: lanes 2k and 2k + 1 exchange
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
 }
-template <uint32_t U>
-__device__ __forceinline__ void velocity_pair(const VelRecBuf& r, float4* sb, uint32_t e0, uint32_t n, uint4 bd,
+template <uint32_t U, class SB>
+__device__ __forceinline__ void velocity_pair(const VelRecBuf& r, const SB& sb, uint32_t e0, uint32_t n, uint4 bd,
                                               uint32_t side, float& dsq) {
     float4 rr4[U], n4[U], t14[U], t24[U], imp[U];
     float2 fb[U];
@@ -460,7 +497,7 @@ __device__ __forceinline__ void velocity_pair(const VelRecBuf& r, float4* sb, ui
             imp[k] = r.imp[e];
         }
     const uint32_t body = side ? bd.y : bd.x;
-    const PackedBody X = load_pb(sb, body);
+    const PackedBody X = sb.load(body);
     const m3 ii = X.ii;
     f3 v = X.v, w = X.w;
     const float zero = 0.0f;
@@ -517,12 +554,12 @@ __device__ __forceinline__ void velocity_pair(const VelRecBuf& r, float4* sb, ui
                 dsq += pd;
             }
         }
-    if (bd.z & (side ? 4u : 2u)) store_vw(sb, body, v, w, ii);
+    if (bd.z & (side ? 4u : 2u)) sb.store(body, v, w, ii);
 }
 
 // SolveContactVelocity, the scalar path (ContactSolver.cpp:137-181).
-template <uint32_t U>
-__device__ __forceinline__ void scalar_unit(const VelRecBuf& r, float4* sb, uint32_t e0, uint32_t n, uint4 bd,
+template <uint32_t U, class SB>
+__device__ __forceinline__ void scalar_unit(const VelRecBuf& r, const SB& sb, uint32_t e0, uint32_t n, uint4 bd,
                                             float& dsq) {
     float4 ra4[U], rb4[U], n4[U], t14[U], t24[U], imp[U];
     float2 fb[U];
@@ -538,7 +575,7 @@ __device__ __forceinline__ void scalar_unit(const VelRecBuf& r, float4* sb, uint
             fb[k] = r.fb[e];
             imp[k] = r.imp[e];
         }
-    const PackedBody A = load_pb(sb, bd.x), B = load_pb(sb, bd.y);
+    const PackedBody A = sb.load(bd.x), B = sb.load(bd.y);
     const m3 ia = A.ii, ib = B.ii;
     f3 va = A.v, wa = A.w, vb = B.v, wb = B.w;
     const bool da = bd.z & 2u, db = bd.z & 4u;
@@ -582,13 +619,13 @@ __device__ __forceinline__ void scalar_unit(const VelRecBuf& r, float4* sb, uint
             r.imp[e0 + k] = im;
             dsq += pd;
         }
-    if (da) store_vw(sb, bd.x, va, wa, ia);
-    if (db) store_vw(sb, bd.y, vb, wb, ib);
+    if (da) sb.store(bd.x, va, wa, ia);
+    if (db) sb.store(bd.y, vb, wb, ib);
 }
 
 // SolveBiasConstraint (ContactSolver.cpp:257-290).
-template <uint32_t U>
-__device__ __forceinline__ void bias_unit(const WsRecBuf& r, float4* ps, uint32_t e0, uint32_t n, uint4 bd) {
+template <uint32_t U, class SB>
+__device__ __forceinline__ void bias_unit(const WsRecBuf& r, const SB& ps, uint32_t e0, uint32_t n, uint4 bd) {
     float4 qa4[U], qb4[U], n4[U], imp[U];
     float ima[U], imb[U];
 #pragma unroll
@@ -602,7 +639,7 @@ __device__ __forceinline__ void bias_unit(const WsRecBuf& r, float4* ps, uint32_
             ima[k] = r.ra[e].w;
             imb[k] = r.rb[e].w;
         }
-    const PackedBody A = load_pb(ps, bd.x), B = load_pb(ps, bd.y);
+    const PackedBody A = ps.load(bd.x), B = ps.load(bd.y);
     f3 pva = A.v, pwa = A.w, pvb = B.v, pwb = B.w;
     bool moved = false;
 #pragma unroll
@@ -630,8 +667,8 @@ __device__ __forceinline__ void bias_unit(const WsRecBuf& r, float4* ps, uint32_
             moved = true;
         }
     }
-    if (moved && (bd.z & 2u)) store_vw(ps, bd.x, pva, pwa, A.ii);
-    if (moved && (bd.z & 4u)) store_vw(ps, bd.y, pvb, pwb, B.ii);
+    if (moved && (bd.z & 2u)) ps.store(bd.x, pva, pwa, A.ii);
+    if (moved && (bd.z & 4u)) ps.store(bd.y, pvb, pwb, B.ii);
 }
 
 // Level offsets of one group, staged in LDS when they fit (the sweep reads
@@ -690,20 +727,66 @@ __device__ __forceinline__ void unit_apply(const UnitRef& r, uint32_t side, F&& 
     else
         f(r.u, 1u, r.d, side);
 }
-// Device-wide barrier of the cooperating workgroups: monotonic counter.  An
-// agent-scope release fence before the arrival publishes the workgroup's
-// body updates of the level; an acquire fence after the wait makes the
-// others' visible.
-__device__ __forceinline__ void grid_sync(uint32_t* counter, uint32_t target) {
+// Device-wide barriers of the cooperating workgroups.  The sync block
+// (kSyncWords words, cleared before every cooperative launch) holds the
+// arrivals of the launch, the XCC chosen for the sweep and each XCC's
+// arrivals (k_solve_xcd), the level counter and the give-up flag.  Every
+// wait is bounded by `limit` polls: a workgroup that waits longer raises
+// the flag, every waiting workgroup then stops waiting, and the sweep ends
+// early with the flag set for the host to report.
+__device__ __forceinline__ uint32_t load_dev(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Thread 0 only: wait until *w >= target, or the flag is up, or the limit is
+// spent (then raise the flag); false if the flag is up.
+__device__ __forceinline__ bool spin_until(uint32_t* sync, const uint32_t* w, uint32_t target, uint32_t limit) {
+    uint32_t polls = 0;
+    while (load_dev(w) < target) {
+        if (load_dev(sync + kSyncGiveUp)) return false;
+        if (++polls >= limit) {
+            atomicOr(sync + kSyncGiveUp, 1u);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+// Workgroups anywhere on the device: an agent-scope release fence before
+// the arrival publishes the workgroup's body updates of the level (it
+// writes the XCD's dirty L2 lines back), an acquire fence after the wait
+// makes the others' visible.
+__device__ __forceinline__ bool grid_sync(uint32_t* sync, uint32_t target, uint32_t limit) {
+    __shared__ uint32_t ok;
     __syncthreads();
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        atomicAdd(counter, 1u);
-        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
-            __builtin_amdgcn_s_sleep(1);
+        atomicAdd(sync + kSyncLevel, 1u);
+        ok = spin_until(sync, sync + kSyncLevel, target, limit);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
+    return ok != 0u;
+}
+// Workgroups of one XCD, which share its L2: each wave waits until its
+// stores are acknowledged (they are then in the L2), the workgroup meets,
+// and thread 0 arrives and waits.  The body lines are read with
+// device-scope loads (LinesL2), which the L2 serves, so no cache is
+// written back or invalidated between levels.
+__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
+__device__ __forceinline__ bool xcd_sync(uint32_t* sync, uint32_t target, uint32_t limit) {
+    __shared__ uint32_t ok;
+    wait_stores();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(sync + kSyncLevel, 1u);
+        ok = spin_until(sync, sync + kSyncLevel, target, limit);
+    }
+    __syncthreads();
+    return ok != 0u;
+}
+// The XCC (accelerator complex die) this workgroup runs on.
+__device__ __forceinline__ uint32_t xcc_id() {
+    return __builtin_amdgcn_s_getreg(20 | (0 << 6) | ((16 - 1) << 11)) & 0xFu;  // HW_REG_XCC_ID
 }
 
 // ---------------------------------------------------------------------------
@@ -752,8 +835,8 @@ struct LevelStream {
 // interval as velocity step k: the same per-body order of updates with
 // fewer barriers.  `resid` (optional) receives the sum of squared impulse
 // increments of each velocity sweep.
-template <uint32_t UW, uint32_t UV, uint32_t L, class VelF, class Sync>
-__device__ __forceinline__ void velocity_and_bias(const WsRecBuf& wr, float4* sbp, const LevelOffsets& wo, uint2 wrange,
+template <uint32_t UW, uint32_t UV, uint32_t L, class SB, class VelF, class Sync>
+__device__ __forceinline__ void velocity_and_bias(const WsRecBuf& wr, const SB& sbp, const LevelOffsets& wo, uint2 wrange,
                                                   const LevelOffsets& vo, uint2 vrange, const uint4* vbd,
                                                   const Schedule& ws, const Schedule& vel, const SolveParams& sp,
                                                   float* resid, uint32_t parts, uint32_t part, VelF&& vf,
@@ -772,11 +855,11 @@ __device__ __forceinline__ void velocity_and_bias(const WsRecBuf& wr, float4* sb
             ++it;
         }
         if (k < bsteps) b.step([&](uint32_t e, uint32_t n, uint4 bd, uint32_t) { bias_unit<UW>(wr, sbp, e, n, bd); });
-        sync();
+        if (!sync()) return;
     }
 }
-template <uint32_t UW, uint32_t UV, class Sync>
-__device__ __forceinline__ void solve_group(const VelRecBuf& vr, const WsRecBuf& wr, float4* sbv, float4* sbp,
+template <uint32_t UW, uint32_t UV, class SB, class Sync>
+__device__ __forceinline__ void solve_group(const VelRecBuf& vr, const WsRecBuf& wr, const SB& sbv, const SB& sbp,
                                             const Schedule& ws, const Schedule& vel, const SolveParams& sp,
                                             float* resid, uint32_t group, uint32_t parts, uint32_t part,
                                             uint32_t* ws_lds, uint32_t* vel_lds, Sync&& sync) {
@@ -793,7 +876,7 @@ __device__ __forceinline__ void solve_group(const VelRecBuf& vr, const WsRecBuf&
         LevelStream<UW, 1> w(wo, wrange, wr.bd, ws.unit, parts, part);
         for (uint32_t k = wrange.x; k < wrange.y; ++k) {
             w.step([&](uint32_t e, uint32_t n, uint4 bd, uint32_t) { warm_unit<UW>(wr, sbv, e, n, bd, wsf); });
-            sync();
+            if (!sync()) return;
         }
     }
     if (!velocity) return;
@@ -822,8 +905,11 @@ __global__ void __launch_bounds__(solve_threads(UW, UV)) k_solve_groups(VelRecBu
                                                                 Schedule ws, Schedule vel, SolveParams sp,
                                                                 float* resid, uint32_t group0) {
     __shared__ uint32_t ws_lds[kLevelsLds], vel_lds[kLevelsLds];
-    solve_group<UW, UV>(vr, wr, sbv, sbp, ws, vel, sp, resid, group0 + blockIdx.x, 1u, 0u, ws_lds, vel_lds,
-                        [] { __syncthreads(); });
+    solve_group<UW, UV>(vr, wr, Lines{sbv}, Lines{sbp}, ws, vel, sp, resid, group0 + blockIdx.x, 1u, 0u, ws_lds,
+                        vel_lds, [] {
+                            __syncthreads();
+                            return true;
+                        });
 }
 
 // One group solved by gridDim.x cooperating workgroups (the launch makes
@@ -831,14 +917,50 @@ __global__ void __launch_bounds__(solve_threads(UW, UV)) k_solve_groups(VelRecBu
 template <uint32_t UW, uint32_t UV>
 __global__ void __launch_bounds__(solve_threads(UW, UV)) k_solve_coop(VelRecBuf vr, WsRecBuf wr, float4* sbv, float4* sbp,
                                                                  Schedule ws, Schedule vel, SolveParams sp,
-                                                                 float* resid, uint32_t group, uint32_t* counter) {
+                                                                 float* resid, uint32_t group, uint32_t* sync,
+                                                                 uint32_t limit) {
     __shared__ uint32_t ws_lds[kLevelsLds], vel_lds[kLevelsLds];
     uint32_t phase = 0;
-    solve_group<UW, UV>(vr, wr, sbv, sbp, ws, vel, sp, resid, group, gridDim.x, blockIdx.x, ws_lds, vel_lds,
-                        [&] { grid_sync(counter, gridDim.x * ++phase); });
+    solve_group<UW, UV>(vr, wr, Lines{sbv}, Lines{sbp}, ws, vel, sp, resid, group, gridDim.x, blockIdx.x, ws_lds,
+                        vel_lds, [&] { return grid_sync(sync, gridDim.x * ++phase, limit); });
 }
 
-__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
+// The same on the workgroups of one XCD.  The launch is cooperative and
+// holds several workgroups per XCD (they are dealt to the XCDs in turn).
+// Each workgroup reads its XCC, the first to arrive names the XCC of the
+// sweep, and once all have arrived the workgroups of that XCC sweep the
+// group's levels with xcd_sync between them; the others return.
+template <uint32_t UW, uint32_t UV>
+__global__ void __launch_bounds__(solve_threads(UW, UV)) k_solve_xcd(VelRecBuf vr, WsRecBuf wr, float4* sbv, float4* sbp,
+                                                                Schedule ws, Schedule vel, SolveParams sp,
+                                                                float* resid, uint32_t group, uint32_t* sync,
+                                                                uint32_t limit) {
+    __shared__ uint32_t ws_lds[kLevelsLds], vel_lds[kLevelsLds];
+    __shared__ uint32_t role[2];
+    if (threadIdx.x == 0) {
+        const uint32_t x = xcc_id();
+        atomicCAS(sync + kSyncTarget, 0xFFFFFFFFu, x);
+        const uint32_t idx = atomicAdd(sync + kSyncXcc + x, 1u);
+        atomicAdd(sync + kSyncArrive, 1u);
+        const bool ok = spin_until(sync, sync + kSyncArrive, gridDim.x, limit);
+        const uint32_t target = load_dev(sync + kSyncTarget);
+        role[0] = load_dev(sync + kSyncXcc + target);
+        role[1] = ok && x == target ? idx : 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    const uint32_t parts = role[0], part = role[1];
+    if (part == 0xFFFFFFFFu) return;
+    uint32_t phase = 0;
+    solve_group<UW, UV>(vr, wr, LinesL2{sbv}, LinesL2{sbp}, ws, vel, sp, resid, group, parts, part, ws_lds, vel_lds,
+                        [&] { return xcd_sync(sync, parts * ++phase, limit); });
+}
+
+// Clears a cooperative launch's sync block, the give-up flag included only
+// with `all` (a kernel, not a memset: the launches may be graph-captured).
+__global__ void k_sync_reset(uint32_t* sync, uint32_t all) {
+    const uint32_t i = threadIdx.x;
+    if (i < kSyncWords && (all || i != kSyncGiveUp)) sync[i] = i == kSyncTarget ? 0xFFFFFFFFu : 0u;
+}
 
 // Unit descriptors from the unit bounds and the records' body entries.
 __global__ void k_unit_desc(const uint32_t* unit, const uint4* bd, uint32_t nu, uint4* desc) {
@@ -1018,9 +1140,10 @@ static void launch_groups(hipStream_t s, UnitKernel uk, uint32_t n, uint32_t thr
         return 0;
     });
 }
-static const void* coop_kernel(UnitKernel uk) {
-    return by_units(uk, [](auto w, auto v) {
-        return (const void*)k_solve_coop<decltype(w)::value, decltype(v)::value>;
+static const void* coop_kernel(UnitKernel uk, bool one_xcd) {
+    return by_units(uk, [one_xcd](auto w, auto v) {
+        return one_xcd ? (const void*)k_solve_xcd<decltype(w)::value, decltype(v)::value>
+                       : (const void*)k_solve_coop<decltype(w)::value, decltype(v)::value>;
     });
 }
 
@@ -1038,13 +1161,19 @@ void launch_solve_groups(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr,
     if (bodies & kUnpackBodies) k_unpack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
 }
 
+void launch_sync_reset(hipStream_t s, uint32_t* sync) { k_sync_reset<<<1, 64, 0, s>>>(sync, 1u); }
+
 // Group 0 by `coop_blocks` cooperating workgroups of `coop_threads`, the
-// other groups one workgroup each.
+// other groups one workgroup each.  one_xcd: the cooperating workgroups all
+// run on one XCD (k_solve_xcd, launched with kXcds x coop_blocks workgroups
+// of which one XCD's take part); else anywhere (k_solve_coop).  sync:
+// kSyncWords words; limit: most polls of one wait.
 hipError_t launch_solve_split(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const BodyBuf& b,
                               uint32_t nbodies, float4* packed, const Schedule& ws, const Schedule& vel,
                               const SolveParams& sp, uint32_t groups, uint32_t threads, float* resid,
-                              UnitKernel uk, uint32_t coop_blocks, uint32_t coop_threads, uint32_t* counter,
-                              hipEvent_t coop_begin, hipEvent_t coop_end, uint32_t bodies) {
+                              UnitKernel uk, uint32_t coop_blocks, uint32_t coop_threads, uint32_t* sync,
+                              uint32_t limit, bool one_xcd, hipEvent_t coop_begin, hipEvent_t coop_end,
+                              uint32_t bodies) {
     if (!groups || !nbodies) return hipSuccess;
     // The cooperative sweep reads unit arrays (the velocity one only when it
     // runs the velocity iterations).
@@ -1052,15 +1181,16 @@ hipError_t launch_solve_split(hipStream_t s, const VelRecBuf& vr, const WsRecBuf
     float4* sbv = packed;
     float4* sbp = packed + 4 * size_t(nbodies);
     if (bodies & kPackBodies) k_pack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
-    k_zero_u32<<<1, 1, 0, s>>>(counter);
+    k_sync_reset<<<1, 64, 0, s>>>(sync, 0u);
     VelRecBuf vr_ = vr;
     WsRecBuf wr_ = wr;
     Schedule ws_ = ws, vel_ = vel;
     SolveParams sp_ = sp;
     uint32_t group = 0;
-    void* args[] = {&vr_, &wr_, &sbv, &sbp, &ws_, &vel_, &sp_, &resid, &group, &counter};
+    void* args[] = {&vr_, &wr_, &sbv, &sbp, &ws_, &vel_, &sp_, &resid, &group, &sync, &limit};
     if (coop_begin) hipEventRecord(coop_begin, s);
-    hipError_t e = hipLaunchCooperativeKernel(coop_kernel(uk), dim3(coop_blocks), dim3(coop_threads), args, 0, s);
+    const uint32_t grid = one_xcd ? kXcds * coop_blocks : coop_blocks;
+    hipError_t e = hipLaunchCooperativeKernel(coop_kernel(uk, one_xcd), dim3(grid), dim3(coop_threads), args, 0, s);
     if (e != hipSuccess) return e;
     if (coop_end) hipEventRecord(coop_end, s);
     if (groups > 1) launch_groups(s, uk, groups - 1, threads, vr, wr, sbv, sbp, ws, vel, sp, resid, 1u);

@@ -31,5 +31,16 @@ constexpr UnitKernel unit_kernel(uint32_t largest_ws, uint32_t largest_vel) {
 // registers, and four records of either kind need the register budget of
 // 512 threads (2 waves per SIMD).
 constexpr uint32_t solve_threads(uint32_t UW, uint32_t UV) { return UV <= 2 && UW <= 2 ? 1024u : 512u; }
+// Words of a cooperative sweep's sync block: the launch's arrivals, the XCC
+// chosen for a one-XCD sweep, each XCC's arrivals, the level counter and
+// the give-up flag.
+constexpr uint32_t kSyncWords = 32;
+enum : uint32_t { kSyncArrive = 0, kSyncTarget = 1, kSyncXcc = 2, kSyncLevel = 18, kSyncGiveUp = 19 };
+// XCDs of an MI355X (workgroups are dealt to them in turn); the one-XCD
+// sweep launches this many workgroups per participant.
+constexpr uint32_t kXcds = 8;
+// Default bound on the polls of one wait of a cooperative sweep (a wait
+// sleeps between polls: about a second in all).
+constexpr uint32_t kDefaultSpinLimit = 1u << 21;
 
 }  // namespace kz

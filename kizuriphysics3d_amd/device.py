@@ -40,7 +40,7 @@ EXPORTS = [
     "kzg_narrowphase", "kzg_build_constraints", "kzg_integrate_velocities", "kzg_solve", "kzg_integrate_positions",
     "kzg_step", "kzg_query_sizes", "kzg_download_states", "kzg_download_manifolds", "kzg_download_constraints",
     "kzg_get_residuals", "kzg_get_timings", "kzg_collide_batch", "kzg_collide_batch_device", "kzg_schedule", "kzg_set_warm_start",
-    "kzg_set_iterations", "kzg_set_cooperative", "kzg_set_sweep_units", "kzg_set_early_warm_start", "kzg_set_narrowphase_waves", "kzg_set_island_lds_limit", "kzg_cache_info", "kzg_download_cache", "kzg_upload_cache",
+    "kzg_set_iterations", "kzg_set_cooperative", "kzg_set_sweep_units", "kzg_set_early_warm_start", "kzg_set_narrowphase_waves", "kzg_set_island_lds_limit", "kzg_set_cooperative_placement", "kzg_set_spin_limit", "kzg_cache_info", "kzg_download_cache", "kzg_upload_cache",
     "kzg_mlp_init", "kzg_mlp_set_params", "kzg_mlp_get_params", "kzg_mlp_set_normalization",
     "kzg_mlp_get_normalization", "kzg_mlp_collect", "kzg_mlp_num_samples", "kzg_mlp_get_samples",
     "kzg_mlp_fit_normalization", "kzg_mlp_train", "kzg_mlp_forward", "kzg_mlp_gradients",
@@ -48,6 +48,7 @@ EXPORTS = [
 
 # Warm-start modes (kzg_warm_start in include/kzg.h).
 WARM_CACHE, WARM_COLD, WARM_LEARNED, WARM_LEARNED_MISS, WARM_NO_STORE = 0, 1, 2, 3, 0x100
+COOP_ONE_XCD, COOP_SPREAD = 0, 1
 MLP_PARAMS, MLP_FEATURES, MLP_FEATURE_ROW, MLP_SAMPLE_ROW = 5315, 14, 16, 20
 
 _lib = None
@@ -86,6 +87,8 @@ def library():
             "kzg_set_early_warm_start": ([vp, u32], i32),
             "kzg_set_narrowphase_waves": ([vp, u32], i32),
             "kzg_set_island_lds_limit": ([vp, u32], i32),
+            "kzg_set_cooperative_placement": ([vp, u32], i32),
+            "kzg_set_spin_limit": ([vp, u32], i32),
             "kzg_cache_info": ([vp, vp, vp], i32), "kzg_download_cache": ([vp, vp, C.c_uint64], i32),
             "kzg_upload_cache": ([vp, vp, C.c_uint64, u32, u32], i32),
             "kzg_mlp_init": ([vp, C.c_uint64], i32), "kzg_mlp_set_params": ([vp, vp, u32], i32),
@@ -300,6 +303,14 @@ class DeviceContext:
     def set_narrowphase_waves(self, waves: int):
         """Waves of the EPA / clipping kernel: 0 auto (16 per CU)."""
         self._check(library().kzg_set_narrowphase_waves(self.h, int(waves)), "kzg_set_narrowphase_waves")
+
+    def set_cooperative_placement(self, placement: int):
+        """COOP_ONE_XCD (0, default) or COOP_SPREAD (1)."""
+        self._check(library().kzg_set_cooperative_placement(self.h, int(placement)), "kzg_set_cooperative_placement")
+
+    def set_spin_limit(self, polls: int):
+        """Most polls of one wait of a cooperative sweep (0 default)."""
+        self._check(library().kzg_set_spin_limit(self.h, int(polls)), "kzg_set_spin_limit")
 
     def set_island_lds_limit(self, max_bodies: int):
         """Most bodies for the one-workgroup LDS union-find (0 default, 1:

@@ -65,11 +65,13 @@ def test_plane_pairs_bit_exact():
         assert i is None, f"point {k} differs at pair {np.nonzero(live)[0][i]}"
 
 
-def _trajectory(scene, steps, cooperative=0, units=0, early=0, island_lds=0, np_waves=0, log_every=0):
+def _trajectory(scene, steps, cooperative=0, units=0, early=0, island_lds=0, np_waves=0, log_every=0,
+                placement=D.COOP_ONE_XCD):
     R.init_jobs(0)
     rw = R.RefWorld(scene)
     ctx = _ctx_for(scene, rw)
     ctx.set_cooperative(cooperative)
+    ctx.set_cooperative_placement(placement)
     ctx.set_early_warm_start(early)
     ctx.set_island_lds_limit(island_lds)
     ctx.set_narrowphase_waves(np_waves)
@@ -166,10 +168,41 @@ def test_device_islands_match_host_union_find():
 
 @pytest.mark.parametrize("blocks", [2, 7, 64])
 def test_pile_trajectory_cooperative_bit_exact(blocks):
-    """The largest group's levels split over `blocks` workgroups with a
-    device-wide barrier between levels (forced even where levels are narrow,
-    so some workgroups get empty slices)."""
+    """The largest group's levels split over `blocks` workgroups of one XCD
+    (64 requested: the XCD's 32) with a barrier through that XCD's L2
+    between levels (forced even where levels are narrow, so some workgroups
+    get empty slices)."""
     _trajectory(S.pile(512, seed=0x4B5A0102), 240, cooperative=blocks)
+
+
+@pytest.mark.parametrize("blocks", [7, 64])
+def test_pile_trajectory_cooperative_spread_bit_exact(blocks):
+    """The same split over workgroups anywhere on the device, with
+    agent-scope fences between levels."""
+    _trajectory(S.pile(512, seed=0x4B5A0102), 240, cooperative=blocks, placement=D.COOP_SPREAD)
+
+
+def test_cooperative_sweep_gives_up_and_reports():
+    """A cooperative sweep whose waits are bounded by one poll gives up on
+    every workgroup (each kernel still ends), and the next call that waits
+    for the device reports it; with the bound restored the world steps
+    again."""
+    from kizuriphysics3d_amd.world import PhysicsWorld
+    w = PhysicsWorld(S.pile(4096, seed=0x4B5A0104))
+    ctx = w.device_context()
+    ctx.set_cooperative(8)
+    for _ in range(30):
+        w.step_fixed()
+    ctx.set_spin_limit(1)
+    with pytest.raises(D.KzgError, match="gave up"):
+        for _ in range(3):  # the step that gives up, then the call that reports it
+            w.step_fixed()
+            w.states()
+    ctx.set_spin_limit(0)
+    for _ in range(3):
+        w.step_fixed()
+    assert ctx.timings()["cooperative_blocks"] == 8
+    w.close()
 
 
 @pytest.mark.parametrize("units,blocks", [((1, 1), 0), ((2, 2), 0), ((3, 1), 1), ((4, 4), 5), ((1, 4), 0)])
@@ -212,13 +245,16 @@ def test_early_warm_start_off_equals_on_large_pile():
 
 def test_cooperative_split_invariant_large_pile():
     """On a 4096-body pile the state after every step is the same with one
-    workgroup, the automatic split, 4 and 16 workgroups."""
+    workgroup, the automatic split, 4 and 32 workgroups on one XCD and 16
+    spread over the device."""
     from kizuriphysics3d_amd.world import PhysicsWorld
     scene = S.pile(4096, seed=0x4B5A0104)
     worlds = []
-    for blocks in (1, 0, 4, 16):
+    for blocks, placement in ((1, 0), (0, D.COOP_ONE_XCD), (4, D.COOP_ONE_XCD), (16, D.COOP_SPREAD),
+                              (32, D.COOP_ONE_XCD)):
         w = PhysicsWorld(scene)
         w.device_context().set_cooperative(blocks)
+        w.device_context().set_cooperative_placement(placement)
         worlds.append(w)
     used = set()
     for i in range(300):
