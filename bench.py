@@ -440,6 +440,7 @@ def main():
     early_steps = 0
     levels = []
     phases = np.zeros(13)
+    bpm = bpq = 0.0
     t_start = time.perf_counter()
     for i in range(args.steps):
         w.step_fixed()
@@ -447,6 +448,8 @@ def main():
         tm = device_timings(w)
         points += st["num_contact_points"]
         path_ms += st["contact_path_ms"]
+        bpm += st["broadphase_move_ms"]
+        bpq += st["broadphase_query_ms"]
         # Device work that ran while the host finished the velocity schedule
         # (the early warm start) counts once.
         resident_ms += (tm["narrowphase_ms"] + tm["build_device_ms"] + tm["build_ms"] + tm["integrate_velocities_ms"] +
@@ -490,7 +493,8 @@ def main():
         f"build on the host {ph[1]:.2f} (schedule {ph[6]:.2f}: warm-start order {ph[8]:.2f}, "
         f"velocity order {ph[11]:.2f} after it, waited {ph[9]:.2f}), integrate velocities "
         f"{ph[2]:.2f}, solve {ph[3]:.2f} (sweep kernel {ph[4]:.2f}; early warm start {ph[10]:.2f}), "
-        f"overlapped {ph[12]:.2f}, host broadphase {ph[5]:.2f}")
+        f"overlapped {ph[12]:.2f}, broadphase {ph[5]:.2f} (proxy moves {bpm / args.steps:.2f}, pair query "
+        f"{bpq / args.steps:.2f})")
     if args.dump:
         import ctypes as C
         from kizuriphysics3d_amd import device as D

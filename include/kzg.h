@@ -122,6 +122,34 @@ int kzg_upload_states(kzg_ctx* ctx, const kz_body_state* states, uint32_t num_bo
 /* Step 2: pairs = (slot A, slot B) per broadphase pair, in broadphase order. */
 int kzg_narrowphase(kzg_ctx* ctx, const uint32_t* pairs, uint32_t num_pairs);
 
+/* Step 1's pair query on the device (DynamicAABBTree::QueryPairs +
+ * BroadPhase::UpdatePairs, include/Kizuri/BroadPhase/BroadPhase.h:61-104,
+ * src/BroadPhase/BroadPhase.cpp:368-381).  The caller keeps the tree as the
+ * reference does (proxies created, moved and rebalanced on the host) and
+ * passes its image: one record per node, leaves with child1 = 0xFFFFFFFF,
+ * user = the proxy's body slot, height as the tree keeps it; the proxy
+ * layers and masks indexed by node id.  The pairs come out in the
+ * reference's order, the pairs its query loses included, as body slots, and
+ * stay on the device for kzg_narrowphase_resident; *num_pairs receives
+ * their count.  A tree whose ids, heights or user data break its invariants
+ * is KZG_ERR_ARGUMENT. */
+typedef struct kzg_bp_node {
+    float fat_min[3];
+    uint32_t child1;
+    float fat_max[3];
+    uint32_t child2;
+    float tight_min[3];
+    uint32_t user;
+    float tight_max[3];
+    int32_t height;
+} kzg_bp_node;
+int kzg_broadphase_pairs(kzg_ctx* ctx, const kzg_bp_node* nodes, uint32_t num_nodes, uint32_t root,
+                         const uint32_t* layers, const uint32_t* masks, uint32_t num_layers, uint32_t* num_pairs);
+/* Step 2 over the pairs the last kzg_broadphase_pairs left on the device. */
+int kzg_narrowphase_resident(kzg_ctx* ctx);
+/* The first n pairs the context holds (slot A, slot B each). */
+int kzg_download_pairs(kzg_ctx* ctx, uint32_t* pairs, uint32_t n);
+
 /* Step 3.  warm_start_mode: 0 = manifold cache (reference default),
  * 1 = cold start (the reference's KZ_NOWS switch). */
 int kzg_build_constraints(kzg_ctx* ctx, int warm_start_mode);

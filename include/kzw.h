@@ -35,6 +35,9 @@ typedef struct kzw_stats {
     double broadphase_ms;            /* host: broadphase update + pair list */
     double contact_path_ms;          /* host clock around narrowphase + build + solve (device synchronised) */
     double step_ms;                  /* whole StepFixed, host clock */
+    /* Inside broadphase_ms: the proxy moves (tree upkeep), and the pair
+     * query (host walk, or tree image + device query with its transfers). */
+    double broadphase_move_ms, broadphase_query_ms;
 } kzw_stats;
 
 int kzw_create(kzw_world** out, const kz_world_desc* world, int device);
@@ -58,6 +61,10 @@ int kzw_get_states(kzw_world* w, kz_body_state* out, uint32_t n);
 int kzw_set_states(kzw_world* w, const kz_body_state* in, uint32_t n);
 /* Pairs of the last broadphase update as (slot A, slot B); returns the count. */
 int kzw_get_pairs(kzw_world* w, uint32_t* out, uint32_t cap);
+/* kzw_step's pair query: on the device (kzg_broadphase_pairs over the tree
+ * kept here; default when the world has a device) or on the host (the same
+ * walk, on host threads).  The pairs and their order are the same. */
+int kzw_set_device_broadphase(kzw_world* w, int on);
 int kzw_get_stats(kzw_world* w, kzw_stats* out);
 /* Per body: inverse mass + inverse local inertia (column-major), 10 floats. */
 int kzw_get_mass(kzw_world* w, float* out, uint32_t n);

@@ -194,11 +194,6 @@ int check_sweep(kzg_ctx* c) {
     return KZG_OK;
 }
 
-// A deferred velocity schedule must be complete before anything reads the
-// host arrays it is built from or the schedule itself.
-void join_schedule(kzg_ctx* c) {
-    if (c->sched_builder.pending()) c->sched_builder.finish();
-}
 
 }  // namespace
 
@@ -265,6 +260,12 @@ void kzg_destroy(kzg_ctx* c) {
     c->b_packed_solver.release();
     for (auto* d : {&c->mlp_master, &c->mlp_m, &c->mlp_v, &c->mlp_norm, &c->mlp_partial, &c->mlp_out, &c->mlp_loss})
         d->release();
+    for (auto* d : {&c->bp_layer, &c->bp_mask, &c->bp_counts, &c->bp_sums, &c->bp_counters}) d->release();
+    c->bp_nodes.release();
+    c->bp_items_a.release();
+    c->bp_items_b.release();
+    c->pin_bp.release();
+    c->stage_bp.release();
     c->mlp_w1b.release();
     c->mlp_w2b.release();
     c->mlp_w3b.release();
@@ -410,6 +411,12 @@ int kzg_upload_states(kzg_ctx* c, const kz_body_state* s, uint32_t n) {
     return KZG_OK;
 }
 
+namespace {
+// Step 2 over the n pairs in d_pairs (already on the device or being copied
+// there on the stream).
+int narrowphase_on_device(kzg_ctx* c, uint32_t n);
+}  // namespace
+
 int kzg_narrowphase(kzg_ctx* c, const uint32_t* pairs, uint32_t n) {
     if (!c || (n && !pairs)) return fail(c, KZG_ERR_ARGUMENT, "null pair array");
     if (n > kWorkPairMask) return fail(c, KZG_ERR_UNSUPPORTED, "more than 2^27 - 1 broadphase pairs");
@@ -417,6 +424,27 @@ int kzg_narrowphase(kzg_ctx* c, const uint32_t* pairs, uint32_t n) {
     hipSetDevice(c->device);
     for (uint32_t i = 0; i < 2 * n; ++i)
         if (pairs[i] >= c->nbodies) return fail(c, KZG_ERR_ARGUMENT, "pair " + std::to_string(i / 2) + ": bad body index");
+    c->recorded = 0;
+    c->pairs_resident = false;
+    int rc;
+    if ((rc = upload(c, c->d_pairs, reinterpret_cast<const uint2*>(pairs), n))) return rc;
+    return narrowphase_on_device(c, n);
+}
+
+int kzg_narrowphase_resident(kzg_ctx* c) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (!c->pairs_resident) return fail(c, KZG_ERR_STATE, "kzg_narrowphase_resident needs a kzg_broadphase_pairs");
+    if (c->npairs > kWorkPairMask) return fail(c, KZG_ERR_UNSUPPORTED, "more than 2^27 - 1 broadphase pairs");
+    join_schedule(c);
+    hipSetDevice(c->device);
+    c->recorded = 0;
+    return narrowphase_on_device(c, c->npairs);
+}
+
+}  // extern "C"
+
+namespace {
+int narrowphase_on_device(kzg_ctx* c, uint32_t n) {
     c->npairs = n;
     KZG_HIP(c, c->m_count.ensure(std::max<uint32_t>(n, 1)));
     KZG_HIP(c, c->m_pa.ensure(size_t(std::max<uint32_t>(n, 1)) * kPairPoints));
@@ -432,9 +460,6 @@ int kzg_narrowphase(kzg_ctx* c, const uint32_t* pairs, uint32_t n) {
     KZG_HIP(c, c->uf_parent.ensure(std::max<uint32_t>(c->nbodies, 1)));
     KZG_HIP(c, c->uf_island.ensure(std::max<uint32_t>(c->nbodies, 1)));
     KZG_HIP(c, c->con_island.ensure(std::max<uint32_t>(n, 1)));
-    int rc = 0;
-    c->recorded = 0;
-    if ((rc = upload(c, c->d_pairs, reinterpret_cast<const uint2*>(pairs), n))) return rc;
     KZG_HIP(c, record(c, 0));
     launch_narrowphase(c->stream, c->d_pairs.p, n, c->bodies(), c->shapes(), c->manifolds(), c->work.p,
                        c->work_count.p, c->gjk.p, 0.02f, 1,
@@ -456,6 +481,9 @@ int kzg_narrowphase(kzg_ctx* c, const uint32_t* pairs, uint32_t n) {
     c->phase = 1;
     return KZG_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int kzg_build_constraints(kzg_ctx* c, int warm) {
     if (!c) return KZG_ERR_ARGUMENT;

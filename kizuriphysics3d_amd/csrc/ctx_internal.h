@@ -160,6 +160,13 @@ struct kzg_ctx {
     // pairs and manifolds
     uint32_t npairs = 0;
     DevVec<uint2> d_pairs;
+    bool pairs_resident = false;  // d_pairs holds the last device broadphase query's pairs
+    // device broadphase query (broadphase.hip)
+    DevVec<float4> bp_nodes;
+    DevVec<uint32_t> bp_layer, bp_mask, bp_counts, bp_sums, bp_counters;
+    DevVec<uint2> bp_items_a, bp_items_b;
+    Pinned pin_bp, stage_bp;
+    uint32_t bp_items_cap = 0, bp_pairs_cap = 0;
     DevVec<uint32_t> m_count;
     DevVec<float4> m_pa, m_pb, m_n;
     DevVec<uint32_t> work, work_count;
@@ -310,6 +317,12 @@ template <class T> inline int download(kzg_ctx* c, T* dst, const DevVec<T>& d, s
 }
 
 inline float4 f4(float a, float b, float cc, float d) { return make_float4(a, b, cc, d); }
+
+// A deferred velocity schedule must be complete before anything reads the
+// host arrays it is built from or the schedule itself.
+inline void join_schedule(kzg_ctx* c) {
+    if (c->sched_builder.pending()) c->sched_builder.finish();
+}
 
 inline int ensure_cache(kzg_ctx* c, size_t needed_keys) {
     // Load factor at most 1/2 so that probing always ends quickly.

@@ -430,6 +430,16 @@ bool BroadPhase::restore(const uint32_t* in, size_t n) {
     return true;
 }
 
+void BroadPhase::image(float* out) const {
+    for (const Node& n : nodes_) {
+        out[0] = n.fat.mn.x; out[1] = n.fat.mn.y; out[2] = n.fat.mn.z; out[3] = bitsf(n.child1);
+        out[4] = n.fat.mx.x; out[5] = n.fat.mx.y; out[6] = n.fat.mx.z; out[7] = bitsf(n.child2);
+        out[8] = n.tight.mn.x; out[9] = n.tight.mn.y; out[10] = n.tight.mn.z; out[11] = bitsf(n.user);
+        out[12] = n.tight.mx.x; out[13] = n.tight.mx.y; out[14] = n.tight.mx.z; out[15] = bitsf(uint32_t(n.height));
+        out += 16;
+    }
+}
+
 bool BroadPhase::is_proxy(uint32_t id, uint32_t user) const {
     return id < nodes_.size() && id < layers_.size() && nodes_[id].leaf() && nodes_[id].height == 0 &&
            nodes_[id].user == user;

@@ -61,6 +61,14 @@ public:
     uint32_t user(uint32_t id) const { return nodes_[id].user; }
     // Whether `id` is a leaf of the tree carrying `user` (snapshot checks).
     bool is_proxy(uint32_t id, uint32_t user) const;
+    // The tree as the device query reads it: per node fat box + child1, fat
+    // box + child2, tight box + user, tight box + height (ints as float bits),
+    // 16 words; the root; the proxy layers and masks by node id.
+    void image(float* nodes16) const;
+    uint32_t num_nodes() const { return uint32_t(nodes_.size()); }
+    uint32_t root() const { return root_; }
+    const std::vector<uint32_t>& layers() const { return layers_; }
+    const std::vector<uint32_t>& masks() const { return masks_; }
     // The whole tree as plain words (snapshots): node records, root, free
     // list, proxy layers and masks.  restore() returns false on a malformed
     // image and leaves the tree unchanged.

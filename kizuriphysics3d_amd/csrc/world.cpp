@@ -29,6 +29,11 @@ struct kzw_world {
     std::vector<float> mass;  // 10 per body
     BroadPhase bp;
     std::vector<uint32_t> pairs;  // slot A, slot B
+    // The pair query on the device (kzg_broadphase_pairs): the pairs then
+    // stay on the device, and `pairs` is filled only when asked for.
+    bool device_bp = true, pairs_on_device = false;
+    uint32_t device_pairs = 0;
+    std::vector<float> bp_image;
     bool shapes_dirty = true, bodies_dirty = true;
     kzw_stats stats{};
 };
@@ -188,10 +193,10 @@ int kzw_add_bodies(kzw_world* w, const kz_body_desc* b, uint32_t n) {
     return KZG_OK;
 }
 
-int kzw_update_broadphase(kzw_world* w) {
-    if (!w) return KZG_ERR_ARGUMENT;
-    // UpdateBroadPhase (PhysicsWorld.cpp:334-345): active non-static bodies,
-    // displacement = v * fixedTimestep.
+namespace {
+// UpdateBroadPhase's proxy moves (PhysicsWorld.cpp:334-345): active
+// non-static bodies, displacement = v * fixedTimestep.
+void move_proxies(kzw_world* w) {
     for (uint32_t i = 0; i < w->bodies.size(); ++i) {
         if (w->bodies[i].motion_type == KZ_MOTION_STATIC) continue;
         const kz_body_state& s = w->states[i];
@@ -199,10 +204,43 @@ int kzw_update_broadphase(kzw_world* w) {
                             w->settings.fixed_timestep);
         w->bp.move_proxy(w->proxy[i], world_bounds(*w, i), disp);
     }
+}
+// The pair query on the device over the moved tree; the pairs stay there.
+int device_pairs(kzw_world* w) {
+    const uint32_t nn = w->bp.num_nodes();
+    w->bp_image.resize(16 * size_t(nn));
+    w->bp.image(w->bp_image.data());
+    const auto& lay = w->bp.layers();
+    const auto& msk = w->bp.masks();
+    uint32_t np = 0;
+    int rc = kzg_broadphase_pairs(w->ctx, reinterpret_cast<const kzg_bp_node*>(w->bp_image.data()), nn,
+                                  w->bp.root(), lay.data(), msk.data(), uint32_t(lay.size()), &np);
+    if (rc) return wfail(w, rc, kzg_last_error(w->ctx));
+    w->device_pairs = np;
+    w->pairs_on_device = true;
+    return KZG_OK;
+}
+
+// The pair query on host threads.
+void host_pairs(kzw_world* w) {
     w->bp.update_pairs();
     const auto& pp = w->bp.pairs();
     w->pairs.resize(pp.size());
     for (size_t k = 0; k < pp.size(); ++k) w->pairs[k] = w->bp.user(pp[k]);
+    w->pairs_on_device = false;
+}
+}  // namespace
+
+int kzw_update_broadphase(kzw_world* w) {
+    if (!w) return KZG_ERR_ARGUMENT;
+    move_proxies(w);
+    host_pairs(w);
+    return KZG_OK;
+}
+
+int kzw_set_device_broadphase(kzw_world* w, int on) {
+    if (!w) return KZG_ERR_ARGUMENT;
+    w->device_bp = on != 0;
     return KZG_OK;
 }
 
@@ -212,11 +250,22 @@ int kzw_step(kzw_world* w, float dt) {
     const auto t0 = clock::now();
     int rc = sync_device(w);
     if (rc) return rc;
-    kzw_update_broadphase(w);
+    uint32_t np = 0;
+    move_proxies(w);
+    const auto tmv = clock::now();
+    if (w->device_bp) {
+        if ((rc = device_pairs(w))) return rc;
+        np = w->device_pairs;
+    } else {
+        host_pairs(w);
+        np = uint32_t(w->pairs.size() / 2);
+    }
     const auto t1 = clock::now();
-    const uint32_t np = uint32_t(w->pairs.size() / 2);
-    if ((rc = kzg_narrowphase(w->ctx, w->pairs.data(), np)) || (rc = kzg_build_constraints(w->ctx, 0)) ||
-        (rc = kzg_integrate_velocities(w->ctx, dt)) || (rc = kzg_solve(w->ctx, dt)))
+    w->stats.broadphase_move_ms = std::chrono::duration<double, std::milli>(tmv - t0).count();
+    w->stats.broadphase_query_ms = std::chrono::duration<double, std::milli>(t1 - tmv).count();
+    if ((rc = w->device_bp ? kzg_narrowphase_resident(w->ctx) : kzg_narrowphase(w->ctx, w->pairs.data(), np)) ||
+        (rc = kzg_build_constraints(w->ctx, 0)) || (rc = kzg_integrate_velocities(w->ctx, dt)) ||
+        (rc = kzg_solve(w->ctx, dt)))
         return wfail(w, rc, kzg_last_error(w->ctx));
     kzg_timings tm;
     if ((rc = kzg_get_timings(w->ctx, &tm))) return wfail(w, rc, kzg_last_error(w->ctx));  // synchronises
@@ -255,6 +304,12 @@ int kzw_set_states(kzw_world* w, const kz_body_state* in, uint32_t n) {
 
 int kzw_get_pairs(kzw_world* w, uint32_t* out, uint32_t cap) {
     if (!w) return KZG_ERR_ARGUMENT;
+    if (w->pairs_on_device) {
+        w->pairs.resize(2 * size_t(w->device_pairs));
+        const int rc = kzg_download_pairs(w->ctx, w->pairs.data(), w->device_pairs);
+        if (rc) return wfail(w, rc, kzg_last_error(w->ctx));
+        w->pairs_on_device = false;
+    }
     const uint32_t n = uint32_t(w->pairs.size() / 2);
     if (out) std::memcpy(out, w->pairs.data(), sizeof(uint32_t) * 2 * std::min(n, cap));
     return int(n);

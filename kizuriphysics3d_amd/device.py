@@ -37,7 +37,7 @@ assert KZG_BODY_DTYPE.itemsize == 136
 
 EXPORTS = [
     "kzg_create", "kzg_destroy", "kzg_last_error", "kzg_upload_shapes", "kzg_upload_bodies", "kzg_upload_states",
-    "kzg_narrowphase", "kzg_build_constraints", "kzg_integrate_velocities", "kzg_solve", "kzg_integrate_positions",
+    "kzg_narrowphase", "kzg_broadphase_pairs", "kzg_narrowphase_resident", "kzg_download_pairs", "kzg_build_constraints", "kzg_integrate_velocities", "kzg_solve", "kzg_integrate_positions",
     "kzg_step", "kzg_query_sizes", "kzg_download_states", "kzg_download_manifolds", "kzg_download_constraints",
     "kzg_get_residuals", "kzg_get_timings", "kzg_collide_batch", "kzg_collide_batch_device", "kzg_schedule", "kzg_set_warm_start",
     "kzg_set_iterations", "kzg_set_cooperative", "kzg_set_sweep_units", "kzg_set_early_warm_start", "kzg_set_narrowphase_waves", "kzg_set_island_lds_limit", "kzg_set_cooperative_placement", "kzg_set_spin_limit", "kzg_cache_info", "kzg_download_cache", "kzg_upload_cache",
@@ -73,6 +73,8 @@ def library():
             "kzg_upload_shapes": ([vp, vp, u32, vp, vp, u32, vp, u32], i32),
             "kzg_upload_bodies": ([vp, vp, u32], i32), "kzg_upload_states": ([vp, vp, u32], i32),
             "kzg_narrowphase": ([vp, vp, u32], i32), "kzg_build_constraints": ([vp, i32], i32),
+            "kzg_broadphase_pairs": ([vp, vp, u32, u32, vp, vp, u32, vp], i32),
+            "kzg_narrowphase_resident": ([vp], i32), "kzg_download_pairs": ([vp, vp, u32], i32),
             "kzg_integrate_velocities": ([vp, f32], i32), "kzg_solve": ([vp, f32], i32),
             "kzg_integrate_positions": ([vp, f32], i32), "kzg_step": ([vp, vp, u32, f32], i32),
             "kzg_query_sizes": ([vp, vp, vp, vp], i32), "kzg_download_states": ([vp, vp, u32], i32),

@@ -20,6 +20,7 @@ from . import scenes as S
 STATS_DTYPE = np.dtype([
     ("num_bodies", "<u4"), ("num_broadphase_pairs", "<u4"), ("num_contact_constraints", "<u4"),
     ("num_contact_points", "<u4"), ("broadphase_ms", "<f8"), ("contact_path_ms", "<f8"), ("step_ms", "<f8"),
+    ("broadphase_move_ms", "<f8"), ("broadphase_query_ms", "<f8"),
 ])
 
 _bound = False
@@ -39,6 +40,7 @@ def _lib():
             "kzw_get_mass": ([vp, vp, u32], i32), "kzw_get_hull": ([vp, u32, vp, vp, vp, vp], i32),
             "kzw_device": ([vp], vp),
             "kzw_save": ([vp, vp, C.c_uint64, vp], i32), "kzw_load": ([vp, vp, C.c_uint64, vp, i32], i32),
+            "kzw_set_device_broadphase": ([vp, i32], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -50,7 +52,7 @@ def _lib():
 
 KZW_EXPORTS = ["kzw_create", "kzw_destroy", "kzw_last_error", "kzw_add_shapes", "kzw_add_bodies", "kzw_step",
                "kzw_update_broadphase", "kzw_get_states", "kzw_set_states", "kzw_get_pairs", "kzw_get_stats",
-               "kzw_get_mass", "kzw_get_hull", "kzw_device", "kzw_save", "kzw_load"]
+               "kzw_get_mass", "kzw_get_hull", "kzw_device", "kzw_save", "kzw_load", "kzw_set_device_broadphase"]
 
 
 def _p(a):
@@ -127,6 +129,10 @@ class PhysicsWorld:
 
     def step_fixed(self, dt=S.DT):
         self._check(_lib().kzw_step(self.h, float(dt)), "kzw_step")
+
+    def set_device_broadphase(self, on: bool):
+        """step_fixed's pair query on the device (default) or on host threads."""
+        self._check(_lib().kzw_set_device_broadphase(self.h, 1 if on else 0), "kzw_set_device_broadphase")
 
     def update_broadphase(self):
         self._check(_lib().kzw_update_broadphase(self.h), "kzw_update_broadphase")
