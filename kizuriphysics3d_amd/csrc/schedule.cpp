@@ -373,25 +373,31 @@ void ScheduleBuilder::vel_fit(const ScheduleInput& in) {
             }
             return l;
         };
-        // First usable batch of a list for (a, b); drops dead entries.
-        auto first_fit = [&](OpenList& l, uint32_t a, uint32_t b) -> uint32_t {
-            while (l.head < l.ids.size() && batches[l.ids[l.head]].n >= 4) ++l.head;
-            size_t dead = 0;
+        // First usable batch of a list for (a, b); drops dead entries.  An
+        // entry is dead when its batch is full or, in the list kept for a
+        // non-dynamic body h, holds h: every point that searches that list
+        // touches h, so neither batch can take it again.  (Without the
+        // second rule every floor point rescans the batches earlier floor
+        // points joined.)
+        auto first_fit = [&](OpenList& l, uint32_t h, uint32_t a, uint32_t b) -> uint32_t {
+            auto dead = [&](const FitBatch& B) { return B.n >= 4 || (h != kNone && holds(B, h)); };
+            while (l.head < l.ids.size() && dead(batches[l.ids[l.head]])) ++l.head;
+            size_t ndead = 0;
             uint32_t found = kNone;
             for (size_t k = l.head; k < l.ids.size(); ++k) {
                 const FitBatch& B = batches[l.ids[k]];
-                if (B.n >= 4) {
-                    ++dead;
+                if (dead(B)) {
+                    ++ndead;
                     continue;
                 }
                 if (holds_either(B, a, b)) continue;
                 found = l.ids[k];
                 break;
             }
-            if (dead > 64 && dead * 2 > l.ids.size() - l.head) {
+            if (ndead > 64 && ndead * 2 > l.ids.size() - l.head) {
                 size_t w = 0;
                 for (size_t k = l.head; k < l.ids.size(); ++k)
-                    if (batches[l.ids[k]].n < 4) l.ids[w++] = l.ids[k];
+                    if (!dead(batches[l.ids[k]])) l.ids[w++] = l.ids[k];
                 l.ids.resize(w);
                 l.head = 0;
             }
@@ -406,11 +412,12 @@ void ScheduleBuilder::vel_fit(const ScheduleInput& in) {
             if ((ci.w >> 8) & 1u) continue;  // sensor
             const uint32_t a = ci.x, b = ci.y, np = ci.w & 0xFFu;
             OpenList* cand = &open;
-            if (!dyn[a]) cand = &open_without(a);
-            else if (!dyn[b]) cand = &open_without(b);
+            uint32_t h = kNone;
+            if (!dyn[a]) cand = &open_without(h = a);
+            else if (!dyn[b]) cand = &open_without(h = b);
             uint32_t prev = kNone, prev_lane = 0, run = 0;
             for (uint32_t k = 0; k < np; ++k) {
-                const uint32_t placed = first_fit(*cand, a, b);
+                const uint32_t placed = first_fit(*cand, h, a, b);
                 const uint32_t id = placed == kNone ? uint32_t(batches.size()) : placed;
                 const bool join = (prev != kNone) & (id == prev + 1) & (run < cap);
                 run = join ? run + 1 : 1;
