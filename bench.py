@@ -140,6 +140,34 @@ def cpu_baseline(scene, states, budget_s: float, first_step: int):
                       f"{workers} workers + caller"}
 
 
+def learned_report(res, train, state: str) -> dict:
+    """The warm-start comparison as reported in the bench line: per mode the
+    sweep-increment residual curve and the natural-map residual curve (k =
+    1..8), penetration, and how many sweeps each takes to reach the cache
+    warm start's 8-sweep value."""
+    from kizuriphysics3d_amd import learned as Lw
+    cache8 = float(res["cache"]["residual"][-1])
+    nat8 = float(res["cache"]["natural"][-1]["total"]) if "natural" in res["cache"] else None
+    out = {"definition": "increment: RMS over contact points of the impulse increments of sweep k; natural: RMS of "
+                         "the natural-map residual of the iterate after k sweeps (normal complementarity "
+                         "min(lambda_n, m_n (vRel.n - bias)), projected friction and cone terms); N s",
+           "state": state, "points": res["cache"]["points"], "k": list(range(1, len(res["cache"]["residual"]) + 1)),
+           "train": train}
+    for name, r in res.items():
+        out[name] = [float(x) for x in r["residual"]]
+        out[name + "_penetration"] = r["penetration"]
+        out[name + "_sweeps_to_cache8"] = Lw.iterations_to_reach(r["residual"], cache8)
+        line = f"residual {name:13s} increment k=1..8 {np.array2string(r['residual'], precision=3)}"
+        if "natural" in r:
+            curve = np.array([x["total"] for x in r["natural"]])
+            out[name + "_natural"] = curve.tolist()
+            out[name + "_natural_terms"] = r["natural"][-1]
+            out[name + "_natural_sweeps_to_cache8"] = Lw.iterations_to_reach(curve, nat8)
+            line += f"; natural {np.array2string(curve, precision=3)}"
+        log(line)
+    return out
+
+
 def bench_batched(args):
     """4,096 independent worlds, one captured graph per step."""
     import torch
@@ -510,16 +538,7 @@ def main():
     if learn:
         from kizuriphysics3d_amd import learned as Lw
         res = Lw.compare_warm_starts(w, penetration_iterations=(4, 8))
-        cache8 = float(res["cache"]["residual"][-1])
-        residual = {"definition": "RMS over contact points of the impulse increments of sweep k (N s)",
-                    "state": f"after step {untimed + args.steps}", "points": res["cache"]["points"],
-                    "k": list(range(1, len(res["cache"]["residual"]) + 1)), "train": train}
-        for name, r in res.items():
-            residual[name] = [float(x) for x in r["residual"]]
-            residual[name + "_penetration"] = r["penetration"]
-            residual[name + "_sweeps_to_cache8"] = Lw.iterations_to_reach(r["residual"], cache8)
-            log(f"residual {name:13s} k=1..8 {np.array2string(r['residual'], precision=3)} "
-                f"penetration k=4 {r['penetration'][4]}")
+        residual = learned_report(res, train, f"after step {untimed + args.steps}")
     # The world is closed before the CPU baseline and before interpreter
     # teardown: its helper thread is joined and its device memory, events and
     # stream are released while the runtime is whole.

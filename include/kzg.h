@@ -172,6 +172,17 @@ int kzg_download_constraints(kzg_ctx* ctx, uint32_t* hdr, float* pts);
 /* RMS of the impulse increments of each velocity sweep of the last solve
  * (the PGS residual); returns the number of sweeps written. */
 int kzg_get_residuals(kzg_ctx* ctx, double* out, uint32_t cap);
+/* Natural-map residual of the iterate the last kzg_solve left (call before
+ * kzg_integrate_positions), over the non-sensor contact points, with the
+ * bodies' velocities and the anchors the velocity sweep uses (rA = qA
+ * localAnchorA).  Per point: normal complementarity min(lambda_n, m_n
+ * (vRel . n - bias)); friction lambda_t - clamp(lambda_t - m_t (vRel . t),
+ * +-mu lambda_n) for t1 and t2; cone max(0, |lambda_t| - mu lambda_n) (a part
+ * of the friction term).  bias = restitution bias (+ position bias with
+ * velocity bias).  out[4] = RMS of normal + friction, of normal, of
+ * friction, of cone, in N s.  No reference counterpart (the reference
+ * computes no residual). */
+int kzg_get_natural_residual(kzg_ctx* ctx, double* out);
 int kzg_get_timings(kzg_ctx* ctx, kzg_timings* out);
 
 /* The warm-start cache (the contact cache of PhysicsWorld.cpp:405-452),

@@ -63,6 +63,14 @@ hipError_t resolve_scratch(int device, unsigned char** out) {
     return hipSuccess;
 }
 
+__global__ void k_fill_zero(double* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+void launch_fill_zero(hipStream_t s, double* p, uint32_t n) {
+    if (n) k_fill_zero<<<(n + 255) / 256, 256, 0, s>>>(p, n);
+}
+
 // Clears the work-list counters and the error flag and stores the pair count
 // (a kernel, not a memset: a caller may capture the launches in a graph).
 __global__ void k_scratch_reset(uint32_t* counters, uint32_t n) {
@@ -257,6 +265,7 @@ void kzg_destroy(kzg_ctx* c) {
     c->cache_key.release();
     c->cache_t.release();
     c->resid.release();
+    c->natural.release();
     c->b_packed_solver.release();
     for (auto* d : {&c->mlp_master, &c->mlp_m, &c->mlp_v, &c->mlp_norm, &c->mlp_partial, &c->mlp_out, &c->mlp_loss})
         d->release();
@@ -913,6 +922,27 @@ int kzg_get_residuals(kzg_ctx* c, double* out, uint32_t cap) {
     for (uint32_t i = 0; i < n && i < cap; ++i)
         out[i] = c->resid_points ? std::sqrt(double(r[i]) / double(c->resid_points)) : 0.0;
     return int(std::min(n, cap));
+}
+
+int kzg_get_natural_residual(kzg_ctx* c, double* out) {
+    if (!c || !out) return KZG_ERR_ARGUMENT;
+    if (c->phase != 3)
+        return fail(c, KZG_ERR_STATE, "kzg_get_natural_residual needs a kzg_solve and no kzg_integrate_positions since");
+    hipSetDevice(c->device);
+    KZG_HIP(c, c->natural.ensure(4));
+    launch_fill_zero(c->stream, c->natural.p, 4);
+    launch_natural_residual(c->stream, c->contacts(), c->bodies(), c->npoints, c->world.use_velocity_bias ? 1 : 0,
+                            c->natural.p);
+    KZG_HIP(c, hipGetLastError());
+    double acc[4];
+    int rc = download(c, acc, c->natural, 4);
+    if (rc) return rc;
+    const double n = acc[3] > 0.0 ? acc[3] : 1.0;
+    out[0] = std::sqrt((acc[0] + acc[1]) / n);
+    out[1] = std::sqrt(acc[0] / n);
+    out[2] = std::sqrt(acc[1] / n);
+    out[3] = std::sqrt(acc[2] / n);
+    return KZG_OK;
 }
 
 int kzg_get_timings(kzg_ctx* c, kzg_timings* out) {

@@ -64,6 +64,10 @@ void launch_islands(hipStream_t s, const ContactBuf& c, uint32_t cap, const Body
                     uint32_t lds_limit = 0);
 void launch_cache_store(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const CacheBuf& cache, uint32_t nc);
 void launch_cache_rehash(hipStream_t s, const CacheBuf& src, const CacheBuf& dst);
+// acc[4] (zeroed by the caller): squared sums of the normal, friction and
+// cone terms of the natural-map residual, and the point count.
+void launch_natural_residual(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32_t np, int use_vbias,
+                             double* acc);
 void launch_finish(hipStream_t s, const BodyBuf& b, uint32_t n, float dt, int split);
 void launch_world_inertia(hipStream_t s, const BodyBuf& b, uint32_t n);
 }  // namespace kz
@@ -204,6 +208,7 @@ struct kzg_ctx {
 
     // residuals
     DevVec<float> resid;
+    DevVec<double> natural;  // natural-map residual sums
     uint32_t resid_points = 0;
 
     // staging and timing

@@ -962,6 +962,68 @@ __global__ void k_sync_reset(uint32_t* sync, uint32_t all) {
     if (i < kSyncWords && (all || i != kSyncGiveUp)) sync[i] = i == kSyncTarget ? 0xFFFFFFFFu : 0u;
 }
 
+// ---------------------------------------------------------------------------
+// Natural-map residual of the current iterate (diagnostic; the reference
+// computes none).  Per non-sensor point, with the bodies' velocities as the
+// solve left them and the anchors as the velocity sweep forms them (rA =
+// qA localAnchorA):
+//   normal:   min(lambda_n, m_n (vRel . n - bias))                 (N s)
+//   friction: lambda_t - clamp(lambda_t - m_t (vRel . t), +-mu lambda_n),
+//             for t1 and t2 (what one more projected step would change)
+//   cone:     max(0, |lambda_t| - mu lambda_n), for t1 and t2 (inside the
+//             friction term, reported apart)
+// bias = the restitution bias, plus the position bias with velocity bias.
+// acc (FP64): squared sums of the three terms and the point count.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_natural_residual(ContactBuf c, BodyBuf b, uint32_t np, int use_vbias,
+                                                          double* acc) {
+    __shared__ double red[4][256];
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    double rn = 0.0, rt = 0.0, rc = 0.0, cnt = 0.0;
+    if (p < np) {
+        const uint4 bd = c.body[p];
+        if (!(bd.z & 1u)) {
+            const f4q qa = ldq(b.q[bd.x]), qb = ldq(b.q[bd.y]);
+            const f3 ra = rotate(qa, ld3(c.la[p])), rb = rotate(qb, ld3(c.lb[p]));
+            const f3 va = ld3(b.v[bd.x]), wa = ld3(b.w[bd.x]), vb = ld3(b.v[bd.y]), wb = ld3(b.w[bd.y]);
+            const f3 vr = sub(add(va, cross(wa, ra)), add(vb, cross(wb, rb)));
+            const float4 n4 = c.n[p], t14 = c.t1[p], t24 = c.t2[p], imp = c.imp[p];
+            const float mu = c.la[p].w;
+            const float bias = c.ra[p].w + (use_vbias ? c.rb[p].w : 0.0f);
+            const double ln = imp.x, lim = double(mu) * ln;
+            const double w = double(dot(vr, ld3(n4))) - bias;
+            const double r0 = fmin(ln, double(n4.w) * w);
+            rn = r0 * r0;
+            const double lt[2] = {imp.y, imp.z};
+            const float4 tt[2] = {t14, t24};
+            for (int k = 0; k < 2; ++k) {
+                const double step = lt[k] - double(tt[k].w) * double(dot(vr, ld3(tt[k])));
+                const double proj = fmin(fmax(step, -lim), lim);
+                const double r = lt[k] - proj;
+                const double over = fmax(0.0, fabs(lt[k]) - lim);
+                rt += r * r;
+                rc += over * over;
+            }
+            cnt = 1.0;
+        }
+    }
+    red[0][threadIdx.x] = rn;
+    red[1][threadIdx.x] = rt;
+    red[2][threadIdx.x] = rc;
+    red[3][threadIdx.x] = cnt;
+    __syncthreads();
+    for (uint32_t off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) atomicAdd(acc + threadIdx.x, red[threadIdx.x][0]);
+}
+void launch_natural_residual(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32_t np, int use_vbias,
+                             double* acc) {
+    if (np) k_natural_residual<<<(np + 255) / 256, 256, 0, s>>>(c, b, np, use_vbias, acc);
+}
+
 // Unit descriptors from the unit bounds and the records' body entries.
 __global__ void k_unit_desc(const uint32_t* unit, const uint4* bd, uint32_t nu, uint4* desc) {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;

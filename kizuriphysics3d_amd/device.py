@@ -39,7 +39,7 @@ EXPORTS = [
     "kzg_create", "kzg_destroy", "kzg_last_error", "kzg_upload_shapes", "kzg_upload_bodies", "kzg_upload_states",
     "kzg_narrowphase", "kzg_broadphase_pairs", "kzg_narrowphase_resident", "kzg_download_pairs", "kzg_build_constraints", "kzg_integrate_velocities", "kzg_solve", "kzg_integrate_positions",
     "kzg_step", "kzg_query_sizes", "kzg_download_states", "kzg_download_manifolds", "kzg_download_constraints",
-    "kzg_get_residuals", "kzg_get_timings", "kzg_collide_batch", "kzg_collide_batch_device", "kzg_schedule", "kzg_set_warm_start",
+    "kzg_get_residuals", "kzg_get_natural_residual", "kzg_get_timings", "kzg_collide_batch", "kzg_collide_batch_device", "kzg_schedule", "kzg_set_warm_start",
     "kzg_set_iterations", "kzg_set_cooperative", "kzg_set_sweep_units", "kzg_set_early_warm_start", "kzg_set_narrowphase_waves", "kzg_set_island_lds_limit", "kzg_set_cooperative_placement", "kzg_set_spin_limit", "kzg_cache_info", "kzg_download_cache", "kzg_upload_cache",
     "kzg_mlp_init", "kzg_mlp_set_params", "kzg_mlp_get_params", "kzg_mlp_set_normalization",
     "kzg_mlp_get_normalization", "kzg_mlp_collect", "kzg_mlp_num_samples", "kzg_mlp_get_samples",
@@ -80,6 +80,7 @@ def library():
             "kzg_query_sizes": ([vp, vp, vp, vp], i32), "kzg_download_states": ([vp, vp, u32], i32),
             "kzg_download_manifolds": ([vp, vp, vp], i32), "kzg_download_constraints": ([vp, vp, vp], i32),
             "kzg_get_residuals": ([vp, vp, u32], i32), "kzg_get_timings": ([vp, vp], i32),
+            "kzg_get_natural_residual": ([vp, vp], i32),
             "kzg_collide_batch": ([vp, vp, vp, vp, vp, u32, f32, vp, vp, vp], i32),
             "kzg_collide_batch_device": ([vp, vp, vp, vp, vp, u32, f32, vp, vp], i32),
             "kzg_schedule": ([u32, vp, u32, vp, u32, u32, vp, vp, vp, vp, vp, vp], i32),
@@ -279,6 +280,14 @@ class DeviceContext:
         out = np.zeros(64, np.float64)
         k = self._check(library().kzg_get_residuals(self.h, _p(out), 64), "kzg_get_residuals")
         return out[:k].copy()
+
+    def natural_residual(self) -> dict:
+        """Natural-map residual of the iterate the last solve left (before
+        integrate_positions): RMS of normal + friction, normal, friction and
+        cone terms (N s)."""
+        out = np.zeros(4, np.float64)
+        self._check(library().kzg_get_natural_residual(self.h, _p(out)), "kzg_get_natural_residual")
+        return {"total": float(out[0]), "normal": float(out[1]), "friction": float(out[2]), "cone": float(out[3])}
 
     def timings(self) -> dict:
         t = np.zeros((), TIMINGS_DTYPE)

@@ -6,9 +6,11 @@ its manifold cache (src/World/PhysicsWorld.cpp:492-530) or, with KZ_NOWS,
 with zeros (:493).  The network (kzg_mlp_* in include/kzg.h) predicts them
 from contact-frame features instead, for every point (WARM_LEARNED) or only
 where the cache has no match (WARM_LEARNED_MISS).  The residual after k
-velocity iterations is the RMS over contact points of the impulse increments
-of sweep k (kzg_get_residuals), so one 8-iteration solve gives the curve for
-k = 1..8.
+velocity iterations is measured two ways: the RMS over contact points of the
+impulse increments of sweep k (kzg_get_residuals: one 8-iteration solve gives
+the curve for k = 1..8), and the natural-map residual of the iterate after k
+sweeps (kzg_get_natural_residual: normal complementarity, projected friction
+and cone terms, one solve per k).
 """
 from __future__ import annotations
 
@@ -40,13 +42,16 @@ def penetration(ctx: D.DeviceContext, pairs: np.ndarray) -> dict:
 
 def compare_warm_starts(world, modes=(D.WARM_COLD, D.WARM_CACHE, D.WARM_LEARNED, D.WARM_LEARNED_MISS),
                         iterations: int = 8, position_iterations: int = 2, penetration_iterations=(),
-                        dt: float = S.DT) -> dict:
+                        natural: bool = True, dt: float = S.DT) -> dict:
     """From the world's current state, one contact-path step per warm-start
     mode (the cache is left as it is), each restarting from the same body
-    states.  Returns per mode the residual after k = 1..iterations sweeps
-    ("residual") and, for each k in penetration_iterations, the penetration
-    statistics after a step solved with k velocity iterations (measured over
-    the step's broadphase pairs).  The world's state is restored afterwards."""
+    states.  Returns per mode the sweep-increment residual after k =
+    1..iterations sweeps ("residual"), with natural=True the natural-map
+    residual of the iterate after k sweeps ("natural": a solve with k
+    velocity iterations each, kzg_get_natural_residual), and, for each k in
+    penetration_iterations, the penetration statistics after a step solved
+    with k velocity iterations (measured over the step's broadphase pairs).
+    The world's state is restored afterwards."""
     ctx = world.device_context()
     snap = world.states()
     world.update_broadphase()
@@ -63,6 +68,17 @@ def compare_warm_starts(world, modes=(D.WARM_COLD, D.WARM_CACHE, D.WARM_LEARNED,
             ctx.solve(dt)
             res["residual"] = ctx.residuals()[:iterations]
             res["points"] = ctx.sizes()[2]
+            if natural:
+                nat = []
+                for k in range(1, iterations + 1):
+                    ctx.set_iterations(k, position_iterations)
+                    ctx.upload_states(snap)
+                    ctx.narrowphase(pairs)
+                    ctx.build_constraints(mode | D.WARM_NO_STORE)
+                    ctx.integrate_velocities(dt)
+                    ctx.solve(dt)
+                    nat.append(ctx.natural_residual())
+                res["natural"] = nat
             pen = {}
             for k in penetration_iterations:
                 ctx.set_iterations(k, position_iterations)

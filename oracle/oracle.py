@@ -172,3 +172,50 @@ def collide_batch(pb: S.PairBatch, derived: Derived, max_separation=0.02, precis
                             _p(derived.hull_tris), _p(sa), _p(sb), _p(xa), _p(xb), n, float(max_separation),
                             _p(counts), _p(pts), threads)
     return counts, pts
+
+
+def _qrot(q: np.ndarray, v: np.ndarray) -> np.ndarray:
+    """Rotate rows of v by quaternions q (xyzw), float64: v + 2w(u x v) + 2u x (u x v)."""
+    u, w = q[:, :3], q[:, 3:4]
+    t = 2.0 * np.cross(u, v)
+    return v + w * t + np.cross(u, t)
+
+
+def natural_residual(hdr: np.ndarray, pts: np.ndarray, states: np.ndarray, velocity_bias: bool = False) -> dict:
+    """Natural-map residual of a solved iterate, in float64 (the device's
+    kzg_get_natural_residual restated).  hdr / pts: a constraint dump
+    (body A, body B, points, flags; 44 floats per point); states: the bodies
+    after the solve (velocities) with the rotations the solve used.  Per
+    non-sensor point, rA = qA localAnchorA, vRel = (vA + wA x rA) - (vB + wB x rB):
+      normal   min(lambda_n, m_n (vRel . n - bias))
+      friction lambda_t - clamp(lambda_t - m_t (vRel . t), +-mu lambda_n) for t1, t2
+      cone     max(0, |lambda_t| - mu lambda_n) for t1, t2
+    bias = restitution bias (+ position bias with velocity bias).  Returns
+    the RMS values (N s) of normal + friction, normal, friction and cone."""
+    k = hdr[:, 2].astype(np.int64)
+    live = (np.arange(8)[None, :] < k[:, None]) & ((hdr[:, 3] & 1) == 0)[:, None]
+    ci, pi = np.nonzero(live)
+    P = pts[ci, pi].astype(np.float64)
+    a, b = hdr[ci, 0], hdr[ci, 1]
+    q = lambda i: np.stack([states["rotation"][i][:, j] for j in range(4)], 1).astype(np.float64)
+    ra = _qrot(q(a), P[:, 12:15])
+    rb = _qrot(q(b), P[:, 15:18])
+    va, wa = states["linear_velocity"][a].astype(np.float64), states["angular_velocity"][a].astype(np.float64)
+    vb, wb = states["linear_velocity"][b].astype(np.float64), states["angular_velocity"][b].astype(np.float64)
+    vr = (va + np.cross(wa, ra)) - (vb + np.cross(wb, rb))
+    n, t1, t2 = P[:, 18:21], P[:, 21:24], P[:, 24:27]
+    mn, mt = P[:, 27], (P[:, 28], P[:, 29])
+    ln, lt = P[:, 30], (P[:, 31], P[:, 32])
+    bias = P[:, 34] + (P[:, 35] if velocity_bias else 0.0)
+    mu = P[:, 37]
+    lim = mu * ln
+    rn = np.minimum(ln, mn * ((vr * n).sum(1) - bias))
+    rt = np.zeros_like(rn)
+    rc = np.zeros_like(rn)
+    for tk, mk, lk in ((t1, mt[0], lt[0]), (t2, mt[1], lt[1])):
+        step = lk - mk * (vr * tk).sum(1)
+        rt += (lk - np.clip(step, -lim, lim)) ** 2
+        rc += np.maximum(0.0, np.abs(lk) - lim) ** 2
+    cnt = max(len(rn), 1)
+    return {"total": float(np.sqrt(((rn ** 2).sum() + rt.sum()) / cnt)), "normal": float(np.sqrt((rn ** 2).sum() / cnt)),
+            "friction": float(np.sqrt(rt.sum() / cnt)), "cone": float(np.sqrt(rc.sum() / cnt))}

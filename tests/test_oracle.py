@@ -152,6 +152,39 @@ def test_plane_pair_closed_forms():
         assert abs(p[0][k]["separation"] - (v[i][1] + np.float32(0.24))) < 1e-6
 
 
+def test_natural_residual_closed_form_sphere_on_plane():
+    """A 2 kg sphere landing on the plane y = 0 at 1 m/s, restitution 0:
+    before any velocity sweep (0 iterations, cold start) the normal term is
+    min(0, m (vRel . n)) = -m |v| with |v| = 1 + g dt and n = +y (plane B to
+    sphere A), the friction and cone terms 0; one sweep solves the single
+    point exactly, so the residual drops to rounding."""
+    out = []
+    for iters in (0, 1):
+        sc = S.Scene("drop", S.world_settings(iters, 0))
+        plane = sc.add_shape(S.SHAPE_PLANE, (0.0, 1.0, 0.0, 0.0))
+        sc.add_body(S.default_body(plane, S.MOTION_STATIC))
+        ball = sc.add_shape(S.SHAPE_SPHERE, (0.5,))
+        bd = S.default_body(ball)
+        bd["position"] = (0.0, 0.499, 0.0)
+        bd["linear_velocity"] = (0.0, -1.0, 0.0)
+        bd["linear_damping"] = 0.0
+        bd["angular_damping"] = 0.0
+        sc.add_body(bd)
+        # inverse mass 1/2; inverse inertia 1 / (2/5 m r^2) = 5 on each axis
+        mass = np.zeros((2, 10), np.float32)
+        mass[1, 0] = 0.5
+        mass[1, 1] = mass[1, 5] = mass[1, 9] = 5.0
+        ow = O.OracleWorld(sc, O.Derived.pack(sc.shape_array(), {}, mass))
+        ow.step(np.array([[0, 1]], np.uint32), stop_after=4)
+        hdr, pts = ow.constraints()
+        assert len(hdr) == 1 and hdr[0, 2] == 1
+        out.append(O.natural_residual(hdr, pts, ow.states()))
+    v = 1.0 + float(np.float32(9.81) * np.float32(S.DT))
+    assert abs(out[0]["normal"] - 2.0 * v) < 1e-4, out[0]
+    assert out[0]["friction"] < 1e-9 and out[0]["cone"] < 1e-9
+    assert out[1]["total"] < 1e-5, out[1]
+
+
 def test_fp64_rerun_one_step_error():
     """FP32 against the FP64 re-run, one step at a time from the same state.
 
