@@ -57,6 +57,7 @@ typedef struct kzg_body {
     float inverse_inertia_local[9]; /* column-major */
     float friction, restitution, linear_damping, angular_damping;
     float gravity_factor, max_linear_velocity, max_angular_velocity;
+    uint32_t allow_sleeping;     /* BodySettings::allowSleeping */
 } kzg_body;
 
 /* Built convex hulls: per shape the range of its vertices and triangles in
@@ -157,6 +158,40 @@ int kzg_integrate_velocities(kzg_ctx* ctx, float dt);
 /* Step 6. */
 int kzg_solve(kzg_ctx* ctx, float dt);
 int kzg_integrate_positions(kzg_ctx* ctx, float dt);
+
+/* Step 8: PhysicsWorld::UpdateSleeping (PhysicsWorld.cpp:866-951), after
+ * kzg_integrate_positions.  Sleep timers of the active bodies, union-find
+ * over the last step's contact constraints with the island indices the
+ * reference keeps on its bodies (active list positions; a sleeping body keeps
+ * the index it had when it fell asleep), then every dynamic body of an
+ * island whose smallest timer reached the threshold has its velocities
+ * zeroed and leaves the active list (BodyManager::DeactivateBody,
+ * src/Body/BodyManager.cpp:148-160: swap-remove), in the reference's loop
+ * order.  A no-op in a world without sleeping (allow_sleeping = 0).
+ *
+ * With sleeping the context keeps the active list (BodyManager.h:63, 79):
+ * kzg_narrowphase wakes sleeping bodies that a moving body touches
+ * (BuildContactConstraints, PhysicsWorld.cpp:473-490: pushed onto the list in
+ * constraint order), the integrations skip sleeping bodies (:760, :787,
+ * :857), and the islands follow BuildIslands (:587-664) over the active list:
+ * compact indices for its dynamic bodies, while a sleeping body's contacts
+ * are routed through the stale index it keeps, as in the reference. */
+int kzg_update_sleeping(kzg_ctx* ctx, float dt);
+/* The active body list in order, as body slots: *count receives its
+ * length; at most cap entries are written. */
+int kzg_get_active_bodies(kzg_ctx* ctx, uint32_t* out, uint32_t cap, uint32_t* count);
+/* Replace the active list: the listed bodies are active in this order, every
+ * other body sleeps with zero velocities; sleep timers are zero and island
+ * indices unset.  That is the state that creating the bodies in slot order
+ * and deactivating the unlisted ones leaves (BodyManager::CreateBody /
+ * DeactivateBody, BodyManager.cpp:53-59, 148-160; the snapshot loader,
+ * src/Serialization/Serializer.cpp:230-234), when the list is in slot order.
+ * kzg_upload_bodies resets the list to every body in slot order. */
+int kzg_set_active_bodies(kzg_ctx* ctx, const uint32_t* list, uint32_t count);
+/* WorldSettings::sleepTimeThreshold / sleepLinearThreshold /
+ * sleepAngularThreshold (include/Kizuri/World/PhysicsWorld.h:25-27);
+ * defaults 0.5 s, 0.05 m/s, 0.1 rad/s. */
+int kzg_set_sleep_thresholds(kzg_ctx* ctx, float time, float linear, float angular);
 
 /* All of the above in StepFixed order. */
 int kzg_step(kzg_ctx* ctx, const uint32_t* pairs, uint32_t num_pairs, float dt);

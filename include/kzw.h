@@ -38,6 +38,7 @@ typedef struct kzw_stats {
     /* Inside broadphase_ms: the proxy moves (tree upkeep), and the pair
      * query (host walk, or tree image + device query with its transfers). */
     double broadphase_move_ms, broadphase_query_ms;
+    uint32_t num_active_bodies;      /* WorldStats::numActiveBodies */
 } kzw_stats;
 
 int kzw_create(kzw_world** out, const kz_world_desc* world, int device);

@@ -5,6 +5,11 @@
 //
 // Three launches over tiles of 1024 pairs: per-tile counts, a scan of the
 // tile counts (one workgroup), and the emission with a scan inside the tile.
+//
+// Also the islands of that table (BuildIslands, :587-664), and, for worlds
+// with sleeping, the active list's device side: the wake-ups of the build
+// (:473-490), the island indices over the list, and UpdateSleeping's timers,
+// islands and decisions (:866-951).
 #include <hip/hip_runtime.h>
 
 #include "kz_device.h"
@@ -256,7 +261,342 @@ __global__ void k_con_island(ContactBuf c, const uint32_t* totals, BodyBuf b, co
     con_island[ci] = island[parent[x]];
 }
 
+// ---- Islands over the active list ------------------------------------------
+// BuildIslands as the reference runs it over its active list: body x's
+// union-find element is isl[x] (the island index the reference keeps on the
+// body: compact index of an active dynamic body, the stale value of a
+// sleeping one), valid below nd = sleep[kSlDynamic].  Unions join dynamic
+// pairs whose two indices are valid; a constraint goes to the island of A
+// when A is dynamic with a valid index, else to that of B, else to none
+// (PhysicsWorld.cpp:612-654).  Roots are the smallest elements, so islands
+// are numbered by their smallest element as the first-appearance scan of
+// the reference numbers them.
+enum : uint32_t { kSlActive = 0, kSlDynamic = 1, kSlWoken = 2, kSlGiveUp = 3 };
+constexpr uint32_t kNoIndex = 0xFFFFFFFFu;
+
+__global__ void k_uf_union_indexed(ContactBuf c, const uint32_t* totals, BodyBuf b, const uint32_t* isl,
+                                   const uint32_t* sleep, uint32_t* parent) {
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= totals[0]) return;
+    const uint4 in = c.info[ci];
+    if (!dyn_body(b, in.x) || !dyn_body(b, in.y)) return;
+    const uint32_t nd = sleep[kSlDynamic];
+    uint32_t x = isl[in.x], y = isl[in.y];
+    if (x >= nd || y >= nd) return;
+    for (;;) {
+        x = uf_find_halve(parent, x);
+        y = uf_find_halve(parent, y);
+        if (x == y) return;
+        const uint32_t hi = x > y ? x : y, lo = x > y ? y : x;
+        if (atomicCAS(parent + hi, hi, lo) == hi) return;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_uf_roots_indexed(uint32_t* parent, const uint32_t* sleep, uint2* tile) {
+    __shared__ uint32_t sh[32];
+    const uint32_t i = blockIdx.x * kTile + threadIdx.x, nd = sleep[kSlDynamic];
+    uint32_t root = kNoIndex;
+    if (i < nd) {
+        root = uf_find(parent, i);
+        parent[i] = root;
+    }
+    uint2 tot;
+    tile_scan(root == i ? 1u : 0u, 0u, tot, sh);
+    if (threadIdx.x == 0) tile[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024) k_island_ids_indexed(const uint32_t* parent, const uint32_t* sleep,
+                                                             const uint2* tile, uint32_t* island) {
+    __shared__ uint32_t sh[32];
+    const uint32_t i = blockIdx.x * kTile + threadIdx.x;
+    const uint32_t f = (i < sleep[kSlDynamic] && parent[i] == i) ? 1u : 0u;
+    uint2 tot;
+    const uint2 e = tile_scan(f, 0u, tot, sh);
+    if (f) island[i] = tile[blockIdx.x].x + e.x;
+}
+
+__global__ void k_con_island_indexed(ContactBuf c, const uint32_t* totals, BodyBuf b, const uint32_t* isl,
+                                     const uint32_t* sleep, const uint32_t* parent, const uint32_t* island,
+                                     uint32_t* con_island) {
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= totals[0]) return;
+    const uint4 in = c.info[ci];
+    const uint32_t nd = sleep[kSlDynamic];
+    auto island_of = [&](uint32_t x) {
+        if (!dyn_body(b, x)) return kNoIndex;
+        const uint32_t e = isl[x];
+        return e < nd ? island[parent[e]] : kNoIndex;
+    };
+    uint32_t is = island_of(in.x);
+    if (is == kNoIndex) is = island_of(in.y);
+    con_island[ci] = is;
+}
+
+// ---- Activation and sleeping ------------------------------------------------
+// Exclusive count of `f` over one workgroup of 1024 threads (16 waves);
+// total receives the workgroup's count.  sh holds 16 words.
+__device__ __forceinline__ uint32_t block_rank(bool f, uint32_t& total, uint32_t* sh) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t m = __ballot(f);
+    const uint32_t below = uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+    if (lane == 0) sh[wave] = uint32_t(__popcll(m));
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+    for (uint32_t w = 0; w < blockDim.x / 64; ++w) {
+        const uint32_t v = sh[w];
+        base += w < wave ? v : 0u;
+        tot += v;
+    }
+    __syncthreads();
+    total = tot;
+    return base + below;
+}
+__device__ __forceinline__ bool sleeper(uint32_t m) { return meta_motion(m) == 2u && meta_asleep(m); }
+
+// BuildIslands' compact indices (PhysicsWorld.cpp:591-604): walking the
+// active list, a dynamic body gets the next index, any other body none.
+// One workgroup; sleep[kSlDynamic] receives the count.
+__global__ void __launch_bounds__(1024) k_island_index(const uint32_t* act, uint32_t* sleep, BodyBuf b,
+                                                       uint32_t* isl) {
+    __shared__ uint32_t sh[16];
+    const uint32_t n = sleep[kSlActive];
+    uint32_t base = 0;
+    for (uint32_t i0 = 0; i0 < n; i0 += blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        const uint32_t x = i < n ? act[i] : 0u;
+        const bool f = i < n && dyn_body(b, x);
+        uint32_t tot;
+        const uint32_t r = block_rank(f, tot, sh);
+        if (i < n) isl[x] = f ? base + r : kNoIndex;
+        base += tot;
+    }
+    if (threadIdx.x == 0) sleep[kSlDynamic] = base;
+}
+
+// The wake-ups of BuildContactConstraints (PhysicsWorld.cpp:473-490): walking
+// the constraints in order, a sleeping dynamic body whose partner is moving
+// (an active dynamic body, or a kinematic one with velocity) is activated,
+// and counts as moving for the constraints after it.  As a fixed point: the
+// step t(x) at which x moves is 0 for bodies moving from the start, and for
+// a sleeping body the first constraint k (as k + 1) whose partner y has
+// t(y) < k + 1.  Relaxing t(x) = min(t(x), k + 1) over all constraints until
+// nothing changes never goes below the walk's value (a lowered value has a
+// partner moving earlier, which the walk would have seen) and reaches it, so
+// the order of the relaxation does not matter.
+__global__ void k_wake_init(BodyBuf b, uint32_t nb, uint32_t* t, uint32_t* sleep) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) sleep[kSlGiveUp] = 0u;
+    if (i >= nb) return;
+    const uint32_t m = b.meta[i], mo = meta_motion(m);
+    uint32_t v = kNoIndex;
+    if (mo == 2u) {
+        v = meta_asleep(m) ? kNoIndex : 0u;
+    } else if (mo == 1u) {
+        const float4 lv = b.v[i], av = b.w[i];
+        if (lv.x * lv.x + lv.y * lv.y + lv.z * lv.z > 0.0f || av.x * av.x + av.y * av.y + av.z * av.z > 0.0f) v = 0u;
+    }
+    t[i] = v;
+}
+
+__device__ __forceinline__ uint32_t load_agent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One workgroup relaxes until a round changes nothing; after `limit` rounds
+// that still change something it sets the give-up flag and stops.
+__global__ void __launch_bounds__(1024) k_wake_relax(ContactBuf c, const uint32_t* totals, BodyBuf b, uint32_t* t,
+                                                     uint32_t* sleep, uint32_t limit) {
+    __shared__ uint32_t changed;
+    const uint32_t nc = totals[0];
+    for (uint32_t round = 0;; ++round) {
+        if (threadIdx.x == 0) changed = 0u;
+        __syncthreads();
+        for (uint32_t ci = threadIdx.x; ci < nc; ci += blockDim.x) {
+            const uint4 in = c.info[ci];
+            const bool sa = sleeper(b.meta[in.x]), sb = sleeper(b.meta[in.y]);
+            if (!sa && !sb) continue;
+            const uint32_t k1 = ci + 1u;
+            if (sa && load_agent(t + in.y) < k1 && atomicMin(t + in.x, k1) > k1) atomicOr(&changed, 1u);
+            if (sb && load_agent(t + in.x) < k1 && atomicMin(t + in.y, k1) > k1) atomicOr(&changed, 1u);
+        }
+        __syncthreads();
+        const bool more = changed != 0u;
+        __syncthreads();
+        if (!more) return;
+        if (round + 1u >= limit) {
+            if (threadIdx.x == 0) sleep[kSlGiveUp] = 1u;
+            return;
+        }
+    }
+}
+
+// The woken bodies onto the active list in constraint order (one constraint
+// wakes at most one body: a body woken by its partner leaves the partner
+// moving, so not asleep), then activated: asleep flag cleared, sleep timer 0
+// (BodyManager::ActivateBody, src/Body/BodyManager.cpp:138-146).
+__global__ void __launch_bounds__(1024) k_wake_append(ContactBuf c, const uint32_t* totals, BodyBuf b,
+                                                      const uint32_t* t, uint32_t* act, uint32_t* sleep,
+                                                      float* timer) {
+    __shared__ uint32_t sh[16];
+    const uint32_t nc = totals[0], n0 = sleep[kSlActive];
+    uint32_t base = 0;
+    for (uint32_t c0 = 0; c0 < nc; c0 += blockDim.x) {
+        const uint32_t ci = c0 + threadIdx.x;
+        uint32_t w = kNoIndex;
+        if (ci < nc) {
+            const uint4 in = c.info[ci];
+            if (sleeper(b.meta[in.x]) && t[in.x] == ci + 1u) w = in.x;
+            else if (sleeper(b.meta[in.y]) && t[in.y] == ci + 1u) w = in.y;
+        }
+        uint32_t tot;
+        const uint32_t r = block_rank(w != kNoIndex, tot, sh);
+        if (w != kNoIndex) act[n0 + base + r] = w;
+        base += tot;
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < base; k += blockDim.x) {
+        const uint32_t w = act[n0 + k];
+        b.meta[w] &= ~kMetaAsleep;
+        timer[w] = 0.0f;
+    }
+    if (threadIdx.x == 0) {
+        sleep[kSlActive] = n0 + base;
+        sleep[kSlWoken] = base;
+    }
+}
+
+// UpdateSleeping (PhysicsWorld.cpp:866-951) over the n bodies of the active
+// list.  Timers (:874-889) and the island indices set to list positions
+// (:895-898), union-find elements and island minima reset.
+__global__ void k_sleep_timers(const uint32_t* act, uint32_t n, BodyBuf b, float* timer, uint32_t* isl,
+                               uint32_t* parent, uint32_t* tmin, float dt, float lin2, float ang2) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t x = act[i], m = b.meta[x];
+    float tm = 0.0f;
+    if (meta_motion(m) == 2u && meta_may_sleep(m)) {
+        const float4 lv = b.v[x], av = b.w[x];
+        const float ls = lv.x * lv.x + lv.y * lv.y + lv.z * lv.z, as = av.x * av.x + av.y * av.y + av.z * av.z;
+        if (ls < lin2 && as < ang2) tm = timer[x] + dt;
+    }
+    timer[x] = tm;
+    isl[x] = i;
+    parent[i] = i;
+    tmin[i] = __float_as_uint(1.0e18f);  // math::kBigNumber
+}
+// Unions over the step's contact constraints with both bodies dynamic and
+// both indices below n (:900-908); sleeping bodies keep stale indices.
+__global__ void k_sleep_union(ContactBuf c, const uint32_t* totals, BodyBuf b, const uint32_t* isl, uint32_t n,
+                              uint32_t* parent) {
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= totals[0]) return;
+    const uint4 in = c.info[ci];
+    if (!dyn_body(b, in.x) || !dyn_body(b, in.y)) return;
+    uint32_t x = isl[in.x], y = isl[in.y];
+    if (x >= n || y >= n) return;
+    for (;;) {
+        x = uf_find_halve(parent, x);
+        y = uf_find_halve(parent, y);
+        if (x == y) return;
+        const uint32_t hi = x > y ? x : y, lo = x > y ? y : x;
+        if (atomicCAS(parent + hi, hi, lo) == hi) return;
+    }
+}
+// Smallest timer of each island (:921-930); timers are non-negative, so
+// their bit patterns order as the values do.
+__global__ void k_sleep_min(const uint32_t* act, uint32_t n, BodyBuf b, const float* timer, const uint32_t* parent,
+                            uint32_t* tmin) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t x = act[i], m = b.meta[x];
+    if (meta_motion(m) != 2u || !meta_may_sleep(m)) return;
+    atomicMin(tmin + uf_find(parent, i), __float_as_uint(timer[x]));
+}
+// The decisions (:932-941): position i of the list as the loop starts holds
+// the body the reference's loop reads there (its swap-removes only write
+// positions it has passed), so q[i] = that body falls asleep.  Its
+// velocities are zeroed and it is flagged asleep here; the host replays the
+// swap-removes on its copy of the list.
+__global__ void k_sleep_decide(const uint32_t* act, uint32_t n, BodyBuf b, const uint32_t* parent,
+                               const uint32_t* tmin, float thr, uint8_t* q) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t x = act[i], m = b.meta[x];
+    const bool d = meta_motion(m) == 2u && meta_may_sleep(m) && __uint_as_float(tmin[uf_find(parent, i)]) >= thr;
+    q[i] = d ? 1u : 0u;
+    if (d) {
+        b.meta[x] = m | kMetaAsleep;
+        b.v[x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        b.w[x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+// A new active list (kzg_set_active_bodies): flags from `active` (1 per
+// body), sleeping bodies' velocities zeroed, timers 0, island indices unset.
+__global__ void k_set_activity(BodyBuf b, uint32_t nb, const uint8_t* active, float* timer, uint32_t* isl) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    const uint32_t m = b.meta[i];
+    if (active[i]) {
+        b.meta[i] = m & ~kMetaAsleep;
+    } else {
+        b.meta[i] = m | kMetaAsleep;
+        b.v[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        b.w[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    timer[i] = 0.0f;
+    isl[i] = kNoIndex;
+}
+
 }  // namespace
+
+// Compact island indices of the active list d_act (length sleep[0]).
+void launch_island_index(hipStream_t s, const uint32_t* act, uint32_t* sleep, const BodyBuf& b, uint32_t* isl) {
+    k_island_index<<<1, 1024, 0, s>>>(act, sleep, b, isl);
+}
+
+// Islands over the active list's indices (see k_uf_union_indexed); the
+// outputs as launch_islands gives them.  parent and island hold nb entries.
+void launch_islands_indexed(hipStream_t s, const ContactBuf& c, uint32_t cap, const BodyBuf& b, uint32_t nb,
+                            const uint32_t* isl, const uint32_t* sleep, uint32_t* parent, uint32_t* island,
+                            uint2* tile, uint32_t* totals, uint32_t* con_island) {
+    const uint32_t tiles = (nb + kTile - 1) / kTile;
+    const uint32_t cb = (cap + 255) / 256;
+    if (nb) k_uf_init<<<(nb + 255) / 256, 256, 0, s>>>(parent, nb);
+    if (cb) k_uf_union_indexed<<<cb, 256, 0, s>>>(c, totals, b, isl, sleep, parent);
+    if (tiles) k_uf_roots_indexed<<<tiles, kTile, 0, s>>>(parent, sleep, tile);
+    k_compact_scan<<<1, kTile, 0, s>>>(tile, tiles, totals + 2);
+    if (tiles) k_island_ids_indexed<<<tiles, kTile, 0, s>>>(parent, sleep, tile, island);
+    if (cb) k_con_island_indexed<<<cb, 256, 0, s>>>(c, totals, b, isl, sleep, parent, island, con_island);
+}
+
+// The wake pass of a build over the constraint table (totals[0]
+// constraints): woken bodies appended to act (capacity nb), sleep[0..2]
+// updated; `limit` bounds the relaxation rounds.
+void launch_wake(hipStream_t s, const ContactBuf& c, const uint32_t* totals, const BodyBuf& b, uint32_t nb,
+                 uint32_t* t, uint32_t* act, uint32_t* sleep, float* timer, uint32_t limit) {
+    if (!nb) return;
+    k_wake_init<<<(nb + 255) / 256, 256, 0, s>>>(b, nb, t, sleep);
+    k_wake_relax<<<1, 1024, 0, s>>>(c, totals, b, t, sleep, limit);
+    k_wake_append<<<1, 1024, 0, s>>>(c, totals, b, t, act, sleep, timer);
+}
+
+// UpdateSleeping's device part over the first n entries of act; q[i] = 1
+// where the body at position i falls asleep.  parent and tmin hold n entries.
+void launch_sleep_decide(hipStream_t s, const ContactBuf& c, uint32_t cap, const uint32_t* totals, const BodyBuf& b,
+                         const uint32_t* act, uint32_t n, float* timer, uint32_t* isl, uint32_t* parent,
+                         uint32_t* tmin, uint8_t* q, float dt, float lin2, float ang2, float thr) {
+    if (!n) return;
+    const uint32_t nbk = (n + 255) / 256, cb = (cap + 255) / 256;
+    k_sleep_timers<<<nbk, 256, 0, s>>>(act, n, b, timer, isl, parent, tmin, dt, lin2, ang2);
+    if (cb) k_sleep_union<<<cb, 256, 0, s>>>(c, totals, b, isl, n, parent);
+    k_sleep_min<<<nbk, 256, 0, s>>>(act, n, b, timer, parent, tmin);
+    k_sleep_decide<<<nbk, 256, 0, s>>>(act, n, b, parent, tmin, thr, q);
+}
+
+void launch_set_activity(hipStream_t s, const BodyBuf& b, uint32_t nb, const uint8_t* active, float* timer,
+                         uint32_t* isl) {
+    if (nb) k_set_activity<<<(nb + 255) / 256, 256, 0, s>>>(b, nb, active, timer, isl);
+}
 
 // Islands of the constraint table built by launch_compact: con_island[ci]
 // for ci < totals[0], and totals[2] = number of islands (every dynamic body

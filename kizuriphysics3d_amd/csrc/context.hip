@@ -202,10 +202,158 @@ int check_sweep(kzg_ctx* c) {
     return KZG_OK;
 }
 
+// ---- Active list (src/Body/BodyManager.cpp:138-160) -------------------------
+constexpr uint32_t kNoBody = 0xFFFFFFFFu;
+
+// The listed bodies active in this order, every other body asleep with zero
+// velocities; timers zero, island indices unset.
+int set_active_list(kzg_ctx* c, const uint32_t* list, uint32_t n) {
+    const uint32_t nb = c->nbodies;
+    std::vector<uint8_t> active(nb, 0);
+    c->h_act.assign(std::max<uint32_t>(nb, 1), kNoBody);
+    c->h_act_pos.assign(nb, kNoBody);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t x = list[i];
+        if (x >= nb || active[x])
+            return fail(c, KZG_ERR_ARGUMENT, "active list: entry " + std::to_string(i) + " is out of range or repeated");
+        active[x] = 1;
+        c->h_act[i] = x;
+        c->h_act_pos[x] = i;
+    }
+    c->h_act_n = n;
+    const size_t nb1 = std::max<uint32_t>(nb, 1);
+    for (auto* d : {&c->d_act, &c->d_island, &c->d_wake, &c->sl_parent, &c->sl_min}) KZG_HIP(c, d->ensure(nb1));
+    KZG_HIP(c, c->d_timer.ensure(nb1));
+    KZG_HIP(c, c->sl_q.ensure(nb1));
+    KZG_HIP(c, c->pin_sleep.ensure(nb1 + 64));
+    const uint32_t head[4] = {n, 0u, 0u, 0u};
+    int rc;
+    if ((rc = upload(c, c->d_act, list, n)) || (rc = upload(c, c->d_sleep, head, 4)) ||
+        (rc = upload(c, c->sl_q, active.data(), nb)))
+        return rc;
+    launch_set_activity(c->stream, c->bodies(), nb, c->sl_q.p, c->d_timer.p, c->d_island.p);
+    KZG_HIP(c, hipGetLastError());
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    return KZG_OK;
+}
+
+// After the build's first wait: the bodies its wake-up pass put on the
+// device list go onto the host's copy, in the same order.
+int take_woken(kzg_ctx* c) {
+    const uint32_t* sl = c->pin_sleep.as<uint32_t>();
+    if (!c->wake_launched) return KZG_OK;
+    if (sl[3]) return fail(c, KZG_ERR_DEVICE, "the wake-up pass did not settle within its round limit");
+    const uint32_t w = sl[2], n0 = c->h_act_n;
+    if (sl[0] != n0 + w || n0 + w > c->nbodies) return fail(c, KZG_ERR_DEVICE, "active list out of step with the device");
+    if (!w) return KZG_OK;
+    KZG_HIP(c, c->stage2.ensure(sizeof(uint32_t) * w));
+    KZG_HIP(c, hipMemcpyAsync(c->stage2.p, c->d_act.p + n0, sizeof(uint32_t) * w, hipMemcpyDeviceToHost, c->stream));
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    const uint32_t* woken = c->stage2.as<uint32_t>();
+    for (uint32_t k = 0; k < w; ++k) {
+        const uint32_t x = woken[k];
+        if (x >= c->nbodies || c->h_act_pos[x] != kNoBody) return fail(c, KZG_ERR_DEVICE, "bad woken body");
+        c->h_act[n0 + k] = x;
+        c->h_act_pos[x] = n0 + k;
+    }
+    c->h_act_n = n0 + w;
+    return KZG_OK;
+}
+
+// The reference solves islands one after another (PhysicsWorld.cpp:563-566
+// without worker threads); the sweeps run them side by side, which is the
+// same while no dynamic body is in two islands.  A sleeping body reached
+// through a stale index can be (a contact goes to its partner's island
+// when the body's own index is out of range).
+int check_shared_bodies(kzg_ctx* c) {
+    c->h_owner.assign(c->nbodies, kNoBody);
+    for (uint32_t ci = 0; ci < c->ncons; ++ci) {
+        const uint32_t is = c->h_con_island[ci];
+        if (is == kNoBody) continue;
+        for (const uint32_t x : {c->h_info[ci].x, c->h_info[ci].y}) {
+            if (!c->h_dynamic[x]) continue;
+            uint32_t& o = c->h_owner[x];
+            if (o == kNoBody) o = is;
+            else if (o != is)
+                return fail(c, KZG_ERR_UNSUPPORTED, "body " + std::to_string(x) + " is in islands " +
+                                                        std::to_string(o) + " and " + std::to_string(is));
+        }
+    }
+    return KZG_OK;
+}
 
 }  // namespace
 
 extern "C" {
+
+int kzg_update_sleeping(kzg_ctx* c, float dt) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (!c->world.allow_sleeping) return KZG_OK;  // PhysicsWorld.cpp:867
+    if (c->phase != 0) return fail(c, KZG_ERR_STATE, "kzg_update_sleeping follows kzg_integrate_positions");
+    hipSetDevice(c->device);
+    const uint32_t n = c->h_act_n;
+    if (!n) return KZG_OK;
+    launch_sleep_decide(c->stream, c->contacts(), c->ncons, c->compact_tot.p, c->bodies(), c->d_act.p, n,
+                        c->d_timer.p, c->d_island.p, c->sl_parent.p, c->sl_min.p, c->sl_q.p, dt,
+                        c->sleep_lin * c->sleep_lin, c->sleep_ang * c->sleep_ang, c->sleep_time);
+    KZG_HIP(c, hipGetLastError());
+    KZG_HIP(c, hipMemcpyAsync(c->pin_sleep.p, c->sl_q.p, n, hipMemcpyDeviceToHost, c->stream));
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    const uint8_t* q = c->pin_sleep.as<uint8_t>();
+    // The reference's loop (:932-941) with DeactivateBody's swap-remove;
+    // position i holds, when the loop reaches it, the body that was there
+    // when the loop started.
+    c->h_orig.assign(c->h_act.begin(), c->h_act.begin() + n);
+    uint32_t size = n;
+    bool any = false;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!q[i]) continue;
+        const uint32_t x = c->h_orig[i], p = c->h_act_pos[x];
+        if (p == kNoBody || p >= size) return fail(c, KZG_ERR_DEVICE, "active list out of step with the device");
+        const uint32_t last = c->h_act[size - 1];
+        c->h_act[p] = last;
+        c->h_act_pos[last] = p;
+        c->h_act_pos[x] = kNoBody;
+        --size;
+        any = true;
+    }
+    if (!any) return KZG_OK;
+    c->h_act_n = size;
+    int rc;
+    if ((rc = upload(c, c->d_act, c->h_act.data(), size)) || (rc = upload(c, c->d_sleep, &size, 1))) return rc;
+    return KZG_OK;
+}
+
+int kzg_get_active_bodies(kzg_ctx* c, uint32_t* out, uint32_t cap, uint32_t* count) {
+    if (!c || (cap && !out)) return KZG_ERR_ARGUMENT;
+    join_schedule(c);
+    const uint32_t n = c->h_act_n;
+    if (count) *count = n;
+    if (cap) std::memcpy(out, c->h_act.data(), sizeof(uint32_t) * std::min(n, cap));
+    return KZG_OK;
+}
+
+int kzg_set_active_bodies(kzg_ctx* c, const uint32_t* list, uint32_t n) {
+    if (!c || (n && !list)) return KZG_ERR_ARGUMENT;
+    if (n > c->nbodies) return fail(c, KZG_ERR_ARGUMENT, "active list longer than the body table");
+    join_schedule(c);
+    hipSetDevice(c->device);
+    int rc = set_active_list(c, list, n);
+    if (rc) return rc;
+    c->sleep_model = true;  // the build's wake-ups and islands follow the list from now on
+    return KZG_OK;
+}
+
+int kzg_set_sleep_thresholds(kzg_ctx* c, float time, float linear, float angular) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (!(time >= 0.0f) || !(linear >= 0.0f) || !(angular >= 0.0f) || !std::isfinite(time) ||
+        !std::isfinite(linear) || !std::isfinite(angular))
+        return fail(c, KZG_ERR_ARGUMENT, "sleep thresholds must be finite and non-negative");
+    c->sleep_time = time;
+    c->sleep_lin = linear;
+    c->sleep_ang = angular;
+    return KZG_OK;
+}
 
 int kzg_create(kzg_ctx** out, const kz_world_desc* world, int device) {
     if (!out || !world) return KZG_ERR_ARGUMENT;
@@ -297,6 +445,10 @@ void kzg_destroy(kzg_ctx* c) {
     c->uf_parent.release();
     c->uf_island.release();
     c->con_island.release();
+    for (auto* d : {&c->d_act, &c->d_island, &c->d_wake, &c->d_sleep, &c->sl_parent, &c->sl_min}) d->release();
+    c->d_timer.release();
+    c->sl_q.release();
+    c->pin_sleep.release();
     for (auto& e : c->ev)
         if (e) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -374,7 +526,8 @@ int kzg_upload_bodies(kzg_ctx* c, const kzg_body* b, uint32_t n) {
                                          d.inverse_inertia_local[3 * col + 2], 0.0f);
         mat[i] = f4(d.friction, d.restitution, d.linear_damping, d.angular_damping);
         lim[i] = f4(d.gravity_factor, d.max_linear_velocity, d.max_angular_velocity, 0.0f);
-        meta[i] = d.shape | (d.motion_type << 24) | ((d.is_sensor ? 1u : 0u) << 26);
+        meta[i] = d.shape | (d.motion_type << 24) | ((d.is_sensor ? 1u : 0u) << 26) |
+                  (d.allow_sleeping ? kMetaMaySleep : 0u);
         packed[i] = d.packed_id;
         c->h_packed[i] = d.packed_id;
         c->h_dynamic[i] = d.motion_type == 2;
@@ -395,6 +548,13 @@ int kzg_upload_bodies(kzg_ctx* c, const kzg_body* b, uint32_t n) {
     KZG_HIP(c, hipGetLastError());
     KZG_HIP(c, hipStreamSynchronize(c->stream));
     c->phase = 0;
+    c->ncons = 0;
+    // Every body on the active list in slot order (CreateBody,
+    // src/Body/BodyManager.cpp:59); island indices unset (Body.h:281).
+    std::vector<uint32_t> all(n);
+    for (uint32_t i = 0; i < n; ++i) all[i] = i;
+    if ((rc = set_active_list(c, all.data(), n))) return rc;
+    c->sleep_model = c->world.allow_sleeping != 0;
     return KZG_OK;
 }
 
@@ -480,8 +640,23 @@ int narrowphase_on_device(kzg_ctx* c, uint32_t n) {
     // right behind the narrowphase.
     launch_compact(c->stream, c->m_count.p, n, c->d_pairs.p, c->bodies(), c->contacts(), c->compact_tile.p,
                    c->compact_tot.p);
-    launch_islands(c->stream, c->contacts(), n, c->bodies(), c->nbodies, c->uf_parent.p, c->uf_island.p,
-                   c->compact_tile.p, c->compact_tot.p, c->con_island.p, c->island_lds_limit);
+    if (c->sleep_model) {
+        // The build's wake-ups (only while some body sleeps), then the
+        // islands over the active list's compact indices.
+        c->wake_launched = c->h_act_n < c->nbodies;
+        if (c->wake_launched)
+            launch_wake(c->stream, c->contacts(), c->compact_tot.p, c->bodies(), c->nbodies, c->d_wake.p,
+                        c->d_act.p, c->d_sleep.p, c->d_timer.p, c->nbodies + 2u);
+        launch_island_index(c->stream, c->d_act.p, c->d_sleep.p, c->bodies(), c->d_island.p);
+        launch_islands_indexed(c->stream, c->contacts(), n, c->bodies(), c->nbodies, c->d_island.p, c->d_sleep.p,
+                               c->uf_parent.p, c->uf_island.p, c->compact_tile.p, c->compact_tot.p,
+                               c->con_island.p);
+        KZG_HIP(c, hipMemcpyAsync(c->pin_sleep.p, c->d_sleep.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                  c->stream));
+    } else {
+        launch_islands(c->stream, c->contacts(), n, c->bodies(), c->nbodies, c->uf_parent.p, c->uf_island.p,
+                       c->compact_tile.p, c->compact_tot.p, c->con_island.p, c->island_lds_limit);
+    }
     KZG_HIP(c, hipGetLastError());
     KZG_HIP(c, record(c, 13));
     uint32_t* tot = c->pin_totals.as<uint32_t>();
@@ -515,6 +690,7 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     c->ncons = tot[0];
     c->npoints = tot[1];
     const uint32_t islands = tot[2];
+    if (c->sleep_model && (rc0 = take_woken(c))) return rc0;
     const uint32_t cache_used = tot[4];
     const uint32_t np = c->npoints;
     int rc;
@@ -549,6 +725,7 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     KZG_HIP(c, hipEventSynchronize(c->ev[12]));
     // Host time of the build proper starts once the table is here.
     const auto t0 = std::chrono::steady_clock::now();
+    if (c->sleep_model && (rc = check_shared_bodies(c))) return rc;
     ScheduleInput si;
     si.num_bodies = c->nbodies;
     si.dynamic = c->h_dynamic.data();
@@ -776,7 +953,8 @@ int kzg_step(kzg_ctx* c, const uint32_t* pairs, uint32_t n, float dt) {
     if ((rc = kzg_build_constraints(c, c->warm_mode))) return rc;
     if ((rc = kzg_integrate_velocities(c, dt))) return rc;
     if ((rc = kzg_solve(c, dt))) return rc;
-    return kzg_integrate_positions(c, dt);
+    if ((rc = kzg_integrate_positions(c, dt))) return rc;
+    return kzg_update_sleeping(c, dt);
 }
 
 int kzg_query_sizes(kzg_ctx* c, uint32_t* np, uint32_t* nc, uint32_t* npts) {

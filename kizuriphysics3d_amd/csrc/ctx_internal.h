@@ -62,6 +62,17 @@ void launch_compact(hipStream_t s, const uint32_t* m_count, uint32_t n, const ui
 void launch_islands(hipStream_t s, const ContactBuf& c, uint32_t cap, const BodyBuf& b, uint32_t nb,
                     uint32_t* parent, uint32_t* island, uint2* tile, uint32_t* totals, uint32_t* con_island,
                     uint32_t lds_limit = 0);
+void launch_island_index(hipStream_t s, const uint32_t* act, uint32_t* sleep, const BodyBuf& b, uint32_t* isl);
+void launch_islands_indexed(hipStream_t s, const ContactBuf& c, uint32_t cap, const BodyBuf& b, uint32_t nb,
+                            const uint32_t* isl, const uint32_t* sleep, uint32_t* parent, uint32_t* island,
+                            uint2* tile, uint32_t* totals, uint32_t* con_island);
+void launch_wake(hipStream_t s, const ContactBuf& c, const uint32_t* totals, const BodyBuf& b, uint32_t nb,
+                 uint32_t* t, uint32_t* act, uint32_t* sleep, float* timer, uint32_t limit);
+void launch_sleep_decide(hipStream_t s, const ContactBuf& c, uint32_t cap, const uint32_t* totals, const BodyBuf& b,
+                         const uint32_t* act, uint32_t n, float* timer, uint32_t* isl, uint32_t* parent,
+                         uint32_t* tmin, uint8_t* q, float dt, float lin2, float ang2, float thr);
+void launch_set_activity(hipStream_t s, const BodyBuf& b, uint32_t nb, const uint8_t* active, float* timer,
+                         uint32_t* isl);
 void launch_cache_store(hipStream_t s, const ContactBuf& c, const BodyBuf& b, const CacheBuf& cache, uint32_t nc);
 void launch_cache_rehash(hipStream_t s, const CacheBuf& src, const CacheBuf& dst);
 // acc[4] (zeroed by the caller): squared sums of the normal, friction and
@@ -160,6 +171,24 @@ struct kzg_ctx {
     uint32_t coop_used = 1;          // workgroups the last solve gave group 0
     uint32_t coop_ws_points = 0, coop_vel_points = 0;
     DevVec<uint32_t> b_meta, b_packed;
+
+    // Activation (BodyManager's active list, src/Body/BodyManager.cpp:138-160)
+    // and sleeping (PhysicsWorld::UpdateSleeping, PhysicsWorld.cpp:866-951).
+    // The host keeps the list: entries past h_act_n are the stale values
+    // swap-removes leave there.  d_act is the device's copy; d_sleep holds
+    // its length, the count of compact island indices, the bodies woken by
+    // the last build and the wake pass's give-up flag.
+    bool sleep_model = false;        // the active list may differ from every body in slot order
+    float sleep_time = 0.5f, sleep_lin = 0.05f, sleep_ang = 0.1f;
+    std::vector<uint32_t> h_act, h_act_pos;  // list, and each body's position in it (kNone: asleep)
+    uint32_t h_act_n = 0;
+    std::vector<uint8_t> h_may_sleep;
+    DevVec<uint32_t> d_act, d_island, d_wake, d_sleep, sl_parent, sl_min;
+    DevVec<float> d_timer;
+    DevVec<uint8_t> sl_q;
+    Pinned pin_sleep;                // d_sleep after a build; UpdateSleeping's decisions
+    bool wake_launched = false;      // this step's build ran the wake-up pass
+    std::vector<uint32_t> h_owner, h_orig;  // host scratch: island of each body, the list before UpdateSleeping
 
     // pairs and manifolds
     uint32_t npairs = 0;

@@ -31,10 +31,15 @@ __host__ __device__ inline uint32_t work_class(uint32_t ta, uint32_t tb) {
     return 4u * (ta == 4u ? 3u : ta) + (tb == 4u ? 3u : tb);
 }
 
-// Body metadata word: shape index (bits 0-23), motion type (24-25), sensor (26).
+// Body metadata word: shape index (bits 0-23), motion type (24-25), sensor
+// (26), asleep (27: the body is off the active list, ActivationState::
+// Inactive), sleeping allowed (28: BodySettings::allowSleeping).
+constexpr uint32_t kMetaAsleep = 1u << 27, kMetaMaySleep = 1u << 28;
 __host__ __device__ inline uint32_t meta_shape(uint32_t m) { return m & 0x00FFFFFFu; }
 __host__ __device__ inline uint32_t meta_motion(uint32_t m) { return (m >> 24) & 3u; }
 __host__ __device__ inline bool meta_sensor(uint32_t m) { return (m >> 26) & 1u; }
+__host__ __device__ inline bool meta_asleep(uint32_t m) { return (m & kMetaAsleep) != 0u; }
+__host__ __device__ inline bool meta_may_sleep(uint32_t m) { return (m & kMetaMaySleep) != 0u; }
 
 // Maximum manifold points kept per pair.  Every in-scope narrowphase path
 // (plane, sphere, GJK/EPA, polyhedral clipping) reduces to at most four
@@ -68,7 +73,7 @@ struct BodyBuf {
     float4* il;       // inverse local inertia, 3 columns per body
     float4* mat;      // friction, restitution, linear damping, angular damping
     float4* lim;      // gravity factor, max linear velocity, max angular velocity, 0
-    uint32_t* meta;   // shape | motion << 24 | sensor << 26
+    uint32_t* meta;   // shape | motion << 24 | sensor << 26 | asleep << 27 | may sleep << 28
     uint32_t* packed; // packed BodyID
 };
 

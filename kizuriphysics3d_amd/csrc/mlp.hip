@@ -240,7 +240,8 @@ __global__ void __launch_bounds__(256) k_mlp_warm(MlpBuf m, WarmArgs w) {
                 w.c_imp[p] = imp;
                 const uint32_t e = w.vel_pos[p];
                 if (e != 0xFFFFFFFFu) w.vr_imp[e] = imp;
-                w.wr_imp[w.ws_pos[p]] = imp;
+                const uint32_t q = w.ws_pos[p];
+                if (q != 0xFFFFFFFFu) w.wr_imp[q] = imp;
             }
         }
     }

@@ -309,6 +309,9 @@ void ScheduleBuilder::ws_levels(const ScheduleInput& in) {
         const U4& ci = in.info[c];
         const bool da = dyn[ci.x] != 0, db = dyn[ci.y] != 0;
         if (!da && !db) continue;
+        // With sleeping bodies a constraint can be in no island: the
+        // reference never solves it.
+        if (in.island_of_con && in.island_of_con[c] == kNone) continue;
         const uint32_t np = ci.w & 0xFFu;
         for (uint32_t k0 = 0; k0 < np; k0 += cap) {
             const uint32_t l = 1 + std::max(da ? bl[ci.x] : 0u, db ? bl[ci.y] : 0u);

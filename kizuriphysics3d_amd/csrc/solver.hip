@@ -33,7 +33,9 @@ __device__ __forceinline__ m3 ldm(const float4* c) { return m3{ld3(c[0]), ld3(c[
 // ---------------------------------------------------------------------------
 __global__ void k_integrate_velocities(BodyBuf b, uint32_t n, float dt, float gx, float gy, float gz) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || meta_motion(b.meta[i]) != 2u) return;
+    if (i >= n) return;
+    const uint32_t m = b.meta[i];
+    if (meta_motion(m) != 2u || meta_asleep(m)) return;  // the active list's dynamic bodies
     const float4 x = b.x[i], lim = b.lim[i], mat = b.mat[i];
     f3 v = ld3(b.v[i]), w = ld3(b.w[i]);
     const m3 iw = ldm(b.iw + 3 * i);
@@ -224,6 +226,7 @@ __global__ void k_prepare(ContactBuf c, BodyBuf b, uint32_t np, const uint32_t* 
         vr.imp[e] = make_float4(imp.x, imp.y, imp.z, 0.0f);
     }
     const uint32_t w = ws_pos[p];
+    if (w == 0xFFFFFFFFu) return;  // in no island: solved nowhere, keeps the clamped impulses
     wr.bd[w] = bdo;
     wr.ra[w] = st3(ra, xa.w);
     wr.rb[w] = st3(rb, xb.w);
@@ -1047,7 +1050,8 @@ __global__ void k_writeback(VelRecBuf vr, WsRecBuf wr, ContactBuf c, const uint3
         const float4 v = vr.imp[e];
         imp.x = v.x; imp.y = v.y; imp.z = v.z;
     }
-    imp.w = wr.imp[ws_pos[p]].w;
+    const uint32_t w = ws_pos[p];
+    if (w != 0xFFFFFFFFu) imp.w = wr.imp[w].w;
     c.imp[p] = imp;
 }
 
@@ -1106,8 +1110,8 @@ __global__ void k_cache_rehash(CacheBuf src, CacheBuf dst) {
 __global__ void k_finish(BodyBuf b, uint32_t n, float dt, int split_impulse) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t mo = meta_motion(b.meta[i]);
-    if (mo == 0u) return;
+    const uint32_t m = b.meta[i], mo = meta_motion(m);
+    if (mo == 0u || meta_asleep(m)) return;  // the active list's non-static bodies
     float4 x = b.x[i];
     f3 t = ld3(x);
     f4q q = ldq(b.q[i]);

@@ -36,6 +36,10 @@ struct kzw_world {
     std::vector<float> bp_image;
     bool shapes_dirty = true, bodies_dirty = true;
     kzw_stats stats{};
+    std::vector<uint32_t> active;  // the context's active list (BodyManager's), read each step
+    // Bodies the last loaded snapshot marks inactive: the list the body
+    // upload sets is then every other body in slot order.
+    std::vector<uint8_t> load_inactive;
 };
 
 namespace {
@@ -97,6 +101,7 @@ int sync_device(kzw_world* w) {
             o.friction = d.friction;
             o.restitution = d.restitution;
             o.linear_damping = d.linear_damping;
+            o.allow_sleeping = d.allow_sleeping;
             o.angular_damping = d.angular_damping;
             o.gravity_factor = d.gravity_factor;
             o.max_linear_velocity = d.max_linear_velocity;
@@ -104,6 +109,16 @@ int sync_device(kzw_world* w) {
         }
         int rc = kzg_upload_bodies(w->ctx, b.data(), uint32_t(b.size()));
         if (rc) return wfail(w, rc, kzg_last_error(w->ctx));
+        if (!w->load_inactive.empty()) {
+            // Serializer.cpp:230-234: CreateBody pushes each body, then an
+            // inactive one is removed again at once (it is the list's tail).
+            std::vector<uint32_t> list;
+            for (uint32_t i = 0; i < uint32_t(b.size()); ++i)
+                if (i >= w->load_inactive.size() || !w->load_inactive[i]) list.push_back(i);
+            rc = kzg_set_active_bodies(w->ctx, list.data(), uint32_t(list.size()));
+            if (rc) return wfail(w, rc, kzg_last_error(w->ctx));
+            w->load_inactive.clear();
+        }
         w->bodies_dirty = false;
     }
     return KZG_OK;
@@ -119,8 +134,6 @@ int kzw_create(kzw_world** out, const kz_world_desc* ws, int device) {
     auto* w = new kzw_world();
     w->settings = *ws;
     w->device = device;
-    if (ws->allow_sleeping)
-        return delete w, KZG_ERR_UNSUPPORTED;  // the device path does not model sleeping
     if (device >= 0) {
         int rc = kzg_create(&w->ctx, ws, device);
         if (rc) {
@@ -194,16 +207,27 @@ int kzw_add_bodies(kzw_world* w, const kz_body_desc* b, uint32_t n) {
 }
 
 namespace {
-// UpdateBroadPhase's proxy moves (PhysicsWorld.cpp:334-345): active
-// non-static bodies, displacement = v * fixedTimestep.
-void move_proxies(kzw_world* w) {
-    for (uint32_t i = 0; i < w->bodies.size(); ++i) {
+// UpdateBroadPhase's proxy moves (PhysicsWorld.cpp:334-345): the active
+// list's non-static bodies in list order, displacement = v * fixedTimestep.
+int move_proxies(kzw_world* w) {
+    const uint32_t nb = uint32_t(w->bodies.size());
+    w->active.resize(nb);
+    uint32_t na = nb;
+    if (w->ctx) {
+        int rc = kzg_get_active_bodies(w->ctx, w->active.data(), nb, &na);
+        if (rc) return wfail(w, rc, kzg_last_error(w->ctx));
+    } else {
+        for (uint32_t i = 0; i < nb; ++i) w->active[i] = i;
+    }
+    for (uint32_t k = 0; k < na; ++k) {
+        const uint32_t i = w->active[k];
         if (w->bodies[i].motion_type == KZ_MOTION_STATIC) continue;
         const kz_body_state& s = w->states[i];
         const f3 disp = mul(f3{s.linear_velocity[0], s.linear_velocity[1], s.linear_velocity[2]},
                             w->settings.fixed_timestep);
         w->bp.move_proxy(w->proxy[i], world_bounds(*w, i), disp);
     }
+    return KZG_OK;
 }
 // The pair query on the device over the moved tree; the pairs stay there.
 int device_pairs(kzw_world* w) {
@@ -233,7 +257,8 @@ void host_pairs(kzw_world* w) {
 
 int kzw_update_broadphase(kzw_world* w) {
     if (!w) return KZG_ERR_ARGUMENT;
-    move_proxies(w);
+    int rc = move_proxies(w);
+    if (rc) return rc;
     host_pairs(w);
     return KZG_OK;
 }
@@ -251,7 +276,7 @@ int kzw_step(kzw_world* w, float dt) {
     int rc = sync_device(w);
     if (rc) return rc;
     uint32_t np = 0;
-    move_proxies(w);
+    if ((rc = move_proxies(w))) return rc;
     const auto tmv = clock::now();
     if (w->device_bp) {
         if ((rc = device_pairs(w))) return rc;
@@ -270,7 +295,7 @@ int kzw_step(kzw_world* w, float dt) {
     kzg_timings tm;
     if ((rc = kzg_get_timings(w->ctx, &tm))) return wfail(w, rc, kzg_last_error(w->ctx));  // synchronises
     const auto t2 = clock::now();
-    if ((rc = kzg_integrate_positions(w->ctx, dt)) ||
+    if ((rc = kzg_integrate_positions(w->ctx, dt)) || (rc = kzg_update_sleeping(w->ctx, dt)) ||
         (rc = kzg_download_states(w->ctx, w->states.data(), uint32_t(w->states.size()))))
         return wfail(w, rc, kzg_last_error(w->ctx));
     const auto t3 = clock::now();
@@ -280,6 +305,7 @@ int kzw_step(kzw_world* w, float dt) {
     w->stats.num_broadphase_pairs = np;
     w->stats.num_contact_constraints = nc;
     w->stats.num_contact_points = npts;
+    kzg_get_active_bodies(w->ctx, nullptr, 0, &w->stats.num_active_bodies);
     w->stats.broadphase_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     w->stats.contact_path_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
     w->stats.step_ms = std::chrono::duration<double, std::milli>(t3 - t0).count();
@@ -410,7 +436,7 @@ struct Reader {
 
 // One KZP1 body: the shape (ShapeType code, parameters, hull points) and the
 // BodySettings fields in the reference's order.
-void write_kzp1_body(Writer& w, const kzw_world& W, uint32_t i) {
+void write_kzp1_body(Writer& w, const kzw_world& W, uint32_t i, bool active) {
     const kz_body_desc& d = W.bodies[i];
     const kz_body_state& s = W.states[i];
     const auto& sh = W.shapes[d.shape];
@@ -446,13 +472,14 @@ void write_kzp1_body(Writer& w, const kzw_world& W, uint32_t i) {
     w.put(uint64_t(0));  // userData
     w.put(d.mass_override);
     w.put(d.density);
-    w.put(uint8_t(1));  // active
+    w.put(uint8_t(active ? 1 : 0));  // Body::IsActive
 }
 
 struct Kzp1Body {
     kz_shape_desc shape{};
     std::vector<float> points;
     kz_body_desc body{};
+    uint8_t active = 1;
 };
 
 bool read_kzp1_body(Reader& r, Kzp1Body& o) {
@@ -499,6 +526,7 @@ bool read_kzp1_body(Reader& r, Kzp1Body& o) {
     d.motion_type = motion;
     d.allow_sleeping = sleeping;
     d.is_sensor = sensor;
+    o.active = active;
     return true;
 }
 
@@ -513,7 +541,16 @@ int kzw_save(kzw_world* w, uint8_t* out, uint64_t cap, uint64_t* size) {
     wr.put(kKzp1Magic);
     wr.put(uint32_t(1));
     wr.put(nb);
-    for (uint32_t i = 0; i < nb; ++i) write_kzp1_body(wr, *w, i);
+    std::vector<uint8_t> active(nb, 1);
+    if (w->ctx) {
+        std::vector<uint32_t> list(nb);
+        uint32_t na = 0;
+        int rc = kzg_get_active_bodies(w->ctx, list.data(), nb, &na);
+        if (rc) return wfail(w, rc, kzg_last_error(w->ctx));
+        active.assign(nb, 0);
+        for (uint32_t k = 0; k < na && k < nb; ++k) active[list[k]] = 1;
+    }
+    for (uint32_t i = 0; i < nb; ++i) write_kzp1_body(wr, *w, i, active[i] != 0);
     wr.put(kKzx1Magic);
     wr.put(uint32_t(1));
     size_t blk = wr.begin_block(tag4("WSET"));
@@ -666,6 +703,16 @@ int kzw_load(kzw_world** out, const uint8_t* data, uint64_t size, const kz_world
     }
     if ((rc = kzw_add_bodies(w, descs.data(), nb))) return bad(rc);
     std::memcpy(w->states.data(), states.data(), sizeof(kz_body_state) * nb);
+    w->load_inactive.assign(nb, 0);
+    bool any_inactive = false;
+    for (uint32_t i = 0; i < nb; ++i) {
+        w->load_inactive[i] = kb[i].active == 0;
+        any_inactive |= kb[i].active == 0;
+        // DeactivateBody zeroes the velocities (BodyManager.cpp:152-153).
+        if (kb[i].active == 0)
+            for (int k = 0; k < 3; ++k) w->states[i].linear_velocity[k] = w->states[i].angular_velocity[k] = 0.0f;
+    }
+    if (!any_inactive) w->load_inactive.clear();
     if (bpht) {
         Reader t{bpht, lbpht};
         uint64_t words = 0;

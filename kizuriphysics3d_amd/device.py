@@ -22,6 +22,7 @@ KZG_BODY_DTYPE = np.dtype([
     ("angular_velocity", "<f4", (3,)), ("inverse_mass", "<f4"), ("inverse_inertia_local", "<f4", (9,)),
     ("friction", "<f4"), ("restitution", "<f4"), ("linear_damping", "<f4"), ("angular_damping", "<f4"),
     ("gravity_factor", "<f4"), ("max_linear_velocity", "<f4"), ("max_angular_velocity", "<f4"),
+    ("allow_sleeping", "<u4"),
 ])
 HULL_RANGE_DTYPE = np.dtype([("vert_offset", "<u4"), ("vert_count", "<u4"), ("tri_offset", "<u4"), ("tri_count", "<u4")])
 TIMINGS_DTYPE = np.dtype([
@@ -33,11 +34,12 @@ TIMINGS_DTYPE = np.dtype([
     ("schedule_ws_ms", "<f4"), ("schedule_wait_ms", "<f4"), ("early_warm_start_ms", "<f4"),
     ("schedule_velocity_after_ms", "<f4"), ("overlap_ms", "<f4"),
 ])
-assert KZG_BODY_DTYPE.itemsize == 136
+assert KZG_BODY_DTYPE.itemsize == 140
 
 EXPORTS = [
     "kzg_create", "kzg_destroy", "kzg_last_error", "kzg_upload_shapes", "kzg_upload_bodies", "kzg_upload_states",
     "kzg_narrowphase", "kzg_broadphase_pairs", "kzg_narrowphase_resident", "kzg_download_pairs", "kzg_build_constraints", "kzg_integrate_velocities", "kzg_solve", "kzg_integrate_positions",
+    "kzg_update_sleeping", "kzg_get_active_bodies", "kzg_set_active_bodies", "kzg_set_sleep_thresholds",
     "kzg_step", "kzg_query_sizes", "kzg_download_states", "kzg_download_manifolds", "kzg_download_constraints",
     "kzg_get_residuals", "kzg_get_natural_residual", "kzg_get_timings", "kzg_collide_batch", "kzg_collide_batch_device", "kzg_schedule", "kzg_set_warm_start",
     "kzg_set_iterations", "kzg_set_cooperative", "kzg_set_sweep_units", "kzg_set_early_warm_start", "kzg_set_narrowphase_waves", "kzg_set_island_lds_limit", "kzg_set_cooperative_placement", "kzg_set_spin_limit", "kzg_cache_info", "kzg_download_cache", "kzg_upload_cache",
@@ -77,6 +79,8 @@ def library():
             "kzg_narrowphase_resident": ([vp], i32), "kzg_download_pairs": ([vp, vp, u32], i32),
             "kzg_integrate_velocities": ([vp, f32], i32), "kzg_solve": ([vp, f32], i32),
             "kzg_integrate_positions": ([vp, f32], i32), "kzg_step": ([vp, vp, u32, f32], i32),
+            "kzg_update_sleeping": ([vp, f32], i32), "kzg_get_active_bodies": ([vp, vp, u32, vp], i32),
+            "kzg_set_active_bodies": ([vp, vp, u32], i32), "kzg_set_sleep_thresholds": ([vp, f32, f32, f32], i32),
             "kzg_query_sizes": ([vp, vp, vp, vp], i32), "kzg_download_states": ([vp, vp, u32], i32),
             "kzg_download_manifolds": ([vp, vp, vp], i32), "kzg_download_constraints": ([vp, vp, vp], i32),
             "kzg_get_residuals": ([vp, vp, u32], i32), "kzg_get_timings": ([vp, vp], i32),
@@ -135,7 +139,7 @@ def device_bodies(bodies: np.ndarray, mass: np.ndarray) -> np.ndarray:
     out["packed_id"] = np.arange(len(bodies), dtype=np.uint32)
     for f in ("shape", "motion_type", "is_sensor", "position", "rotation", "linear_velocity", "angular_velocity",
               "friction", "restitution", "linear_damping", "angular_damping", "gravity_factor",
-              "max_linear_velocity", "max_angular_velocity"):
+              "max_linear_velocity", "max_angular_velocity", "allow_sleeping"):
         out[f] = bodies[f]
     out["inverse_mass"] = mass[:, 0]
     out["inverse_inertia_local"] = mass[:, 1:10]
@@ -238,6 +242,27 @@ class DeviceContext:
     def integrate_positions(self, dt=S.DT):
         self._check(library().kzg_integrate_positions(self.h, float(dt)), "kzg_integrate_positions")
 
+    def update_sleeping(self, dt=S.DT):
+        """UpdateSleeping (a no-op unless the world allows sleeping)."""
+        self._check(library().kzg_update_sleeping(self.h, float(dt)), "kzg_update_sleeping")
+
+    def active_bodies(self) -> np.ndarray:
+        """The active body list in order (body slots)."""
+        n = C.c_uint32(0)
+        self._check(library().kzg_get_active_bodies(self.h, None, 0, C.byref(n)), "kzg_get_active_bodies")
+        out = np.zeros(max(n.value, 1), np.uint32)
+        self._check(library().kzg_get_active_bodies(self.h, _p(out), n.value, C.byref(n)), "kzg_get_active_bodies")
+        return out[:n.value]
+
+    def set_active_bodies(self, bodies):
+        lst = np.ascontiguousarray(bodies, np.uint32)
+        self._check(library().kzg_set_active_bodies(self.h, _p(lst) if len(lst) else None, len(lst)),
+                    "kzg_set_active_bodies")
+
+    def set_sleep_thresholds(self, time=0.5, linear=0.05, angular=0.1):
+        self._check(library().kzg_set_sleep_thresholds(self.h, float(time), float(linear), float(angular)),
+                    "kzg_set_sleep_thresholds")
+
     def step(self, pairs: np.ndarray, dt=S.DT, stop_after: int = 5, warm_start_mode: int = 0):
         """StepFixed's contact path over `pairs`; stop_after as in the oracle."""
         self.narrowphase(pairs)
@@ -249,6 +274,7 @@ class DeviceContext:
             self.solve(dt)
         if stop_after >= 5:
             self.integrate_positions(dt)
+            self.update_sleeping(dt)
 
     def sizes(self):
         a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()

@@ -325,6 +325,25 @@ def small_pile(num_bodies: int, rng: SplitMix64, name: str = "pile") -> Scene:
     return sc
 
 
+def sleeping_pile(num_bodies: int, seed: int = 0x4B5A0108, late: int = 4, late_height: float = 30.0) -> Scene:
+    """small_pile with sleeping on (the reference's default settings: 0.5 s
+    below 0.05 m/s and 0.1 rad/s) and ``late`` more bodies released
+    ``late_height`` above the pile's top, so that they land on a pile that
+    has fallen asleep and wake it."""
+    rng = SplitMix64(seed)
+    sc = small_pile(num_bodies, rng, f"sleeping_pile{num_bodies}")
+    sc.world["allow_sleeping"] = 1
+    width = max(4.0, math.sqrt(num_bodies / 4.0) * 1.6)
+    half = 0.5 * width
+    top = 1.0 + 1.6 * num_bodies / (width * width) * 4.0
+    for i in range(late):
+        b = random_convex_body(sc, rng, i % 4)
+        b["position"] = (rng.uniform(-half + 0.8, half - 0.8), top + late_height + 2.0 * i,
+                         rng.uniform(-half + 0.8, half - 0.8))
+        sc.add_body(b)
+    return sc
+
+
 def batched_worlds(num_worlds: int = 4096, seed: int = 0x4B5A0004, min_bodies: int = 16,
                    max_bodies: int = 512) -> list:
     """Independent worlds of uneven size: body counts log-uniform in

@@ -20,8 +20,8 @@ from . import scenes as S
 STATS_DTYPE = np.dtype([
     ("num_bodies", "<u4"), ("num_broadphase_pairs", "<u4"), ("num_contact_constraints", "<u4"),
     ("num_contact_points", "<u4"), ("broadphase_ms", "<f8"), ("contact_path_ms", "<f8"), ("step_ms", "<f8"),
-    ("broadphase_move_ms", "<f8"), ("broadphase_query_ms", "<f8"),
-])
+    ("broadphase_move_ms", "<f8"), ("broadphase_query_ms", "<f8"), ("num_active_bodies", "<u4"),
+], align=True)
 
 _bound = False
 

@@ -31,6 +31,7 @@
 #include <thread>
 #include <type_traits>
 #include <utility>
+#include <unordered_map>
 #include <vector>
 
 #define private public
@@ -356,6 +357,17 @@ uint32_t kzr_get_islands(void* wp, uint32_t* islandOfConstraint) {
     for (u32 i = 0; i < w.mIslands.Size(); ++i)
         for (u32 c : w.mIslands[i].contacts) islandOfConstraint[c] = i;
     return u32(w.mIslands.Size());
+}
+
+// The active body list in order, as body slots (BodyManager::GetActiveBodyList);
+// returns its length.
+uint32_t kzr_get_active(void* wp, uint32_t* out) {
+    RefWorld* rw = static_cast<RefWorld*>(wp);
+    std::unordered_map<u32, uint32_t> slot;
+    for (size_t i = 0; i < rw->ids.size(); ++i) slot[rw->ids[i].Packed()] = uint32_t(i);
+    const auto& act = rw->world.mBodies.GetActiveBodyList();
+    for (u32 i = 0; i < act.Size(); ++i) out[i] = slot.at(act[i].Packed());
+    return u32(act.Size());
 }
 
 uint64_t kzr_state_hash(void* wp) { return static_cast<RefWorld*>(wp)->world.ComputeStateHash(); }

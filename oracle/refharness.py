@@ -46,6 +46,7 @@ def lib():
         L.kzr_num_constraints.argtypes = [vp]; L.kzr_num_constraints.restype = u32
         L.kzr_get_constraints.argtypes = [vp, vp, vp]
         L.kzr_get_islands.argtypes = [vp, vp]; L.kzr_get_islands.restype = u32
+        L.kzr_get_active.argtypes = [vp, vp]; L.kzr_get_active.restype = u32
         L.kzr_state_hash.argtypes = [vp]; L.kzr_state_hash.restype = C.c_uint64
         L.kzr_hull_create.argtypes = [vp, u32, vp, vp, vp, vp]; L.kzr_hull_create.restype = i32
         L.kzr_collide_batch.argtypes = [vp, u32, vp, vp, vp, vp, vp, u32, f32, vp, vp, i32]
@@ -161,6 +162,11 @@ class RefWorld:
         if n:
             lib().kzr_get_constraints(self.h, _p(hdr), _p(pts))
         return hdr, pts
+
+    def active(self) -> np.ndarray:
+        out = np.zeros(max(self.n, 1), np.uint32)
+        k = lib().kzr_get_active(self.h, _p(out))
+        return out[:k]
 
     def islands(self):
         n = lib().kzr_num_constraints(self.h)
