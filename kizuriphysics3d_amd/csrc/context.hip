@@ -265,7 +265,7 @@ int take_woken(kzg_ctx* c) {
 // same while no dynamic body is in two islands.  A sleeping body reached
 // through a stale index can be (a contact goes to its partner's island
 // when the body's own index is out of range).
-int check_shared_bodies(kzg_ctx* c) {
+bool shares_bodies(kzg_ctx* c) {
     c->h_owner.assign(c->nbodies, kNoBody);
     for (uint32_t ci = 0; ci < c->ncons; ++ci) {
         const uint32_t is = c->h_con_island[ci];
@@ -274,10 +274,125 @@ int check_shared_bodies(kzg_ctx* c) {
             if (!c->h_dynamic[x]) continue;
             uint32_t& o = c->h_owner[x];
             if (o == kNoBody) o = is;
-            else if (o != is)
-                return fail(c, KZG_ERR_UNSUPPORTED, "body " + std::to_string(x) + " is in islands " +
-                                                        std::to_string(o) + " and " + std::to_string(is));
+            else if (o != is) return true;
         }
+    }
+    return false;
+}
+
+// Generations: island I gets 1 + the largest generation of an earlier
+// island it shares a dynamic body with (0 if none).  Islands of one
+// generation share no body and keep, body by body, the island order, so
+// solving the generations one after another, each as one solve over its
+// islands, updates every body in the reference's order.  Each generation's
+// schedule is built as for a whole step (its islands only) and the
+// schedules go up side by side: records at disjoint positions, levels,
+// groups and units at per-generation offsets.
+int schedule_generations(kzg_ctx* c, ScheduleInput si, uint32_t islands) {
+    const uint32_t nc = c->ncons, np = c->npoints;
+    std::vector<uint32_t> gen(islands, 0), used(c->nbodies, 0), off(islands + 1, 0), cons(nc);
+    for (uint32_t ci = 0; ci < nc; ++ci)
+        if (c->h_con_island[ci] != kNoBody) ++off[c->h_con_island[ci] + 1];
+    for (uint32_t i = 0; i < islands; ++i) off[i + 1] += off[i];
+    std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+    for (uint32_t ci = 0; ci < nc; ++ci)
+        if (c->h_con_island[ci] != kNoBody) cons[fill[c->h_con_island[ci]]++] = ci;
+    uint32_t ng = 1;
+    for (uint32_t is = 0; is < islands; ++is) {
+        uint32_t g = 0;
+        for (uint32_t k = off[is]; k < off[is + 1]; ++k)
+            for (const uint32_t x : {c->h_info[cons[k]].x, c->h_info[cons[k]].y})
+                if (c->h_dynamic[x]) g = std::max(g, used[x]);
+        gen[is] = g;
+        ng = std::max(ng, g + 1);
+        for (uint32_t k = off[is]; k < off[is + 1]; ++k)
+            for (const uint32_t x : {c->h_info[cons[k]].x, c->h_info[cons[k]].y})
+                if (c->h_dynamic[x]) used[x] = g + 1;
+    }
+    std::vector<uint32_t> ws_off, ws_unit, vel_off, vel_unit, ws_pos(np, kNoBody), vel_pos(np, kNoBody);
+    std::vector<U2> ws_grp, vel_grp;
+    c->gens.assign(ng, kzg_ctx::GenPlan{});
+    c->h_gen_mask.resize(nc);
+    uint32_t ws_rec = 0, vel_rec = 0, ws_ud = 0, vel_ud = 0;
+    const uint32_t vel_lanes = c->world.use_simd_solver ? 2u : 1u;
+    for (uint32_t g = 0; g < ng; ++g) {
+        for (uint32_t ci = 0; ci < nc; ++ci) {
+            const uint32_t is = c->h_con_island[ci];
+            c->h_gen_mask[ci] = (is != kNoBody && gen[is] == g) ? is : kNoBody;
+        }
+        si.island_of_con = c->h_gen_mask.data();
+        ScheduleOutput out;
+        c->sched_builder.build(si, out);
+        kzg_ctx::GenPlan& G = c->gens[g];
+        G.ws_loff = uint32_t(ws_off.size());
+        G.ws_goff = uint32_t(ws_grp.size());
+        G.ws_uoff = uint32_t(ws_unit.size());
+        G.ws_udoff = ws_ud;
+        G.ws_units = out.ws.units;
+        G.vel_loff = uint32_t(vel_off.size());
+        G.vel_goff = uint32_t(vel_grp.size());
+        G.vel_uoff = uint32_t(vel_unit.size());
+        G.vel_udoff = vel_ud;
+        G.vel_units = out.vel.units;
+        G.groups = out.groups;
+        G.width = std::max(out.ws.max_width, vel_lanes * out.vel.max_width);
+        G.largest_ws = out.ws.largest_unit;
+        G.largest_vel = out.vel.largest_unit;
+        ws_off.insert(ws_off.end(), out.ws.off.begin(), out.ws.off.end());
+        ws_grp.insert(ws_grp.end(), out.ws.grp.begin(), out.ws.grp.end());
+        for (uint32_t u : out.ws.unit) ws_unit.push_back(u + ws_rec);
+        vel_off.insert(vel_off.end(), out.vel.off.begin(), out.vel.off.end());
+        vel_grp.insert(vel_grp.end(), out.vel.grp.begin(), out.vel.grp.end());
+        for (uint32_t u : out.vel.unit) vel_unit.push_back(u + vel_rec);
+        for (uint32_t p = 0; p < np; ++p) {
+            if (out.ws.pos[p] != kNoBody) ws_pos[p] = out.ws.pos[p] + ws_rec;
+            if (out.vel.pos[p] != kNoBody) vel_pos[p] = out.vel.pos[p] + vel_rec;
+        }
+        ws_rec += uint32_t(out.ws.idx.size());
+        vel_rec += uint32_t(out.vel.idx.size());
+        ws_ud += out.ws.units;
+        vel_ud += out.vel.units;
+        if (g == 0) c->sched = std::move(out);  // the step's reported schedule
+    }
+    KZG_HIP(c, c->ws_udesc.ensure(std::max<uint32_t>(ws_ud, 1)));
+    KZG_HIP(c, c->vel_udesc.ensure(std::max<uint32_t>(vel_ud, 1)));
+    const size_t bytes = sizeof(uint32_t) * (ws_off.size() + vel_off.size() + ws_unit.size() + vel_unit.size() +
+                                             2 * size_t(np)) +
+                         sizeof(U2) * (ws_grp.size() + vel_grp.size()) + 10 * 256;
+    KZG_HIP(c, c->stage_b.ensure(bytes));
+    size_t o = 0;
+    int rc;
+    if ((rc = upload_async(c, c->ws_off, c->stage_b, o, ws_off.data(), ws_off.size())) ||
+        (rc = upload_async(c, c->ws_grp, c->stage_b, o, reinterpret_cast<const uint2*>(ws_grp.data()), ws_grp.size())) ||
+        (rc = upload_async(c, c->ws_pos, c->stage_b, o, ws_pos.data(), ws_pos.size())) ||
+        (rc = upload_async(c, c->vel_off, c->stage_b, o, vel_off.data(), vel_off.size())) ||
+        (rc = upload_async(c, c->vel_grp, c->stage_b, o, reinterpret_cast<const uint2*>(vel_grp.data()), vel_grp.size())) ||
+        (rc = upload_async(c, c->vel_pos, c->stage_b, o, vel_pos.data(), vel_pos.size())) ||
+        (rc = upload_async(c, c->ws_unit, c->stage_b, o, ws_unit.data(), ws_unit.size())) ||
+        (rc = upload_async(c, c->vel_unit, c->stage_b, o, vel_unit.data(), vel_unit.size())))
+        return rc;
+    return KZG_OK;
+}
+
+// The solve of a step scheduled in generations: one group-kernel solve per
+// generation (warm start, velocity and bias sweeps), in order.
+int solve_generations(kzg_ctx* c, const SolveParams& sp) {
+    KZG_HIP(c, c->b_packed_solver.ensure(8 * size_t(std::max<uint32_t>(c->nbodies, 1))));
+    for (const kzg_ctx::GenPlan& G : c->gens) {
+        if (!G.ws_units && !G.vel_units) continue;
+        launch_unit_desc(c->stream, c->ws_unit.p + G.ws_uoff, c->ws_records().bd, G.ws_units,
+                         c->ws_udesc.p + G.ws_udoff);
+        launch_unit_desc(c->stream, c->vel_unit.p + G.vel_uoff, c->vel_records().bd, G.vel_units,
+                         c->vel_udesc.p + G.vel_udoff);
+        const Schedule ws{c->ws_off.p + G.ws_loff, c->ws_grp.p + G.ws_goff, c->ws_udesc.p + G.ws_udoff};
+        const Schedule vel{c->vel_off.p + G.vel_loff, c->vel_grp.p + G.vel_goff, c->vel_udesc.p + G.vel_udoff};
+        const UnitKernel uk = unit_kernel(G.largest_ws, G.largest_vel);
+        uint32_t threads = 64;
+        while (threads < G.width && threads < solve_threads(uk.ws, uk.vel)) threads *= 2;
+        launch_solve_groups(c->stream, c->vel_records(), c->ws_records(), c->bodies(), c->nbodies,
+                            c->b_packed_solver.p, ws, vel, sp, G.groups, threads, c->resid.p, uk,
+                            kPackBodies | kUnpackBodies);
+        KZG_HIP(c, hipGetLastError());
     }
     return KZG_OK;
 }
@@ -725,7 +840,8 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     KZG_HIP(c, hipEventSynchronize(c->ev[12]));
     // Host time of the build proper starts once the table is here.
     const auto t0 = std::chrono::steady_clock::now();
-    if (c->sleep_model && (rc = check_shared_bodies(c))) return rc;
+    const bool shared = c->sleep_model && shares_bodies(c);
+    c->gens.clear();
     ScheduleInput si;
     si.num_bodies = c->nbodies;
     si.dynamic = c->h_dynamic.data();
@@ -747,6 +863,16 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
                            : c->early_ws_mode == 2 ? ScheduleBuilder::kDeferAlways
                                                    : ScheduleBuilder::kDeferLarge;
     const auto ts = std::chrono::steady_clock::now();
+    if (shared) {
+        c->early_ws = false;
+        c->vel_after_ms = 0.0;
+        if ((rc = schedule_generations(c, si, islands))) return rc;
+        const auto te = std::chrono::steady_clock::now();
+        c->build_host_ms = std::chrono::duration<double, std::milli>(te - t0).count();
+        c->schedule_host_ms = std::chrono::duration<double, std::milli>(te - ts).count();
+        c->phase = 2;
+        return KZG_OK;
+    }
     c->early_ws = c->sched_builder.start(si, c->sched, defer);
     c->vel_after_ms = 0.0;
     // Host compute of the build ends here; what follows stages the
@@ -827,6 +953,10 @@ int kzg_solve(kzg_ctx* c, float dt) {
     KZG_HIP(c, hipMemsetAsync(c->resid.p, 0, sizeof(float) * std::max<uint32_t>(w.velocity_iterations, 1), c->stream));
     SolveParams sp{dt, w.warm_start_factor, w.velocity_iterations, w.position_iterations, w.use_simd_solver,
                    w.use_velocity_bias};
+    if (!c->gens.empty()) {
+        int rc = solve_generations(c, sp);
+        if (rc) return rc;
+    } else {
     KZG_HIP(c, c->ws_udesc.ensure(std::max<uint32_t>(c->sched.ws.units, 1)));
     launch_unit_desc(c->stream, c->ws_unit.p, c->ws_records().bd, c->sched.ws.units, c->ws_udesc.p);
     Schedule ws{c->ws_off.p, c->ws_grp.p, c->ws_udesc.p};
@@ -913,6 +1043,7 @@ int kzg_solve(kzg_ctx* c, float dt) {
                                 early ? kUnpackBodies : kPackBodies | kUnpackBodies);
         }
     }
+    }  // one solve over all islands
     KZG_HIP(c, hipGetLastError());
     if (c->coop_launched) {
         KZG_HIP(c, hipMemcpyAsync(c->pin_totals.as<uint32_t>() + 5, c->coop_sync.p + kSyncGiveUp, sizeof(uint32_t),

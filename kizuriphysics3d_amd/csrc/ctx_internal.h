@@ -189,6 +189,17 @@ struct kzg_ctx {
     Pinned pin_sleep;                // d_sleep after a build; UpdateSleeping's decisions
     bool wake_launched = false;      // this step's build ran the wake-up pass
     std::vector<uint32_t> h_owner, h_orig;  // host scratch: island of each body, the list before UpdateSleeping
+    // Islands that share a dynamic body (a sleeping body reached through
+    // stale indices) are solved in generations, one solve each, in order:
+    // per generation the offsets of its level, group and unit arrays in the
+    // uploaded schedule arrays, its unit count, groups and launch shape.
+    struct GenPlan {
+        uint32_t ws_loff, ws_goff, ws_uoff, ws_udoff, ws_units;
+        uint32_t vel_loff, vel_goff, vel_uoff, vel_udoff, vel_units;
+        uint32_t groups, width, largest_ws, largest_vel;
+    };
+    std::vector<GenPlan> gens;  // empty: one solve over all islands
+    std::vector<uint32_t> h_gen_mask;
 
     // pairs and manifolds
     uint32_t npairs = 0;

@@ -344,6 +344,33 @@ def sleeping_pile(num_bodies: int, seed: int = 0x4B5A0108, late: int = 4, late_h
     return sc
 
 
+def sleeping_field(grid: int = 10, seed: int = 7, half: float = 0.5) -> Scene:
+    """Sleeping on: a grid x grid field of box stacks 1-3 high on a ground
+    plane (1.6 apart), and grid*grid/3 more boxes released one above the
+    other at random spots of the field, 0.9 apart from 6 m up.  The stacks
+    fall asleep within a second and the late boxes wake them one by one,
+    so most of the world sleeps while the rest moves: the case where the
+    reference routes contacts of sleeping bodies through stale island
+    indices."""
+    sc = Scene(f"sleeping_field{grid}", world_settings(8, 2))
+    sc.world["allow_sleeping"] = 1
+    _ground_plane(sc)
+    box = sc.add_shape(SHAPE_BOX, (half, half, half))
+    rng = SplitMix64(seed)
+    for gx in range(grid):
+        for gz in range(grid):
+            for level in range(1 + rng.integer(0, 3)):
+                b = default_body(box)
+                b["position"] = (gx * 1.6 + rng.uniform(-0.02, 0.02), half + 2.0 * half * level,
+                                 gz * 1.6 + rng.uniform(-0.02, 0.02))
+                sc.add_body(b)
+    for k in range(grid * grid // 3):
+        b = default_body(box)
+        b["position"] = (rng.uniform(0.0, (grid - 1) * 1.6), 6.0 + 0.9 * k, rng.uniform(0.0, (grid - 1) * 1.6))
+        sc.add_body(b)
+    return sc
+
+
 def batched_worlds(num_worlds: int = 4096, seed: int = 0x4B5A0004, min_bodies: int = 16,
                    max_bodies: int = 512) -> list:
     """Independent worlds of uneven size: body counts log-uniform in

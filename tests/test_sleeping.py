@@ -188,6 +188,21 @@ def _pyramid_with_late_boxes():
     return sc
 
 
+def _shared_bodies(rw) -> int:
+    """Dynamic bodies the reference's last BuildIslands put in two islands."""
+    hdr, _ = rw.constraints()
+    _, isl = rw.islands()
+    dyn = rw.mass()[:, 0] > 0
+    owner, shared = {}, set()
+    for c in range(len(isl)):
+        if isl[c] == NONE:
+            continue
+        for x in (int(hdr[c, 0]), int(hdr[c, 1])):
+            if dyn[x] and owner.setdefault(x, int(isl[c])) != isl[c]:
+                shared.add(x)
+    return len(shared)
+
+
 def _sleeping_trajectory(scene, steps, **opts):
     from kizuriphysics3d_amd import device as D
     from oracle import refharness as R
@@ -200,11 +215,13 @@ def _sleeping_trajectory(scene, steps, **opts):
     ctx.upload_bodies(D.device_bodies(scene.body_array(), rw.mass()[:, :10]))
     if "cooperative" in opts:
         ctx.set_cooperative(opts["cooperative"])
-    slept = woke = 0
+    slept = woke = shared_steps = 0
     prev = len(scene.bodies)
     for i in range(steps):
         rw.step_phases()
         ctx.step(rw.pairs())
+        if opts.get("count_shared"):
+            shared_steps += _shared_bodies(rw) > 0
         ra, ga = rw.active(), ctx.active_bodies()
         assert np.array_equal(ra, ga), f"step {i}: active lists differ ({len(ra)} vs {len(ga)})"
         rs, gs = rw.states(), ctx.states()
@@ -214,8 +231,8 @@ def _sleeping_trajectory(scene, steps, **opts):
         woke += max(0, len(ra) - prev)
         prev = len(ra)
     print(f"{scene.name}: {steps} steps bit-exact, {slept} sleep / {woke} wake events, "
-          f"{len(scene.bodies) - prev} asleep at the end", flush=True)
-    return slept, woke
+          f"{len(scene.bodies) - prev} asleep at the end, {shared_steps} steps with shared bodies", flush=True)
+    return (slept, woke, shared_steps) if opts.get("count_shared") else (slept, woke)
 
 
 @pytest.mark.gpu
@@ -230,3 +247,73 @@ def test_sleeping_pile_trajectory_bit_exact():
 def test_sleeping_pyramid_trajectory_bit_exact():
     slept, woke = _sleeping_trajectory(_pyramid_with_late_boxes(), 400)
     assert slept > 0 and woke > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.reference
+def test_sleeping_field_islands_sharing_a_body_bit_exact():
+    """The box field: sleeping bodies routed through stale indices put one
+    dynamic body into two islands on some steps (the reference solves those
+    islands one after another); the device solves them in generations."""
+    slept, woke, shared = _sleeping_trajectory(S.sleeping_field(10), 300, count_shared=True)
+    assert slept > 0 and woke > 0 and shared > 0
+
+
+def _world_vs_reference(ours, ref, steps, label):
+    from tests.helpers import first_mismatch
+    ctx = ours.device_context()
+    asleep = 0
+    for i in range(steps):
+        ref.step()
+        ours.step_fixed()
+        ra, ga = ref.active(), ctx.active_bodies()
+        assert np.array_equal(ra, ga), f"{label} step {i}: active lists differ ({len(ra)} vs {len(ga)})"
+        j = first_mismatch(ref.states(), ours.states())
+        assert j is None, f"{label} step {i}: body {j} differs"
+        asleep = max(asleep, ours.num_bodies - len(ga))
+        assert ours.stats()["num_active_bodies"] == len(ga)
+    return asleep
+
+
+@pytest.mark.gpu
+@pytest.mark.reference
+def test_world_with_sleeping_matches_reference():
+    """The whole step through the world layer (its own broadphase tree, the
+    pair query on the device, proxies moved in active-list order as
+    UpdateBroadPhase moves them, PhysicsWorld.cpp:333-345) beside the
+    reference on the sleeping box field."""
+    from kizuriphysics3d_amd.world import PhysicsWorld
+    from oracle import refharness as R
+    scene = S.sleeping_field(8)
+    R.init_jobs(0)
+    asleep = _world_vs_reference(PhysicsWorld(scene), R.RefWorld(scene), 240, "field")
+    assert asleep > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.reference
+def test_kzp1_resume_with_sleeping_bodies():
+    """A KZP1 record saved by the reference while part of the world sleeps:
+    both load it (inactive bodies removed from the active list as the loader
+    creates them, velocities zeroed, Serializer.cpp:230-234) and step on
+    bit-identically, active lists included; the record this library writes
+    back carries the same activity bytes."""
+    from kizuriphysics3d_amd.world import PhysicsWorld
+    from oracle import refharness as R
+    scene = S.sleeping_field(8)
+    R.init_jobs(0)
+    rw = R.RefWorld(scene)
+    for _ in range(90):
+        rw.step()
+    assert len(rw.active()) < len(scene.bodies)
+    rec = rw.save_kzp1()
+    ref2 = R.RefWorld.load_kzp1(rec, scene.world)
+    ours = PhysicsWorld.load(rec, world=scene.world)
+    assert np.array_equal(ref2.active(), ours.device_context().active_bodies())
+    _world_vs_reference(ours, ref2, 120, "resume")
+    # Reloading a record written after the run gives both the same list.
+    rec_ref = ref2.save_kzp1()
+    again = PhysicsWorld.load(rec_ref, world=scene.world)
+    ref3 = R.RefWorld.load_kzp1(rec_ref, scene.world)
+    assert np.array_equal(again.device_context().active_bodies(), ref3.active())
+    assert np.array_equal(np.sort(ref3.active()), np.sort(ref2.active()))
