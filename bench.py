@@ -168,6 +168,36 @@ def learned_report(res, train, state: str) -> dict:
     return out
 
 
+def bench_learned(args):
+    """The learned warm start at its own workload: the random convex pile at
+    65,536 bodies, settled, trained on the rows of its last steps, and the
+    residual curves of every start on one state (learned.study)."""
+    from kizuriphysics3d_amd import learned as Lw
+    bodies = args.bodies if args.bodies != 32768 else 65536
+    clips = tuple(float(x) for x in args.learn_clips.split(","))
+    t0 = time.time()
+    res = Lw.study(bodies=bodies, settle=args.settle, collect=args.learn_collect, train_steps=args.learn_steps,
+                   batch=args.learn_batch, clips=clips, converged_iterations=64, log=log)
+    rows = Lw.study_summary(res)
+    for name, r in rows.items():
+        log(f"{name:22s} natural {np.array2string(np.array(r['natural']), precision=2)} "
+            f"-> cache@8 in {r['sweeps_to_cache8_natural']} (increment: {r['sweeps_to_cache8_increment']})")
+    best = min((r["sweeps_to_cache8_natural"] or 9, name) for name, r in rows.items() if name.startswith("learned_"))
+    return {
+        "metric": "sweeps for the learned warm start to reach the cache warm start's 8-sweep natural residual "
+                  f"({bodies}-body random convex pile)",
+        "value": best[0], "unit": "sweeps", "n_gpus": 1, "steps": 1, "warmup": args.settle,
+        "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 solve, bf16 MLP weights", "data": "synthetic",
+        "config": {"workload": f"learned warm start, random convex pile {bodies} bodies, settled {args.settle} "
+                               f"steps, rows from steps {res['collected_over_steps']}", "best": best[1],
+                   "seconds": time.time() - t0},
+        "targets": res["targets"], "variants": {k: {"loss_first": v["loss_first"], "loss_last": v["loss_last"]}
+                                                for k, v in res["variants"].items()},
+        "curves": rows,
+    }
+
+
 def bench_batched(args):
     """4,096 independent worlds, one captured graph per step."""
     import torch
@@ -389,6 +419,7 @@ def main():
     ap.add_argument("--learn-collect", type=int, default=100, help="last warm-up steps that collect training rows")
     ap.add_argument("--learn-steps", type=int, default=2000, help="Adam steps on the device")
     ap.add_argument("--learn-batch", type=int, default=16384)
+    ap.add_argument("--learn-clips", default="0,8,2", help="learned workload: target clips to train with")
     ap.add_argument("--cooperative", type=int, default=0,
                     help="workgroups sharing the largest island's levels (0 auto, 1 one per group)")
     ap.add_argument("--placement", choices=("one-xcd", "spread"), default="one-xcd",
@@ -396,7 +427,7 @@ def main():
     ap.add_argument("--np-waves", type=int, default=0, help="narrowphase EPA / clipping waves (0: auto)")
     ap.add_argument("--units-vel", type=int, default=0, help="most points per velocity sweep unit (0: default)")
     ap.add_argument("--units-ws", type=int, default=0, help="most points per warm-start/bias sweep unit (0: default)")
-    ap.add_argument("--workload", choices=("pile", "batched", "narrowphase", "copy"), default="pile")
+    ap.add_argument("--workload", choices=("pile", "batched", "narrowphase", "copy", "learned"), default="pile")
     ap.add_argument("--copy-mib", type=int, default=1024, help="copy workload: buffer size")
     ap.add_argument("--worlds", type=int, default=4096, help="batched workload: number of worlds")
     ap.add_argument("--pairs", type=int, default=1 << 20, help="narrowphase workload: number of pairs")
@@ -413,6 +444,9 @@ def main():
         return
     if args.workload == "copy":
         print(json.dumps(bench_copy(args)), flush=True)
+        return
+    if args.workload == "learned":
+        print(json.dumps(bench_learned(args)), flush=True)
         return
 
     import torch  # device synchronisation around the timed region

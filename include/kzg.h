@@ -366,6 +366,11 @@ int kzg_mlp_fit_normalization(kzg_ctx* ctx);
  * rows drawn from the ring by a hash of (seed, step, row); loss = mean
  * squared error over the batch and the 3 outputs; losses[steps] optional. */
 int kzg_mlp_train(kzg_ctx* ctx, uint32_t steps, uint32_t batch, float lr, uint32_t seed, float* losses);
+/* Training targets clamped to [-clip, clip] (units of s) by kzg_mlp_train;
+ * 0 (the default) trains on them as stored.  A few separated pairs whose
+ * clipped manifolds carry impulses of hundreds of s otherwise dominate the
+ * squared error.  No reference counterpart (the learned warm start is new). */
+int kzg_mlp_set_target_clip(kzg_ctx* ctx, float clip);
 /* Inference on feature rows (n x 16 floats): out = (y0, y1, y2, 0) per row. */
 int kzg_mlp_forward(kzg_ctx* ctx, const float* features, uint32_t n, float* out);
 /* Gradient of the mean squared error over sample rows (n x 20 floats) with

@@ -1688,6 +1688,13 @@ int kzg_mlp_fit_normalization(kzg_ctx* c) {
     return kzg_mlp_set_normalization(c, mean, istd);
 }
 
+int kzg_mlp_set_target_clip(kzg_ctx* c, float clip) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (!(clip >= 0.0f) || !std::isfinite(clip)) return fail(c, KZG_ERR_ARGUMENT, "target clip must be finite and >= 0");
+    c->mlp_clip = clip;
+    return KZG_OK;
+}
+
 int kzg_mlp_train(kzg_ctx* c, uint32_t steps, uint32_t batch, float lr, uint32_t seed, float* losses) {
     if (!c) return KZG_ERR_ARGUMENT;
     if (!c->mlp_ready) return fail(c, KZG_ERR_STATE, "no network weights");
@@ -1704,7 +1711,8 @@ int kzg_mlp_train(kzg_ctx* c, uint32_t steps, uint32_t batch, float lr, uint32_t
         ++c->adam_t;
         AdamParams ap{lr, b1, b2, 1e-8f, float(1.0 - std::pow(double(b1), double(c->adam_t))),
                       float(1.0 - std::pow(double(b2), double(c->adam_t))), 1.0f / (3.0f * float(batch)), rows};
-        launch_mlp_grad(c->stream, c->mlp(), c->samples.p, held, batch, 0, seed + 0x9E3779B9u * s, c->mlp_partial.p);
+        launch_mlp_grad(c->stream, c->mlp(), c->samples.p, held, batch, 0, seed + 0x9E3779B9u * s, c->mlp_partial.p,
+                        c->mlp_clip);
         launch_mlp_adam(c->stream, c->mlp(), c->mlp_partial.p, ap, c->mlp_loss.p + s);
     }
     KZG_HIP(c, hipGetLastError());
@@ -1735,7 +1743,7 @@ int kzg_mlp_gradients(kzg_ctx* c, const float* samples, uint32_t n, float* grad,
     if ((rc = upload(c, c->x_samples, reinterpret_cast<const float4*>(samples), 5 * size_t(n)))) return rc;
     const uint32_t rows = mlp_grad_blocks(n);
     KZG_HIP(c, c->mlp_partial.ensure(size_t(rows) * (kMlpParams + 1)));
-    launch_mlp_grad(c->stream, c->mlp(), c->x_samples.p, n, n, 1, 0, c->mlp_partial.p);
+    launch_mlp_grad(c->stream, c->mlp(), c->x_samples.p, n, n, 1, 0, c->mlp_partial.p, c->mlp_clip);
     launch_mlp_reduce(c->stream, c->mlp_partial.p, rows, c->mlp_out.p);
     KZG_HIP(c, hipGetLastError());
     std::vector<float> g(kMlpParams + 1);

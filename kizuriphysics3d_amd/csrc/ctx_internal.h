@@ -278,6 +278,7 @@ struct kzg_ctx {
     DevVec<float> mlp_master, mlp_m, mlp_v, mlp_norm, mlp_partial, mlp_out, mlp_loss;
     DevVec<uint16_t> mlp_w1b, mlp_w2b, mlp_w3b;
     uint32_t adam_t = 0;
+    float mlp_clip = 0.0f;     // kzg_mlp_set_target_clip: 0 none
     DevVec<float4> feat;       // feature rows of the current step, 4 float4 per point
     DevVec<float4> base;       // the cache's clamped impulses of the current step, 1 float4 per point
     DevVec<float4> samples;    // training ring, 5 float4 per sample

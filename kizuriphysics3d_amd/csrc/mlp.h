@@ -69,8 +69,9 @@ void launch_mlp_collect(hipStream_t s, const float4* feat, const float4* base, c
 // gradients (and summed squared error) into partial[blocks][kMlpParams + 1].
 // identity != 0 takes samples 0..batch-1 in order, else hashed indices in [0, n).
 uint32_t mlp_grad_blocks(uint32_t batch);
+// clip > 0 clamps the targets to [-clip, clip] (0: as stored).
 void launch_mlp_grad(hipStream_t s, const MlpBuf& m, const float4* samples, uint32_t n, uint32_t batch,
-                     int identity, uint32_t seed, float* partial);
+                     int identity, uint32_t seed, float* partial, float clip = 0.0f);
 void launch_mlp_reduce(hipStream_t s, const float* partial, uint32_t rows, float* out);
 void launch_mlp_adam(hipStream_t s, const MlpBuf& m, const float* partial, const AdamParams& ap, float* loss_out);
 

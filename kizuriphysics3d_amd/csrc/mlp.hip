@@ -296,6 +296,7 @@ struct GradArgs {
     uint32_t n, batch, tpb, seed;
     int identity;
     float* partial;         // [gridDim.x][kMlpParams + 1]
+    float clip;             // > 0: targets clamped to [-clip, clip]
 };
 
 __global__ void __launch_bounds__(256) k_mlp_grad(MlpBuf m, GradArgs ga) {
@@ -328,7 +329,8 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpBuf m, GradArgs ga) {
         const float4 tg = valid ? ga.samples[size_t(idx) * 5 + 4] : z4;
         Act a;
         mlp_forward(L.w, xin, lane, a);
-        const float tv[4] = {tg.x, tg.y, tg.z, 0.0f};
+        auto cl = [&](float t) { return ga.clip > 0.0f ? fminf(fmaxf(t, -ga.clip), ga.clip) : t; };
+        const float tv[4] = {cl(tg.x), cl(tg.y), cl(tg.z), 0.0f};
         float dy[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -502,9 +504,9 @@ static uint32_t grad_tpb(uint32_t batch) { return std::max<uint32_t>(4, cdiv(cdi
 uint32_t mlp_grad_blocks(uint32_t batch) { return cdiv(cdiv(batch, 16), grad_tpb(batch)); }
 
 void launch_mlp_grad(hipStream_t s, const MlpBuf& m, const float4* samples, uint32_t n, uint32_t batch,
-                     int identity, uint32_t seed, float* partial) {
+                     int identity, uint32_t seed, float* partial, float clip) {
     const uint32_t tpb = grad_tpb(batch);
-    GradArgs ga{samples, n, batch, tpb, seed, identity, partial};
+    GradArgs ga{samples, n, batch, tpb, seed, identity, partial, clip};
     k_mlp_grad<<<mlp_grad_blocks(batch), 256, 0, s>>>(m, ga);
 }
 

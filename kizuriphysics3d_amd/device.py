@@ -45,7 +45,7 @@ EXPORTS = [
     "kzg_set_iterations", "kzg_set_cooperative", "kzg_set_sweep_units", "kzg_set_early_warm_start", "kzg_set_narrowphase_waves", "kzg_set_island_lds_limit", "kzg_set_cooperative_placement", "kzg_set_spin_limit", "kzg_cache_info", "kzg_download_cache", "kzg_upload_cache",
     "kzg_mlp_init", "kzg_mlp_set_params", "kzg_mlp_get_params", "kzg_mlp_set_normalization",
     "kzg_mlp_get_normalization", "kzg_mlp_collect", "kzg_mlp_num_samples", "kzg_mlp_get_samples",
-    "kzg_mlp_fit_normalization", "kzg_mlp_train", "kzg_mlp_forward", "kzg_mlp_gradients",
+    "kzg_mlp_fit_normalization", "kzg_mlp_train", "kzg_mlp_set_target_clip", "kzg_mlp_forward", "kzg_mlp_gradients",
 ]
 
 # Warm-start modes (kzg_warm_start in include/kzg.h).
@@ -102,7 +102,7 @@ def library():
             "kzg_mlp_get_params": ([vp, vp, u32], i32), "kzg_mlp_set_normalization": ([vp, vp, vp], i32),
             "kzg_mlp_get_normalization": ([vp, vp, vp], i32), "kzg_mlp_collect": ([vp, u32], i32),
             "kzg_mlp_num_samples": ([vp, vp, vp], i32), "kzg_mlp_get_samples": ([vp, u32, u32, vp], i32),
-            "kzg_mlp_fit_normalization": ([vp], i32), "kzg_mlp_train": ([vp, u32, u32, f32, u32, vp], i32),
+            "kzg_mlp_fit_normalization": ([vp], i32), "kzg_mlp_set_target_clip": ([vp, f32], i32), "kzg_mlp_train": ([vp, u32, u32, f32, u32, vp], i32),
             "kzg_mlp_forward": ([vp, vp, u32, vp], i32), "kzg_mlp_gradients": ([vp, vp, u32, vp, vp], i32),
         }
         for name, (args, res) in sig.items():
@@ -405,6 +405,22 @@ class DeviceContext:
         self._check(library().kzg_mlp_train(self.h, int(steps), int(batch), float(lr), int(seed) & 0xFFFFFFFF,
                                             _p(losses)), "kzg_mlp_train")
         return losses[:steps]
+
+    def cache_image(self):
+        """(image bytes, capacity, used) of the warm-start cache."""
+        cap, used = C.c_uint32(), C.c_uint32()
+        self._check(library().kzg_cache_info(self.h, C.byref(cap), C.byref(used)), "kzg_cache_info")
+        img = np.zeros(max(cap.value, 1) * 108, np.uint8)
+        self._check(library().kzg_download_cache(self.h, _p(img), cap.value * 108), "kzg_download_cache")
+        return img[:cap.value * 108], cap.value, used.value
+
+    def set_cache_image(self, image):
+        img, cap, used = image
+        self._check(library().kzg_upload_cache(self.h, _p(img), len(img), cap, used), "kzg_upload_cache")
+
+    def mlp_set_target_clip(self, clip: float = 0.0):
+        """Clamp training targets to [-clip, clip] (0: off) in mlp_train and mlp_gradients."""
+        self._check(library().kzg_mlp_set_target_clip(self.h, float(clip)), "kzg_mlp_set_target_clip")
 
     def mlp_forward(self, features) -> np.ndarray:
         f = np.ascontiguousarray(features, np.float32).reshape(-1, MLP_FEATURE_ROW)

@@ -42,7 +42,7 @@ def penetration(ctx: D.DeviceContext, pairs: np.ndarray) -> dict:
 
 def compare_warm_starts(world, modes=(D.WARM_COLD, D.WARM_CACHE, D.WARM_LEARNED, D.WARM_LEARNED_MISS),
                         iterations: int = 8, position_iterations: int = 2, penetration_iterations=(),
-                        natural: bool = True, dt: float = S.DT) -> dict:
+                        natural: bool = True, dt: float = S.DT, converged_iterations: int = 0) -> dict:
     """From the world's current state, one contact-path step per warm-start
     mode (the cache is left as it is), each restarting from the same body
     states.  Returns per mode the sweep-increment residual after k =
@@ -51,6 +51,10 @@ def compare_warm_starts(world, modes=(D.WARM_COLD, D.WARM_CACHE, D.WARM_LEARNED,
     velocity iterations each, kzg_get_natural_residual), and, for each k in
     penetration_iterations, the penetration statistics after a step solved
     with k velocity iterations (measured over the step's broadphase pairs).
+    With converged_iterations > 0 a last entry, "converged", starts from the
+    impulses a solve of the same step with that many sweeps reaches: the
+    bound of any warm start that predicts each point's impulses (the cache
+    is swapped for one holding those impulses and restored after).
     The world's state is restored afterwards."""
     ctx = world.device_context()
     snap = world.states()
@@ -91,6 +95,39 @@ def compare_warm_starts(world, modes=(D.WARM_COLD, D.WARM_CACHE, D.WARM_LEARNED,
                 pen[int(k)] = penetration(ctx, pairs)
             res["penetration"] = pen
             out[MODE_NAMES.get(mode, str(mode))] = res
+        if converged_iterations:
+            image = ctx.cache_image()
+            try:
+                ctx.set_iterations(converged_iterations, position_iterations)
+                ctx.upload_states(snap)
+                ctx.narrowphase(pairs)
+                ctx.build_constraints(D.WARM_CACHE)  # stores this step's converged impulses
+                ctx.integrate_velocities(dt)
+                ctx.solve(dt)
+                res = {}
+                ctx.set_iterations(iterations, position_iterations)
+                ctx.upload_states(snap)
+                ctx.narrowphase(pairs)
+                ctx.build_constraints(D.WARM_CACHE | D.WARM_NO_STORE)
+                ctx.integrate_velocities(dt)
+                ctx.solve(dt)
+                res["residual"] = ctx.residuals()[:iterations]
+                res["points"] = ctx.sizes()[2]
+                if natural:
+                    nat = []
+                    for k in range(1, iterations + 1):
+                        ctx.set_iterations(k, position_iterations)
+                        ctx.upload_states(snap)
+                        ctx.narrowphase(pairs)
+                        ctx.build_constraints(D.WARM_CACHE | D.WARM_NO_STORE)
+                        ctx.integrate_velocities(dt)
+                        ctx.solve(dt)
+                        nat.append(ctx.natural_residual())
+                    res["natural"] = nat
+                res["penetration"] = {}
+                out["converged"] = res
+            finally:
+                ctx.set_cache_image(image)
     finally:
         ctx.set_iterations(world_iterations(world)[0], world_iterations(world)[1])
         ctx.upload_states(snap)
@@ -109,7 +146,7 @@ def iterations_to_reach(curve: np.ndarray, target: float):
 
 
 def train_on_world(world, collect_steps: int, train_steps: int, batch: int = 4096, lr: float = 1e-3,
-                   capacity: int = 1 << 21, seed: int = 0x4B5A0005, init: bool = True) -> np.ndarray:
+                   capacity: int = 1 << 21, seed: int = 0x4B5A0005, init: bool = True, clip: float = 0.0) -> np.ndarray:
     """Step the world `collect_steps` times with the cache warm start while
     collecting (features, final impulses) per contact point, fit the feature
     normalisation, then run `train_steps` Adam steps on the device.  Returns
@@ -121,6 +158,77 @@ def train_on_world(world, collect_steps: int, train_steps: int, batch: int = 409
     for _ in range(collect_steps):
         world.step_fixed()
     ctx.mlp_fit_normalization()
+    ctx.mlp_set_target_clip(clip)
     losses = ctx.mlp_train(train_steps, batch=batch, lr=lr, seed=seed)
     ctx.mlp_collect(0)
     return losses
+
+
+def study(bodies: int = 65536, settle: int = 600, collect: int = 100, train_steps: int = 3000,
+          batch: int = 16384, lr: float = 1e-3, clips=(0.0, 8.0, 2.0), converged_iterations: int = 64,
+          seed: int = 0x4B5A0005, log=print) -> dict:
+    """The learned warm start on the random convex pile of `bodies` bodies:
+    settle it (the last `collect` steps collect training rows), then, on the
+    same state, the residual curves of the cold start, the cache, a start from
+    the impulses of a `converged_iterations`-sweep solve (the bound of a
+    per-point predictor), and the network trained from one initialisation
+    with each target clip in `clips` (learned everywhere, and on cache misses
+    only)."""
+    from .world import PhysicsWorld
+    w = PhysicsWorld(S.pile(bodies))
+    ctx = w.device_context()
+    ctx.mlp_init(seed)
+    for i in range(settle):
+        if i == max(settle - collect, 0):
+            ctx.mlp_collect(1 << 24)
+        w.step_fixed()
+        if (i + 1) % 100 == 0:
+            st = w.stats()
+            log(f"settle {i + 1}/{settle}: {st['num_contact_points']} points, step {st['step_ms']:.1f} ms")
+    _, held = ctx.mlp_num_samples()
+    rows = ctx.mlp_samples(0, min(int(held), 1 << 20))
+    tg = rows[:, 16:19]
+    target_stats = {"rows": int(held), "abs_p50": float(np.percentile(np.abs(tg), 50)),
+                    "abs_p99": float(np.percentile(np.abs(tg), 99)), "abs_max": float(np.abs(tg).max()),
+                    "hit_fraction": float(rows[:, 13].mean())}
+    log(f"training rows: {target_stats}")
+    ctx.mlp_fit_normalization()
+    norm = ctx.mlp_normalization()
+    out = {"bodies": bodies, "settle": settle, "collected_over_steps": f"{settle - collect + 1}-{settle}",
+           "targets": target_stats}
+    base = compare_warm_starts(w, modes=(D.WARM_COLD, D.WARM_CACHE), converged_iterations=converged_iterations)
+    out["cold"], out["cache"], out["converged"] = base["cold"], base["cache"], base["converged"]
+    out["variants"] = {}
+    for clip in clips:
+        ctx.mlp_init(seed)
+        ctx.mlp_set_normalization(*norm)
+        ctx.mlp_set_target_clip(clip)
+        losses = ctx.mlp_train(train_steps, batch=batch, lr=lr, seed=seed)
+        r = compare_warm_starts(w, modes=(D.WARM_LEARNED, D.WARM_LEARNED_MISS))
+        out["variants"][f"clip{clip:g}"] = {"clip": clip, "loss_first": float(losses[:10].mean()),
+                                           "loss_last": float(losses[-100:].mean()), **r}
+        log(f"clip {clip:g}: loss {losses[:10].mean():.3f} -> {losses[-100:].mean():.3f}")
+    ctx.mlp_collect(0)
+    w.close()
+    return out
+
+
+def study_summary(res: dict) -> dict:
+    """Per start: increment and natural curves, and the sweeps each takes to
+    reach the cache's 8-sweep value of both."""
+    cache_inc = float(res["cache"]["residual"][-1])
+    cache_nat = float(res["cache"]["natural"][-1]["total"])
+    rows = {}
+
+    def add(name, r):
+        nat = np.array([x["total"] for x in r["natural"]])
+        rows[name] = {"increment": [float(x) for x in r["residual"]], "natural": nat.tolist(),
+                      "sweeps_to_cache8_increment": iterations_to_reach(r["residual"], cache_inc),
+                      "sweeps_to_cache8_natural": iterations_to_reach(nat, cache_nat),
+                      "natural_terms_k8": r["natural"][-1]}
+    for name in ("cold", "cache", "converged"):
+        add(name, res[name])
+    for vname, v in res["variants"].items():
+        add(f"learned_{vname}", v["learned"])
+        add(f"learned_miss_{vname}", v["learned_miss"])
+    return rows

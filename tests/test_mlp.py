@@ -166,6 +166,31 @@ def test_mlp_gradients_match_oracle():
     assert np.linalg.norm(g - gw) / np.linalg.norm(gw) > 0.05
 
 
+@pytest.mark.gpu
+def test_mlp_gradients_with_target_clip():
+    """kzg_mlp_set_target_clip: the device gradient equals the oracle's on
+    targets clamped to [-clip, clip], and differs from the unclamped one."""
+    p, mean, istd, s = _random_problem(1000, 13, scale=0.2)
+    s[:, 16:19] *= 5.0
+    clip = 2.0
+    assert (np.abs(s[:, 16:19]) > clip).mean() > 0.3
+    ctx = _ctx()
+    ctx.mlp_set_params(p)
+    ctx.mlp_set_normalization(mean, istd)
+    ctx.mlp_set_target_clip(clip)
+    g, loss = ctx.mlp_gradients(s)
+    sc = s.copy()
+    sc[:, 16:19] = np.clip(sc[:, 16:19], -clip, clip)
+    lo, go = M.loss_and_grad(p, mean, istd, sc, emulate=True)
+    assert loss == pytest.approx(lo, rel=1e-4)
+    assert np.linalg.norm(g - go) / np.linalg.norm(go) <= 1e-3
+    _, gu = M.loss_and_grad(p, mean, istd, s, emulate=True)
+    assert np.linalg.norm(g - gu) / np.linalg.norm(gu) > 0.05
+    ctx.mlp_set_target_clip(0.0)
+    g0, _ = ctx.mlp_gradients(s)
+    assert np.linalg.norm(g0 - gu) / np.linalg.norm(gu) <= 1e-3
+
+
 def _small_pile(n=1024, seed=0x4B5A0510, y_range=(1.0, 16.0)):
     from kizuriphysics3d_amd.world import PhysicsWorld
     return PhysicsWorld(S.pile(n, seed=seed, width=12.0, y_range=y_range))
