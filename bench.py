@@ -541,7 +541,9 @@ def main():
     # island's group when it ran, else the per-group sweep kernel (the span
     # then also holds the body pack/unpack kernels).
     if coop_ms > 0:
-        rl_kernel, achieved = "k_solve_coop", coop_bytes / (coop_ms / 1e3) / 1e9
+        # One-XCD placement launches the cooperative sweep as k_solve_xcd.
+        rl_kernel = "k_solve_xcd" if args.placement == "one-xcd" else "k_solve_coop"
+        achieved = coop_bytes / (coop_ms / 1e3) / 1e9
     else:
         rl_kernel, achieved = "k_solve_groups", solve_bytes / (solve_kernel_ms / 1e3) / 1e9
     lv = np.array(levels)
@@ -608,7 +610,7 @@ def main():
                    "contacts_per_s_with_pcie": pcie_value},
         "roofline": {"bound": "hbm", "kernel": rl_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": pmc_traffic("pile32k_k_solve_coop_pmc.md") if rl_kernel == "k_solve_coop" else None,
+                     "traffic": pmc_traffic(f"pile32k_{rl_kernel}_pmc.md") if coop_ms > 0 else None,
                      # Per step: with the early warm start the sweep is two cooperative launches (the
                      # warm start, then the velocity iterations with the bias alongside), and
                      # `achieved` and `traffic` cover both.
