@@ -430,8 +430,11 @@ bool BroadPhase::restore(const uint32_t* in, size_t n) {
     return true;
 }
 
-void BroadPhase::image(float* out) const {
-    for (const Node& n : nodes_) {
+void BroadPhase::image(float* out) const { image(out, 0, uint32_t(nodes_.size())); }
+
+void BroadPhase::image(float* out, uint32_t first, uint32_t end) const {
+    for (uint32_t k = first; k < end; ++k) {
+        const Node& n = nodes_[k];
         out[0] = n.fat.mn.x; out[1] = n.fat.mn.y; out[2] = n.fat.mn.z; out[3] = bitsf(n.child1);
         out[4] = n.fat.mx.x; out[5] = n.fat.mx.y; out[6] = n.fat.mx.z; out[7] = bitsf(n.child2);
         out[8] = n.tight.mn.x; out[9] = n.tight.mn.y; out[10] = n.tight.mn.z; out[11] = bitsf(n.user);
@@ -464,14 +467,73 @@ uint32_t BroadPhase::create_proxy(const Aabb& tight, uint32_t user, uint32_t lay
 }
 
 bool BroadPhase::move_proxy(uint32_t id, const Aabb& tight, f3 disp) {
+    if (set_tight(id, tight)) return false;
+    reinsert(id, disp);
+    return true;
+}
+
+bool BroadPhase::set_tight(uint32_t id, const Aabb& tight) {
     nodes_[id].tight = tight;
-    if (contains(nodes_[id].fat, tight)) return false;
+    return contains(nodes_[id].fat, tight);
+}
+
+void BroadPhase::reinsert(uint32_t id, f3 disp) {
     remove_leaf(id);
+    const Aabb tight = nodes_[id].tight;
     const f3 m{0.1f, 0.1f, 0.1f};
     const f3 d = vabs(disp);
     nodes_[id].fat = Aabb{sub(sub(tight.mn, m), d), add(add(tight.mx, m), d)};
     insert_leaf(id);
-    return true;
+}
+
+HostPool::HostPool(unsigned workers) {
+    for (unsigned k = 0; k < workers; ++k) threads_.emplace_back([this, k] { loop(k + 1); });
+}
+
+HostPool::~HostPool() {
+    {
+        std::lock_guard<std::mutex> lk(m_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : threads_) t.join();
+}
+
+void HostPool::dispatch(std::function<void(uint32_t)>* job, uint32_t workers) {
+    {
+        std::lock_guard<std::mutex> lk(m_);
+        job_ = job;
+        pending_ = workers;
+        ++gen_;
+    }
+    cv_.notify_all();
+}
+
+void HostPool::wait() {
+    std::unique_lock<std::mutex> lk(m_);
+    done_.wait(lk, [this] { return pending_ == 0; });
+    job_ = nullptr;
+}
+
+// Every worker runs every job once: run() returns only when all have, so a
+// worker never misses a generation.
+void HostPool::loop(uint32_t k) {
+    uint64_t seen = 0;
+    for (;;) {
+        std::function<void(uint32_t)>* job;
+        {
+            std::unique_lock<std::mutex> lk(m_);
+            cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            job = job_;
+        }
+        (*job)(k);
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
 }
 
 void BroadPhase::insert_leaf(uint32_t leaf) {

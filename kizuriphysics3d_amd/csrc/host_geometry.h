@@ -8,6 +8,10 @@
 
 #include <stdint.h>
 
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "kz_math.h"
@@ -56,6 +60,12 @@ class BroadPhase {
 public:
     uint32_t create_proxy(const Aabb& tight, uint32_t user, uint32_t layer, uint32_t mask);
     bool move_proxy(uint32_t id, const Aabb& tight, f3 displacement);
+    // move_proxy in two halves: set_tight stores the leaf's tight box and
+    // tells whether its fat box still holds it (nothing else to do); else
+    // reinsert gives the leaf its new fat box and puts it back in the tree.
+    // set_tight touches only that leaf, so it may run for many leaves at once.
+    bool set_tight(uint32_t id, const Aabb& tight);
+    void reinsert(uint32_t id, f3 displacement);
     void update_pairs();
     const std::vector<uint32_t>& pairs() const { return pairs_; }  // proxy A, proxy B, ...
     uint32_t user(uint32_t id) const { return nodes_[id].user; }
@@ -65,6 +75,7 @@ public:
     // box + child2, tight box + user, tight box + height (ints as float bits),
     // 16 words; the root; the proxy layers and masks by node id.
     void image(float* nodes16) const;
+    void image(float* nodes16, uint32_t first, uint32_t end) const;  // nodes [first, end) at nodes16
     uint32_t num_nodes() const { return uint32_t(nodes_.size()); }
     uint32_t root() const { return root_; }
     const std::vector<uint32_t>& layers() const { return layers_; }
@@ -99,6 +110,42 @@ private:
     static constexpr size_t kParallelLeaves = 8192;  // smaller trees are walked on the calling thread
     static constexpr size_t kSplitEntries = 4096;    // entries handed to the threads
     void query_step(std::vector<Entry>& stack, std::vector<uint32_t>& out) const;
+};
+
+// A few persistent host threads for per-body loops: run(n, f) calls
+// f(begin, end) over contiguous chunks of [0, n), one chunk on the calling
+// thread, and returns when all are done.
+class HostPool {
+public:
+    explicit HostPool(unsigned workers);
+    ~HostPool();
+    HostPool(const HostPool&) = delete;
+    HostPool& operator=(const HostPool&) = delete;
+    template <class F> void run(uint32_t n, F&& f) {
+        const uint32_t parts = uint32_t(threads_.size()) + 1;
+        if (parts == 1 || n < 4096) {
+            f(0u, n);
+            return;
+        }
+        std::function<void(uint32_t)> job = [&f, n, parts](uint32_t k) {
+            f(uint32_t(uint64_t(n) * k / parts), uint32_t(uint64_t(n) * (k + 1) / parts));
+        };
+        dispatch(&job, parts - 1);
+        job(0);
+        wait();
+    }
+
+private:
+    void dispatch(std::function<void(uint32_t)>* job, uint32_t workers);
+    void wait();
+    void loop(uint32_t k);
+    std::vector<std::thread> threads_;
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    std::function<void(uint32_t)>* job_ = nullptr;
+    uint32_t pending_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
 };
 
 }  // namespace kz

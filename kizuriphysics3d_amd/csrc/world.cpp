@@ -1,9 +1,12 @@
 // world.cpp - the host world: bodies, shapes, broadphase, and the per-step
 // orchestration of PhysicsWorld::StepFixed (src/World/PhysicsWorld.cpp:245-329)
 // with steps 2-6 on the device.  Flat C-ABI in include/kzw.h.
+#include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "host_geometry.h"
@@ -37,6 +40,9 @@ struct kzw_world {
     bool shapes_dirty = true, bodies_dirty = true;
     kzw_stats stats{};
     std::vector<uint32_t> active;  // the context's active list (BodyManager's), read each step
+    std::vector<uint8_t> moved;    // per active-list entry: left its fat box
+    std::unique_ptr<HostPool> pool;  // host threads for the per-body loops of the broadphase update
+    int host_threads = 1;            // kzw_set_host_threads
     // Bodies the last loaded snapshot marks inactive: the list the body
     // upload sets is then every other body in slot order.
     std::vector<uint8_t> load_inactive;
@@ -207,8 +213,17 @@ int kzw_add_bodies(kzw_world* w, const kz_body_desc* b, uint32_t n) {
 }
 
 namespace {
+HostPool& pool(kzw_world* w) {
+    if (!w->pool) w->pool = std::make_unique<HostPool>(unsigned(std::max(w->host_threads, 1)) - 1u);
+    return *w->pool;
+}
+
 // UpdateBroadPhase's proxy moves (PhysicsWorld.cpp:334-345): the active
 // list's non-static bodies in list order, displacement = v * fixedTimestep.
+// A move that stays inside the leaf's fat box only stores the tight box and
+// touches nothing another move reads, so those run on host threads; the
+// leaves that left their fat boxes are then re-inserted one after another
+// in list order, which is the order the tree sees in the reference.
 int move_proxies(kzw_world* w) {
     const uint32_t nb = uint32_t(w->bodies.size());
     w->active.resize(nb);
@@ -219,13 +234,24 @@ int move_proxies(kzw_world* w) {
     } else {
         for (uint32_t i = 0; i < nb; ++i) w->active[i] = i;
     }
+    w->moved.resize(na);
+    pool(w).run(na, [w](uint32_t b, uint32_t e) {
+        for (uint32_t k = b; k < e; ++k) {
+            const uint32_t i = w->active[k];
+            if (w->bodies[i].motion_type == KZ_MOTION_STATIC) {
+                w->moved[k] = 0;
+                continue;
+            }
+            w->moved[k] = w->bp.set_tight(w->proxy[i], world_bounds(*w, i)) ? 0 : 1;
+        }
+    });
     for (uint32_t k = 0; k < na; ++k) {
+        if (!w->moved[k]) continue;
         const uint32_t i = w->active[k];
-        if (w->bodies[i].motion_type == KZ_MOTION_STATIC) continue;
         const kz_body_state& s = w->states[i];
         const f3 disp = mul(f3{s.linear_velocity[0], s.linear_velocity[1], s.linear_velocity[2]},
                             w->settings.fixed_timestep);
-        w->bp.move_proxy(w->proxy[i], world_bounds(*w, i), disp);
+        w->bp.reinsert(w->proxy[i], disp);
     }
     return KZG_OK;
 }
@@ -233,7 +259,8 @@ int move_proxies(kzw_world* w) {
 int device_pairs(kzw_world* w) {
     const uint32_t nn = w->bp.num_nodes();
     w->bp_image.resize(16 * size_t(nn));
-    w->bp.image(w->bp_image.data());
+    float* img = w->bp_image.data();
+    pool(w).run(nn, [w, img](uint32_t b, uint32_t e) { w->bp.image(img + 16 * size_t(b), b, e); });
     const auto& lay = w->bp.layers();
     const auto& msk = w->bp.masks();
     uint32_t np = 0;
@@ -260,6 +287,13 @@ int kzw_update_broadphase(kzw_world* w) {
     int rc = move_proxies(w);
     if (rc) return rc;
     host_pairs(w);
+    return KZG_OK;
+}
+
+int kzw_set_host_threads(kzw_world* w, int threads) {
+    if (!w || threads < 1 || threads > 256) return KZG_ERR_ARGUMENT;
+    w->host_threads = threads;
+    w->pool.reset();
     return KZG_OK;
 }
 

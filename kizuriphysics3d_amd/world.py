@@ -40,7 +40,7 @@ def _lib():
             "kzw_get_mass": ([vp, vp, u32], i32), "kzw_get_hull": ([vp, u32, vp, vp, vp, vp], i32),
             "kzw_device": ([vp], vp),
             "kzw_save": ([vp, vp, C.c_uint64, vp], i32), "kzw_load": ([vp, vp, C.c_uint64, vp, i32], i32),
-            "kzw_set_device_broadphase": ([vp, i32], i32),
+            "kzw_set_device_broadphase": ([vp, i32], i32), "kzw_set_host_threads": ([vp, i32], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -52,7 +52,8 @@ def _lib():
 
 KZW_EXPORTS = ["kzw_create", "kzw_destroy", "kzw_last_error", "kzw_add_shapes", "kzw_add_bodies", "kzw_step",
                "kzw_update_broadphase", "kzw_get_states", "kzw_set_states", "kzw_get_pairs", "kzw_get_stats",
-               "kzw_get_mass", "kzw_get_hull", "kzw_device", "kzw_save", "kzw_load", "kzw_set_device_broadphase"]
+               "kzw_get_mass", "kzw_get_hull", "kzw_device", "kzw_save", "kzw_load", "kzw_set_device_broadphase",
+               "kzw_set_host_threads"]
 
 
 def _p(a):
@@ -129,6 +130,9 @@ class PhysicsWorld:
 
     def step_fixed(self, dt=S.DT):
         self._check(_lib().kzw_step(self.h, float(dt)), "kzw_step")
+
+    def set_host_threads(self, threads: int = 1):
+        self._check(_lib().kzw_set_host_threads(self.h, int(threads)), "kzw_set_host_threads")
 
     def set_device_broadphase(self, on: bool):
         """step_fixed's pair query on the device (default) or on host threads."""

@@ -111,3 +111,21 @@ def test_large_broadphase_pair_order_matches_the_reference():
     rp, op = rw.pairs(), ours.pairs()
     assert len(rp) > 10000
     assert np.array_equal(rp, op)
+
+
+def test_broadphase_update_is_the_same_on_any_number_of_host_threads():
+    """kzw_set_host_threads: the proxies' tight boxes and fat-box tests run
+    on host threads, the re-insertions in list order, so the tree and the
+    pair list are the same as on one thread, step after step."""
+    sc = S.pile(6000, seed=0x4B5A0111, width=24.0, y_range=(0.5, 12.0))
+    worlds = [PhysicsWorld(sc, device=-1) for _ in range(2)]
+    worlds[1].set_host_threads(4)
+    st = worlds[0].states()
+    rng = np.random.default_rng(3)
+    for step in range(8):
+        st["position"] += rng.normal(0.0, 0.05, st["position"].shape).astype(np.float32)
+        for w in worlds:
+            w.set_states(st)
+            w.update_broadphase()
+        a, b = worlds[0].pairs(), worlds[1].pairs()
+        assert len(a) > 1000 and np.array_equal(a, b), f"update {step}: pair lists differ"
