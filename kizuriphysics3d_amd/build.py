@@ -5,9 +5,6 @@ Compile flags that matter for results:
   (no -ffast-math)       IEEE semantics, correctly rounded division and sqrt
   -Xclang -target-feature -Xclang -packed-fp32-ops
                          no packed FP32 instructions
-  -Xarch_host -march=x86-64-v3 (host-only sources)
-                         AVX2 vectors for the host loops; with contraction off
-                         and no reassociation every result is the same
 """
 from __future__ import annotations
 
@@ -34,9 +31,7 @@ def _compile(src: str) -> str:
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
     cmd = ["hipcc", "-c"] + FLAGS
     if src.endswith(".cpp"):
-        # Host-only sources: AVX2 for the schedule's and the broadphase's
-        # loops (no FMA contraction, no reassociation: the same results).
-        cmd += ["-x", "hip", "-Xarch_host", "-march=x86-64-v3"]
+        cmd += ["-x", "hip"]
     cmd += [os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
