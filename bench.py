@@ -177,7 +177,7 @@ def bench_learned(args):
     clips = tuple(float(x) for x in args.learn_clips.split(","))
     t0 = time.time()
     res = Lw.study(bodies=bodies, settle=args.settle, collect=args.learn_collect, train_steps=args.learn_steps,
-                   batch=args.learn_batch, clips=clips, converged_iterations=64, log=log)
+                   batch=args.learn_batch, clips=clips, converged_iterations=64, targets=args.learn_targets, log=log)
     rows = Lw.study_summary(res)
     for name, r in rows.items():
         log(f"{name:22s} natural {np.array2string(np.array(r['natural']), precision=2)} "
@@ -190,7 +190,8 @@ def bench_learned(args):
         "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 solve, bf16 MLP weights", "data": "synthetic",
         "config": {"workload": f"learned warm start, random convex pile {bodies} bodies, settled {args.settle} "
-                               f"steps, rows from steps {res['collected_over_steps']}", "best": best[1],
+                               f"steps, rows from steps {res['collected_over_steps']} ({args.learn_targets} targets)",
+                   "best": best[1],
                    "seconds": time.time() - t0},
         "targets": res["targets"], "variants": {k: {"loss_first": v["loss_first"], "loss_last": v["loss_last"]}
                                                 for k, v in res["variants"].items()},
@@ -420,6 +421,8 @@ def main():
     ap.add_argument("--learn-steps", type=int, default=2000, help="Adam steps on the device")
     ap.add_argument("--learn-batch", type=int, default=16384)
     ap.add_argument("--learn-clips", default="0,8,2", help="learned workload: target clips to train with")
+    ap.add_argument("--learn-targets", choices=("solver", "converged"), default="solver",
+                    help="learned workload: train on the step's own impulses or on a 64-sweep side solve's")
     ap.add_argument("--cooperative", type=int, default=0,
                     help="workgroups sharing the largest island's levels (0 auto, 1 one per group)")
     ap.add_argument("--placement", choices=("one-xcd", "spread"), default="one-xcd",

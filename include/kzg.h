@@ -265,13 +265,15 @@ int kzg_collide_batch_device(kzg_ctx* ctx, const uint32_t* shape_a, const uint32
  * match.  The sums are clamped as Prepare clamps cached impulses
  * (ContactSolver.cpp:107-110).  KZG_WARM_NO_STORE, or-ed into a mode,
  * makes the following kzg_solve leave the cache untouched (for comparing
- * modes on one state). */
+ * modes on one state); KZG_WARM_NO_COLLECT makes it append no training rows
+ * while collection is on (kzg_mlp_collect). */
 enum kzg_warm_start {
     KZG_WARM_CACHE = 0,
     KZG_WARM_COLD = 1,
     KZG_WARM_LEARNED = 2,
     KZG_WARM_LEARNED_MISS = 3,
-    KZG_WARM_NO_STORE = 0x100
+    KZG_WARM_NO_STORE = 0x100,
+    KZG_WARM_NO_COLLECT = 0x200
 };
 /* Mode kzg_step uses (default KZG_WARM_CACHE). */
 int kzg_set_warm_start(kzg_ctx* ctx, int mode);

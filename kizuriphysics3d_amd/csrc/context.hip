@@ -788,12 +788,13 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
     if (!c) return KZG_ERR_ARGUMENT;
     if (c->phase != 1) return fail(c, KZG_ERR_STATE, "kzg_build_constraints needs a preceding kzg_narrowphase");
     const int mode = warm & 0xFF;
-    if ((warm & ~(0xFF | KZG_WARM_NO_STORE)) || mode > KZG_WARM_LEARNED_MISS)
+    if ((warm & ~(0xFF | KZG_WARM_NO_STORE | KZG_WARM_NO_COLLECT)) || mode > KZG_WARM_LEARNED_MISS)
         return fail(c, KZG_ERR_ARGUMENT, "unknown warm-start mode " + std::to_string(warm));
     if ((mode == KZG_WARM_LEARNED || mode == KZG_WARM_LEARNED_MISS) && !c->mlp_ready)
         return fail(c, KZG_ERR_STATE, "learned warm start needs network weights (kzg_mlp_init or kzg_mlp_set_params)");
     c->cur_mode = mode;
     c->no_store = (warm & KZG_WARM_NO_STORE) != 0;
+    c->no_collect = (warm & KZG_WARM_NO_COLLECT) != 0;
     join_schedule(c);
     hipSetDevice(c->device);
     // Constraint and point totals and the cache occupancy (the wait is the
@@ -932,7 +933,8 @@ int kzg_solve(kzg_ctx* c, float dt) {
     const kz_world_desc& w = c->world;
     const bool learned = c->cur_mode == KZG_WARM_LEARNED || c->cur_mode == KZG_WARM_LEARNED_MISS;
     float4 *feat = nullptr, *base = nullptr;
-    if ((learned || c->sample_cap) && c->npoints) {
+    const bool collect = c->sample_cap && !c->no_collect;
+    if ((learned || collect) && c->npoints) {
         KZG_HIP(c, c->feat.ensure(4 * size_t(c->npoints)));
         KZG_HIP(c, c->base.ensure(c->npoints));
         feat = c->feat.p;
@@ -1053,7 +1055,7 @@ int kzg_solve(kzg_ctx* c, float dt) {
     KZG_HIP(c, record(c, 6));
     launch_writeback(c->stream, c->vel_records(), c->ws_records(), c->contacts(), c->vel_pos.p, c->ws_pos.p,
                      c->npoints);
-    if (c->sample_cap && c->npoints) {
+    if (collect && c->npoints) {
         // Training samples: this step's features with its final impulses.
         const uint32_t n = std::min(c->npoints, c->sample_cap);
         launch_mlp_collect(c->stream, feat, base, c->p_imp.p, n, c->samples.p, c->sample_cap,

@@ -289,6 +289,7 @@ struct kzg_ctx {
     int warm_mode = KZG_WARM_CACHE;  // kzg_step's mode
     int cur_mode = KZG_WARM_CACHE;   // mode of the step being built and solved
     bool no_store = false;
+    bool no_collect = false;  // this step's solve appends no training rows
 
     // allocation-free standalone narrowphase: this device's copy of the
     // library's scratch array (resolved in kzg_create) and a pinned word for

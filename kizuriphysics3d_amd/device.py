@@ -49,7 +49,7 @@ EXPORTS = [
 ]
 
 # Warm-start modes (kzg_warm_start in include/kzg.h).
-WARM_CACHE, WARM_COLD, WARM_LEARNED, WARM_LEARNED_MISS, WARM_NO_STORE = 0, 1, 2, 3, 0x100
+WARM_CACHE, WARM_COLD, WARM_LEARNED, WARM_LEARNED_MISS, WARM_NO_STORE, WARM_NO_COLLECT = 0, 1, 2, 3, 0x100, 0x200
 COOP_ONE_XCD, COOP_SPREAD = 0, 1
 MLP_PARAMS, MLP_FEATURES, MLP_FEATURE_ROW, MLP_SAMPLE_ROW = 5315, 14, 16, 20
 

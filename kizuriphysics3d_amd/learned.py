@@ -72,6 +72,7 @@ def compare_warm_starts(world, modes=(D.WARM_COLD, D.WARM_CACHE, D.WARM_LEARNED,
             ctx.solve(dt)
             res["residual"] = ctx.residuals()[:iterations]
             res["points"] = ctx.sizes()[2]
+            _side_solve_done(ctx, snap, dt)
             if natural:
                 nat = []
                 for k in range(1, iterations + 1):
@@ -82,6 +83,7 @@ def compare_warm_starts(world, modes=(D.WARM_COLD, D.WARM_CACHE, D.WARM_LEARNED,
                     ctx.integrate_velocities(dt)
                     ctx.solve(dt)
                     nat.append(ctx.natural_residual())
+                    _side_solve_done(ctx, snap, dt)
                 res["natural"] = nat
             pen = {}
             for k in penetration_iterations:
@@ -93,6 +95,7 @@ def compare_warm_starts(world, modes=(D.WARM_COLD, D.WARM_CACHE, D.WARM_LEARNED,
                 ctx.solve(dt)
                 ctx.integrate_positions(dt)
                 pen[int(k)] = penetration(ctx, pairs)
+                ctx.upload_states(snap)
             res["penetration"] = pen
             out[MODE_NAMES.get(mode, str(mode))] = res
         if converged_iterations:
@@ -104,6 +107,7 @@ def compare_warm_starts(world, modes=(D.WARM_COLD, D.WARM_CACHE, D.WARM_LEARNED,
                 ctx.build_constraints(D.WARM_CACHE)  # stores this step's converged impulses
                 ctx.integrate_velocities(dt)
                 ctx.solve(dt)
+                _side_solve_done(ctx, snap, dt)
                 res = {}
                 ctx.set_iterations(iterations, position_iterations)
                 ctx.upload_states(snap)
@@ -113,6 +117,7 @@ def compare_warm_starts(world, modes=(D.WARM_COLD, D.WARM_CACHE, D.WARM_LEARNED,
                 ctx.solve(dt)
                 res["residual"] = ctx.residuals()[:iterations]
                 res["points"] = ctx.sizes()[2]
+                _side_solve_done(ctx, snap, dt)
                 if natural:
                     nat = []
                     for k in range(1, iterations + 1):
@@ -123,6 +128,7 @@ def compare_warm_starts(world, modes=(D.WARM_COLD, D.WARM_CACHE, D.WARM_LEARNED,
                         ctx.integrate_velocities(dt)
                         ctx.solve(dt)
                         nat.append(ctx.natural_residual())
+                        _side_solve_done(ctx, snap, dt)
                     res["natural"] = nat
                 res["penetration"] = {}
                 out["converged"] = res
@@ -132,6 +138,15 @@ def compare_warm_starts(world, modes=(D.WARM_COLD, D.WARM_CACHE, D.WARM_LEARNED,
         ctx.set_iterations(world_iterations(world)[0], world_iterations(world)[1])
         ctx.upload_states(snap)
     return out
+
+
+def _side_solve_done(ctx, snap, dt: float = S.DT):
+    """After a solve that is not the world's step: integrate the positions,
+    which clears the bias sweeps' pseudo velocities as the end of a step
+    does, and put the states back.  (A state upload leaves pseudo
+    velocities alone: between steps they are zero.)"""
+    ctx.integrate_positions(dt)
+    ctx.upload_states(snap)
 
 
 def world_iterations(world):
@@ -164,16 +179,43 @@ def train_on_world(world, collect_steps: int, train_steps: int, batch: int = 409
     return losses
 
 
+def _step_with_converged_rows(w, ctx, iterations: int, dt: float = S.DT):
+    """One world step whose training rows come from a side solve of the same
+    contacts with `iterations` sweeps (the cache untouched); the step itself
+    then runs as kzw_step runs it and appends no rows."""
+    vel_it, pos_it = world_iterations(w)
+    snap = w.states()
+    w.update_broadphase()
+    pairs = w.pairs()
+    ctx.set_iterations(iterations, pos_it)
+    ctx.upload_states(snap)
+    ctx.narrowphase(pairs)
+    ctx.build_constraints(D.WARM_CACHE | D.WARM_NO_STORE)
+    ctx.integrate_velocities(dt)
+    ctx.solve(dt)
+    _side_solve_done(ctx, snap, dt)
+    ctx.set_iterations(vel_it, pos_it)
+    ctx.narrowphase(pairs)
+    ctx.build_constraints(D.WARM_CACHE | D.WARM_NO_COLLECT)
+    ctx.integrate_velocities(dt)
+    ctx.solve(dt)
+    ctx.integrate_positions(dt)
+    ctx.update_sleeping(dt)
+    w.set_states(ctx.states())
+
+
 def study(bodies: int = 65536, settle: int = 600, collect: int = 100, train_steps: int = 3000,
           batch: int = 16384, lr: float = 1e-3, clips=(0.0, 8.0, 2.0), converged_iterations: int = 64,
-          seed: int = 0x4B5A0005, log=print) -> dict:
+          seed: int = 0x4B5A0005, targets: str = "solver", log=print) -> dict:
     """The learned warm start on the random convex pile of `bodies` bodies:
     settle it (the last `collect` steps collect training rows), then, on the
     same state, the residual curves of the cold start, the cache, a start from
     the impulses of a `converged_iterations`-sweep solve (the bound of a
     per-point predictor), and the network trained from one initialisation
     with each target clip in `clips` (learned everywhere, and on cache misses
-    only)."""
+    only).  targets: "solver" trains on the impulses the step's own solve
+    stores; "converged" on those of a `converged_iterations`-sweep side solve
+    of each collected step."""
     from .world import PhysicsWorld
     w = PhysicsWorld(S.pile(bodies))
     ctx = w.device_context()
@@ -181,7 +223,10 @@ def study(bodies: int = 65536, settle: int = 600, collect: int = 100, train_step
     for i in range(settle):
         if i == max(settle - collect, 0):
             ctx.mlp_collect(1 << 24)
-        w.step_fixed()
+        if targets == "converged" and i >= settle - collect:
+            _step_with_converged_rows(w, ctx, converged_iterations)
+        else:
+            w.step_fixed()
         if (i + 1) % 100 == 0:
             st = w.stats()
             log(f"settle {i + 1}/{settle}: {st['num_contact_points']} points, step {st['step_ms']:.1f} ms")
@@ -195,7 +240,7 @@ def study(bodies: int = 65536, settle: int = 600, collect: int = 100, train_step
     ctx.mlp_fit_normalization()
     norm = ctx.mlp_normalization()
     out = {"bodies": bodies, "settle": settle, "collected_over_steps": f"{settle - collect + 1}-{settle}",
-           "targets": target_stats}
+           "target_kind": targets, "targets": target_stats}
     base = compare_warm_starts(w, modes=(D.WARM_COLD, D.WARM_CACHE), converged_iterations=converged_iterations)
     out["cold"], out["cache"], out["converged"] = base["cold"], base["cache"], base["converged"]
     out["variants"] = {}

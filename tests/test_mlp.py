@@ -307,3 +307,27 @@ def test_learned_warm_start_gains_over_cold_start():
     assert cache[7] < cold[7]
     assert learned[7] < cold[7] and miss[7] < cold[7]
     assert learned[7] <= cache[7] and miss[7] <= cache[7]
+
+
+@pytest.mark.gpu
+def test_no_collect_flag_appends_no_rows():
+    """KZG_WARM_NO_COLLECT: a solve built with it appends no training rows;
+    the next one without it does."""
+    w = _small_pile(256)
+    ctx = w.device_context()
+    for _ in range(60):
+        w.step_fixed()
+    ctx.mlp_collect(1 << 16)
+    w.update_broadphase()
+    pairs = w.pairs()
+    ctx.narrowphase(pairs)
+    ctx.build_constraints(D.WARM_CACHE | D.WARM_NO_STORE | D.WARM_NO_COLLECT)
+    ctx.integrate_velocities()
+    ctx.solve()
+    assert ctx.mlp_num_samples() == (0, 0)
+    ctx.narrowphase(pairs)
+    ctx.build_constraints(D.WARM_CACHE | D.WARM_NO_STORE)
+    ctx.integrate_velocities()
+    ctx.solve()
+    written, held = ctx.mlp_num_samples()
+    assert written == held == ctx.sizes()[2] > 0
