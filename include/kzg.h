@@ -146,6 +146,29 @@ typedef struct kzg_bp_node {
 } kzg_bp_node;
 int kzg_broadphase_pairs(kzg_ctx* ctx, const kzg_bp_node* nodes, uint32_t num_nodes, uint32_t root,
                          const uint32_t* layers, const uint32_t* masks, uint32_t num_layers, uint32_t* num_pairs);
+/* Step 1's proxy moves on the device (UpdateBroadPhase, PhysicsWorld.cpp:333-345,
+ * DynamicAABBTree::MoveProxy, src/BroadPhase/BroadPhase.cpp:58-75), against
+ * the tree image the last kzg_broadphase_pairs / _update left on the device:
+ * for every non-static body of the active list, its world bounds (the local
+ * bounds registered with kzg_broadphase_set_proxies under its pose,
+ * Shape::GetWorldBounds' arithmetic) become its leaf's tight box in that
+ * image, and the bodies whose tight box left their leaf's fat box are
+ * reported: *count of them, with out[7 * k] = the body's position in the
+ * active list and out[7 * k + 1 .. 6] = its tight box (min, max), in no
+ * particular order (at most cap are written).  The caller re-inserts those
+ * leaves in active-list order, as MoveProxy would, and sends the changed
+ * nodes with kzg_broadphase_pairs_update. */
+int kzg_broadphase_set_proxies(kzg_ctx* ctx, const float* local_bounds, const uint32_t* proxies, uint32_t num_bodies);
+int kzg_broadphase_move(kzg_ctx* ctx, float* out, uint32_t cap, uint32_t* count);
+/* The pair query over the device's tree image after replacing n node
+ * records (ids[k] gets records[k]); num_nodes must be the image's size and
+ * root the tree's root.  Leaf tight boxes written by kzg_broadphase_move
+ * stay unless a record replaces them. */
+int kzg_broadphase_pairs_update(kzg_ctx* ctx, const uint32_t* ids, const kzg_bp_node* records, uint32_t n,
+                                uint32_t num_nodes, uint32_t root, uint32_t* num_pairs);
+/* The device's tree image (num_nodes records), e.g. to bring a host copy's
+ * leaf tight boxes up to date after device moves. */
+int kzg_broadphase_download(kzg_ctx* ctx, kzg_bp_node* out, uint32_t num_nodes);
 /* Step 2 over the pairs the last kzg_broadphase_pairs left on the device. */
 int kzg_narrowphase_resident(kzg_ctx* ctx);
 /* The first n pairs the context holds (slot A, slot B each). */

@@ -39,6 +39,7 @@ typedef struct kzw_stats {
      * query (host walk, or tree image + device query with its transfers). */
     double broadphase_move_ms, broadphase_query_ms;
     uint32_t num_active_bodies;      /* WorldStats::numActiveBodies */
+    uint32_t broadphase_reinserted;  /* proxies that left their fat boxes this step (MoveProxy's re-insertions) */
 } kzw_stats;
 
 int kzw_create(kzw_world** out, const kz_world_desc* world, int device);
@@ -62,9 +63,12 @@ int kzw_get_states(kzw_world* w, kz_body_state* out, uint32_t n);
 int kzw_set_states(kzw_world* w, const kz_body_state* in, uint32_t n);
 /* Pairs of the last broadphase update as (slot A, slot B); returns the count. */
 int kzw_get_pairs(kzw_world* w, uint32_t* out, uint32_t cap);
-/* kzw_step's pair query: on the device (kzg_broadphase_pairs over the tree
- * kept here; default when the world has a device) or on the host (the same
- * walk, on host threads).  The pairs and their order are the same. */
+/* kzw_step's broadphase: 1 (default with a device) the pair query on the
+ * device over an image of the tree kept here, with the proxies' bounds and
+ * fat-box tests on the device too (kzg_broadphase_move; the re-insertions
+ * stay here, in order); 2 the same query with the proxy moves on the host;
+ * 0 everything on the host (the walk on host threads).  The pairs and their
+ * order are the same. */
 int kzw_set_device_broadphase(kzw_world* w, int on);
 /* Host threads for the broadphase update's per-body loops (the proxies'
  * tight boxes and fat-box tests; the re-insertions stay in list order) and

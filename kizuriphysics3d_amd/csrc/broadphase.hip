@@ -322,6 +322,64 @@ __global__ void k_bp_finish(BpCounters* cnt, uint32_t cap) {
     if (threadIdx.x == 0) cnt->pairs = min(cnt->pairs, cap);
 }
 
+// ---- proxy moves ------------------------------------------------------------
+__device__ __forceinline__ f3 xyz(const float4& v) { return f3{v.x, v.y, v.z}; }
+// MoveProxy's first half for the active list's non-static bodies: the world
+// bounds as Shape::GetWorldBounds forms them (the local box's centre rotated
+// and moved, its half extents through |R|; the same expressions as the host's
+// transform_aabb) become the leaf's tight box in the image; a leaf whose fat
+// box no longer contains it is reported (moved[0] = count, moved[1] = flags,
+// records of 7 words from moved[2]).
+__global__ void k_bp_move(BodyBuf b, const float4* local, const uint32_t* proxy, const uint32_t* act, uint32_t n,
+                          float4* node, uint32_t num_nodes, uint32_t* moved, uint32_t cap) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t x = act[i];
+    if (meta_motion(b.meta[x]) == 0u) return;
+    const uint32_t id = proxy[x];
+    if (id >= num_nodes) {
+        atomicOr(moved + 1, kBpBadTree);
+        return;
+    }
+    const f3 lmn = xyz(local[2 * size_t(x)]), lmx = xyz(local[2 * size_t(x) + 1]);
+    const f3 c = mul(add(lmn, lmx), 0.5f);
+    const f3 e = mul(sub(lmx, lmn), 0.5f);
+    const float4 q4 = b.q[x];
+    const f4q q{q4.x, q4.y, q4.z, q4.w};
+    const m3 m = to_mat3(q);
+    const f3 nc = add(rotate(q, c), xyz(b.x[x]));
+    const f3 ne{fabs_(m.c0.x) * e.x + fabs_(m.c1.x) * e.y + fabs_(m.c2.x) * e.z,
+                fabs_(m.c0.y) * e.x + fabs_(m.c1.y) * e.y + fabs_(m.c2.y) * e.z,
+                fabs_(m.c0.z) * e.x + fabs_(m.c1.z) * e.y + fabs_(m.c2.z) * e.z};
+    const f3 tmn = sub(nc, ne), tmx = add(nc, ne);
+    float4* rec = node + 4 * size_t(id);
+    const float4 f0 = rec[0], f1 = rec[1];
+    float4 r2 = rec[2], r3 = rec[3];
+    r2.x = tmn.x; r2.y = tmn.y; r2.z = tmn.z;
+    r3.x = tmx.x; r3.y = tmx.y; r3.z = tmx.z;
+    rec[2] = r2;
+    rec[3] = r3;
+    const bool inside = f0.x <= tmn.x && f1.x >= tmx.x && f0.y <= tmn.y && f1.y >= tmx.y && f0.z <= tmn.z &&
+                        f1.z >= tmx.z;
+    if (inside) return;
+    const uint32_t k = atomicAdd(moved, 1u);
+    if (k >= cap) return;
+    uint32_t* o = moved + 2 + 7 * size_t(k);
+    o[0] = i;
+    o[1] = __float_as_uint(tmn.x); o[2] = __float_as_uint(tmn.y); o[3] = __float_as_uint(tmn.z);
+    o[4] = __float_as_uint(tmx.x); o[5] = __float_as_uint(tmx.y); o[6] = __float_as_uint(tmx.z);
+}
+__global__ void k_bp_move_reset(uint32_t* moved) {
+    if (threadIdx.x < 2) moved[threadIdx.x] = 0u;
+}
+// Replaced node records into the image.
+__global__ void k_bp_scatter(const uint32_t* ids, const float4* rec, uint32_t n, float4* node) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4* d = node + 4 * size_t(ids[i]);
+    for (int k = 0; k < 4; ++k) d[k] = rec[4 * size_t(i) + k];
+}
+
 }  // namespace
 
 // Query of one tree image: rounds = how many refinement rounds the root's
@@ -353,29 +411,13 @@ void launch_bp_query(hipStream_t s, const BpTree& t, uint32_t root, uint32_t rou
 
 }  // namespace kz
 
-extern "C" {
-
-int kzg_broadphase_pairs(kzg_ctx* c, const kzg_bp_node* nodes, uint32_t num_nodes, uint32_t root,
-                         const uint32_t* layers, const uint32_t* masks, uint32_t num_layers, uint32_t* num_pairs) {
-    if (!c || (num_nodes && !nodes) || (num_layers && (!layers || !masks)))
-        return fail(c, KZG_ERR_ARGUMENT, "kzg_broadphase_pairs: null array");
-    if (num_nodes >= kPairFlag || num_layers > num_nodes)
-        return fail(c, KZG_ERR_ARGUMENT, "kzg_broadphase_pairs: bad node / layer count");
-    if (root != kNullNode && root >= num_nodes) return fail(c, KZG_ERR_ARGUMENT, "kzg_broadphase_pairs: bad root");
-    join_schedule(c);
-    hipSetDevice(c->device);
-    const int32_t h = root == kNullNode ? 0 : std::max(nodes[root].height, 0);
+namespace {
+// The query over the image on the device (num_nodes records, num_layers
+// layers / masks), with the rounds the root's height needs.
+int query_image(kzg_ctx* c, uint32_t num_nodes, uint32_t num_layers, uint32_t root, int32_t root_height,
+                uint32_t* num_pairs) {
+    const int32_t h = root == kNullNode ? 0 : std::max(root_height, 0);
     const uint32_t rounds = uint32_t(std::max<int64_t>(0, 2 * int64_t(h) - int64_t(kWalkHeight)));
-    // The image goes up through one pinned buffer, with no wait until the
-    // query's counters come back.
-    int rc;
-    KZG_HIP(c, c->stage_bp.ensure(16 * size_t(num_nodes) * sizeof(float) + 2 * sizeof(uint32_t) * num_layers + 768));
-    size_t off = 0;
-    if ((rc = upload_async(c, c->bp_nodes, c->stage_bp, off, reinterpret_cast<const float4*>(nodes),
-                           4 * size_t(num_nodes))) ||
-        (rc = upload_async(c, c->bp_layer, c->stage_bp, off, layers, num_layers)) ||
-        (rc = upload_async(c, c->bp_mask, c->stage_bp, off, masks, num_layers)))
-        return rc;
     KZG_HIP(c, c->bp_counters.ensure(4));
     KZG_HIP(c, c->pin_bp.ensure(4 * sizeof(uint32_t)));
     const BpTree t{c->bp_nodes.p, c->bp_layer.p, c->bp_mask.p, num_nodes, num_layers, c->nbodies};
@@ -397,9 +439,11 @@ int kzg_broadphase_pairs(kzg_ctx* c, const kzg_bp_node* nodes, uint32_t num_node
         KZG_HIP(c, hipMemcpyAsync(pin, cnt, sizeof(BpCounters), hipMemcpyDeviceToHost, c->stream));
         KZG_HIP(c, hipStreamSynchronize(c->stream));
         const BpCounters got{pin[0], pin[1], pin[2], pin[3]};
-        if (got.flags & kBpBadTree)
+        if (got.flags & kBpBadTree) {
+            c->bp_num_nodes = 0;  // the image is not to be reused
             return fail(c, KZG_ERR_ARGUMENT, "kzg_broadphase_pairs: malformed tree image (a node id, a height or "
                                              "a proxy's body outside its range)");
+        }
         if (got.flags & kBpOverflow) {
             if (attempt >= 8) return fail(c, KZG_ERR_DEVICE, "kzg_broadphase_pairs: lists keep overflowing");
             // The pair count is exact once the walk counted; the item lists
@@ -410,9 +454,126 @@ int kzg_broadphase_pairs(kzg_ctx* c, const kzg_bp_node* nodes, uint32_t num_node
         }
         c->npairs = got.pairs;
         c->pairs_resident = true;
+        c->bp_root = root;
+        c->bp_root_height = h;
         if (num_pairs) *num_pairs = got.pairs;
         return KZG_OK;
     }
+}
+}  // namespace
+
+extern "C" {
+
+int kzg_broadphase_pairs(kzg_ctx* c, const kzg_bp_node* nodes, uint32_t num_nodes, uint32_t root,
+                         const uint32_t* layers, const uint32_t* masks, uint32_t num_layers, uint32_t* num_pairs) {
+    if (!c || (num_nodes && !nodes) || (num_layers && (!layers || !masks)))
+        return fail(c, KZG_ERR_ARGUMENT, "kzg_broadphase_pairs: null array");
+    if (num_nodes >= kPairFlag || num_layers > num_nodes)
+        return fail(c, KZG_ERR_ARGUMENT, "kzg_broadphase_pairs: bad node / layer count");
+    if (root != kNullNode && root >= num_nodes) return fail(c, KZG_ERR_ARGUMENT, "kzg_broadphase_pairs: bad root");
+    join_schedule(c);
+    hipSetDevice(c->device);
+    // The image goes up through one pinned buffer, with no wait until the
+    // query's counters come back.
+    int rc;
+    KZG_HIP(c, c->stage_bp.ensure(16 * size_t(num_nodes) * sizeof(float) + 2 * sizeof(uint32_t) * num_layers + 768));
+    size_t off = 0;
+    if ((rc = upload_async(c, c->bp_nodes, c->stage_bp, off, reinterpret_cast<const float4*>(nodes),
+                           4 * size_t(num_nodes))) ||
+        (rc = upload_async(c, c->bp_layer, c->stage_bp, off, layers, num_layers)) ||
+        (rc = upload_async(c, c->bp_mask, c->stage_bp, off, masks, num_layers)))
+        return rc;
+    c->bp_num_nodes = num_nodes;
+    c->bp_num_layers = num_layers;
+    return query_image(c, num_nodes, num_layers, root, root == kNullNode ? 0 : nodes[root].height, num_pairs);
+}
+
+int kzg_broadphase_pairs_update(kzg_ctx* c, const uint32_t* ids, const kzg_bp_node* records, uint32_t n,
+                                uint32_t num_nodes, uint32_t root, uint32_t* num_pairs) {
+    if (!c || (n && (!ids || !records))) return fail(c, KZG_ERR_ARGUMENT, "kzg_broadphase_pairs_update: null array");
+    if (!c->bp_num_nodes || num_nodes != c->bp_num_nodes)
+        return fail(c, KZG_ERR_STATE, "kzg_broadphase_pairs_update: no image of that size on the device "
+                                      "(kzg_broadphase_pairs first)");
+    if (root != kNullNode && root >= num_nodes)
+        return fail(c, KZG_ERR_ARGUMENT, "kzg_broadphase_pairs_update: bad root");
+    int32_t h = root == c->bp_root ? c->bp_root_height : -1;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (ids[k] >= num_nodes) return fail(c, KZG_ERR_ARGUMENT, "kzg_broadphase_pairs_update: bad node id");
+        if (ids[k] == root) h = records[k].height;
+    }
+    if (root != kNullNode && h < 0)
+        return fail(c, KZG_ERR_ARGUMENT, "kzg_broadphase_pairs_update: a new root must come with its record");
+    join_schedule(c);
+    hipSetDevice(c->device);
+    if (n) {
+        int rc;
+        KZG_HIP(c, c->stage_bp.ensure(n * (sizeof(uint32_t) + sizeof(kzg_bp_node)) + 512));
+        size_t off = 0;
+        if ((rc = upload_async(c, c->bp_upd_ids, c->stage_bp, off, ids, n)) ||
+            (rc = upload_async(c, c->bp_upd_rec, c->stage_bp, off, reinterpret_cast<const float4*>(records),
+                               4 * size_t(n))))
+            return rc;
+        k_bp_scatter<<<(n + 255) / 256, 256, 0, c->stream>>>(c->bp_upd_ids.p, c->bp_upd_rec.p, n, c->bp_nodes.p);
+        KZG_HIP(c, hipGetLastError());
+    }
+    return query_image(c, num_nodes, c->bp_num_layers, root, h, num_pairs);
+}
+
+int kzg_broadphase_set_proxies(kzg_ctx* c, const float* local, const uint32_t* proxies, uint32_t num_bodies) {
+    if (!c || (num_bodies && (!local || !proxies))) return fail(c, KZG_ERR_ARGUMENT, "null array");
+    if (num_bodies != c->nbodies) return fail(c, KZG_ERR_ARGUMENT, "one record per body of the body table");
+    std::vector<float4> lb(2 * size_t(num_bodies));
+    for (uint32_t i = 0; i < num_bodies; ++i) {
+        lb[2 * size_t(i)] = make_float4(local[6 * size_t(i)], local[6 * size_t(i) + 1], local[6 * size_t(i) + 2], 0.0f);
+        lb[2 * size_t(i) + 1] =
+            make_float4(local[6 * size_t(i) + 3], local[6 * size_t(i) + 4], local[6 * size_t(i) + 5], 0.0f);
+    }
+    join_schedule(c);
+    hipSetDevice(c->device);
+    int rc;
+    if ((rc = upload(c, c->bp_local, lb.data(), lb.size())) || (rc = upload(c, c->bp_proxy, proxies, num_bodies)))
+        return rc;
+    c->bp_proxies = num_bodies;
+    return KZG_OK;
+}
+
+int kzg_broadphase_move(kzg_ctx* c, float* out, uint32_t cap, uint32_t* count) {
+    if (!c || (cap && !out) || !count) return KZG_ERR_ARGUMENT;
+    if (!c->bp_num_nodes) return fail(c, KZG_ERR_STATE, "kzg_broadphase_move: no tree image on the device");
+    if (c->bp_proxies != c->nbodies || !c->nbodies)
+        return fail(c, KZG_ERR_STATE, "kzg_broadphase_move: proxies not registered (kzg_broadphase_set_proxies)");
+    join_schedule(c);
+    hipSetDevice(c->device);
+    const uint32_t n = c->h_act_n;
+    KZG_HIP(c, c->bp_moved.ensure(2 + 7 * size_t(std::max<uint32_t>(n, 1))));
+    KZG_HIP(c, c->pin_bp.ensure(4 * sizeof(uint32_t)));
+    k_bp_move_reset<<<1, 64, 0, c->stream>>>(c->bp_moved.p);
+    if (n)
+        k_bp_move<<<(n + 255) / 256, 256, 0, c->stream>>>(c->bodies(), c->bp_local.p, c->bp_proxy.p, c->d_act.p, n,
+                                                          c->bp_nodes.p, c->bp_num_nodes, c->bp_moved.p, n);
+    KZG_HIP(c, hipGetLastError());
+    uint32_t* pin = c->pin_bp.as<uint32_t>();
+    KZG_HIP(c, hipMemcpyAsync(pin, c->bp_moved.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    KZG_HIP(c, hipStreamSynchronize(c->stream));
+    if (pin[1] & kBpBadTree) return fail(c, KZG_ERR_ARGUMENT, "kzg_broadphase_move: a proxy id outside the image");
+    const uint32_t m = pin[0];
+    *count = m;
+    const uint32_t w = std::min(m, cap);
+    if (w) {
+        KZG_HIP(c, c->stage2.ensure(7 * sizeof(uint32_t) * size_t(w)));
+        KZG_HIP(c, hipMemcpyAsync(c->stage2.p, c->bp_moved.p + 2, 7 * sizeof(uint32_t) * size_t(w),
+                                  hipMemcpyDeviceToHost, c->stream));
+        KZG_HIP(c, hipStreamSynchronize(c->stream));
+        std::memcpy(out, c->stage2.p, 7 * sizeof(uint32_t) * size_t(w));
+    }
+    return KZG_OK;
+}
+
+int kzg_broadphase_download(kzg_ctx* c, kzg_bp_node* out, uint32_t num_nodes) {
+    if (!c || (num_nodes && !out)) return KZG_ERR_ARGUMENT;
+    if (num_nodes != c->bp_num_nodes) return fail(c, KZG_ERR_ARGUMENT, "kzg_broadphase_download: image size");
+    hipSetDevice(c->device);
+    return download(c, reinterpret_cast<float4*>(out), c->bp_nodes, 4 * size_t(num_nodes));
 }
 
 int kzg_download_pairs(kzg_ctx* c, uint32_t* out, uint32_t n) {

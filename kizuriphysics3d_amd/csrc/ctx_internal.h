@@ -211,6 +211,17 @@ struct kzg_ctx {
     DevVec<uint2> bp_items_a, bp_items_b;
     Pinned pin_bp, stage_bp;
     uint32_t bp_items_cap = 0, bp_pairs_cap = 0;
+    // the tree image kept on the device between queries, and the proxy
+    // moves on the device (kzg_broadphase_move)
+    uint32_t bp_num_nodes = 0, bp_num_layers = 0;  // of the image on the device (0: none)
+    DevVec<float4> bp_local;     // per body: local bounds min, max
+    DevVec<uint32_t> bp_proxy;   // per body: proxy (leaf node) id
+    DevVec<uint32_t> bp_moved;   // [0] count, then (active position, 6 floats) records
+    DevVec<uint32_t> bp_upd_ids; // records of kzg_broadphase_pairs_update
+    DevVec<float4> bp_upd_rec;
+    uint32_t bp_proxies = 0;     // bodies with proxies registered
+    uint32_t bp_root = 0xFFFFFFFFu;  // root and its height at the last query
+    int32_t bp_root_height = 0;
     DevVec<uint32_t> m_count;
     DevVec<float4> m_pa, m_pb, m_n;
     DevVec<uint32_t> work, work_count;

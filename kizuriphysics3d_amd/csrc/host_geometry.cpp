@@ -332,9 +332,11 @@ uint32_t BroadPhase::allocate() {
         free_ = nodes_[id].next;
         nodes_[id] = Node{};
         nodes_[id].next = kNull;
+        touch(id);
         return id;
     }
     nodes_.push_back(Node{});
+    touch(uint32_t(nodes_.size() - 1));
     return uint32_t(nodes_.size() - 1);
 }
 
@@ -343,6 +345,21 @@ void BroadPhase::free_node(uint32_t id) {
     nodes_[id].next = free_;
     nodes_[id].height = -1;
     free_ = id;
+    touch(id);
+}
+
+void BroadPhase::touch(uint32_t id) {
+    if (id == kNull) return;
+    if (id >= mark_.size()) mark_.resize(std::max<size_t>(nodes_.size(), size_t(id) + 1), 0);
+    if (!mark_[id]) {
+        mark_[id] = 1;
+        dirty_.push_back(id);
+    }
+}
+
+void BroadPhase::clear_dirty() {
+    for (uint32_t id : dirty_) mark_[id] = 0;
+    dirty_.clear();
 }
 
 namespace {
@@ -478,6 +495,7 @@ bool BroadPhase::set_tight(uint32_t id, const Aabb& tight) {
 }
 
 void BroadPhase::reinsert(uint32_t id, f3 disp) {
+    touch(id);
     remove_leaf(id);
     const Aabb tight = nodes_[id].tight;
     const f3 m{0.1f, 0.1f, 0.1f};
@@ -562,6 +580,7 @@ void BroadPhase::insert_leaf(uint32_t leaf) {
     const uint32_t sibling = index;
     const uint32_t old_parent = nodes_[sibling].parent;
     const uint32_t np = allocate();
+    touch(old_parent);
     nodes_[np].parent = old_parent;
     nodes_[np].fat = merge(la, nodes_[sibling].fat);
     nodes_[np].tight = nodes_[np].fat;
@@ -580,6 +599,7 @@ void BroadPhase::insert_leaf(uint32_t leaf) {
     index = nodes_[leaf].parent;
     while (index != kNull) {
         index = balance(index);
+        touch(index);
         const uint32_t c1 = nodes_[index].child1, c2 = nodes_[index].child2;
         nodes_[index].height = 1 + std::max(nodes_[c1].height, nodes_[c2].height);
         nodes_[index].fat = merge(nodes_[c1].fat, nodes_[c2].fat);
@@ -596,6 +616,7 @@ void BroadPhase::remove_leaf(uint32_t leaf) {
     const uint32_t grand = nodes_[parent].parent;
     const uint32_t sibling = nodes_[parent].child1 == leaf ? nodes_[parent].child2 : nodes_[parent].child1;
     if (grand != kNull) {
+        touch(grand);
         if (nodes_[grand].child1 == parent) nodes_[grand].child1 = sibling;
         else nodes_[grand].child2 = sibling;
         nodes_[sibling].parent = grand;
@@ -603,6 +624,7 @@ void BroadPhase::remove_leaf(uint32_t leaf) {
         uint32_t index = grand;
         while (index != kNull) {
             index = balance(index);
+            touch(index);
             const uint32_t c1 = nodes_[index].child1, c2 = nodes_[index].child2;
             nodes_[index].fat = merge(nodes_[c1].fat, nodes_[c2].fat);
             nodes_[index].height = 1 + std::max(nodes_[c1].height, nodes_[c2].height);
@@ -623,6 +645,20 @@ uint32_t BroadPhase::balance(uint32_t ia) {
     Node& B = nodes_[ib];
     Node& C = nodes_[ic];
     const int32_t bal = C.height - B.height;
+    // A rotation rewrites A, the child that takes its place (internal: it is
+    // two levels taller than its sibling) and the parent's child pointer;
+    // the other child and the grandchildren only get new parents, which no
+    // image record holds.  (A leaf must not be marked: its tight box may be
+    // newer elsewhere than here.)
+    if (bal > 1) {
+        touch(ia);
+        touch(ic);
+        touch(A.parent);
+    } else if (bal < -1) {
+        touch(ia);
+        touch(ib);
+        touch(A.parent);
+    }
     if (bal > 1) {
         const uint32_t i_f = C.child1, i_g = C.child2;
         Node& F = nodes_[i_f];

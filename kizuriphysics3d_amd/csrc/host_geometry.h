@@ -76,6 +76,11 @@ public:
     // 16 words; the root; the proxy layers and masks by node id.
     void image(float* nodes16) const;
     void image(float* nodes16, uint32_t first, uint32_t end) const;  // nodes [first, end) at nodes16
+    // The nodes whose image record changed since clear_dirty() (leaf tight
+    // boxes stored by set_tight are not counted: they are the caller's to
+    // track), for updating an image kept elsewhere.
+    const std::vector<uint32_t>& dirty() const { return dirty_; }
+    void clear_dirty();
     uint32_t num_nodes() const { return uint32_t(nodes_.size()); }
     uint32_t root() const { return root_; }
     const std::vector<uint32_t>& layers() const { return layers_; }
@@ -96,6 +101,9 @@ private:
     };
     uint32_t allocate();
     void free_node(uint32_t id);
+    void touch(uint32_t id);
+    std::vector<uint32_t> dirty_;
+    std::vector<uint8_t> mark_;
     void insert_leaf(uint32_t leaf);
     void remove_leaf(uint32_t leaf);
     uint32_t balance(uint32_t a);

@@ -35,12 +35,21 @@ struct kzw_world {
     // The pair query on the device (kzg_broadphase_pairs): the pairs then
     // stay on the device, and `pairs` is filled only when asked for.
     bool device_bp = true, pairs_on_device = false;
+    bool host_moves = false;  // kzw_set_device_broadphase mode 2
+    uint32_t reinserted = 0;
     uint32_t device_pairs = 0;
     std::vector<float> bp_image;
     bool shapes_dirty = true, bodies_dirty = true;
     kzw_stats stats{};
     std::vector<uint32_t> active;  // the context's active list (BodyManager's), read each step
     std::vector<uint8_t> moved;    // per active-list entry: left its fat box
+    // The device keeps the tree image between steps (kzg_broadphase_move /
+    // _update): current while dev_image; leaf tight boxes then live there
+    // and the host's are stale (tight_stale) until synchronised.
+    bool dev_image = false, dev_proxies = false, tight_stale = false;
+    uint32_t dev_nodes = 0;
+    std::vector<uint32_t> moved_rec, upd_ids;
+    std::vector<float> upd_rec;
     std::unique_ptr<HostPool> pool;  // host threads for the per-body loops of the broadphase update
     int host_threads = 1;            // kzw_set_host_threads
     // Bodies the last loaded snapshot marks inactive: the list the body
@@ -115,6 +124,7 @@ int sync_device(kzw_world* w) {
         }
         int rc = kzg_upload_bodies(w->ctx, b.data(), uint32_t(b.size()));
         if (rc) return wfail(w, rc, kzg_last_error(w->ctx));
+        w->dev_proxies = false;
         if (!w->load_inactive.empty()) {
             // Serializer.cpp:230-234: CreateBody pushes each body, then an
             // inactive one is removed again at once (it is the list's tail).
@@ -247,6 +257,7 @@ int move_proxies(kzw_world* w) {
     });
     for (uint32_t k = 0; k < na; ++k) {
         if (!w->moved[k]) continue;
+        ++w->reinserted;
         const uint32_t i = w->active[k];
         const kz_body_state& s = w->states[i];
         const f3 disp = mul(f3{s.linear_velocity[0], s.linear_velocity[1], s.linear_velocity[2]},
@@ -255,6 +266,81 @@ int move_proxies(kzw_world* w) {
     }
     return KZG_OK;
 }
+// The host's leaf tight boxes from the device's image (after device moves).
+int sync_tight(kzw_world* w) {
+    if (!w->tight_stale) return KZG_OK;
+    if (!w->ctx || w->dev_nodes == 0) return wfail(w, KZG_ERR_STATE, "no device tree image to read tight boxes from");
+    w->bp_image.resize(16 * size_t(w->dev_nodes));
+    int rc = kzg_broadphase_download(w->ctx, reinterpret_cast<kzg_bp_node*>(w->bp_image.data()), w->dev_nodes);
+    if (rc) return wfail(w, rc, kzg_last_error(w->ctx));
+    for (uint32_t i = 0; i < w->bodies.size(); ++i) {
+        const float* r = w->bp_image.data() + 16 * size_t(w->proxy[i]);
+        w->bp.set_tight(w->proxy[i], Aabb{f3{r[8], r[9], r[10]}, f3{r[12], r[13], r[14]}});
+    }
+    w->tight_stale = false;
+    return KZG_OK;
+}
+
+// The proxy moves on the device (kzg_broadphase_move): the leaves that left
+// their fat boxes come back with their tight boxes and are re-inserted here
+// in active-list order; the nodes that changed go up with the query.
+int device_moves(kzw_world* w) {
+    const uint32_t nb = uint32_t(w->bodies.size());
+    w->active.resize(nb);
+    uint32_t na = nb, m = 0;
+    int rc = kzg_get_active_bodies(w->ctx, w->active.data(), nb, &na);
+    if (rc) return wfail(w, rc, kzg_last_error(w->ctx));
+    w->moved_rec.resize(7 * size_t(std::max<uint32_t>(na, 1)));
+    rc = kzg_broadphase_move(w->ctx, reinterpret_cast<float*>(w->moved_rec.data()), na, &m);
+    if (rc) return wfail(w, rc, kzg_last_error(w->ctx));
+    if (m > na) return wfail(w, KZG_ERR_DEVICE, "more moved proxies than active bodies");
+    // MoveProxy in list order for the leaves that left their fat boxes.
+    std::vector<uint32_t> order(m);
+    for (uint32_t k = 0; k < m; ++k) order[k] = k;
+    std::sort(order.begin(), order.end(),
+              [&](uint32_t a, uint32_t b) { return w->moved_rec[7 * size_t(a)] < w->moved_rec[7 * size_t(b)]; });
+    w->bp.clear_dirty();
+    for (uint32_t k : order) {
+        const uint32_t* r = w->moved_rec.data() + 7 * size_t(k);
+        if (r[0] >= na) return wfail(w, KZG_ERR_DEVICE, "moved proxy outside the active list");
+        const uint32_t i = w->active[r[0]];
+        float t[6];
+        std::memcpy(t, r + 1, sizeof(t));
+        w->bp.set_tight(w->proxy[i], Aabb{f3{t[0], t[1], t[2]}, f3{t[3], t[4], t[5]}});
+        const kz_body_state& s = w->states[i];
+        w->bp.reinsert(w->proxy[i], mul(f3{s.linear_velocity[0], s.linear_velocity[1], s.linear_velocity[2]},
+                                        w->settings.fixed_timestep));
+    }
+    w->reinserted = m;
+    w->tight_stale = true;
+    return KZG_OK;
+}
+
+// The pair query over the device's image with the changed nodes replaced;
+// a tree that grew goes up whole (its leaves' tight boxes read back first).
+int device_pairs_update(kzw_world* w) {
+    const uint32_t nn = w->bp.num_nodes();
+    if (nn != w->dev_nodes) {
+        int rc = sync_tight(w);
+        if (rc) return rc;
+        w->dev_image = false;
+        return KZG_OK;
+    }
+    const auto& dirty = w->bp.dirty();
+    const uint32_t n = uint32_t(dirty.size());
+    w->upd_ids.assign(dirty.begin(), dirty.end());
+    w->upd_rec.resize(16 * size_t(std::max<uint32_t>(n, 1)));
+    for (uint32_t k = 0; k < n; ++k) w->bp.image(w->upd_rec.data() + 16 * size_t(k), dirty[k], dirty[k] + 1);
+    uint32_t np = 0;
+    int rc = kzg_broadphase_pairs_update(w->ctx, w->upd_ids.data(), reinterpret_cast<const kzg_bp_node*>(w->upd_rec.data()),
+                                         n, nn, w->bp.root(), &np);
+    if (rc) return wfail(w, rc, kzg_last_error(w->ctx));
+    w->bp.clear_dirty();
+    w->device_pairs = np;
+    w->pairs_on_device = true;
+    return KZG_OK;
+}
+
 // The pair query on the device over the moved tree; the pairs stay there.
 int device_pairs(kzw_world* w) {
     const uint32_t nn = w->bp.num_nodes();
@@ -269,6 +355,20 @@ int device_pairs(kzw_world* w) {
     if (rc) return wfail(w, rc, kzg_last_error(w->ctx));
     w->device_pairs = np;
     w->pairs_on_device = true;
+    w->bp.clear_dirty();
+    w->dev_image = true;
+    w->dev_nodes = nn;
+    if (!w->dev_proxies) {
+        std::vector<float> local(6 * w->bodies.size());
+        for (size_t i = 0; i < w->bodies.size(); ++i) {
+            const Aabb& l = w->shapes[w->bodies[i].shape].local;
+            const float v[6] = {l.mn.x, l.mn.y, l.mn.z, l.mx.x, l.mx.y, l.mx.z};
+            std::memcpy(&local[6 * i], v, sizeof(v));
+        }
+        rc = kzg_broadphase_set_proxies(w->ctx, local.data(), w->proxy.data(), uint32_t(w->bodies.size()));
+        if (rc) return wfail(w, rc, kzg_last_error(w->ctx));
+        w->dev_proxies = true;
+    }
     return KZG_OK;
 }
 
@@ -284,7 +384,10 @@ void host_pairs(kzw_world* w) {
 
 int kzw_update_broadphase(kzw_world* w) {
     if (!w) return KZG_ERR_ARGUMENT;
-    int rc = move_proxies(w);
+    int rc = sync_tight(w);
+    if (rc) return rc;
+    w->dev_image = false;  // the host moves the tree from here on
+    rc = move_proxies(w);
     if (rc) return rc;
     host_pairs(w);
     return KZG_OK;
@@ -298,8 +401,9 @@ int kzw_set_host_threads(kzw_world* w, int threads) {
 }
 
 int kzw_set_device_broadphase(kzw_world* w, int on) {
-    if (!w) return KZG_ERR_ARGUMENT;
+    if (!w || on < 0 || on > 2) return KZG_ERR_ARGUMENT;
     w->device_bp = on != 0;
+    w->host_moves = on == 2;
     return KZG_OK;
 }
 
@@ -310,12 +414,22 @@ int kzw_step(kzw_world* w, float dt) {
     int rc = sync_device(w);
     if (rc) return rc;
     uint32_t np = 0;
-    if ((rc = move_proxies(w))) return rc;
+    const bool dev_moves = w->device_bp && !w->host_moves && w->dev_image && w->dev_proxies;
+    w->reinserted = 0;
+    if (dev_moves) {
+        if ((rc = device_moves(w))) return rc;
+    } else {
+        if ((rc = sync_tight(w))) return rc;
+        if ((rc = move_proxies(w))) return rc;
+        w->dev_image = false;  // host moves: the image goes up whole
+    }
     const auto tmv = clock::now();
     if (w->device_bp) {
-        if ((rc = device_pairs(w))) return rc;
+        if (dev_moves && (rc = device_pairs_update(w))) return rc;
+        if (!w->dev_image && (rc = device_pairs(w))) return rc;
         np = w->device_pairs;
     } else {
+        w->dev_image = false;
         host_pairs(w);
         np = uint32_t(w->pairs.size() / 2);
     }
@@ -339,6 +453,7 @@ int kzw_step(kzw_world* w, float dt) {
     w->stats.num_broadphase_pairs = np;
     w->stats.num_contact_constraints = nc;
     w->stats.num_contact_points = npts;
+    w->stats.broadphase_reinserted = w->reinserted;
     kzg_get_active_bodies(w->ctx, nullptr, 0, &w->stats.num_active_bodies);
     w->stats.broadphase_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     w->stats.contact_path_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
@@ -610,6 +725,8 @@ int kzw_save(kzw_world* w, uint8_t* out, uint64_t cap, uint64_t* size) {
     wr.end_block(blk);
     blk = wr.begin_block(tag4("BPHT"));
     std::vector<uint32_t> tree;
+    int rc0 = sync_tight(w);  // leaf tight boxes as the device moved them
+    if (rc0) return rc0;
     w->bp.save(tree);
     wr.put(uint64_t(tree.size()));
     wr.bytes(tree.data(), sizeof(uint32_t) * tree.size());

@@ -38,7 +38,8 @@ assert KZG_BODY_DTYPE.itemsize == 140
 
 EXPORTS = [
     "kzg_create", "kzg_destroy", "kzg_last_error", "kzg_upload_shapes", "kzg_upload_bodies", "kzg_upload_states",
-    "kzg_narrowphase", "kzg_broadphase_pairs", "kzg_narrowphase_resident", "kzg_download_pairs", "kzg_build_constraints", "kzg_integrate_velocities", "kzg_solve", "kzg_integrate_positions",
+    "kzg_narrowphase", "kzg_broadphase_pairs", "kzg_broadphase_pairs_update", "kzg_broadphase_set_proxies",
+    "kzg_broadphase_move", "kzg_broadphase_download", "kzg_narrowphase_resident", "kzg_download_pairs", "kzg_build_constraints", "kzg_integrate_velocities", "kzg_solve", "kzg_integrate_positions",
     "kzg_update_sleeping", "kzg_get_active_bodies", "kzg_set_active_bodies", "kzg_set_sleep_thresholds",
     "kzg_step", "kzg_query_sizes", "kzg_download_states", "kzg_download_manifolds", "kzg_download_constraints",
     "kzg_get_residuals", "kzg_get_natural_residual", "kzg_get_timings", "kzg_collide_batch", "kzg_collide_batch_device", "kzg_schedule", "kzg_set_warm_start",
@@ -77,6 +78,9 @@ def library():
             "kzg_narrowphase": ([vp, vp, u32], i32), "kzg_build_constraints": ([vp, i32], i32),
             "kzg_broadphase_pairs": ([vp, vp, u32, u32, vp, vp, u32, vp], i32),
             "kzg_narrowphase_resident": ([vp], i32), "kzg_download_pairs": ([vp, vp, u32], i32),
+            "kzg_broadphase_pairs_update": ([vp, vp, vp, u32, u32, u32, vp], i32),
+            "kzg_broadphase_set_proxies": ([vp, vp, vp, u32], i32),
+            "kzg_broadphase_move": ([vp, vp, u32, vp], i32), "kzg_broadphase_download": ([vp, vp, u32], i32),
             "kzg_integrate_velocities": ([vp, f32], i32), "kzg_solve": ([vp, f32], i32),
             "kzg_integrate_positions": ([vp, f32], i32), "kzg_step": ([vp, vp, u32, f32], i32),
             "kzg_update_sleeping": ([vp, f32], i32), "kzg_get_active_bodies": ([vp, vp, u32, vp], i32),

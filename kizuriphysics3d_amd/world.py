@@ -21,6 +21,7 @@ STATS_DTYPE = np.dtype([
     ("num_bodies", "<u4"), ("num_broadphase_pairs", "<u4"), ("num_contact_constraints", "<u4"),
     ("num_contact_points", "<u4"), ("broadphase_ms", "<f8"), ("contact_path_ms", "<f8"), ("step_ms", "<f8"),
     ("broadphase_move_ms", "<f8"), ("broadphase_query_ms", "<f8"), ("num_active_bodies", "<u4"),
+    ("broadphase_reinserted", "<u4"),
 ], align=True)
 
 _bound = False
@@ -134,9 +135,9 @@ class PhysicsWorld:
     def set_host_threads(self, threads: int = 1):
         self._check(_lib().kzw_set_host_threads(self.h, int(threads)), "kzw_set_host_threads")
 
-    def set_device_broadphase(self, on: bool):
+    def set_device_broadphase(self, on):
         """step_fixed's pair query on the device (default) or on host threads."""
-        self._check(_lib().kzw_set_device_broadphase(self.h, 1 if on else 0), "kzw_set_device_broadphase")
+        self._check(_lib().kzw_set_device_broadphase(self.h, int(on)), "kzw_set_device_broadphase")
 
     def update_broadphase(self):
         self._check(_lib().kzw_update_broadphase(self.h), "kzw_update_broadphase")
