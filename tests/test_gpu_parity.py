@@ -182,6 +182,20 @@ def test_pile_trajectory_cooperative_spread_bit_exact(blocks):
     _trajectory(S.pile(512, seed=0x4B5A0102), 240, cooperative=blocks, placement=D.COOP_SPREAD)
 
 
+@pytest.mark.parametrize("placement", [D.COOP_ONE_XCD, D.COOP_SPREAD])
+@pytest.mark.parametrize("simd,vbias", [(0, 0), (1, 1), (0, 1)])
+def test_pile_trajectory_solver_variants_bit_exact(placement, simd, vbias):
+    """SolverSettings::useSIMDSolver off (the scalar sweep in constraint
+    order, ContactSolver.cpp:137-181) and useVelocityBias on (no split
+    impulse: the position bias rides in the velocity bias, no bias sweeps),
+    split over cooperating workgroups both ways: the trajectory stays the
+    reference's."""
+    sc = S.pile(512, seed=0x4B5A0102)
+    sc.world["use_simd_solver"] = simd
+    sc.world["use_velocity_bias"] = vbias
+    _trajectory(sc, 200, cooperative=6, placement=placement)
+
+
 def test_cooperative_sweep_gives_up_and_reports():
     """A cooperative sweep whose waits are bounded by one poll gives up on
     every workgroup (each kernel still ends), and the next call that waits
