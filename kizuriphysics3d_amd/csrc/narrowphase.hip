@@ -1055,23 +1055,40 @@ __device__ __forceinline__ bool shape_face(const WShape& w, const ShapeBuf& sb, 
     return false;
 }
 
-// Sutherland-Hodgman against one plane, keeping d <= 0 (CollideShape.cpp:111-126).
-// Run by lane 0 on the LDS polygons.
+// Sutherland-Hodgman against one plane, keeping d <= 0 (CollideShape.cpp:111-126),
+// across the wave: lane i takes edge (i, i + 1) of the input polygon and
+// emits its start vertex when inside and the crossing point when the edge
+// crosses the plane.  Ballot prefix counts give each emitted point its place
+// in the serial loop's output order; that loop stops appending at
+// 2 * kMaxFaceVerts points, so points placed beyond are dropped.  Returns the
+// output count (wave-uniform).  The caller orders the LDS writes before the
+// next reads.
 __device__ __forceinline__ int clip_plane(const float (*in)[3], int nin, f3 pn, float pd, float (*out)[3]) {
-    int n = 0;
     if (nin == 0) return 0;
-    for (int i = 0; i < nin; ++i) {
+    const int i = lane_id();
+    const bool live = i < nin;
+    bool keep = false, cross_ = false;
+    f3 cur{0.0f, 0.0f, 0.0f}, p{0.0f, 0.0f, 0.0f};
+    if (live) {
         const int j = (i + 1) % nin;
-        const f3 cur{in[i][0], in[i][1], in[i][2]}, nxt{in[j][0], in[j][1], in[j][2]};
+        cur = f3{in[i][0], in[i][1], in[i][2]};
+        const f3 nxt{in[j][0], in[j][1], in[j][2]};
         const float dc = dot(pn, cur) - pd, dn = dot(pn, nxt) - pd;
-        if (dc <= 0.0f && n < 2 * kMaxFaceVerts) { out[n][0] = cur.x; out[n][1] = cur.y; out[n][2] = cur.z; ++n; }
-        if ((dc <= 0.0f) != (dn <= 0.0f)) {
+        keep = dc <= 0.0f;
+        cross_ = (dc <= 0.0f) != (dn <= 0.0f);
+        if (cross_) {
             const float t = dc / (dc - dn);
-            const f3 p = add(cur, mul(sub(nxt, cur), t));
-            if (n < 2 * kMaxFaceVerts) { out[n][0] = p.x; out[n][1] = p.y; out[n][2] = p.z; ++n; }
+            p = add(cur, mul(sub(nxt, cur), t));
         }
     }
-    return n;
+    const unsigned long long mk = __ballot(keep), mc = __ballot(cross_);
+    const int at = int(prefix_count(mk) + prefix_count(mc));
+    constexpr int cap = 2 * kMaxFaceVerts;
+    if (keep && at < cap) { out[at][0] = cur.x; out[at][1] = cur.y; out[at][2] = cur.z; }
+    const int at1 = at + (keep ? 1 : 0);
+    if (cross_ && at1 < cap) { out[at1][0] = p.x; out[at1][1] = p.y; out[at1][2] = p.z; }
+    const int n = __popcll(mk) + __popcll(mc);
+    return n < cap ? n : cap;
 }
 
 // One result contact into the wave's LDS output slot k (lane 0 writes).
@@ -1101,40 +1118,48 @@ __device__ __forceinline__ int collide_polyhedral(const WShape& A, const WShape&
     const int ic = ref_a ? cb : ca;
     const f3 rn = pick(ref_a, na, nb);
     const float rd = dot(rn, f3{rv[0][0], rv[0][1], rv[0][2]});
-    int count = 0;
-    if (lane_id() == 0) {
-        int cur = 0, np = 0;
-        for (int i = 0; i < ic; ++i) { L.poly[0][np][0] = iv[i][0]; L.poly[0][np][1] = iv[i][1]; L.poly[0][np][2] = iv[i][2]; ++np; }
-        f3 cen{0.0f, 0.0f, 0.0f};
-        for (int i = 0; i < rc; ++i) cen = add(cen, f3{rv[i][0], rv[i][1], rv[i][2]});
-        cen = divs(cen, float(rc));
-        for (int i = 0; i < rc; ++i) {
-            const int i1 = (i + 1) % rc;
-            const f3 v0{rv[i][0], rv[i][1], rv[i][2]}, v1{rv[i1][0], rv[i1][1], rv[i1][2]};
-            f3 sn = cross(sub(v1, v0), rn);
-            const float l = len(sn);
-            if (l < 1.0e-9f) continue;
-            sn = divs(sn, l);
-            if (dot(sn, sub(v0, cen)) < 0.0f) sn = neg(sn);
-            np = clip_plane(L.poly[cur], np, sn, dot(sn, v0), L.poly[cur ^ 1]);
-            cur ^= 1;
-            if (np == 0) break;
-        }
-        int k = 0;
-        for (int i = 0; i < np && k < 8; ++i) {
-            const f3 p{L.poly[cur][i][0], L.poly[cur][i][1], L.poly[cur][i][2]};
-            const float sep = dot(rn, p) - rd;
-            if (sep > max_sep) continue;
-            const f3 proj = sub(p, mul(rn, sep));
-            const f3 pa = pick(ref_a, proj, p), pb = pick(ref_a, p, proj);
-            float* c = L.cand[k];
-            c[0] = pa.x; c[1] = pa.y; c[2] = pa.z; c[3] = pb.x; c[4] = pb.y; c[5] = pb.z;
-            c[6] = normal.x; c[7] = normal.y; c[8] = normal.z; c[9] = sep; c[10] = __uint_as_float(uint32_t(i));
-            ++k;
-        }
-        count = k;
+    // The incident face, clipped by the side planes of the reference face
+    // (every lane computes the planes), then the points within max_sep
+    // projected onto the reference plane, at most 8 in order.
+    const int lane = lane_id();
+    int cur = 0, np = ic;
+    if (lane < ic) { L.poly[0][lane][0] = iv[lane][0]; L.poly[0][lane][1] = iv[lane][1]; L.poly[0][lane][2] = iv[lane][2]; }
+    __syncthreads();
+    f3 cen{0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < rc; ++i) cen = add(cen, f3{rv[i][0], rv[i][1], rv[i][2]});
+    cen = divs(cen, float(rc));
+    for (int i = 0; i < rc; ++i) {
+        const int i1 = (i + 1) % rc;
+        const f3 v0{rv[i][0], rv[i][1], rv[i][2]}, v1{rv[i1][0], rv[i1][1], rv[i1][2]};
+        f3 sn = cross(sub(v1, v0), rn);
+        const float l = len(sn);
+        if (l < 1.0e-9f) continue;
+        sn = divs(sn, l);
+        if (dot(sn, sub(v0, cen)) < 0.0f) sn = neg(sn);
+        np = clip_plane(L.poly[cur], np, sn, dot(sn, v0), L.poly[cur ^ 1]);
+        cur ^= 1;
+        __syncthreads();
+        if (np == 0) break;
     }
-    count = __shfl(count, 0, 64);
+    bool keep = false;
+    float sep = 0.0f;
+    f3 pt{0.0f, 0.0f, 0.0f};
+    if (lane < np) {
+        pt = f3{L.poly[cur][lane][0], L.poly[cur][lane][1], L.poly[cur][lane][2]};
+        sep = dot(rn, pt) - rd;
+        keep = !(sep > max_sep);
+    }
+    const unsigned long long mk = __ballot(keep);
+    const int k = int(prefix_count(mk));
+    if (keep && k < 8) {
+        const f3 proj = sub(pt, mul(rn, sep));
+        const f3 pa = pick(ref_a, proj, pt), pb = pick(ref_a, pt, proj);
+        float* c = L.cand[k];
+        c[0] = pa.x; c[1] = pa.y; c[2] = pa.z; c[3] = pb.x; c[4] = pb.y; c[5] = pb.z;
+        c[6] = normal.x; c[7] = normal.y; c[8] = normal.z; c[9] = sep; c[10] = __uint_as_float(uint32_t(lane));
+    }
+    const int kept = __popcll(mk);
+    const int count = kept < 8 ? kept : 8;
     __syncthreads();
     const int outn = count > 4 ? 4 : count;
     // Reduction to four points: cand[j * (count - 1) / 3] (CollideShape.cpp:216-224).
