@@ -351,29 +351,41 @@ int kzg_set_island_lds_limit(kzg_ctx* ctx, uint32_t max_bodies);
 int kzg_set_early_warm_start(kzg_ctx* ctx, uint32_t mode);
 
 /* ---- Learned warm start (no reference counterpart) ---------------------
- * A per-contact-point MLP 14 -> 64 -> 64 -> 3 with ReLU hidden layers,
+ * A per-contact-point MLP 16 -> 64 -> 64 -> 3 with ReLU hidden layers,
  * BF16 weights and FP32 accumulation on the matrix cores; it replaces the
  * impulse initialisation at PhysicsWorld.cpp:514-530 and is trained on the
  * impulses the solver stores at :568-581.
- * Feature row, 16 floats per point: 0 normal relative velocity, 1-2
- * relative velocity along t1 / t2, 3-5 normal, 6 depth, 7-8 inverse masses
- * A / B, 9-10 (r x n).I^-1(r x n) for A / B, 11-12 the mean of that along
- * t1 and t2 for A / B, 13 cache hit (0/1), 14 impulse scale s = normal mass
- * * |gravity| * dt, 15 zero.  Inputs are features 0-13, normalised with a
- * per-feature mean and inverse standard deviation.  Outputs y are impulse
- * corrections in units of s on top of the cache's clamped impulses c (zero
- * where the cache has no match): lambda_n = max(c_n + y0 s, 0),
+ * Feature row, 16 floats per point, all network inputs: 0 normal relative
+ * velocity, 1-2 relative velocity along t1 / t2, 3-5 normal, 6 depth, 7-8
+ * inverse masses A / B, 9-10 (r x n).I^-1(r x n) for A / B, 11-12 the mean
+ * of that along t1 and t2 for A / B, 13 cache hit (0/1), 14-15 the loads of
+ * bodies A / B with KZG_FEATURES_LOAD (kzg_set_learned_features), else 0.
+ * Inputs are normalised with a per-feature mean and inverse standard
+ * deviation (a constant feature gets inverse deviation 0).  Outputs y are
+ * impulse corrections in units of the point's impulse scale s = normal mass
+ * * |gravity| * dt on top of the cache's clamped impulses c (zero where the
+ * cache has no match): lambda_n = max(c_n + y0 s, 0),
  * lambda_t = clamp(c_t + y1,2 s, +-mu lambda_n).  KZG_WARM_LEARNED applies
  * the correction everywhere, KZG_WARM_LEARNED_MISS only where c is zero by
  * a miss.
- * Parameters (FP32 master copy), in this order: W1[64][14], b1[64],
+ * Parameters (FP32 master copy), in this order: W1[64][16], b1[64],
  * W2[64][64], b2[64], W3[3][64], b3[3] (row = output unit). */
-#define KZG_MLP_PARAMS 5315
-#define KZG_MLP_FEATURES 14
+#define KZG_MLP_PARAMS 5443
+#define KZG_MLP_FEATURES 16
 #define KZG_MLP_FEATURE_ROW 16
 #define KZG_MLP_SAMPLE_ROW 20   /* feature row + target correction y (3) + 0 */
 /* He-uniform weights from a SplitMix64 seed, zero biases, Adam reset. */
 int kzg_mlp_init(kzg_ctx* ctx, uint64_t seed);
+/* Which optional features kzg_solve writes into features 14-15 (0: none,
+ * the network sees the 14 contact-frame features alone).
+ * KZG_FEATURES_LOAD: the load of each of the point's bodies, the sum over
+ * all its contact points of the cached normal impulse clamped at 0, in units
+ * of the body's weight impulse m |gravity| dt (0 for a static or kinematic
+ * body); summed in fixed point, so the same in any order.  It tells the
+ * network how much of the stack a body carries, which no contact-frame
+ * feature shows. */
+#define KZG_FEATURES_LOAD 1u
+int kzg_set_learned_features(kzg_ctx* ctx, uint32_t flags);
 int kzg_mlp_set_params(kzg_ctx* ctx, const float* params, uint32_t n);
 int kzg_mlp_get_params(kzg_ctx* ctx, float* params, uint32_t n);
 int kzg_mlp_set_normalization(kzg_ctx* ctx, const float* mean, const float* inv_std);

@@ -555,6 +555,7 @@ void kzg_destroy(kzg_ctx* c) {
     c->pin_info.release();
     c->pin_flag.release();
     c->coop_sync.release();
+    c->body_load.release();
     c->compact_tile.release();
     c->compact_tot.release();
     c->uf_parent.release();
@@ -942,11 +943,16 @@ int kzg_solve(kzg_ctx* c, float dt) {
     }
     const float g = std::sqrt(w.gravity[0] * w.gravity[0] + w.gravity[1] * w.gravity[1] + w.gravity[2] * w.gravity[2]);
     const bool early = c->early_ws;
+    unsigned long long* load = nullptr;
+    if (feat && (c->mlp_features & KZG_FEATURES_LOAD)) {
+        KZG_HIP(c, c->body_load.ensure(std::max<uint32_t>(c->nbodies, 1)));
+        load = c->body_load.p;
+    }
     // Early warm start: the velocity records wait for their schedule.
     launch_prepare(c->stream, c->contacts(), c->bodies(), c->npoints, dt, w.restitution_threshold, w.baumgarte,
                    w.penetration_slop, w.max_bias_velocity, w.use_simd_solver ? 1 : 0, w.use_velocity_bias ? 1 : 0,
                    c->vel_records(), c->ws_records(), early ? nullptr : c->vel_pos.p, c->ws_pos.p,
-                   (g > 0.0f ? g : 1.0f) * dt, feat, base);
+                   (g > 0.0f ? g : 1.0f) * dt, feat, base, load, c->nbodies);
     KZG_HIP(c, hipGetLastError());
     if (learned)
         launch_mlp_warm(c->stream, c->mlp(), feat, base, c->npoints, c->cur_mode == KZG_WARM_LEARNED_MISS ? 1 : 0,
@@ -1535,7 +1541,7 @@ int mlp_alloc(kzg_ctx* c) {
     KZG_HIP(c, c->mlp_w1b.ensure(64 * 16));
     KZG_HIP(c, c->mlp_w2b.ensure(64 * 64));
     KZG_HIP(c, c->mlp_w3b.ensure(16 * 64));
-    // Zero padding of the BF16 operands (columns 14-15 of W1, rows 3-15 of W3).
+    // Zero padding of the BF16 operands (rows 3-15 of W3).
     KZG_HIP(c, hipMemsetAsync(c->mlp_w1b.p, 0, 64 * 16 * sizeof(uint16_t), c->stream));
     KZG_HIP(c, hipMemsetAsync(c->mlp_w3b.p, 0, 16 * 64 * sizeof(uint16_t), c->stream));
     std::vector<float> norm(32, 0.0f);
@@ -1688,6 +1694,13 @@ int kzg_mlp_fit_normalization(kzg_ctx* c) {
         istd[f] = sd > 1e-6 ? float(1.0 / sd) : 0.0f;  // a constant feature is switched off
     }
     return kzg_mlp_set_normalization(c, mean, istd);
+}
+
+int kzg_set_learned_features(kzg_ctx* c, uint32_t flags) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (flags & ~uint32_t(KZG_FEATURES_LOAD)) return fail(c, KZG_ERR_ARGUMENT, "unknown feature flags");
+    c->mlp_features = flags;
+    return KZG_OK;
 }
 
 int kzg_mlp_set_target_clip(kzg_ctx* c, float clip) {

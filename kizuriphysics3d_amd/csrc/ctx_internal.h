@@ -35,7 +35,7 @@ void launch_build(hipStream_t s, const ContactBuf& c, const ManifoldBuf& m, cons
 void launch_prepare(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32_t np, float dt, float rthr,
                     float baum, float slop, float maxb, int use_simd, int use_vbias, const VelRecBuf& vr,
                     const WsRecBuf& wr, const uint32_t* vel_pos, const uint32_t* ws_pos, float gdt, float4* feat,
-                    float4* base);
+                    float4* base, unsigned long long* load = nullptr, uint32_t nbodies = 0);
 void launch_writeback(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const ContactBuf& c,
                       const uint32_t* vel_pos, const uint32_t* ws_pos, uint32_t np);
 // `bodies`: kPackBodies / kUnpackBodies (kz_device.h).
@@ -290,6 +290,8 @@ struct kzg_ctx {
     DevVec<uint16_t> mlp_w1b, mlp_w2b, mlp_w3b;
     uint32_t adam_t = 0;
     float mlp_clip = 0.0f;     // kzg_mlp_set_target_clip: 0 none
+    uint32_t mlp_features = 0; // kzg_set_learned_features: KZG_FEATURES_LOAD fills features 14-15
+    DevVec<unsigned long long> body_load;  // per-body load sums of the load features
     DevVec<float4> feat;       // feature rows of the current step, 4 float4 per point
     DevVec<float4> base;       // the cache's clamped impulses of the current step, 1 float4 per point
     DevVec<float4> samples;    // training ring, 5 float4 per sample

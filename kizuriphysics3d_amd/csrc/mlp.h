@@ -1,4 +1,4 @@
-// mlp.h - learned warm start: a per-contact-point MLP 14 -> 64 -> 64 -> 3
+// mlp.h - learned warm start: a per-contact-point MLP 16 -> 64 -> 64 -> 3
 // (ReLU hidden layers) with BF16 weights and FP32 accumulation on the
 // gfx950 matrix cores.
 //
@@ -8,7 +8,8 @@
 // (src/World/PhysicsWorld.cpp:514-530), and is trained on the impulses the
 // solver stores at the end of a step (PhysicsWorld.cpp:568-581).
 //
-// Feature row (16 floats per point, written by the Prepare kernel):
+// Feature row (16 floats per point, written by the Prepare kernel; all 16
+// are network inputs):
 //    0  normal relative velocity (vA + wA x rA - vB - wB x rB) . n
 //    1  relative velocity . t1        2  relative velocity . t2
 //    3-5 normal n (world)             6  penetration depth (-separation)
@@ -16,11 +17,16 @@
 //    9  (rA x n) . IA^-1 (rA x n)    10  (rB x n) . IB^-1 (rB x n)
 //   11  mean over t1, t2 of (rA x t) . IA^-1 (rA x t)
 //   12  the same for B               13  1 if the manifold cache supplied the point, else 0
-//   14  impulse scale s = normal mass * |gravity| * dt (not a network input)
-//   15  0
-// Network outputs y are impulse corrections in units of s on top of the
-// cache's clamped impulses c (zero where the cache has no match, which
-// feature 13 tells the network): lambda_n = max(c_n + y0 s, 0),
+//   14  load of body A, 15  load of body B: with the load features on, the
+//       body's summed cached normal impulses (clamped at 0) over all its
+//       contact points, in units of its weight impulse m |gravity| dt (0 for
+//       a static or kinematic body); else 0, which leaves the 14 features
+//       above as the network's only information.
+// The impulse scale s = normal mass * |gravity| * dt of the point rides in
+// the w slot of its base row (the cache's clamped impulses).  Network
+// outputs y are impulse corrections in units of s on top of the cache's
+// clamped impulses c (zero where the cache has no match, which feature 13
+// tells the network): lambda_n = max(c_n + y0 s, 0),
 // lambda_t1,2 = clamp(c_t1,2 + y1,2 s, -mu lambda_n, mu lambda_n).  Training
 // targets are (final impulses - c) / s.
 #pragma once
@@ -30,23 +36,23 @@
 
 namespace kz {
 
-constexpr int kMlpIn = 14, kMlpHid = 64, kMlpOut = 3;
-constexpr int kMlpW1 = 0;                                // [64][14] row-major (unit, feature)
+constexpr int kMlpIn = 16, kMlpHid = 64, kMlpOut = 3;
+constexpr int kMlpW1 = 0;                                // [64][16] row-major (unit, feature)
 constexpr int kMlpB1 = kMlpW1 + kMlpHid * kMlpIn;        // [64]
 constexpr int kMlpW2 = kMlpB1 + kMlpHid;                 // [64][64] (unit, input unit)
 constexpr int kMlpB2 = kMlpW2 + kMlpHid * kMlpHid;       // [64]
 constexpr int kMlpW3 = kMlpB2 + kMlpHid;                 // [3][64]
 constexpr int kMlpB3 = kMlpW3 + kMlpOut * kMlpHid;       // [3]
-constexpr int kMlpParams = kMlpB3 + kMlpOut;             // 5315
+constexpr int kMlpParams = kMlpB3 + kMlpOut;             // 5443
 constexpr int kFeatRow = 16;                             // floats per feature row
 constexpr int kSampleRow = 20;                           // feature row + (3 target corrections / s, 0)
 
 struct MlpBuf {
     float* master;     // [kMlpParams] FP32 master weights
-    uint16_t* w1b;     // [64][16] BF16, columns 14-15 zero
+    uint16_t* w1b;     // [64][16] BF16
     uint16_t* w2b;     // [64][64] BF16
     uint16_t* w3b;     // [16][64] BF16, rows 3-15 zero
-    float* norm;       // [32] feature mean[16], inverse std[16] (entries 14-15 zero)
+    float* norm;       // [32] feature mean[16], inverse std[16]
     float* adam_m;     // [kMlpParams]
     float* adam_v;     // [kMlpParams]
 };

@@ -1,5 +1,5 @@
 // mlp.hip - learned warm start on the gfx950 matrix cores: inference,
-// training (forward + backward + Adam) of the 14 -> 64 -> 64 -> 3 network.
+// training (forward + backward + Adam) of the 16 -> 64 -> 64 -> 3 network.
 //
 // Matrix layout.  A wave works on a tile of 16 contact points with
 // v_mfma_f32_16x16x16_bf16: D[16x16] += A[16x16] B[16x16].  Lane l holds
@@ -228,10 +228,10 @@ __global__ void __launch_bounds__(256) k_mlp_warm(MlpBuf m, WarmArgs w) {
         Act a;
         mlp_forward(L, xin, lane, a);
         if (valid && g == 0) {
-            const float4 tail = w.feat[size_t(p) * 4 + 3];  // feature 12, cache hit, scale, 0
+            const float4 tail = w.feat[size_t(p) * 4 + 3];  // feature 12, cache hit, loads A and B
             if (!(w.only_miss && tail.y != 0.0f)) {
-                const float s = tail.z;
-                const float4 b0 = w.base[p];
+                const float4 b0 = w.base[p];  // clamped cache impulses, impulse scale s
+                const float s = b0.w;
                 const float ln = fmaxf(b0.x + a.y[0] * s, 0.0f);
                 const float mf = w.c_la[p].w * ln;
                 const float t1 = fminf(fmaxf(b0.y + a.y[1] * s, -mf), mf);
@@ -262,8 +262,8 @@ __global__ void k_mlp_collect(const float4* feat, const float4* base, const floa
         f[k] = feat[size_t(p) * 4 + k];
         samples[row + k] = f[k];
     }
-    const float s = f[3].z;
     const float4 imp = c_imp[p], b0 = base[p];
+    const float s = b0.w;
     samples[row + 4] = s > 0.0f ? make_float4((imp.x - b0.x) / s, (imp.y - b0.y) / s, (imp.z - b0.z) / s, 0.0f)
                                 : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }

@@ -129,8 +129,37 @@ struct PrepParams {
     int use_simd, use_velocity_bias;
     float gdt;      // |gravity| * dt (or dt without gravity): the impulse scale per unit normal mass
     float4* feat;   // optional: learned warm-start feature rows, 4 float4 per point
-    float4* base;   // with feat: the clamped cache impulses (zero on a miss), 1 float4 per point
+    float4* base;   // with feat: the clamped cache impulses (zero on a miss) and the impulse scale, 1 float4 per point
+    const unsigned long long* load;  // optional, with feat: per-body loads (k_body_load), else features 14-15 are 0
 };
+
+// Load features of the learned warm start: for every contact point, its
+// cached normal impulse clamped at 0 (as Prepare clamps it), over each
+// dynamic body's weight impulse m |g| dt, summed per body in 44.20 fixed
+// point (integer sums: the same in any order).  acc[nbodies] cleared first.
+constexpr double kLoadScale = 1048576.0;
+__global__ void k_body_load(ContactBuf c, BodyBuf b, uint32_t np, float gdt, unsigned long long* acc) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const uint4 bd = c.body[p];
+    const float ln = fmax_(c.imp[p].x, 0.0f);
+    if (!(ln > 0.0f)) return;
+    const uint32_t xs[2] = {bd.x, bd.y};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const uint32_t x = xs[k];
+        if (meta_motion(b.meta[x]) != 2u) continue;
+        const double w = double(ln) * double(b.x[x].w) / double(gdt);
+        if (w > 0.0 && w < 1.0e12) atomicAdd(acc + x, (unsigned long long)(w * kLoadScale + 0.5));
+    }
+}
+__global__ void k_fill_u64(unsigned long long* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+__device__ __forceinline__ float body_load(const unsigned long long* acc, uint32_t x) {
+    return acc ? float(double(acc[x]) / kLoadScale) : 0.0f;
+}
 __device__ __forceinline__ float proj_inertia(const m3& inv_i, f3 r, f3 d) {
     const f3 x = cross(r, d);
     return dot(x, mmul(inv_i, x));
@@ -184,14 +213,15 @@ __global__ void k_prepare(ContactBuf c, BodyBuf b, uint32_t np, const uint32_t* 
         row[1] = make_float4(n.y, n.z, -sep, xa.w);
         row[2] = make_float4(xb.w, proj_inertia(ia, ra, n), proj_inertia(ib, rb, n),
                              0.5f * (proj_inertia(ia, ra, t1) + proj_inertia(ia, ra, t2)));
-        row[3] = make_float4(0.5f * (proj_inertia(ib, rb, t1) + proj_inertia(ib, rb, t2)), imp.w, nm * pp.gdt, 0.0f);
+        row[3] = make_float4(0.5f * (proj_inertia(ib, rb, t1) + proj_inertia(ib, rb, t2)), imp.w,
+                             body_load(pp.load, bd.x), body_load(pp.load, bd.y));
     }
     const float mf = cf * imp.x;
     imp.y = fclamp(imp.y, -mf, mf);
     imp.z = fclamp(imp.z, -mf, mf);
     imp.x = fmax_(imp.x, 0.0f);
     imp.w = 0.0f;
-    if (pp.feat) pp.base[p] = imp;
+    if (pp.feat) pp.base[p] = make_float4(imp.x, imp.y, imp.z, nm * pp.gdt);
     uint4 bdo = bd;
     const uint32_t ma = meta_motion(b.meta[bd.x]), mb = meta_motion(b.meta[bd.y]);
     bdo.z = (bd.z & 1u) | (ma == 2u ? 2u : 0u) | (mb == 2u ? 4u : 0u);
@@ -1164,8 +1194,12 @@ void launch_build(hipStream_t s, const ContactBuf& c, const ManifoldBuf& m, cons
 void launch_prepare(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32_t np, float dt, float rthr,
                     float baum, float slop, float maxb, int use_simd, int use_vbias, const VelRecBuf& vr,
                     const WsRecBuf& wr, const uint32_t* vel_pos, const uint32_t* ws_pos, float gdt, float4* feat,
-                    float4* base) {
-    PrepParams pp{dt, rthr, baum, slop, maxb, use_simd, use_vbias, gdt, feat, base};
+                    float4* base, unsigned long long* load, uint32_t nbodies) {
+    if (feat && load && np && nbodies) {
+        k_fill_u64<<<blocks(nbodies, 256), 256, 0, s>>>(load, nbodies);
+        k_body_load<<<blocks(np, 256), 256, 0, s>>>(c, b, np, gdt, load);
+    }
+    PrepParams pp{dt, rthr, baum, slop, maxb, use_simd, use_vbias, gdt, feat, base, feat ? load : nullptr};
     if (np) k_prepare<<<blocks(np, 256), 256, 0, s>>>(c, b, np, nullptr, pp, vr, wr, vel_pos, ws_pos);
 }
 void launch_velocity_records(hipStream_t s, const ContactBuf& c, const WsRecBuf& wr, const VelRecBuf& vr,
@@ -1285,7 +1319,7 @@ void launch_build_counted(hipStream_t s, const ContactBuf& c, const ManifoldBuf&
 void launch_prepare_counted(hipStream_t s, const ContactBuf& c, const BodyBuf& b, uint32_t cap, const uint32_t* np_dev,
                             float dt, float rthr, float baum, float slop, float maxb, int use_simd, int use_vbias,
                             const VelRecBuf& vr, const WsRecBuf& wr, const uint32_t* vel_pos, const uint32_t* ws_pos) {
-    PrepParams pp{dt, rthr, baum, slop, maxb, use_simd, use_vbias, 0.0f, nullptr, nullptr};
+    PrepParams pp{dt, rthr, baum, slop, maxb, use_simd, use_vbias, 0.0f, nullptr, nullptr, nullptr};
     if (cap) k_prepare<<<blocks(cap, 256), 256, 0, s>>>(c, b, cap, np_dev, pp, vr, wr, vel_pos, ws_pos);
 }
 void launch_writeback_counted(hipStream_t s, const VelRecBuf& vr, const WsRecBuf& wr, const ContactBuf& c,

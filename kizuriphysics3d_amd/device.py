@@ -46,13 +46,14 @@ EXPORTS = [
     "kzg_set_iterations", "kzg_set_cooperative", "kzg_set_sweep_units", "kzg_set_early_warm_start", "kzg_set_narrowphase_waves", "kzg_set_island_lds_limit", "kzg_set_cooperative_placement", "kzg_set_spin_limit", "kzg_cache_info", "kzg_download_cache", "kzg_upload_cache",
     "kzg_mlp_init", "kzg_mlp_set_params", "kzg_mlp_get_params", "kzg_mlp_set_normalization",
     "kzg_mlp_get_normalization", "kzg_mlp_collect", "kzg_mlp_num_samples", "kzg_mlp_get_samples",
-    "kzg_mlp_fit_normalization", "kzg_mlp_train", "kzg_mlp_set_target_clip", "kzg_mlp_forward", "kzg_mlp_gradients",
+    "kzg_mlp_fit_normalization", "kzg_mlp_train", "kzg_mlp_set_target_clip", "kzg_set_learned_features", "kzg_mlp_forward", "kzg_mlp_gradients",
 ]
 
 # Warm-start modes (kzg_warm_start in include/kzg.h).
 WARM_CACHE, WARM_COLD, WARM_LEARNED, WARM_LEARNED_MISS, WARM_NO_STORE, WARM_NO_COLLECT = 0, 1, 2, 3, 0x100, 0x200
 COOP_ONE_XCD, COOP_SPREAD = 0, 1
-MLP_PARAMS, MLP_FEATURES, MLP_FEATURE_ROW, MLP_SAMPLE_ROW = 5315, 14, 16, 20
+MLP_PARAMS, MLP_FEATURES, MLP_FEATURE_ROW, MLP_SAMPLE_ROW = 5443, 16, 16, 20
+FEATURES_LOAD = 1
 
 _lib = None
 
@@ -106,7 +107,7 @@ def library():
             "kzg_mlp_get_params": ([vp, vp, u32], i32), "kzg_mlp_set_normalization": ([vp, vp, vp], i32),
             "kzg_mlp_get_normalization": ([vp, vp, vp], i32), "kzg_mlp_collect": ([vp, u32], i32),
             "kzg_mlp_num_samples": ([vp, vp, vp], i32), "kzg_mlp_get_samples": ([vp, u32, u32, vp], i32),
-            "kzg_mlp_fit_normalization": ([vp], i32), "kzg_mlp_set_target_clip": ([vp, f32], i32), "kzg_mlp_train": ([vp, u32, u32, f32, u32, vp], i32),
+            "kzg_mlp_fit_normalization": ([vp], i32), "kzg_mlp_set_target_clip": ([vp, f32], i32), "kzg_set_learned_features": ([vp, u32], i32), "kzg_mlp_train": ([vp, u32, u32, f32, u32, vp], i32),
             "kzg_mlp_forward": ([vp, vp, u32, vp], i32), "kzg_mlp_gradients": ([vp, vp, u32, vp, vp], i32),
         }
         for name, (args, res) in sig.items():
@@ -425,6 +426,11 @@ class DeviceContext:
     def mlp_set_target_clip(self, clip: float = 0.0):
         """Clamp training targets to [-clip, clip] (0: off) in mlp_train and mlp_gradients."""
         self._check(library().kzg_mlp_set_target_clip(self.h, float(clip)), "kzg_mlp_set_target_clip")
+
+    def set_learned_features(self, flags: int = 0):
+        """FEATURES_LOAD (1): features 14-15 carry the loads of the point's
+        bodies (summed cached normal impulses in body weights); 0: zero."""
+        self._check(library().kzg_set_learned_features(self.h, int(flags)), "kzg_set_learned_features")
 
     def mlp_forward(self, features) -> np.ndarray:
         f = np.ascontiguousarray(features, np.float32).reshape(-1, MLP_FEATURE_ROW)

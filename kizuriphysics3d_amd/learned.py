@@ -206,7 +206,7 @@ def _step_with_converged_rows(w, ctx, iterations: int, dt: float = S.DT):
 
 def study(bodies: int = 65536, settle: int = 600, collect: int = 100, train_steps: int = 3000,
           batch: int = 16384, lr: float = 1e-3, clips=(0.0, 8.0, 2.0), converged_iterations: int = 64,
-          seed: int = 0x4B5A0005, targets: str = "solver", log=print) -> dict:
+          seed: int = 0x4B5A0005, targets: str = "solver", load_features: bool = True, log=print) -> dict:
     """The learned warm start on the random convex pile of `bodies` bodies:
     settle it (the last `collect` steps collect training rows), then, on the
     same state, the residual curves of the cold start, the cache, a start from
@@ -215,11 +215,15 @@ def study(bodies: int = 65536, settle: int = 600, collect: int = 100, train_step
     with each target clip in `clips` (learned everywhere, and on cache misses
     only).  targets: "solver" trains on the impulses the step's own solve
     stores; "converged" on those of a `converged_iterations`-sweep side solve
-    of each collected step."""
+    of each collected step.  load_features: the rows carry the body-load
+    features (14-15), and every clip is trained twice, without them (their
+    inverse deviations set to 0, which is the 14-feature network) and with
+    them (variant names ending in "_load")."""
     from .world import PhysicsWorld
     w = PhysicsWorld(S.pile(bodies))
     ctx = w.device_context()
     ctx.mlp_init(seed)
+    ctx.set_learned_features(D.FEATURES_LOAD if load_features else 0)
     for i in range(settle):
         if i == max(settle - collect, 0):
             ctx.mlp_collect(1 << 24)
@@ -244,15 +248,20 @@ def study(bodies: int = 65536, settle: int = 600, collect: int = 100, train_step
     base = compare_warm_starts(w, modes=(D.WARM_COLD, D.WARM_CACHE), converged_iterations=converged_iterations)
     out["cold"], out["cache"], out["converged"] = base["cold"], base["cache"], base["converged"]
     out["variants"] = {}
+    mean, istd = norm
+    no_load = np.array(istd, np.float32)
+    no_load[14:16] = 0.0
     for clip in clips:
-        ctx.mlp_init(seed)
-        ctx.mlp_set_normalization(*norm)
-        ctx.mlp_set_target_clip(clip)
-        losses = ctx.mlp_train(train_steps, batch=batch, lr=lr, seed=seed)
-        r = compare_warm_starts(w, modes=(D.WARM_LEARNED, D.WARM_LEARNED_MISS))
-        out["variants"][f"clip{clip:g}"] = {"clip": clip, "loss_first": float(losses[:10].mean()),
-                                           "loss_last": float(losses[-100:].mean()), **r}
-        log(f"clip {clip:g}: loss {losses[:10].mean():.3f} -> {losses[-100:].mean():.3f}")
+        for load in ((False, True) if load_features else (False,)):
+            ctx.mlp_init(seed)
+            ctx.mlp_set_normalization(mean, istd if load else no_load)
+            ctx.mlp_set_target_clip(clip)
+            losses = ctx.mlp_train(train_steps, batch=batch, lr=lr, seed=seed)
+            r = compare_warm_starts(w, modes=(D.WARM_LEARNED, D.WARM_LEARNED_MISS))
+            name = f"clip{clip:g}" + ("_load" if load else "")
+            out["variants"][name] = {"clip": clip, "load_features": load, "loss_first": float(losses[:10].mean()),
+                                     "loss_last": float(losses[-100:].mean()), **r}
+            log(f"{name}: loss {losses[:10].mean():.3f} -> {losses[-100:].mean():.3f}")
     ctx.mlp_collect(0)
     w.close()
     return out

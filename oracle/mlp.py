@@ -17,15 +17,15 @@ impulses stored at :568-581).  Two modes:
 * ``emulate=False`` is the same network in float64 without rounding: the
   mathematical definition that finite differences check.
 
-Parameter layout (flat, FP32): W1[64][14], b1[64], W2[64][64], b2[64],
-W3[3][64], b3[3].  Sample rows: 16 feature floats (features 0-13 are inputs,
-14 is the impulse scale) + 3 targets + 0.
+Parameter layout (flat, FP32): W1[64][16], b1[64], W2[64][64], b2[64],
+W3[3][64], b3[3].  Sample rows: 16 feature floats (all inputs; 14-15 are the
+optional body-load features, else 0) + 3 targets + 0.
 """
 from __future__ import annotations
 
 import numpy as np
 
-N_IN, N_HID, N_OUT = 14, 64, 3
+N_IN, N_HID, N_OUT = 16, 64, 3
 OFF_W1 = 0
 OFF_B1 = OFF_W1 + N_HID * N_IN
 OFF_W2 = OFF_B1 + N_HID
@@ -33,7 +33,7 @@ OFF_B2 = OFF_W2 + N_HID * N_HID
 OFF_W3 = OFF_B2 + N_HID
 OFF_B3 = OFF_W3 + N_OUT * N_HID
 N_PARAMS = OFF_B3 + N_OUT
-assert N_PARAMS == 5315
+assert N_PARAMS == 5443
 
 
 def bf16(x) -> np.ndarray:
@@ -151,11 +151,27 @@ def sample_hash(seed: int, q: np.ndarray) -> np.ndarray:
     return h
 
 
+def body_loads(body_a, body_b, cached_normal, inv_mass, dynamic, gdt):
+    """Load features 14-15 of every point (float64): for each dynamic body,
+    the sum over all its points of the cached normal impulse clamped at 0,
+    times its inverse mass over |gravity| dt (the weight impulse); 0 for a
+    static or kinematic body."""
+    lam = np.maximum(np.asarray(cached_normal, np.float64), 0.0)
+    nb = len(inv_mass)
+    acc = np.zeros(nb)
+    for bodies in (body_a, body_b):
+        w = np.where(dynamic[bodies], lam * inv_mass[bodies] / gdt, 0.0)
+        np.add.at(acc, bodies, w)
+    acc = np.where(dynamic, acc, 0.0)
+    return acc[body_a], acc[body_b]
+
+
 def contact_features(ra, rb, n, t1, t2, sep, inv_mass_a, inv_mass_b, inv_inertia_a, inv_inertia_b,
-                     va, wa, vb, wb, hit, normal_mass, gdt):
+                     va, wa, vb, wb, hit, load_a=0.0, load_b=0.0):
     """Feature rows (n x 16, float64) from contact-point and body data, as
     defined for the learned warm start: velocities are those after velocity
-    integration, inverse inertias are world-space 3x3 matrices."""
+    integration, inverse inertias are world-space 3x3 matrices; load_a /
+    load_b are the body-load features (body_loads) or 0 without them."""
     def proj(I, r, d):
         x = np.cross(r, d)
         return np.einsum("ni,nij,nj->n", x, I, x)
@@ -173,5 +189,6 @@ def contact_features(ra, rb, n, t1, t2, sep, inv_mass_a, inv_mass_b, inv_inertia
     out[:, 11] = 0.5 * (proj(inv_inertia_a, ra, t1) + proj(inv_inertia_a, ra, t2))
     out[:, 12] = 0.5 * (proj(inv_inertia_b, rb, t1) + proj(inv_inertia_b, rb, t2))
     out[:, 13] = hit
-    out[:, 14] = normal_mass * gdt
+    out[:, 14] = load_a
+    out[:, 15] = load_b
     return out

@@ -28,7 +28,6 @@ def _random_problem(n, seed, scale=0.4):
     istd = rng.uniform(0.5, 2.0, M.N_IN).astype(np.float32)
     s = np.zeros((n, 20), np.float32)
     s[:, :M.N_IN] = rng.normal(0.0, 1.0, (n, M.N_IN))
-    s[:, 14] = rng.uniform(0.1, 2.0, n)
     s[:, 16:19] = rng.normal(0.0, 1.0, (n, 3))
     return p, mean, istd, s
 
@@ -95,9 +94,8 @@ def test_adam_first_step_closed_form():
 
 def test_features_definition_on_a_resting_box():
     # Unit-mass box resting on a static floor, contact point straight below
-    # the centre: rA parallel to n, so the normal projected inertia is 0,
-    # the normal relative velocity is the body's vertical velocity, and the
-    # scale is normal mass (= 1 kg) * g * dt.
+    # the centre: rA parallel to n, so the normal projected inertia is 0 and
+    # the normal relative velocity is the body's vertical velocity.
     ra = np.array([[0.0, -0.5, 0.0]])
     rb = np.array([[0.0, 0.5, 0.0]])
     n = np.array([[0.0, 1.0, 0.0]])
@@ -107,13 +105,37 @@ def test_features_definition_on_a_resting_box():
     Z = np.zeros((1, 3, 3))
     f = M.contact_features(ra, rb, n, t1, t2, np.array([-0.01]), np.array([1.0]), np.array([0.0]), I, Z,
                            np.array([[0.0, -0.16, 0.0]]), np.zeros((1, 3)), np.zeros((1, 3)), np.zeros((1, 3)),
-                           np.array([1.0]), np.array([1.0]), 9.81 / 60)
+                           np.array([1.0]))
     assert f[0, 0] == pytest.approx(-0.16)
     assert f[0, 1] == 0 and f[0, 2] == 0
     assert f[0, 6] == pytest.approx(0.01)
     assert f[0, 9] == 0.0                       # r x n = 0
     assert f[0, 11] == pytest.approx(6 * 0.25)  # |r x t|^2 = 0.25 for both tangents
-    assert f[0, 14] == pytest.approx(9.81 / 60)
+    assert f[0, 14] == 0 and f[0, 15] == 0      # no load features
+
+
+def test_body_loads_of_a_resting_stack():
+    # Two boxes stacked on a static floor (body 0), at rest: the floor holds
+    # both weights over 4 points, the boxes press on each other with the top
+    # box's weight over 4 points.  Each box's load is the sum of its points'
+    # normal impulses over its own weight impulse: the top box 1, the bottom
+    # box (m_bot + m_top) / m_bot from the floor + m_top / m_bot from the top
+    # box, the floor 0.
+    g, dt = 9.81, 1 / 60
+    m_bot, m_top = 2.0, 0.5
+    inv_mass = np.array([0.0, 1 / m_bot, 1 / m_top])
+    dynamic = np.array([False, True, True])
+    a = np.array([1] * 4 + [2] * 4)   # bottom on floor, top on bottom
+    b = np.array([0] * 4 + [1] * 4)
+    lam = np.array([(m_bot + m_top) * g * dt / 4] * 4 + [m_top * g * dt / 4] * 4)
+    la, lb = M.body_loads(a, b, lam, inv_mass, dynamic, g * dt)
+    assert la[:4] == pytest.approx([(m_bot + m_top) / m_bot + m_top / m_bot] * 4)
+    assert la[4:] == pytest.approx([1.0] * 4)
+    assert lb[:4] == pytest.approx([0.0] * 4)
+    assert lb[4:] == pytest.approx([(m_bot + m_top) / m_bot + m_top / m_bot] * 4)
+    # negative cached impulses count as 0 (Prepare clamps them)
+    la2, _ = M.body_loads(a, b, np.where(np.arange(8) == 0, -5.0, lam), inv_mass, dynamic, g * dt)
+    assert la2[0] == pytest.approx(la[0] - lam[0] / m_bot / (g * dt))
 
 
 # ---------------------------------------------------------------------------
@@ -197,7 +219,8 @@ def _small_pile(n=1024, seed=0x4B5A0510, y_range=(1.0, 16.0)):
 
 
 @pytest.mark.gpu
-def test_feature_rows_and_targets_match_definition():
+@pytest.mark.parametrize("load", [False, True])
+def test_feature_rows_and_targets_match_definition(load):
     w = _small_pile()
     ctx = w.device_context()
     for _ in range(150):  # the bodies have landed: most contacts persist from the last step
@@ -206,8 +229,10 @@ def test_feature_rows_and_targets_match_definition():
     w.update_broadphase()
     pairs = w.pairs()
     ctx.mlp_collect(1 << 16)
+    ctx.set_learned_features(D.FEATURES_LOAD if load else 0)
     ctx.narrowphase(pairs)
     ctx.build_constraints(D.WARM_CACHE | D.WARM_NO_STORE)
+    cached = ctx.constraints()[1]  # the cache's impulses, before Prepare clamps them
     ctx.integrate_velocities()
     st = ctx.states()  # velocities after integration, the ones Prepare reads
     ctx.solve()
@@ -234,17 +259,23 @@ def test_feature_rows_and_targets_match_definition():
                            mass[a, 0], mass[b, 0], Iw[a], Iw[b],
                            st["linear_velocity"][a], st["angular_velocity"][a],
                            st["linear_velocity"][b], st["angular_velocity"][b],
-                           rows[:, 13], P[:, 27], np.float32(9.81) * np.float32(1 / 60))
-    for k in list(range(13)) + [14]:
+                           rows[:, 13])
+    gdt = np.float32(9.81) * np.float32(1 / 60)
+    if load:
+        la, lb = M.body_loads(a, b, cached[live][:, 30].astype(np.float64), mass[:, 0].astype(np.float64), dyn,
+                              float(gdt))
+        f[:, 14], f[:, 15] = la, lb
+        assert la.max() > 2.0  # bodies low in the pile carry several weights
+    for k in range(16):
         tol = 1e-4 * max(1.0, np.abs(f[:, k]).max())
         err = np.abs(rows[:, k] - f[:, k]).max()
         assert err <= tol, (k, err, tol)
     assert set(np.unique(rows[:, 13])) <= {0.0, 1.0}
     assert rows[:, 13].mean() > 0.5  # a settling pile: most points come from the cache
-    assert np.all(rows[:, 15] == 0) and np.all(rows[:, 19] == 0)
-    # targets: (final impulses - the cache's) in units of the scale; where
-    # the cache had no match its impulses are zero
-    s = rows[:, 14]
+    assert np.all(rows[:, 19] == 0)
+    # targets: (final impulses - the cache's) in units of the scale s =
+    # normal mass * |g| dt; where the cache had no match its impulses are zero
+    s = (P[:, 27].astype(np.float32) * gdt).astype(np.float32)
     miss = rows[:, 13] == 0
     for k, col in ((16, 30), (17, 31), (18, 32)):
         expect = np.where(s > 0, (P[:, col].astype(np.float32) / np.where(s > 0, s, 1)).astype(np.float32), 0)
@@ -268,7 +299,7 @@ def test_mlp_train_step_is_adam_on_the_device_gradient():
     assert held > 2000
     rows = ctx.mlp_samples(0, held)
     mean, istd = ctx.mlp_normalization()
-    fm = rows[:, :14].astype(np.float64)
+    fm = rows[:, :16].astype(np.float64)
     assert np.allclose(mean, fm.mean(0), rtol=1e-4, atol=1e-5)
     p0 = ctx.mlp_params()
     batch, seed, lr = 1024, 77, 1e-3
