@@ -38,6 +38,8 @@ void launch_cache_store_counted(hipStream_t s, const ContactBuf& c, const BodyBu
 namespace {
 
 constexpr uint32_t kWorldBodies = 1024;   // LDS tables of the schedule kernel
+static_assert((kWorldBodies & (kWorldBodies - 1u)) == 0 && kWorldBodies <= 0x8000u,
+              "body indices are masked into the tables and carry a flag in bit 15");
 constexpr uint32_t kWorldBatches = 2048;
 constexpr uint32_t kWorldLevels = 4096;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
@@ -203,9 +205,10 @@ struct alignas(16) SchedLds {
     uint16_t last_ws[kWorldBodies], last_vel[kWorldBodies], hint[kWorldBodies];
     uint32_t dyn[kWorldBodies / 32];
     uint4 stage[64];
-    uint16_t ba[kWorldBatches * 4], bb[kWorldBatches * 4];
+    uint16_t ba[kWorldBatches * 4], bb[kWorldBatches * 4];  // 8-byte aligned: one quad per batch
     uint16_t bm[kWorldBatches];  // lanes used | island root << 3
 };
+static_assert(offsetof(SchedLds, ba) % 8 == 0 && offsetof(SchedLds, bb) % 8 == 0, "lane quads are read as 8 bytes");
 
 __device__ __forceinline__ bool is_dyn(const SchedLds& L, uint32_t x) { return (L.dyn[x >> 5] >> (x & 31u)) & 1u; }
 __device__ __forceinline__ uint32_t lds_find(const volatile uint32_t* parent, uint32_t x) {
@@ -266,21 +269,32 @@ __global__ void __launch_bounds__(64) k_world_schedule(BatchDev d, ContactBuf c,
                 const uint4 in = L.stage[j];
                 const uint32_t x = in.x - b0, y = in.y - b0, k = in.w & 0xFFu, first = in.z - p0;
                 const bool sensor = (in.w >> 8) & 1u;
-                const bool dx = is_dyn(L, x), dy = is_dyn(L, y);
+                const bool dx = (in.w >> 9) & 1u, dy = (in.w >> 10) & 1u;
+                // The points of a constraint follow each other: the first
+                // takes 1 + the bodies' last level, each next one the level
+                // after (its bodies were last updated by the point before),
+                // capped at kWorldLevels; with no dynamic body every point
+                // takes level 1.  Both entries are read (a static body's is
+                // never used) so that the reads issue together.
+                const uint32_t rx = L.last_ws[x], ry = L.last_ws[y];
+                const uint32_t lx = dx ? rx : 0u, ly = dy ? ry : 0u;
+                const uint32_t l0 = 1u + (lx > ly ? lx : ly);
+                uint32_t l = l0;
                 for (uint32_t q = 0; q < k; ++q) {
-                    const uint32_t lx = dx ? L.last_ws[x] : 0u, ly = dy ? L.last_ws[y] : 0u;
-                    uint32_t l = 1u + (lx > ly ? lx : ly);
+                    l = (dx || dy) ? l0 + q : 1u;
                     if (l > kWorldLevels) {
                         err = kErrLevels;
                         l = kWorldLevels;
                     }
-                    if (dx) L.last_ws[x] = uint16_t(l);
-                    if (dy) L.last_ws[y] = uint16_t(l);
                     d.ws_level[p0 + first + q] = l;
                     if (use_simd && sensor) d.vel_level[p0 + first + q] = 0;
-                    depth_ws = l > depth_ws ? l : depth_ws;
-                    slots += (use_simd && sensor) ? 0u : 1u;
                 }
+                if (k) {
+                    L.last_ws[x] = uint16_t(l);  // a static body's entry is never read as a level
+                    L.last_ws[y] = uint16_t(l);
+                    depth_ws = l > depth_ws ? l : depth_ws;
+                }
+                slots += (use_simd && sensor) ? 0u : k;
             }
         }
         err = __shfl(err, 0);
@@ -290,6 +304,9 @@ __global__ void __launch_bounds__(64) k_world_schedule(BatchDev d, ContactBuf c,
     // world only, so its lane table fits in 4 entries per point.
     uint32_t nbat = 0;
     uint16_t* lane_pt = d.lane_pt + 4 * size_t(p0);
+    // The hint (first non-full batch) of the island of the last point placed,
+    // kept in a register; L.hint holds every island's.
+    uint32_t hint_isl = kNone, hint_val = 0xFFFFu;
     if (use_simd) {
         for (uint32_t base = 0; !err && base < nc; base += 64) {
             const uint32_t cnt = nc - base < 64u ? nc - base : 64u;
@@ -299,28 +316,42 @@ __global__ void __launch_bounds__(64) k_world_schedule(BatchDev d, ContactBuf c,
                 const uint4 in = L.stage[j];
                 if ((in.w >> 8) & 1u) continue;  // sensors are not swept (PhysicsWorld.cpp:683)
                 const uint32_t x = in.x - b0, y = in.y - b0, k = in.w & 0xFFu, first = in.z - p0;
-                const uint32_t isl = is_dyn(L, x) ? L.parent[x] : L.parent[y];
+                const bool dx = (in.w >> 9) & 1u;
+                const uint32_t isl = dx ? L.parent[x] : L.parent[y];
+                // Lane entries: body | dynamic << 15.
+                const uint32_t ex = x | (dx ? 0x8000u : 0u), ey = y | (((in.w >> 10) & 1u) ? 0x8000u : 0u);
                 for (uint32_t q = 0; q < k; ++q) {
-                    const uint32_t h = L.hint[isl];
-                    uint32_t found = kNone, open = kNone;
+                    const uint32_t h = isl == hint_isl ? hint_val : uint32_t(L.hint[isl]);
+                    uint32_t found = kNone, open = kNone, mfound = 0;
                     for (uint32_t s = (h == 0xFFFFu ? nbat : h); s < nbat; s += 64) {
                         const uint32_t jb = s + lane;
                         bool mine = false, fit = false;
+                        uint32_t m = 0;
                         if (jb < nbat) {
-                            const uint32_t m = L.bm[jb], n = m & 7u;
+                            // The batch's lane count and both lane quads at
+                            // once (8 bytes each); lanes past the count hold
+                            // stale entries and are masked.
+                            m = L.bm[jb];
+                            const uint2 qa = reinterpret_cast<const uint2*>(L.ba)[jb];
+                            const uint2 qb = reinterpret_cast<const uint2*>(L.bb)[jb];
+                            const uint32_t n = m & 7u;
                             mine = (m >> 3) == isl && n < 4u;
-                            if (mine) {
-                                fit = true;
-                                for (uint32_t t = 0; t < n; ++t) {
-                                    const uint32_t ma = L.ba[jb * 4 + t], mb = L.bb[jb * 4 + t];
-                                    if (ma == x || ma == y || mb == x || mb == y) fit = false;
-                                }
-                            }
+                            const uint32_t ea[4] = {qa.x & 0x7FFFu, (qa.x >> 16) & 0x7FFFu, qa.y & 0x7FFFu,
+                                                    (qa.y >> 16) & 0x7FFFu};
+                            const uint32_t eb[4] = {qb.x & 0x7FFFu, (qb.x >> 16) & 0x7FFFu, qb.y & 0x7FFFu,
+                                                    (qb.y >> 16) & 0x7FFFu};
+                            bool clash = false;
+#pragma unroll
+                            for (uint32_t t = 0; t < 4; ++t)
+                                clash |= t < n && (ea[t] == x || ea[t] == y || eb[t] == x || eb[t] == y);
+                            fit = mine && !clash;
                         }
                         const unsigned long long om = __ballot(mine), fm = __ballot(fit);
                         if (open == kNone && om) open = s + uint32_t(__ffsll(om)) - 1u;
                         if (fm) {
-                            found = s + uint32_t(__ffsll(fm)) - 1u;
+                            const uint32_t src = uint32_t(__ffsll(fm)) - 1u;
+                            found = s + src;
+                            mfound = __builtin_amdgcn_readlane(m, src);
                             break;
                         }
                     }
@@ -330,18 +361,20 @@ __global__ void __launch_bounds__(64) k_world_schedule(BatchDev d, ContactBuf c,
                             break;
                         }
                         found = nbat++;
-                        if (lane == 0) L.bm[found] = uint16_t(isl << 3);
+                        mfound = isl << 3;
                         if (open == kNone) open = found;
                     }
                     wave_sync();
                     if (lane == 0) {
-                        const uint32_t m = L.bm[found], n = m & 7u;
-                        L.ba[found * 4 + n] = uint16_t(x);
-                        L.bb[found * 4 + n] = uint16_t(y);
+                        const uint32_t n = mfound & 7u;
+                        L.ba[found * 4 + n] = uint16_t(ex);
+                        L.bb[found * 4 + n] = uint16_t(ey);
                         lane_pt[found * 4 + n] = uint16_t(first + q);
-                        L.bm[found] = uint16_t(m + 1u);
+                        L.bm[found] = uint16_t(mfound + 1u);
                         L.hint[isl] = uint16_t(open);
                     }
+                    hint_isl = isl;
+                    hint_val = open;
                     wave_sync();
                 }
             }
@@ -359,20 +392,38 @@ __global__ void __launch_bounds__(64) k_world_schedule(BatchDev d, ContactBuf c,
             if ((L.bm[jb0 + k / 4] & 7u) > k % 4) stage_pt[k] = lane_pt[4 * jb0 + k];
         wave_sync();
         if (lane != 0) continue;
+        // The lanes of a batch share no body, so its points read and write
+        // disjoint level entries: all four are taken at once.
         for (uint32_t jb = jb0; jb < jb0 + nj; ++jb) {
             const uint32_t n = L.bm[jb] & 7u;
-            for (uint32_t t = 0; t < n; ++t) {
-                const uint32_t x = L.ba[jb * 4 + t], y = L.bb[jb * 4 + t], p = stage_pt[(jb - jb0) * 4 + t];
-                const bool dx = is_dyn(L, x), dy = is_dyn(L, y);
-                const uint32_t lx = dx ? L.last_vel[x] : 0u, ly = dy ? L.last_vel[y] : 0u;
-                uint32_t l = 1u + (lx > ly ? lx : ly);
+            const uint2 qa = reinterpret_cast<const uint2*>(L.ba)[jb];
+            const uint2 qb = reinterpret_cast<const uint2*>(L.bb)[jb];
+            const uint2 qp = reinterpret_cast<const uint2*>(stage_pt)[jb - jb0];
+            const uint32_t xs[4] = {qa.x & 0xFFFFu, qa.x >> 16, qa.y & 0xFFFFu, qa.y >> 16};
+            const uint32_t ys[4] = {qb.x & 0xFFFFu, qb.x >> 16, qb.y & 0xFFFFu, qb.y >> 16};
+            const uint32_t ps[4] = {qp.x & 0xFFFFu, qp.x >> 16, qp.y & 0xFFFFu, qp.y >> 16};
+            // Every entry is read, a static body's too (its level is written
+            // and never used); entries past the lane count hold stale values,
+            // masked into the table and never used: no read waits on a flag.
+            uint32_t lv[4];
+#pragma unroll
+            for (uint32_t t = 0; t < 4; ++t) {
+                const uint32_t rx = L.last_vel[xs[t] & (kWorldBodies - 1u)];
+                const uint32_t ry = L.last_vel[ys[t] & (kWorldBodies - 1u)];
+                const uint32_t lx = (xs[t] >> 15) ? rx : 0u, ly = (ys[t] >> 15) ? ry : 0u;
+                lv[t] = 1u + (lx > ly ? lx : ly);
+            }
+#pragma unroll
+            for (uint32_t t = 0; t < 4; ++t) {
+                if (t >= n) continue;
+                uint32_t l = lv[t];
                 if (l > kWorldLevels) {
                     err = kErrLevels;
                     l = kWorldLevels;
                 }
-                if (dx) L.last_vel[x] = uint16_t(l);
-                if (dy) L.last_vel[y] = uint16_t(l);
-                d.vel_level[p0 + p] = l;
+                L.last_vel[xs[t] & (kWorldBodies - 1u)] = uint16_t(l);
+                L.last_vel[ys[t] & (kWorldBodies - 1u)] = uint16_t(l);
+                d.vel_level[p0 + ps[t]] = l;
                 depth_vel = l > depth_vel ? l : depth_vel;
             }
         }
