@@ -434,7 +434,16 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=1024, help="copy workload: buffer size")
     ap.add_argument("--worlds", type=int, default=4096, help="batched workload: number of worlds")
     ap.add_argument("--pairs", type=int, default=1 << 20, help="narrowphase workload: number of pairs")
+    ap.add_argument("--maps", default="", help="write the process's library map (/proc/self/maps) to this file at "
+                                             "the end, to resolve the addresses of a crash report")
     args = ap.parse_args()
+    if args.maps:
+        import atexit
+
+        def write_maps(path=args.maps):
+            with open("/proc/self/maps") as src, open(path, "w") as dst:
+                dst.write(src.read())
+        atexit.register(write_maps)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the contact path of one world runs on one MI355X "
               "(multi-GPU sharding is a design note, not part of this build); use --gpus 1", file=sys.stderr)
