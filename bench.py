@@ -425,6 +425,8 @@ def main():
                     help="learned workload: train on the step's own impulses or on a 64-sweep side solve's")
     ap.add_argument("--cooperative", type=int, default=0,
                     help="workgroups sharing the largest island's levels (0 auto, 1 one per group)")
+    ap.add_argument("--device-bucketing", type=int, choices=(0, 1), default=0,
+                    help="0: the early warm start's velocity schedule is bucketed by level on the device; 1: on the host")
     ap.add_argument("--placement", choices=("one-xcd", "spread"), default="one-xcd",
                     help="where the workgroups sharing the largest island's levels run")
     ap.add_argument("--np-waves", type=int, default=0, help="narrowphase EPA / clipping waves (0: auto)")
@@ -472,6 +474,7 @@ def main():
     ctx = w.device_context()
     ctx.set_cooperative(args.cooperative)
     ctx.set_cooperative_placement(0 if args.placement == "one-xcd" else 1)
+    ctx.set_device_bucketing(args.device_bucketing)
     ctx.set_sweep_units(args.units_vel, args.units_ws)
     ctx.set_narrowphase_waves(args.np_waves)
     if learn:

@@ -317,6 +317,14 @@ int kzg_set_cooperative(kzg_ctx* ctx, uint32_t blocks);
 #define KZG_COOP_ONE_XCD 0u
 #define KZG_COOP_SPREAD 1u
 int kzg_set_cooperative_placement(kzg_ctx* ctx, uint32_t placement);
+/* Where the velocity schedule of an early warm start (kzg_set_early_warm_start)
+ * is bucketed by level: 0 = on the device when one group of islands holds at
+ * least half the points (default), 1 = on the host.  The device takes the
+ * host's level walk per point and builds the level offsets, unit bounds and
+ * record positions in parallel; outside the largest group the units of one
+ * level then sit in another order, which changes no result (they share no
+ * dynamic body). */
+int kzg_set_device_bucketing(kzg_ctx* ctx, uint32_t mode);
 /* Bound on the polls of one wait between levels of a cooperative sweep
  * (0 = default, about a second).  A sweep whose wait runs out stops early
  * on every workgroup, and the next call that waits for the device (build,

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkizuri_amd.so")
 SOURCES = ["narrowphase.hip", "compact.hip", "solver.hip", "context.hip", "schedule.cpp", "host_geometry.cpp", "world.cpp",
-           "mlp.hip", "batch.hip", "broadphase.hip"]
+           "mlp.hip", "batch.hip", "broadphase.hip", "bucket.hip"]
 
 FLAGS = [
     "-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++20", "-ffp-contract=off", "-fno-fast-math",

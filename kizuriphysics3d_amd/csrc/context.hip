@@ -168,7 +168,7 @@ CoopPlan plan_cooperative(const ScheduleOutput& s, uint32_t request, uint32_t ma
         for (uint32_t l = g.x; l < g.y; ++l) widest = std::max(widest, lanes * (ls->off[l + 1] - ls->off[l]));
         units += ls->off[g.y] - ls->off[g.x];
         levels += g.y - g.x;
-        (ls == &s.ws ? p.ws_points : p.vel_points) = ls->unit[ls->off[g.y]] - ls->unit[ls->off[g.x]];
+        (ls == &s.ws ? p.ws_points : p.vel_points) = ls->g0_points;
     }
     if (!levels) return CoopPlan{};
     p.blocks = request ? request
@@ -556,6 +556,10 @@ void kzg_destroy(kzg_ctx* c) {
     c->pin_flag.release();
     c->coop_sync.release();
     c->body_load.release();
+    for (auto* d : {&c->vb_pts, &c->vb_uid, &c->vb_lvl, &c->vb_rank, &c->vb_gl, &c->vb_base, &c->vb_count,
+                    &c->vb_fill, &c->vb_uslot, &c->vb_usize, &c->vb_sums})
+        d->release();
+    c->vb_runs.release();
     c->compact_tile.release();
     c->compact_tot.release();
     c->uf_parent.release();
@@ -875,6 +879,7 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
         c->phase = 2;
         return KZG_OK;
     }
+    c->sched_builder.set_device_bucketing(c->device_bucketing == 0);
     c->early_ws = c->sched_builder.start(si, c->sched, defer);
     c->vel_after_ms = 0.0;
     // Host compute of the build ends here; what follows stages the
@@ -1006,18 +1011,57 @@ int kzg_solve(kzg_ctx* c, float dt) {
         c->build_host_ms += c->vel_after_ms;
         c->schedule_host_ms += c->vel_after_ms;
         const LevelSchedule& vs = c->sched.vel;
-        const size_t bytes = sizeof(uint32_t) * (vs.off.size() + vs.unit.size() + size_t(c->npoints)) +
-                             sizeof(U2) * vs.grp.size() + 5 * 256;
-        KZG_HIP(c, c->stage_v.ensure(bytes));
-        size_t off = 0;
         int rc;
-        if ((rc = upload_async(c, c->vel_off, c->stage_v, off, vs.off.data(), vs.off.size())) ||
-            (rc = upload_async(c, c->vel_grp, c->stage_v, off, reinterpret_cast<const uint2*>(vs.grp.data()),
-                               vs.grp.size())) ||
-            (rc = upload_async(c, c->vel_pos, c->stage_v, off, vs.pos.data(), vs.pos.size())) ||
-            (rc = upload_async(c, c->vel_unit, c->stage_v, off, vs.unit.data(), vs.unit.size())))
-            return rc;
-        KZG_HIP(c, record(c, 16));
+        if (vs.device_bucketed) {
+            // The walk's per-point arrays go up; the device buckets them.
+            const VelSequence q = c->sched_builder.velocity_sequence();
+            const size_t bytes = sizeof(uint32_t) * (4 * size_t(q.n) + q.groups + q.g0_levels) +
+                                 sizeof(U2) * vs.grp.size() + sizeof(uint2) * q.num_runs + 9 * 256;
+            KZG_HIP(c, c->stage_v.ensure(bytes));
+            size_t off = 0;
+            if ((rc = upload_async(c, c->vb_pts, c->stage_v, off, q.pts, q.n)) ||
+                (rc = upload_async(c, c->vb_uid, c->stage_v, off, q.uid, q.n)) ||
+                (rc = upload_async(c, c->vb_lvl, c->stage_v, off, q.lvl, q.n)) ||
+                (rc = upload_async(c, c->vb_rank, c->stage_v, off, q.rank, q.n)) ||
+                (rc = upload_async(c, c->vb_runs, c->stage_v, off, reinterpret_cast<const uint2*>(q.runs),
+                                   q.num_runs)) ||
+                (rc = upload_async(c, c->vb_base, c->stage_v, off, q.group_base, q.groups)) ||
+                (rc = upload_async(c, c->vel_grp, c->stage_v, off, reinterpret_cast<const uint2*>(vs.grp.data()),
+                                   vs.grp.size())))
+                return rc;
+            const size_t n1 = std::max<uint32_t>(q.n, 1), l1 = size_t(vs.levels) + 1, u1 = size_t(vs.units) + 1;
+            for (auto* d : {&c->vb_gl}) KZG_HIP(c, d->ensure(n1));
+            for (auto* d : {&c->vb_count, &c->vb_fill}) KZG_HIP(c, d->ensure(l1));
+            // Group 0's level counts straight into the head of the count array
+            // (sized above: the upload then does not reallocate it).
+            if ((rc = upload_async(c, c->vb_count, c->stage_v, off, q.g0_count, q.g0_levels))) return rc;
+            for (auto* d : {&c->vb_uslot, &c->vb_usize}) KZG_HIP(c, d->ensure(u1));
+            KZG_HIP(c, c->vel_off.ensure(l1));
+            KZG_HIP(c, c->vel_unit.ensure(u1));
+            KZG_HIP(c, c->vel_pos.ensure(std::max<uint32_t>(c->npoints, 1)));
+            KZG_HIP(c, c->vb_sums.ensure(vel_bucket_sums_words(vs.levels, vs.units)));
+            const VelBucketArgs a{c->vb_pts.p, c->vb_uid.p, c->vb_lvl.p, q.n, c->vb_runs.p, q.num_runs,
+                                  c->vb_base.p, c->vb_rank.p, q.g0_levels, vs.levels, vs.units, c->npoints,
+                                  c->vb_gl.p, c->vb_count.p,
+                                  c->vb_fill.p, c->vb_uslot.p, c->vb_usize.p, c->vel_off.p, c->vel_unit.p,
+                                  c->vel_pos.p, c->vb_sums.p};
+            // The upload ends here; the bucketing is the solve's device work.
+            KZG_HIP(c, record(c, 16));
+            launch_vel_bucket(c->stream, a);
+            KZG_HIP(c, hipGetLastError());
+        } else {
+            const size_t bytes = sizeof(uint32_t) * (vs.off.size() + vs.unit.size() + size_t(c->npoints)) +
+                                 sizeof(U2) * vs.grp.size() + 5 * 256;
+            KZG_HIP(c, c->stage_v.ensure(bytes));
+            size_t off = 0;
+            if ((rc = upload_async(c, c->vel_off, c->stage_v, off, vs.off.data(), vs.off.size())) ||
+                (rc = upload_async(c, c->vel_grp, c->stage_v, off, reinterpret_cast<const uint2*>(vs.grp.data()),
+                                   vs.grp.size())) ||
+                (rc = upload_async(c, c->vel_pos, c->stage_v, off, vs.pos.data(), vs.pos.size())) ||
+                (rc = upload_async(c, c->vel_unit, c->stage_v, off, vs.unit.data(), vs.unit.size())))
+                return rc;
+            KZG_HIP(c, record(c, 16));
+        }
         launch_velocity_records(c->stream, c->contacts(), c->ws_records(), c->vel_records(), c->vel_pos.p,
                                 c->ws_pos.p, c->npoints, w.use_simd_solver ? 1 : 0, w.use_velocity_bias ? 1 : 0);
     }
@@ -1495,6 +1539,13 @@ int kzg_set_cooperative_placement(kzg_ctx* c, uint32_t placement) {
     if (placement != KZG_COOP_ONE_XCD && placement != KZG_COOP_SPREAD)
         return fail(c, KZG_ERR_ARGUMENT, "placement is KZG_COOP_ONE_XCD or KZG_COOP_SPREAD");
     c->coop_placement = placement;
+    return KZG_OK;
+}
+
+int kzg_set_device_bucketing(kzg_ctx* c, uint32_t mode) {
+    if (!c) return KZG_ERR_ARGUMENT;
+    if (mode > 1) return fail(c, KZG_ERR_ARGUMENT, "device bucketing mode is 0 (auto) or 1 (off)");
+    c->device_bucketing = mode;
     return KZG_OK;
 }
 

@@ -55,6 +55,32 @@ hipError_t launch_solve_split(hipStream_t s, const VelRecBuf& vr, const WsRecBuf
                               uint32_t bodies = kPackBodies | kUnpackBodies);
 // Clears a sync block of kSyncWords words, its give-up flag included.
 void launch_sync_reset(hipStream_t s, uint32_t* sync);
+// The velocity sequence's bucketing on the device (bucket.hip): inputs from
+// the host's level walk (ScheduleBuilder::velocity_sequence), scratch, and
+// the outputs the sweeps read (level offsets, unit bounds, point records).
+struct VelBucketArgs {
+    const uint32_t* pts;
+    const uint32_t* uid;
+    const uint32_t* lvl;
+    uint32_t n;
+    const uint2* runs;
+    uint32_t num_runs;
+    const uint32_t* group_base;
+    const uint32_t* rank;  // [n] place in its level of a group-0 unit's first point, else 0xFFFFFFFF
+    uint32_t g0_levels;    // count[0 .. g0_levels) holds group 0's counts already
+    uint32_t levels, units, num_points;
+    uint32_t* gl;      // [n]
+    uint32_t* count;   // [levels]
+    uint32_t* fill;    // [levels]
+    uint32_t* uslot;   // [units]
+    uint32_t* usize;   // [units]
+    uint32_t* off;     // [levels + 1] out
+    uint32_t* unit;    // [units + 1] out
+    uint32_t* pos;     // [num_points] out
+    uint32_t* sums;    // [vel_bucket_sums_words(levels, units)] scan scratch
+};
+uint32_t vel_bucket_sums_words(uint32_t levels, uint32_t units);
+void launch_vel_bucket(hipStream_t s, const VelBucketArgs& a);
 void launch_compact(hipStream_t s, const uint32_t* m_count, uint32_t n, const uint2* pairs, const BodyBuf& b,
                     const ContactBuf& c, uint2* tile, uint32_t* totals);
 // lds_limit: most bodies for the one-workgroup LDS union-find (0: the
@@ -164,6 +190,11 @@ struct kzg_ctx {
     DevVec<uint32_t> coop_sync;      // sync block of the cooperative solve (kSyncWords words)
     uint32_t coop_blocks = 0;        // kzg_set_cooperative: 0 auto, 1 off, n workgroups
     uint32_t coop_placement = 0;     // kzg_set_cooperative_placement: KZG_COOP_ONE_XCD or KZG_COOP_SPREAD
+    uint32_t device_bucketing = 0;   // kzg_set_device_bucketing: 0 auto (on), 1 off
+    // Device bucketing of the velocity sequence: the walk's per-point arrays,
+    // island runs and group bases (uploaded), and the bucketing's scratch.
+    DevVec<uint32_t> vb_pts, vb_uid, vb_lvl, vb_rank, vb_gl, vb_base, vb_count, vb_fill, vb_uslot, vb_usize, vb_sums;
+    DevVec<uint2> vb_runs;
     uint32_t spin_limit = 0;         // kzg_set_spin_limit: 0 default
     bool coop_launched = false;      // this step's solve made a cooperative launch
     uint32_t spin_limit_or_default() const { return spin_limit ? spin_limit : kDefaultSpinLimit; }

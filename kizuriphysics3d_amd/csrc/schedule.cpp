@@ -129,6 +129,8 @@ void ScheduleBuilder::prefix_levels(Seq& s, uint32_t total, LevelSchedule& out) 
 
 void ScheduleBuilder::place_points(Seq& s, uint32_t num_points, LevelSchedule& out) {
     for (uint32_t k = 0; k < s.nu; ++k) out.unit[k + 1] += out.unit[k];
+    out.device_bucketed = false;
+    out.g0_points = out.grp.empty() ? 0u : out.unit[out.off[out.grp[0].y]] - out.unit[out.off[out.grp[0].x]];
     out.idx.resize(s.n);
     out.pos.assign(num_points, kNone);
     for (uint32_t i = 0; i < s.n; ++i) {
@@ -467,9 +469,23 @@ void ScheduleBuilder::vel_levels(const ScheduleInput& in) {
     uint32_t* last = vel_body_level_.data();
     uint32_t *pts = s.pts.data(), *uid = s.uid.data(), *lvl = s.lvl.data();
     uint32_t n = 0, nu = 0, x = 0;
+    // Per group: the deepest level; group 0: units per level (its levels
+    // cannot outnumber the batches); every other group counts into a scratch
+    // row that is never read.
+    vel_depth_.assign(groups_, 0);
+    vel_g0_count_.assign(size_t(batches_.size()) + 2, 0);
+    std::vector<uint32_t>& scratch = vel_group_base_;  // reused below as the group bases
+    scratch.assign(size_t(batches_.size()) + 2, 0);
+    uint32_t jmax = 0;
+    vel_rank_.resize(s.pts.size());
+    uint32_t* rank = vel_rank_.data();
     for (uint32_t is = 0; is < ni; ++is) {
         const uint32_t n0 = n;
         uint32_t pb = n;  // first point of the previous batch
+        const uint32_t grp = group_of_island_[is];
+        uint32_t* count = grp == 0 ? vel_g0_count_.data() : scratch.data();
+        const uint32_t rank_mask = grp == 0 ? 0u : kNone;  // group 0's units take their rank
+        uint32_t deepest = 0;
         for (; x < fit_island_end_[is]; ++x) {
             const FitBatch& B = batches[x];
             const uint32_t cb = n;
@@ -489,14 +505,78 @@ void ScheduleBuilder::vel_levels(const ScheduleInput& in) {
                 pts[n] = B.pt[q];
                 uid[n] = ux;
                 lvl[n] = l;
+                // A unit's place among its level's units (group 0): the count
+                // of the level before it.
+                const uint32_t before = count[l];
+                count[l] = before + 1 - uint32_t(join);
+                rank[n] = before | rank_mask;
+                deepest = deepest > l ? deepest : l;
+                jmax = jmax > (ux & 3u) ? jmax : (ux & 3u);
                 ++n;
             }
             pb = cb;
         }
-        if (n > n0) s.runs.push_back(U2{n, group_of_island_[is]});
+        if (n > n0) s.runs.push_back(U2{n, grp});
+        vel_depth_[grp] = std::max(vel_depth_[grp], deepest);
     }
     s.n = n;
     s.nu = nu;
+    vel_jmax_ = jmax;
+}
+
+// The host's share of a velocity sequence bucketed on the device: group
+// level ranges, group 0's level offsets (for the launch plan), counts.
+void ScheduleBuilder::vel_lite(uint32_t num_points, LevelSchedule& out) {
+    (void)num_points;
+    Seq& s = vel_;
+    out.device_bucketed = true;
+    out.largest_unit = s.nu ? vel_jmax_ + 1 : 0;
+    out.grp.assign(groups_, U2{0, 0});
+    vel_group_base_.assign(groups_, 0);
+    uint32_t total = 0;
+    out.max_depth = 0;
+    for (uint32_t g = 0; g < groups_; ++g) {
+        vel_group_base_[g] = total;
+        out.grp[g] = U2{total, total + vel_depth_[g]};
+        total += vel_depth_[g];
+        out.max_depth = std::max(out.max_depth, vel_depth_[g]);
+    }
+    out.levels = total;
+    out.units = s.nu;
+    const uint32_t d0 = groups_ ? vel_depth_[0] : 0u;
+    out.off.assign(d0 + 1, 0);
+    out.max_width = 0;
+    for (uint32_t l = 0; l < d0; ++l) {
+        const uint32_t c = vel_g0_count_[l + 1];
+        out.max_width = std::max(out.max_width, c);
+        out.off[l + 1] = out.off[l] + c;
+    }
+    uint32_t g0 = 0;
+    for (uint32_t is = 0; is < uint32_t(island_points_.size()); ++is)
+        if (group_of_island_[is] == 0) g0 += island_points_[is];
+    out.g0_points = g0;
+    out.idx.clear();
+    out.pos.clear();
+    out.unit.clear();
+}
+
+VelSequence ScheduleBuilder::velocity_sequence() const {
+    VelSequence v;
+    v.pts = vel_.pts.data();
+    v.uid = vel_.uid.data();
+    v.lvl = vel_.lvl.data();
+    v.n = vel_.n;
+    v.units = vel_.nu;
+    v.runs = vel_.runs.data();
+    v.num_runs = uint32_t(vel_.runs.size());
+    v.group_base = vel_group_base_.data();
+    v.groups = groups_;
+    v.rank = vel_rank_.data();
+    v.g0_count = vel_g0_count_.data() + 1;
+    v.g0_levels = groups_ ? vel_depth_[0] : 0u;
+    v.levels = 0;
+    for (uint32_t g = 0; g < groups_; ++g) v.levels += vel_depth_[g];
+    return v;
 }
 
 void ScheduleBuilder::build(const ScheduleInput& in, ScheduleOutput& out) {
@@ -532,13 +612,21 @@ bool ScheduleBuilder::start(const ScheduleInput& in_arg, ScheduleOutput& out, ui
         flatten(ws_, groups_, num_points, out.ws);
         ws_ms = std::chrono::duration<double, std::milli>(clock::now() - t0).count();
     };
-    auto vel_chain = [this, &in, &out, num_points] {
+    auto vel_chain = [this, &in, &out, num_points](bool may_defer_bucketing) {
         vel_body_level_.assign(in.num_bodies, 0);
         vel_.runs.clear();
         fit_island_end_.assign(island_points_.size(), 0);
         vel_fit(in);
         vel_levels(in);
-        flatten_points(vel_, groups_, num_points, out.vel);
+        // Device bucketing pays where one group dominates (the launch plan
+        // and thread counts are taken from group 0's levels).
+        uint32_t g0 = 0;
+        for (uint32_t is = 0; is < uint32_t(island_points_.size()); ++is)
+            if (group_of_island_[is] == 0) g0 += island_points_[is];
+        if (may_defer_bucketing && device_bucket_ && 2u * g0 >= vel_.n)
+            vel_lite(num_points, out.vel);
+        else
+            flatten_points(vel_, groups_, num_points, out.vel);
     };
     const bool parallel = num_points >= kParallelPoints;
     const bool simd = in.use_simd;
@@ -548,7 +636,7 @@ bool ScheduleBuilder::start(const ScheduleInput& in_arg, ScheduleOutput& out, ui
         // velocity sequence on the helper and the warm-start one here.
         islands(in, out);
         run_helper([this, vel_chain] {
-            vel_chain();
+            vel_chain(true);
             vel_done_ = clock::now();
         });
         ws_chain(false);
@@ -560,7 +648,7 @@ bool ScheduleBuilder::start(const ScheduleInput& in_arg, ScheduleOutput& out, ui
     islands(in, out);
     groups_ready_.store(true, std::memory_order_release);
     if (!parallel) ws_chain(false);
-    if (simd) vel_chain();
+    if (simd) vel_chain(false);
     const auto tw = clock::now();
     if (parallel) wait_helper();
     wait_ms = std::chrono::duration<double, std::milli>(clock::now() - tw).count();

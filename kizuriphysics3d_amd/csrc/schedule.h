@@ -43,6 +43,33 @@ struct LevelSchedule {
     uint32_t levels = 0;      // total levels over all groups
     uint32_t max_depth = 0;   // deepest group
     uint32_t max_width = 0;   // widest level, in units
+    uint32_t g0_points = 0;   // points of group 0
+    // Left for the device to bucket (ScheduleBuilder::set_device_bucketing):
+    // idx, pos and unit are empty, off holds group 0's levels only, and
+    // max_width is group 0's widest level.
+    bool device_bucketed = false;
+};
+
+// The velocity sequence as the level walk leaves it, for bucketing on the
+// device: per point in sequence order its point index, unit << 2 | index in
+// the unit (units numbered in the order of their first points) and level
+// (1-based, within its group); the runs of consecutive points of one island
+// (end point, group); and each group's first global level.
+struct VelSequence {
+    const uint32_t* pts = nullptr;
+    const uint32_t* uid = nullptr;
+    const uint32_t* lvl = nullptr;
+    uint32_t n = 0, units = 0;
+    const U2* runs = nullptr;
+    uint32_t num_runs = 0;
+    const uint32_t* group_base = nullptr;  // [groups]
+    uint32_t groups = 0, levels = 0;
+    // Per point: for the first point of a unit of group 0, the unit's place
+    // among its level's units in sequence order; 0xFFFFFFFF for other groups
+    // (the device assigns those).  Group 0's units per level [g0_levels].
+    const uint32_t* rank = nullptr;
+    const uint32_t* g0_count = nullptr;
+    uint32_t g0_levels = 0;
 };
 
 struct ScheduleInput {
@@ -99,6 +126,11 @@ public:
     bool start(const ScheduleInput& in, ScheduleOutput& out, uint32_t defer);
     void finish();
     bool pending() const { return pending_; }
+    // With on, a deferred velocity sequence whose group 0 holds at least
+    // half the points stops after the level walk (out.vel.device_bucketed):
+    // the caller buckets it on the device from velocity_sequence().
+    void set_device_bucketing(bool on) { device_bucket_ = on; }
+    VelSequence velocity_sequence() const;
     // Of the last build: the warm-start sequence's time, the time spent
     // waiting for the other thread's sequence, and (deferred builds) the
     // velocity sequence's time after start() returned.
@@ -149,6 +181,12 @@ private:
     ScheduleInput in_{};
     std::chrono::steady_clock::time_point ws_ready_, vel_done_;
     std::vector<uint32_t> fit_island_end_;  // per island: end of its batches
+    // Level walk of the velocity sequence: deepest level of each group, the
+    // largest unit index, and group 0's units per level (index level).
+    std::vector<uint32_t> vel_depth_, vel_g0_count_, vel_group_base_, vel_rank_;
+    uint32_t vel_jmax_ = 0;
+    bool device_bucket_ = false;
+    void vel_lite(uint32_t num_points, LevelSchedule& out);
 };
 
 }  // namespace kz

@@ -43,7 +43,7 @@ EXPORTS = [
     "kzg_update_sleeping", "kzg_get_active_bodies", "kzg_set_active_bodies", "kzg_set_sleep_thresholds",
     "kzg_step", "kzg_query_sizes", "kzg_download_states", "kzg_download_manifolds", "kzg_download_constraints",
     "kzg_get_residuals", "kzg_get_natural_residual", "kzg_get_timings", "kzg_collide_batch", "kzg_collide_batch_device", "kzg_schedule", "kzg_set_warm_start",
-    "kzg_set_iterations", "kzg_set_cooperative", "kzg_set_sweep_units", "kzg_set_early_warm_start", "kzg_set_narrowphase_waves", "kzg_set_island_lds_limit", "kzg_set_cooperative_placement", "kzg_set_spin_limit", "kzg_cache_info", "kzg_download_cache", "kzg_upload_cache",
+    "kzg_set_iterations", "kzg_set_cooperative", "kzg_set_sweep_units", "kzg_set_early_warm_start", "kzg_set_narrowphase_waves", "kzg_set_island_lds_limit", "kzg_set_cooperative_placement", "kzg_set_spin_limit", "kzg_set_device_bucketing", "kzg_cache_info", "kzg_download_cache", "kzg_upload_cache",
     "kzg_mlp_init", "kzg_mlp_set_params", "kzg_mlp_get_params", "kzg_mlp_set_normalization",
     "kzg_mlp_get_normalization", "kzg_mlp_collect", "kzg_mlp_num_samples", "kzg_mlp_get_samples",
     "kzg_mlp_fit_normalization", "kzg_mlp_train", "kzg_mlp_set_target_clip", "kzg_set_learned_features", "kzg_mlp_forward", "kzg_mlp_gradients",
@@ -100,7 +100,7 @@ def library():
             "kzg_set_narrowphase_waves": ([vp, u32], i32),
             "kzg_set_island_lds_limit": ([vp, u32], i32),
             "kzg_set_cooperative_placement": ([vp, u32], i32),
-            "kzg_set_spin_limit": ([vp, u32], i32),
+            "kzg_set_spin_limit": ([vp, u32], i32), "kzg_set_device_bucketing": ([vp, u32], i32),
             "kzg_cache_info": ([vp, vp, vp], i32), "kzg_download_cache": ([vp, vp, C.c_uint64], i32),
             "kzg_upload_cache": ([vp, vp, C.c_uint64, u32, u32], i32),
             "kzg_mlp_init": ([vp, C.c_uint64], i32), "kzg_mlp_set_params": ([vp, vp, u32], i32),
@@ -349,6 +349,11 @@ class DeviceContext:
     def set_cooperative_placement(self, placement: int):
         """COOP_ONE_XCD (0, default) or COOP_SPREAD (1)."""
         self._check(library().kzg_set_cooperative_placement(self.h, int(placement)), "kzg_set_cooperative_placement")
+
+    def set_device_bucketing(self, mode: int):
+        """0 (default): an early warm start's velocity schedule is bucketed
+        by level on the device; 1: on the host."""
+        self._check(library().kzg_set_device_bucketing(self.h, int(mode)), "kzg_set_device_bucketing")
 
     def set_spin_limit(self, polls: int):
         """Most polls of one wait of a cooperative sweep (0 default)."""

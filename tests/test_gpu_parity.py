@@ -238,20 +238,24 @@ def test_pile_trajectory_early_warm_start_bit_exact(blocks):
 def test_early_warm_start_off_equals_on_large_pile():
     """On a 4096-body pile, with the largest group split over cooperating
     workgroups, the state after every step is the same with the early warm
-    start off and forced on."""
+    start off and forced on, the latter with its velocity schedule bucketed
+    on the device (default) and on the host."""
     from kizuriphysics3d_amd.world import PhysicsWorld
     scene = S.pile(4096, seed=0x4B5A0104)
     worlds = []
-    for mode in (1, 2):
+    for mode, bucketing in ((1, 0), (2, 0), (2, 1)):
         w = PhysicsWorld(scene)
         w.device_context().set_early_warm_start(mode)
+        w.device_context().set_device_bucketing(bucketing)
         worlds.append(w)
     early = 0
     for i in range(300):
         for w in worlds:
             w.step_fixed()
-        j = first_mismatch(worlds[0].states(), worlds[1].states())
-        assert j is None, f"step {i}: body {j} differs with the early warm start"
+        base = worlds[0].states()
+        for k, w in enumerate(worlds[1:]):
+            j = first_mismatch(base, w.states())
+            assert j is None, f"step {i}: body {j} differs with the early warm start (world {k + 1})"
         early += worlds[1].device_context().timings()["early_warm_start_ms"] > 0
     assert early > 0, "the warm start never started early"
     print(f"early warm start on {early} of 300 steps", flush=True)
