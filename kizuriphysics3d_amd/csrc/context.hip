@@ -551,12 +551,13 @@ void kzg_destroy(kzg_ctx* c) {
     c->stage_a.release();
     c->stage_b.release();
     c->stage_v.release();
+    c->stage_seq.release();
     c->pin_totals.release();
     c->pin_info.release();
     c->pin_flag.release();
     c->coop_sync.release();
     c->body_load.release();
-    for (auto* d : {&c->vb_pts, &c->vb_uid, &c->vb_lvl, &c->vb_rank, &c->vb_gl, &c->vb_base, &c->vb_count,
+    for (auto* d : {&c->vb_pts, &c->vb_uid, &c->vb_lvl, &c->vb_rank, &c->vb_seq, &c->vb_gl, &c->vb_base, &c->vb_count,
                     &c->vb_fill, &c->vb_uslot, &c->vb_usize, &c->vb_sums})
         d->release();
     c->vb_runs.release();
@@ -880,6 +881,16 @@ int kzg_build_constraints(kzg_ctx* c, int warm) {
         return KZG_OK;
     }
     c->sched_builder.set_device_bucketing(c->device_bucketing == 0);
+    // A velocity schedule bucketed on the device leaves the level walk's
+    // arrays in pinned memory; the last upload from there has completed (the
+    // stream has passed the table download above).
+    if (c->device_bucketing == 0 && defer != ScheduleBuilder::kDeferNever) {
+        const size_t cap = std::max<uint32_t>(c->npoints, 1);
+        KZG_HIP(c, c->stage_seq.ensure(4 * sizeof(uint32_t) * cap));
+        c->sched_builder.set_vel_sink(c->stage_seq.as<uint32_t>(), cap);
+    } else {
+        c->sched_builder.set_vel_sink(nullptr, 0);
+    }
     c->early_ws = c->sched_builder.start(si, c->sched, defer);
     c->vel_after_ms = 0.0;
     // Host compute of the build ends here; what follows stages the
@@ -1015,15 +1026,32 @@ int kzg_solve(kzg_ctx* c, float dt) {
         if (vs.device_bucketed) {
             // The walk's per-point arrays go up; the device buckets them.
             const VelSequence q = c->sched_builder.velocity_sequence();
-            const size_t bytes = sizeof(uint32_t) * (4 * size_t(q.n) + q.groups + q.g0_levels) +
+            const size_t bytes = sizeof(uint32_t) * ((q.in_sink ? 0 : 4 * size_t(q.n)) + q.groups + q.g0_levels) +
                                  sizeof(U2) * vs.grp.size() + sizeof(uint2) * q.num_runs + 9 * 256;
             KZG_HIP(c, c->stage_v.ensure(bytes));
             size_t off = 0;
-            if ((rc = upload_async(c, c->vb_pts, c->stage_v, off, q.pts, q.n)) ||
-                (rc = upload_async(c, c->vb_uid, c->stage_v, off, q.uid, q.n)) ||
-                (rc = upload_async(c, c->vb_lvl, c->stage_v, off, q.lvl, q.n)) ||
-                (rc = upload_async(c, c->vb_rank, c->stage_v, off, q.rank, q.n)) ||
-                (rc = upload_async(c, c->vb_runs, c->stage_v, off, reinterpret_cast<const uint2*>(q.runs),
+            const uint32_t *d_pts, *d_uid, *d_lvl, *d_rank;
+            if (q.in_sink) {
+                // The walk wrote the four arrays into stage_seq: one transfer.
+                KZG_HIP(c, c->vb_seq.ensure(4 * q.stride));
+                KZG_HIP(c, hipMemcpyAsync(c->vb_seq.p, c->stage_seq.p, 4 * sizeof(uint32_t) * q.stride,
+                                          hipMemcpyHostToDevice, c->stream));
+                d_pts = c->vb_seq.p;
+                d_uid = d_pts + q.stride;
+                d_lvl = d_pts + 2 * q.stride;
+                d_rank = d_pts + 3 * q.stride;
+            } else {
+                if ((rc = upload_async(c, c->vb_pts, c->stage_v, off, q.pts, q.n)) ||
+                    (rc = upload_async(c, c->vb_uid, c->stage_v, off, q.uid, q.n)) ||
+                    (rc = upload_async(c, c->vb_lvl, c->stage_v, off, q.lvl, q.n)) ||
+                    (rc = upload_async(c, c->vb_rank, c->stage_v, off, q.rank, q.n)))
+                    return rc;
+                d_pts = c->vb_pts.p;
+                d_uid = c->vb_uid.p;
+                d_lvl = c->vb_lvl.p;
+                d_rank = c->vb_rank.p;
+            }
+            if ((rc = upload_async(c, c->vb_runs, c->stage_v, off, reinterpret_cast<const uint2*>(q.runs),
                                    q.num_runs)) ||
                 (rc = upload_async(c, c->vb_base, c->stage_v, off, q.group_base, q.groups)) ||
                 (rc = upload_async(c, c->vel_grp, c->stage_v, off, reinterpret_cast<const uint2*>(vs.grp.data()),
@@ -1040,8 +1068,8 @@ int kzg_solve(kzg_ctx* c, float dt) {
             KZG_HIP(c, c->vel_unit.ensure(u1));
             KZG_HIP(c, c->vel_pos.ensure(std::max<uint32_t>(c->npoints, 1)));
             KZG_HIP(c, c->vb_sums.ensure(vel_bucket_sums_words(vs.levels, vs.units)));
-            const VelBucketArgs a{c->vb_pts.p, c->vb_uid.p, c->vb_lvl.p, q.n, c->vb_runs.p, q.num_runs,
-                                  c->vb_base.p, c->vb_rank.p, q.g0_levels, vs.levels, vs.units, c->npoints,
+            const VelBucketArgs a{d_pts, d_uid, d_lvl, q.n, c->vb_runs.p, q.num_runs,
+                                  c->vb_base.p, d_rank, q.g0_levels, vs.levels, vs.units, c->npoints,
                                   c->vb_gl.p, c->vb_count.p,
                                   c->vb_fill.p, c->vb_uslot.p, c->vb_usize.p, c->vel_off.p, c->vel_unit.p,
                                   c->vel_pos.p, c->vb_sums.p};

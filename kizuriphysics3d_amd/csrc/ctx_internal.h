@@ -194,6 +194,7 @@ struct kzg_ctx {
     // Device bucketing of the velocity sequence: the walk's per-point arrays,
     // island runs and group bases (uploaded), and the bucketing's scratch.
     DevVec<uint32_t> vb_pts, vb_uid, vb_lvl, vb_rank, vb_gl, vb_base, vb_count, vb_fill, vb_uslot, vb_usize, vb_sums;
+    DevVec<uint32_t> vb_seq;  // the four per-point arrays in one block (from stage_seq)
     DevVec<uint2> vb_runs;
     uint32_t spin_limit = 0;         // kzg_set_spin_limit: 0 default
     bool coop_launched = false;      // this step's solve made a cooperative launch
@@ -295,6 +296,7 @@ struct kzg_ctx {
 
     // staging and timing
     Pinned stage, stage2, stage_a, stage_b, stage_v;  // stage_v: a deferred velocity schedule
+    Pinned stage_seq;  // the level walk's per-point arrays of a velocity schedule bucketed on the device
     Pinned pin_totals;  // constraints, points and cache occupancy of the step
     Pinned pin_info;    // the step's constraint table (the schedule's input)
     DevVec<uint2> compact_tile;

@@ -461,13 +461,19 @@ void ScheduleBuilder::vel_fit(const ScheduleInput& in) {
 
 // Levels of the batch sequence, unit by unit.  A joining lane takes the
 // unit of the lane it names in the previous batch (read back from uid).
-void ScheduleBuilder::vel_levels(const ScheduleInput& in) {
+void ScheduleBuilder::vel_levels(const ScheduleInput& in, bool to_sink) {
     const uint8_t* dyn = in.dynamic;
     Seq& s = vel_;
     const uint32_t ni = uint32_t(island_points_.size());
     const FitBatch* batches = batches_.data();
     uint32_t* last = vel_body_level_.data();
+    sunk_ = to_sink;
     uint32_t *pts = s.pts.data(), *uid = s.uid.data(), *lvl = s.lvl.data();
+    if (to_sink) {
+        pts = sink_;
+        uid = sink_ + sink_cap_;
+        lvl = sink_ + 2 * sink_cap_;
+    }
     uint32_t n = 0, nu = 0, x = 0;
     // Per group: the deepest level; group 0: units per level (its levels
     // cannot outnumber the batches); every other group counts into a scratch
@@ -477,8 +483,8 @@ void ScheduleBuilder::vel_levels(const ScheduleInput& in) {
     std::vector<uint32_t>& scratch = vel_group_base_;  // reused below as the group bases
     scratch.assign(size_t(batches_.size()) + 2, 0);
     uint32_t jmax = 0;
-    vel_rank_.resize(s.pts.size());
-    uint32_t* rank = vel_rank_.data();
+    if (!to_sink) vel_rank_.resize(s.pts.size());
+    uint32_t* rank = to_sink ? sink_ + 3 * sink_cap_ : vel_rank_.data();
     for (uint32_t is = 0; is < ni; ++is) {
         const uint32_t n0 = n;
         uint32_t pb = n;  // first point of the previous batch
@@ -565,13 +571,21 @@ VelSequence ScheduleBuilder::velocity_sequence() const {
     v.pts = vel_.pts.data();
     v.uid = vel_.uid.data();
     v.lvl = vel_.lvl.data();
+    v.rank = vel_rank_.data();
+    if (sunk_) {
+        v.pts = sink_;
+        v.uid = sink_ + sink_cap_;
+        v.lvl = sink_ + 2 * sink_cap_;
+        v.rank = sink_ + 3 * sink_cap_;
+        v.in_sink = true;
+        v.stride = sink_cap_;
+    }
     v.n = vel_.n;
     v.units = vel_.nu;
     v.runs = vel_.runs.data();
     v.num_runs = uint32_t(vel_.runs.size());
     v.group_base = vel_group_base_.data();
     v.groups = groups_;
-    v.rank = vel_rank_.data();
     v.g0_count = vel_g0_count_.data() + 1;
     v.g0_levels = groups_ ? vel_depth_[0] : 0u;
     v.levels = 0;
@@ -617,13 +631,14 @@ bool ScheduleBuilder::start(const ScheduleInput& in_arg, ScheduleOutput& out, ui
         vel_.runs.clear();
         fit_island_end_.assign(island_points_.size(), 0);
         vel_fit(in);
-        vel_levels(in);
         // Device bucketing pays where one group dominates (the launch plan
         // and thread counts are taken from group 0's levels).
         uint32_t g0 = 0;
         for (uint32_t is = 0; is < uint32_t(island_points_.size()); ++is)
             if (group_of_island_[is] == 0) g0 += island_points_[is];
-        if (may_defer_bucketing && device_bucket_ && 2u * g0 >= vel_.n)
+        const bool device = may_defer_bucketing && device_bucket_ && 2u * g0 >= num_points;
+        vel_levels(in, device && sink_ && sink_cap_ >= num_points);
+        if (device)
             vel_lite(num_points, out.vel);
         else
             flatten_points(vel_, groups_, num_points, out.vel);

@@ -70,6 +70,10 @@ struct VelSequence {
     const uint32_t* rank = nullptr;
     const uint32_t* g0_count = nullptr;
     uint32_t g0_levels = 0;
+    // The four per-point arrays sit in the sink (ScheduleBuilder::
+    // set_vel_sink) `stride` entries apart, from its start.
+    bool in_sink = false;
+    size_t stride = 0;
 };
 
 struct ScheduleInput {
@@ -130,6 +134,17 @@ public:
     // half the points stops after the level walk (out.vel.device_bucketed):
     // the caller buckets it on the device from velocity_sequence().
     void set_device_bucketing(bool on) { device_bucket_ = on; }
+    // Where the level walk of such a sequence writes its per-point arrays
+    // (point, unit, level, rank; `cap` entries each, one after the other):
+    // memory the caller uploads from, e.g. pinned, so the arrays reach the
+    // device without a copy on another thread, whose reads would also cost
+    // the next walk a transfer of every line back.  Null, or less room than
+    // the points of a build: the builder's own arrays.  The walk writes
+    // there until finish() returns.
+    void set_vel_sink(uint32_t* base, size_t cap) {
+        sink_ = base;
+        sink_cap_ = cap;
+    }
     VelSequence velocity_sequence() const;
     // Of the last build: the warm-start sequence's time, the time spent
     // waiting for the other thread's sequence, and (deferred builds) the
@@ -154,7 +169,7 @@ private:
     uint32_t union_find(const ScheduleInput& in);
     void ws_levels(const ScheduleInput& in);
     void vel_fit(const ScheduleInput& in);
-    void vel_levels(const ScheduleInput& in);
+    void vel_levels(const ScheduleInput& in, bool to_sink = false);
     uint32_t level_bases(Seq& s, uint32_t groups, LevelSchedule& out);
     void prefix_levels(Seq& s, uint32_t total, LevelSchedule& out);
     void place_points(Seq& s, uint32_t num_points, LevelSchedule& out);
@@ -186,6 +201,9 @@ private:
     std::vector<uint32_t> vel_depth_, vel_g0_count_, vel_group_base_, vel_rank_;
     uint32_t vel_jmax_ = 0;
     bool device_bucket_ = false;
+    uint32_t* sink_ = nullptr;
+    size_t sink_cap_ = 0;
+    bool sunk_ = false;  // the last walk wrote to the sink
     void vel_lite(uint32_t num_points, LevelSchedule& out);
 };
 
