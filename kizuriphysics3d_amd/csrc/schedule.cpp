@@ -384,12 +384,18 @@ void ScheduleBuilder::vel_fit(const ScheduleInput& in) {
         // touches h, so neither batch can take it again.  (Without the
         // second rule every floor point rescans the batches earlier floor
         // points joined.)
-        auto first_fit = [&](OpenList& l, uint32_t h, uint32_t a, uint32_t b) -> uint32_t {
+        //   `from` carries a constraint's search from point to point: every
+        // entry before the batch its last point took was dead or held a or b,
+        // and still is, and that batch now holds both, so the next point's
+        // search starts after it (after the list's end when the point opened
+        // a new batch: no entry could take it).
+        auto first_fit = [&](OpenList& l, uint32_t h, uint32_t a, uint32_t b, size_t& from) -> uint32_t {
             auto dead = [&](const FitBatch& B) { return B.n >= 4 || (h != kNone && holds(B, h)); };
             while (l.head < l.ids.size() && dead(batches[l.ids[l.head]])) ++l.head;
             size_t ndead = 0;
             uint32_t found = kNone;
-            for (size_t k = l.head; k < l.ids.size(); ++k) {
+            size_t k = std::max(from, l.head);
+            for (; k < l.ids.size(); ++k) {
                 const FitBatch& B = batches[l.ids[k]];
                 if (dead(B)) {
                     ++ndead;
@@ -399,12 +405,14 @@ void ScheduleBuilder::vel_fit(const ScheduleInput& in) {
                 found = l.ids[k];
                 break;
             }
+            from = found == kNone ? l.ids.size() : k + 1;
             if (ndead > 64 && ndead * 2 > l.ids.size() - l.head) {
                 size_t w = 0;
-                for (size_t k = l.head; k < l.ids.size(); ++k)
-                    if (!dead(batches[l.ids[k]])) l.ids[w++] = l.ids[k];
+                for (size_t q = l.head; q < l.ids.size(); ++q)
+                    if (!dead(batches[l.ids[q]])) l.ids[w++] = l.ids[q];
                 l.ids.resize(w);
                 l.head = 0;
+                from = 0;
             }
             return found;
         };
@@ -421,8 +429,9 @@ void ScheduleBuilder::vel_fit(const ScheduleInput& in) {
             if (!dyn[a]) cand = &open_without(h = a);
             else if (!dyn[b]) cand = &open_without(h = b);
             uint32_t prev = kNone, prev_lane = 0, run = 0;
+            size_t from = 0;
             for (uint32_t k = 0; k < np; ++k) {
-                const uint32_t placed = first_fit(*cand, h, a, b);
+                const uint32_t placed = first_fit(*cand, h, a, b, from);
                 const uint32_t id = placed == kNone ? uint32_t(batches.size()) : placed;
                 const bool join = (prev != kNone) & (id == prev + 1) & (run < cap);
                 run = join ? run + 1 : 1;
