@@ -994,8 +994,10 @@ int kzg_solve(kzg_ctx* c, float dt) {
         while (threads < c->sched.ws.max_width && threads < solve_threads(uk.ws, uk.vel)) threads *= 2;
         const CoopPlan coop = plan_cooperative(c->sched, c->coop_blocks, solve_threads(uk.ws, uk.vel), 1u,
                                                c->coop_placement == KZG_COOP_ONE_XCD, false);
+        // The bias sweeps need only the warm-start schedule too: they run
+        // here as well, and the velocity launch carries no bias levels.
         SolveParams spw = sp;
-        spw.phases = 1u;
+        spw.phases = kPhaseWarm | kPhaseBiasAlone;
         const Schedule none{nullptr, nullptr, nullptr};
         KZG_HIP(c, record(c, 14));
         if (coop.blocks > 1) {
@@ -1013,7 +1015,7 @@ int kzg_solve(kzg_ctx* c, float dt) {
         }
         KZG_HIP(c, hipGetLastError());
         KZG_HIP(c, record(c, 15));
-        sp.phases = 2u;
+        sp.phases = kPhaseVelocity | kPhaseBiasDone;
     }
     if (early) {
         // The velocity schedule: wait for it, send it up, then its records.

@@ -160,10 +160,13 @@ struct SolveParams {
     uint32_t position_iterations;
     uint32_t use_simd;
     uint32_t use_velocity_bias;
-    // Which sweeps a launch runs: 1 the warm start, 2 the velocity
-    // iterations (with the bias sweeps alongside), 3 both in order.
+    // Which sweeps a launch runs (kPhase* bits): the warm start; the bias
+    // sweeps on their own after it; the velocity iterations, with the bias
+    // sweeps alongside unless they ran already.  Default: warm start, then
+    // velocity with bias alongside.
     uint32_t phases = 3;
 };
+constexpr uint32_t kPhaseWarm = 1, kPhaseVelocity = 2, kPhaseBiasAlone = 4, kPhaseBiasDone = 8;
 // The sweeps run on packed copies of the bodies: whether a sweep launch
 // packs them first and unpacks them after (a warm-start-only launch packs,
 // the velocity launch that follows it unpacks).

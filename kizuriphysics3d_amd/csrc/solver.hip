@@ -875,7 +875,8 @@ __device__ __forceinline__ void velocity_and_bias(const WsRecBuf& wr, const SB& 
                                                   float* resid, uint32_t parts, uint32_t part, VelF&& vf,
                                                   Sync&& sync) {
     const uint32_t vsteps = (vrange.y - vrange.x) * sp.velocity_iterations;
-    const uint32_t bsteps = sp.use_velocity_bias ? 0u : (wrange.y - wrange.x) * sp.position_iterations;
+    const uint32_t bsteps =
+        sp.use_velocity_bias || (sp.phases & kPhaseBiasDone) ? 0u : (wrange.y - wrange.x) * sp.position_iterations;
     LevelStream<UV, L> v(vo, vrange, vbd, vel.unit, parts, part);
     LevelStream<UW, 1> b(wo, wrange, wr.bd, ws.unit, parts, part);
     float dsq = 0.0f;
@@ -898,17 +899,29 @@ __device__ __forceinline__ void solve_group(const VelRecBuf& vr, const WsRecBuf&
                                             uint32_t* ws_lds, uint32_t* vel_lds, Sync&& sync) {
     // A warm-start-only launch may run before the velocity schedule exists:
     // it reads nothing of it.
-    const bool velocity = (sp.phases & 2u) != 0;
+    const bool velocity = (sp.phases & kPhaseVelocity) != 0;
     const uint2 wrange = ws.grp[group];
     const uint2 vrange = velocity ? vel.grp[group] : make_uint2(0u, 0u);
     const LevelOffsets wo = stage_offsets(ws, wrange, ws_lds);
     const LevelOffsets vo = velocity ? stage_offsets(vel, vrange, vel_lds) : wo;
     __syncthreads();
     const float wsf = sp.warm_start_factor;
-    if (sp.phases & 1u) {
+    if (sp.phases & kPhaseWarm) {
         LevelStream<UW, 1> w(wo, wrange, wr.bd, ws.unit, parts, part);
         for (uint32_t k = wrange.x; k < wrange.y; ++k) {
             w.step([&](uint32_t e, uint32_t n, uint4 bd, uint32_t) { warm_unit<UW>(wr, sbv, e, n, bd, wsf); });
+            if (!sync()) return;
+        }
+    }
+    // The bias sweeps read and write only pseudo velocities and the bias
+    // impulses, which neither the warm start nor a velocity sweep touches,
+    // so they may also run here, before the velocity iterations: every body
+    // sees its pseudo-velocity updates in the same order.
+    if ((sp.phases & kPhaseBiasAlone) && !sp.use_velocity_bias) {
+        LevelStream<UW, 1> b(wo, wrange, wr.bd, ws.unit, parts, part);
+        const uint32_t bsteps = (wrange.y - wrange.x) * sp.position_iterations;
+        for (uint32_t k = 0; k < bsteps; ++k) {
+            b.step([&](uint32_t e, uint32_t n, uint4 bd, uint32_t) { bias_unit<UW>(wr, sbp, e, n, bd); });
             if (!sync()) return;
         }
     }
@@ -1233,7 +1246,7 @@ static void launch_groups(hipStream_t s, UnitKernel uk, uint32_t n, uint32_t thr
                           const SolveParams& sp, float* resid, uint32_t group0) {
     // Schedules without unit descriptors (the batched worlds) take one-point
     // units; a warm-start-only launch has no velocity schedule and needs none.
-    if (!ws.unit || (!vel.unit && (sp.phases & 2u))) uk = UnitKernel{};
+    if (!ws.unit || (!vel.unit && (sp.phases & kPhaseVelocity))) uk = UnitKernel{};
     by_units(uk, [&](auto w, auto v) {
         k_solve_groups<decltype(w)::value, decltype(v)::value>
             <<<n, threads, 0, s>>>(vr, wr, sbv, sbp, ws, vel, sp, resid, group0);
@@ -1277,7 +1290,7 @@ hipError_t launch_solve_split(hipStream_t s, const VelRecBuf& vr, const WsRecBuf
     if (!groups || !nbodies) return hipSuccess;
     // The cooperative sweep reads unit arrays (the velocity one only when it
     // runs the velocity iterations).
-    if (!ws.unit || (!vel.unit && (sp.phases & 2u))) return hipErrorInvalidValue;
+    if (!ws.unit || (!vel.unit && (sp.phases & kPhaseVelocity))) return hipErrorInvalidValue;
     float4* sbv = packed;
     float4* sbp = packed + 4 * size_t(nbodies);
     if (bodies & kPackBodies) k_pack_bodies<<<blocks(nbodies, 256), 256, 0, s>>>(b, nbodies, sbv, sbp);
