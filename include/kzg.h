@@ -352,10 +352,10 @@ int kzg_set_island_lds_limit(kzg_ctx* ctx, uint32_t max_bodies);
 /* Early warm start (no reference counterpart: an overlap of host and device
  * work inside PhysicsWorld::StepFixed's order).  With the SIMD solver and the
  * cache warm start, kzg_build_constraints may return once the warm-start
- * schedule is built, and kzg_solve then sweeps the warm start on the device
- * while the host finishes the velocity (first-fit) schedule.  mode: 0 = auto
- * (worlds of 16,384 or more contact points), 1 = off, 2 = always.  Results
- * are the same either way. */
+ * schedule is built, and kzg_solve then sweeps the warm start and the bias
+ * (position) iterations on the device while the host finishes the velocity
+ * (first-fit) schedule.  mode: 0 = auto (worlds of 16,384 or more contact
+ * points), 1 = off, 2 = always.  Results are the same either way. */
 int kzg_set_early_warm_start(kzg_ctx* ctx, uint32_t mode);
 
 /* ---- Learned warm start (no reference counterpart) ---------------------
