@@ -906,22 +906,27 @@ __device__ __forceinline__ void solve_group(const VelRecBuf& vr, const WsRecBuf&
     const LevelOffsets vo = velocity ? stage_offsets(vel, vrange, vel_lds) : wo;
     __syncthreads();
     const float wsf = sp.warm_start_factor;
-    if (sp.phases & kPhaseWarm) {
-        LevelStream<UW, 1> w(wo, wrange, wr.bd, ws.unit, parts, part);
-        for (uint32_t k = wrange.x; k < wrange.y; ++k) {
-            w.step([&](uint32_t e, uint32_t n, uint4 bd, uint32_t) { warm_unit<UW>(wr, sbv, e, n, bd, wsf); });
-            if (!sync()) return;
-        }
-    }
     // The bias sweeps read and write only pseudo velocities and the bias
     // impulses, which neither the warm start nor a velocity sweep touches,
     // so they may also run here, before the velocity iterations: every body
-    // sees its pseudo-velocity updates in the same order.
-    if ((sp.phases & kPhaseBiasAlone) && !sp.use_velocity_bias) {
+    // sees its pseudo-velocity updates in the same order.  Bias level k then
+    // shares warm-start level k's interval.  The two sweeps take the same
+    // levels in the same slices, so a record's warm-start unit (which reads
+    // its cached impulses) and its bias unit (which rewrites them with the
+    // bias impulse changed) run on one thread, in that order.
+    const uint32_t wsteps = (sp.phases & kPhaseWarm) ? wrange.y - wrange.x : 0u;
+    const uint32_t bsteps = (sp.phases & kPhaseBiasAlone) && !sp.use_velocity_bias
+                                ? (wrange.y - wrange.x) * sp.position_iterations
+                                : 0u;
+    if (wsteps || bsteps) {
+        LevelStream<UW, 1> w(wo, wrange, wr.bd, ws.unit, parts, part);
         LevelStream<UW, 1> b(wo, wrange, wr.bd, ws.unit, parts, part);
-        const uint32_t bsteps = (wrange.y - wrange.x) * sp.position_iterations;
-        for (uint32_t k = 0; k < bsteps; ++k) {
-            b.step([&](uint32_t e, uint32_t n, uint4 bd, uint32_t) { bias_unit<UW>(wr, sbp, e, n, bd); });
+        const uint32_t steps = wsteps > bsteps ? wsteps : bsteps;
+        for (uint32_t k = 0; k < steps; ++k) {
+            if (k < wsteps)
+                w.step([&](uint32_t e, uint32_t n, uint4 bd, uint32_t) { warm_unit<UW>(wr, sbv, e, n, bd, wsf); });
+            if (k < bsteps)
+                b.step([&](uint32_t e, uint32_t n, uint4 bd, uint32_t) { bias_unit<UW>(wr, sbp, e, n, bd); });
             if (!sync()) return;
         }
     }
